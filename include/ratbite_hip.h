@@ -1,0 +1,207 @@
+/*
+ * ratbite_hip.h — C ABI of the MI355X (gfx950) spectral path-tracing hot path.
+ *
+ * The reference drives its device code through the C++ class
+ * RenderingInterface (src/core/rendering_interface.h:19-226) with a
+ * LaunchParameters block (src/core/launch_parameters.h:16-96), OptiX program
+ * groups and three PTX kernels. This header replaces that boundary with plain
+ * C functions over plain pointers and sizes. Every entry point names the
+ * reference interface it replaces.
+ *
+ * Conventions
+ *  - Every function returns an int: RBH_OK (0) or a negative RBH_ERR_* code;
+ *    rbh_last_error(ctx) then holds a message. Nothing aborts the process.
+ *  - One host thread per context; calls are stream ordered on the context's
+ *    stream. Host buffers are caller owned and copied during the call.
+ *    Device memory is context owned.
+ *  - Geometry and lights are in camera-relative world space (the reference
+ *    translates instances, light triangles and tree nodes by -cameraPos on
+ *    the host, rendering_interface.cpp:986-994, 1116-1125, 1449-1457,
+ *    1484-1486).
+ */
+#ifndef RATBITE_HIP_H
+#define RATBITE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rbh_context* rbh_ctx;
+
+enum {
+    RBH_OK = 0,
+    RBH_ERR_INVALID = -1,   /* bad argument / size / layout */
+    RBH_ERR_HIP = -2,       /* HIP runtime failure */
+    RBH_ERR_STATE = -3,     /* call out of order (e.g. render before geometry) */
+    RBH_ERR_RANGE = -4,     /* size above a documented limit */
+    RBH_ERR_NOMEM = -5,     /* allocation failed */
+};
+
+/* ------------------------------------------------------------------ records
+ * Byte layouts are fixed (asserted in the implementation) so that numpy
+ * structured arrays and ctypes can fill them directly. */
+
+/* MaterialData (src/core/material.h:17-107) without raw device pointers:
+ * index/attribute buffers are the context's geometry buffers, textures are
+ * indices into the rbh_upload_textures table. 152 bytes. */
+typedef struct rbh_material {
+    uint32_t bxdf;            /* bxdfIndexSBT: 0 PureConductor, 1 PureDielectric, 2 ComplexSurface */
+    uint32_t flags;           /* 1 doubleSided, 2 sheenPresent, 4 spectral emission (see below) */
+    uint32_t attributes;      /* AttributeTypeBitfield: 1 NORMAL, 2 FRAME, 4 COLOR, 8 TEX_COORD_1, 16 TEX_COORD_2 */
+    uint32_t attributeStride; /* bytes per vertex in the attribute buffer */
+    uint32_t normalOffset, frameOffset, texCoord1Offset, texCoord2Offset;
+    uint32_t iorSpectrum;     /* indexOfRefractSpectrumDataIndex */
+    uint32_t acSpectrum;      /* absorpCoefSpectrumDataIndex */
+    uint32_t emissionSpectrum;/* emissionSpectrumDataIndex */
+    float mfRoughness, ior, alphaCutoff;
+    uint32_t textures;        /* TextureTypeBitfield */
+    uint32_t texCoordSets;    /* bit i: texture slot i reads TEX_COORD_2 (the *TexCoordSetIndex flags) */
+    int32_t tex[7];           /* baseColor, normal, metRough, transmission, sheenColor, sheenRough, emission */
+    uint32_t factors;         /* FactorTypeBitfield */
+    float baseColor[4];
+    float metalness, roughness, transmission;
+    float sheenColor[3];
+    float sheenRoughness;
+    float emissive[3];        /* emissiveFactor; with flag 4: emissive[0] scales spectra[emissionSpectrum] */
+} rbh_material;
+
+/* A texture (replaces CudaCombinedTexture, src/core/texture.h:237-500).
+ * Filtering is done in software with exact FP32 weights. 48 bytes. */
+typedef struct rbh_texture_desc {
+    uint32_t width, height, depth;
+    uint32_t format;          /* 0 RGBA8, 1 R8, 2 RGBA32F, 3 RGB32F, 4 R32F */
+    uint32_t srgb;            /* decode RGB from sRGB before filtering */
+    uint32_t addr[3];         /* 0 wrap, 1 clamp, 2 mirror */
+    uint32_t filter;          /* 0 nearest, 1 linear */
+    uint32_t pad;
+    uint64_t offset;          /* byte offset of the texels in the blob */
+} rbh_texture_desc;
+
+/* EmissiveTriangleLightData (src/core/light_tree_types.h:40-46), 68 bytes. */
+typedef struct rbh_emissive_tri {
+    float vertices[9];
+    float uvs[6];
+    uint32_t materialIndex;
+    uint32_t primitiveDataIndex;
+} rbh_emissive_tri;
+
+/* LightTree::PackedNode (src/core/light_tree_types.h:119-162), 32 bytes. */
+typedef struct rbh_packed_node {
+    float spatialMean[3];
+    float spatialVariance;
+    uint32_t packedAverageDirection;
+    float sharpness;
+    float flux;
+    uint32_t coreData;
+} rbh_packed_node;
+
+/* LaunchParameters' per-launch state (launch_parameters.h:18-50). */
+typedef struct rbh_launch_params {
+    uint32_t width, height;                      /* ResolutionState */
+    uint32_t maxPathDepth, maxReflectedPathDepth, maxTransmittedPathDepth; /* PathState */
+    uint32_t sampleOffset, sampleCount;          /* SamplingState */
+    float camU[3], camV[3], camW[3];             /* CameraState */
+    float perspectiveScaleW, perspectiveScaleH;
+    uint32_t depthOfField;
+    float aperture, focusDistance;
+    uint32_t flags;                              /* RBH_RENDER_* */
+} rbh_launch_params;
+
+enum {
+    RBH_RENDER_GUIDED = 1,        /* one-sample MIS with the guiding MLP */
+    RBH_RENDER_COLLECT = 2,       /* record guiding training samples */
+    RBH_RENDER_GUIDE_FP8 = 4,     /* FP8 E4M3 guiding inference */
+};
+
+enum {
+    RBH_CDF_FP64 = 1,             /* FP64 prefix sums and normalisation */
+};
+
+typedef struct rbh_stats {
+    uint64_t closestRays;         /* closest-hit queries of the last rbh_render */
+    uint64_t shadowRays;          /* any-hit queries issued (occluded-flag culled rays excluded) */
+    uint64_t pathVertices;        /* shaded path vertices */
+    uint64_t samples;             /* W*H*sampleCount */
+    double renderMs;              /* device time of the last rbh_render */
+    double traceMs;               /* time in closest-hit kernels */
+    double shadowMs;              /* time in any-hit kernels */
+    double shadeMs;               /* time in shading kernels */
+    double cdfMs;                 /* time of the last env-map CDF build kernel */
+    double bvhMs;                 /* time of the last LBVH build */
+    uint32_t traceLaunches;       /* closest-hit kernel launches of the last render */
+    uint32_t bvhDepth;            /* max depth of the LBVH */
+} rbh_stats;
+
+/* ----------------------------------------------------------------- context
+ * Replaces RenderingInterface::RenderingInterface / cleanup
+ * (rendering_interface.cpp:1952-1978, 1880-1950). */
+int rbh_create(int device, rbh_ctx* out);
+int rbh_destroy(rbh_ctx ctx);
+const char* rbh_last_error(rbh_ctx ctx);
+/* cudaEventQuery polling (rendering_interface.cpp:1982-1986): 1 idle, 0 busy */
+int rbh_query(rbh_ctx ctx);
+int rbh_synchronize(rbh_ctx ctx);
+
+/* ----------------------------------------------------------- scene uploads */
+/* LaunchParameters::spectra (n dense 471-sample spectra, spectral.cpp:1277-1286) */
+int rbh_upload_spectra(rbh_ctx ctx, const float* dense, uint32_t n);
+/* sensorSpectralCurveA/B/C and spectralBasisR/G/B
+ * (rendering_interface.cpp:518-527): 3 x 471 floats each */
+int rbh_upload_color_tables(rbh_ctx ctx, const float* xyz, const float* basis);
+/* loadLookUpTables (rendering_interface.cpp:528-622) */
+int rbh_upload_luts(rbh_ctx ctx, const float* conductor32x32, const float* dielOuter32x32x32,
+                    const float* dielInner32x32x32, const float* reflOuter32x32x32,
+                    const float* reflInner32x32x32, const float* sheen32x32x4);
+int rbh_upload_textures(rbh_ctx ctx, const rbh_texture_desc* descs, uint32_t n, const void* blob, uint64_t blobBytes);
+/* fillModelMaterials / addMaterial (rendering_interface.cpp:47-307, 1187-1290) */
+int rbh_upload_materials(rbh_ctx ctx, const rbh_material* mats, uint32_t n);
+/* Vertex, index and attribute buffers (replaces the GAS inputs and the
+ * per-material index/attribute pointers). triLight[i] is the light index of
+ * an emissive triangle (LightPointer TRIANGLE) or 0xFFFFFFFF. */
+int rbh_upload_geometry(rbh_ctx ctx, const float* verts, uint32_t nverts, const uint32_t* indices, uint32_t ntris,
+                        const void* attribs, uint64_t attribBytes, const uint32_t* triMaterial,
+                        const uint32_t* triLight);
+/* optixAccelBuild GAS/IAS (rendering_interface.cpp:733-1064): builds the LBVH */
+int rbh_build_accel(rbh_ctx ctx);
+
+/* ------------------------------------------------------------ light tree */
+/* RenderingInterface::buildLightTree + LightTree::Builder::build
+ * (rendering_interface.cpp:1065-1186, light_tree.cpp:47-79): builds the tree
+ * on the host from n emissive triangles with their flux, uploads nodes,
+ * light pointers, bitmasks and the triangle records. */
+int rbh_build_light_tree(rbh_ctx ctx, const rbh_emissive_tri* tris, const float* flux, uint32_t n);
+/* Read the built tree back: nodes (<= 2n), lightPointers (n), bitmasks (n). */
+int rbh_get_light_tree(rbh_ctx ctx, rbh_packed_node* nodes, uint32_t* nnodes, uint32_t* lightPointers,
+                       uint64_t* bitmasks);
+
+/* ---------------------------------------------------------- environment map
+ * loadEnvironmentMap + buildCDFLatLongRGBUpTo{4,8,16,32}k + invertCDFToIndices
+ * (rendering_interface.cpp:1339-1432, cdf.cu:468-535). texels: w*h*channels
+ * floats (channels 3 or 4), w and h multiples of 4, w <= 32768.
+ * The integral is read back after a stream sync. */
+int rbh_envmap_build(rbh_ctx ctx, const float* texels, int w, int h, int channels, uint32_t flags);
+/* Re-run the CDF build on the resident env map (device-resident input). */
+int rbh_envmap_rebuild(rbh_ctx ctx, uint32_t flags, int iterations);
+int rbh_envmap_read(rbh_ctx ctx, uint16_t* condIdx, uint16_t* margIdx, float* cdfY, double* integral);
+int rbh_envmap_disable(rbh_ctx ctx);
+
+/* ------------------------------------------------------------------ render
+ * optixLaunch(__raygen__main) (rendering_interface.cpp:1873-1879): renders
+ * sampleCount samples per pixel starting at sampleOffset into the FP64
+ * accumulation buffer (re-read when sampleOffset != 0). Asynchronous. */
+int rbh_render(rbh_ctx ctx, const rbh_launch_params* p);
+/* Copy the FP64 (x, y, z, w) accumulation buffer (W*H*4 doubles). */
+int rbh_read_accum(rbh_ctx ctx, double* rgbw);
+int rbh_write_accum(rbh_ctx ctx, const double* rgbw, uint32_t width, uint32_t height);
+/* renderResolve (image.cu:92-121): colour-space matrix (column-major 3x3),
+ * exposure scale 2^(e-7), AgX, RGBA8 out (W*H*4 bytes). */
+int rbh_resolve(rbh_ctx ctx, const float* colorspace, float exposureScale, uint8_t* rgba8);
+int rbh_stats_get(rbh_ctx ctx, rbh_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,166 @@
+"""ctypes binding of libratbite_hip.so (include/ratbite_hip.h).
+
+The library is the product: every compute call goes through it. Loading fails
+loudly when it has not been built; there is no fallback path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libratbite_hip.so")
+
+c_void_p = ctypes.c_void_p
+c_int = ctypes.c_int
+c_uint32 = ctypes.c_uint32
+c_uint64 = ctypes.c_uint64
+c_float = ctypes.c_float
+c_double = ctypes.c_double
+
+RBH_OK = 0
+RBH_ERR_INVALID = -1
+RBH_ERR_HIP = -2
+RBH_ERR_STATE = -3
+RBH_ERR_RANGE = -4
+RBH_ERR_NOMEM = -5
+
+RBH_RENDER_GUIDED = 1
+RBH_RENDER_COLLECT = 2
+RBH_RENDER_GUIDE_FP8 = 4
+RBH_CDF_FP64 = 1
+
+# Record layouts (byte-identical to include/ratbite_hip.h)
+MATERIAL_DTYPE = np.dtype([
+    ("bxdf", "<u4"), ("flags", "<u4"), ("attributes", "<u4"), ("attributeStride", "<u4"),
+    ("normalOffset", "<u4"), ("frameOffset", "<u4"), ("texCoord1Offset", "<u4"), ("texCoord2Offset", "<u4"),
+    ("iorSpectrum", "<u4"), ("acSpectrum", "<u4"), ("emissionSpectrum", "<u4"),
+    ("mfRoughness", "<f4"), ("ior", "<f4"), ("alphaCutoff", "<f4"),
+    ("textures", "<u4"), ("texCoordSets", "<u4"), ("tex", "<i4", (7,)), ("factors", "<u4"),
+    ("baseColor", "<f4", (4,)), ("metalness", "<f4"), ("roughness", "<f4"), ("transmission", "<f4"),
+    ("sheenColor", "<f4", (3,)), ("sheenRoughness", "<f4"), ("emissive", "<f4", (3,)),
+])
+assert MATERIAL_DTYPE.itemsize == 152
+
+TEXTURE_DTYPE = np.dtype([
+    ("width", "<u4"), ("height", "<u4"), ("depth", "<u4"), ("format", "<u4"), ("srgb", "<u4"),
+    ("addr", "<u4", (3,)), ("filter", "<u4"), ("pad", "<u4"), ("offset", "<u8"),
+])
+assert TEXTURE_DTYPE.itemsize == 48
+
+EMISSIVE_TRI_DTYPE = np.dtype([
+    ("vertices", "<f4", (9,)), ("uvs", "<f4", (6,)), ("materialIndex", "<u4"), ("primitiveDataIndex", "<u4"),
+])
+assert EMISSIVE_TRI_DTYPE.itemsize == 68
+
+PACKED_NODE_DTYPE = np.dtype([
+    ("spatialMean", "<f4", (3,)), ("spatialVariance", "<f4"), ("packedAverageDirection", "<u4"),
+    ("sharpness", "<f4"), ("flux", "<f4"), ("coreData", "<u4"),
+])
+assert PACKED_NODE_DTYPE.itemsize == 32
+
+
+class LaunchParams(ctypes.Structure):
+    _fields_ = [
+        ("width", c_uint32), ("height", c_uint32),
+        ("maxPathDepth", c_uint32), ("maxReflectedPathDepth", c_uint32), ("maxTransmittedPathDepth", c_uint32),
+        ("sampleOffset", c_uint32), ("sampleCount", c_uint32),
+        ("camU", c_float * 3), ("camV", c_float * 3), ("camW", c_float * 3),
+        ("perspectiveScaleW", c_float), ("perspectiveScaleH", c_float),
+        ("depthOfField", c_uint32), ("aperture", c_float), ("focusDistance", c_float),
+        ("flags", c_uint32),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("closestRays", c_uint64), ("shadowRays", c_uint64), ("pathVertices", c_uint64), ("samples", c_uint64),
+        ("renderMs", c_double), ("traceMs", c_double), ("shadowMs", c_double), ("shadeMs", c_double),
+        ("cdfMs", c_double), ("bvhMs", c_double), ("traceLaunches", c_uint32), ("bvhDepth", c_uint32),
+    ]
+
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "rbh_create": (c_int, [c_int, ctypes.POINTER(c_void_p)]),
+    "rbh_destroy": (c_int, [c_void_p]),
+    "rbh_last_error": (ctypes.c_char_p, [c_void_p]),
+    "rbh_query": (c_int, [c_void_p]),
+    "rbh_synchronize": (c_int, [c_void_p]),
+    "rbh_upload_spectra": (c_int, [c_void_p, c_void_p, c_uint32]),
+    "rbh_upload_color_tables": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "rbh_upload_luts": (c_int, [c_void_p] + [c_void_p] * 6),
+    "rbh_upload_textures": (c_int, [c_void_p, c_void_p, c_uint32, c_void_p, c_uint64]),
+    "rbh_upload_materials": (c_int, [c_void_p, c_void_p, c_uint32]),
+    "rbh_upload_geometry": (c_int, [c_void_p, c_void_p, c_uint32, c_void_p, c_uint32, c_void_p, c_uint64, c_void_p, c_void_p]),
+    "rbh_build_accel": (c_int, [c_void_p]),
+    "rbh_build_light_tree": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32]),
+    "rbh_get_light_tree": (c_int, [c_void_p, c_void_p, ctypes.POINTER(c_uint32), c_void_p, c_void_p]),
+    "rbh_envmap_build": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_uint32]),
+    "rbh_envmap_rebuild": (c_int, [c_void_p, c_uint32, c_int]),
+    "rbh_envmap_read": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_double)]),
+    "rbh_envmap_disable": (c_int, [c_void_p]),
+    "rbh_render": (c_int, [c_void_p, ctypes.POINTER(LaunchParams)]),
+    "rbh_read_accum": (c_int, [c_void_p, c_void_p]),
+    "rbh_write_accum": (c_int, [c_void_p, c_void_p, c_uint32, c_uint32]),
+    "rbh_resolve": (c_int, [c_void_p, c_void_p, c_float, c_void_p]),
+    "rbh_stats_get": (c_int, [c_void_p, ctypes.POINTER(Stats)]),
+}
+
+# Optional groups, declared in the other include/ headers
+_OPTIONAL_SIGS = {}
+
+_lib = None
+
+
+def declared_symbols():
+    return list(_SIGS) + list(_OPTIONAL_SIGS)
+
+
+def register(sigs):
+    """Adds signatures of another header's entry points."""
+    _OPTIONAL_SIGS.update(sigs)
+    if _lib is not None:
+        _apply(_lib, sigs)
+
+
+MISSING = []
+
+
+def _apply(lib, sigs):
+    for name, (res, args) in sigs.items():
+        try:
+            fn = getattr(lib, name)
+        except AttributeError:
+            MISSING.append(name)
+            continue
+        fn.restype = res
+        fn.argtypes = args
+
+
+def lib():
+    """Loads libratbite_hip.so; raises when it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libratbite_hip.so is not built (run __graft_entry__.build())")
+        l = ctypes.CDLL(LIB_PATH)
+        _apply(l, _SIGS)
+        _apply(l, _OPTIONAL_SIGS)
+        _lib = l
+    return _lib
+
+
+def ptr(a):
+    """Raw data pointer of a contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous")
+    return a.ctypes.data_as(c_void_p)
+
+
+class RbhError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("rbh error %d: %s" % (code, msg))
+        self.code = code

@@ -1,0 +1,154 @@
+"""Python handle over one rbh_ctx: uploads, env-map tables, rendering.
+
+Each method maps one-to-one onto an rbh_* C entry point and raises RbhError
+with the library's message on a non-zero status.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+
+class Context:
+    def __init__(self, device=0):
+        self._l = L.lib()
+        h = ctypes.c_void_p()
+        rc = self._l.rbh_create(device, ctypes.byref(h))
+        if rc != 0:
+            raise L.RbhError(rc, "rbh_create(device=%d) failed" % device)
+        self.h = h
+        self.width = 0
+        self.height = 0
+
+    # --------------------------------------------------------------- basics
+    def close(self):
+        if self.h:
+            self._l.rbh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def check(self, rc):
+        if rc != 0:
+            raise L.RbhError(rc, self._l.rbh_last_error(self.h).decode())
+        return rc
+
+    def call(self, name, *args):
+        return self.check(getattr(self._l, name)(self.h, *args))
+
+    def synchronize(self):
+        self.call("rbh_synchronize")
+
+    def query(self):
+        rc = self._l.rbh_query(self.h)
+        if rc < 0:
+            self.check(rc)
+        return rc == 1
+
+    def stats(self):
+        s = L.Stats()
+        self.call("rbh_stats_get", ctypes.byref(s))
+        return {k: getattr(s, k) for k, _ in L.Stats._fields_}
+
+    # -------------------------------------------------------------- uploads
+    def upload_spectra(self, dense):
+        dense = np.ascontiguousarray(dense, np.float32).reshape(-1, 471)
+        self.call("rbh_upload_spectra", L.ptr(dense), dense.shape[0])
+
+    def upload_color_tables(self, xyz, basis):
+        xyz = np.ascontiguousarray(xyz, np.float32).reshape(3, 471)
+        basis = np.ascontiguousarray(basis, np.float32).reshape(3, 471)
+        self.call("rbh_upload_color_tables", L.ptr(xyz), L.ptr(basis))
+
+    def upload_luts(self, luts):
+        arrs = [np.ascontiguousarray(luts[k], np.float32) for k in
+                ("conductor", "diel_outer", "diel_inner", "refl_outer", "refl_inner", "sheen")]
+        self.call("rbh_upload_luts", *[L.ptr(a) for a in arrs])
+
+    def upload_textures(self, descs, blob):
+        descs = np.ascontiguousarray(descs, L.TEXTURE_DTYPE)
+        blob = np.ascontiguousarray(blob, np.uint8)
+        self.call("rbh_upload_textures", L.ptr(descs) if len(descs) else None, len(descs),
+                  L.ptr(blob) if blob.size else None, blob.size)
+
+    def upload_materials(self, mats):
+        mats = np.ascontiguousarray(mats, L.MATERIAL_DTYPE)
+        self.call("rbh_upload_materials", L.ptr(mats), len(mats))
+
+    def upload_geometry(self, verts, indices, attribs, tri_material, tri_light):
+        verts = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
+        indices = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
+        attribs = np.ascontiguousarray(attribs, np.uint8).reshape(-1)
+        tri_material = np.ascontiguousarray(tri_material, np.uint32)
+        tri_light = np.ascontiguousarray(tri_light, np.uint32)
+        self.call("rbh_upload_geometry", L.ptr(verts), len(verts), L.ptr(indices), len(indices),
+                  L.ptr(attribs) if attribs.size else None, attribs.size, L.ptr(tri_material), L.ptr(tri_light))
+
+    def build_accel(self):
+        self.call("rbh_build_accel")
+
+    def build_light_tree(self, tris, flux):
+        tris = np.ascontiguousarray(tris, L.EMISSIVE_TRI_DTYPE)
+        flux = np.ascontiguousarray(flux, np.float32)
+        self.call("rbh_build_light_tree", L.ptr(tris) if len(tris) else None, L.ptr(flux) if len(flux) else None, len(tris))
+
+    def get_light_tree(self, nlights):
+        nodes = np.zeros(max(1, 2 * nlights), L.PACKED_NODE_DTYPE)
+        lp = np.zeros(max(1, nlights), np.uint32)
+        bm = np.zeros(max(1, nlights), np.uint64)
+        n = ctypes.c_uint32(0)
+        self.call("rbh_get_light_tree", L.ptr(nodes), ctypes.byref(n), L.ptr(lp), L.ptr(bm))
+        return nodes[:n.value], lp[:nlights], bm[:nlights]
+
+    # -------------------------------------------------------------- env map
+    def envmap_build(self, texels, fp64=False):
+        texels = np.ascontiguousarray(texels, np.float32)
+        h, w, ch = texels.shape
+        self.call("rbh_envmap_build", L.ptr(texels), w, h, ch, L.RBH_CDF_FP64 if fp64 else 0)
+        self._env = (w, h)
+
+    def envmap_rebuild(self, fp64=False, iterations=1):
+        self.call("rbh_envmap_rebuild", L.RBH_CDF_FP64 if fp64 else 0, iterations)
+
+    def envmap_read(self):
+        w, h = self._env
+        cond = np.zeros((h, w), np.uint16)
+        marg = np.zeros(h, np.uint16)
+        cdfy = np.zeros(h, np.float32)
+        integral = ctypes.c_double(0)
+        self.call("rbh_envmap_read", L.ptr(cond), L.ptr(marg), L.ptr(cdfy), ctypes.byref(integral))
+        return cond, marg, cdfy, integral.value
+
+    def envmap_disable(self):
+        self.call("rbh_envmap_disable")
+
+    # --------------------------------------------------------------- render
+    def render(self, params):
+        self.width, self.height = params.width, params.height
+        self.call("rbh_render", ctypes.byref(params))
+
+    def read_accum(self):
+        a = np.zeros((self.height, self.width, 4), np.float64)
+        self.call("rbh_read_accum", L.ptr(a))
+        return a
+
+    def write_accum(self, a):
+        a = np.ascontiguousarray(a, np.float64)
+        self.call("rbh_write_accum", L.ptr(a), a.shape[1], a.shape[0])
+
+    def resolve(self, colorspace, exposure_scale):
+        m = np.ascontiguousarray(colorspace, np.float32).reshape(9)
+        out = np.zeros((self.height, self.width, 4), np.uint8)
+        self.call("rbh_resolve", L.ptr(m), float(exposure_scale), L.ptr(out))
+        return out

@@ -1,0 +1,129 @@
+// Host-side context of the HIP path tracer: device buffers, stream, events
+// and error state behind the rbh_* C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <cstdarg>
+#include <string>
+#include <vector>
+#include "../../include/ratbite_hip.h"
+#include "device/types.hpp"
+
+namespace rbh {
+
+// Owning device allocation.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    // Grow-only: keeps the allocation when it is large enough.
+    hipError_t reserve(size_t n) {
+        if (n <= bytes && p) return hipSuccess;
+        release();
+        if (n == 0) return hipSuccess;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) { p = nullptr; return e; }
+        bytes = n;
+        return hipSuccess;
+    }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct EnvMap {
+    bool enabled = false;
+    int width = 0, height = 0, channels = 0;
+    float integral = 0.0f;       // host copy, already divided by w*h
+    DevBuf texels;               // w*h*channels floats
+    DevBuf cond, marg;           // uint16 inverted-CDF indices
+    DevBuf cdfY;                 // h floats (normalised marginal CDF)
+    DevBuf integralDev;          // raw total (float) + double slot
+    DevBuf counter;              // last-block counter
+};
+
+struct LightTreeDev {
+    uint32_t nnodes = 0, nlights = 0;
+    DevBuf nodes, lightPointers, bitmasks, tris;
+    std::vector<rbh_packed_node> hostNodes;
+    std::vector<uint32_t> hostPointers;
+    std::vector<uint64_t> hostMasks;
+};
+
+struct Geometry {
+    uint32_t nverts = 0, ntris = 0;
+    DevBuf verts, indices, attribs, triMaterial, triLight;
+};
+
+struct Accel {
+    bool built = false;
+    uint32_t nnodes = 0, depth = 0;
+    DevBuf nodes;     // BvhNode
+    DevBuf tris;      // TriRecord in leaf order
+    DevBuf scratch;   // build scratch
+};
+
+struct RenderState {
+    uint32_t width = 0, height = 0;
+    DevBuf accum;     // W*H*4 doubles
+    DevBuf paths;     // SoA path state
+    DevBuf queues;    // ray/shadow queues + counters
+    DevBuf image;     // RGBA8 resolve output
+    size_t pathCapacity = 0;
+};
+
+}  // namespace rbh
+
+struct rbh_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    rbh::DevBuf spectra;  uint32_t nspectra = 0;
+    rbh::DevBuf colorTables;  // 6 x 471: X, Y, Z, basis R, G, B
+    bool haveColor = false;
+    rbh::DevBuf luts;         // conductor | dielOut | dielIn | reflOut | reflIn | sheen
+    bool haveLuts = false;
+    rbh::DevBuf textures, texels; uint32_t ntextures = 0;
+    rbh::DevBuf materials; uint32_t nmaterials = 0;
+    rbh::Geometry geom;
+    rbh::Accel accel;
+    rbh::LightTreeDev tree;
+    rbh::EnvMap env;
+    rbh::RenderState rs;
+    rbh_stats stats{};
+    void* guide = nullptr;    // guiding state (guide.hip)
+};
+
+namespace rbh {
+
+inline int fail(rbh_ctx c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+}  // namespace rbh
+
+#define RBH_HIP(ctx, call)                                                                              \
+    do {                                                                                                \
+        hipError_t e_ = (call);                                                                         \
+        if (e_ != hipSuccess) return rbh::fail((ctx), RBH_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+#define RBH_ALLOC(ctx, buf, n)                                                                          \
+    do {                                                                                                \
+        hipError_t e_ = (buf).reserve(n);                                                               \
+        if (e_ != hipSuccess) return rbh::fail((ctx), RBH_ERR_NOMEM, "allocation of %zu bytes failed: %s", (size_t)(n), hipGetErrorString(e_)); \
+    } while (0)

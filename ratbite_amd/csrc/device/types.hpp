@@ -1,0 +1,87 @@
+// Device-side data layouts shared by the kernels and the host context.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace rbh {
+
+constexpr int kLambdaMin = 360;       // src/core/spectral_settings.h:6
+constexpr int kLambdaMax = 830;       // src/core/spectral_settings.h:7
+constexpr int kDense = kLambdaMax - kLambdaMin + 1;
+constexpr int kNL = 4;                // hero wavelengths per path, spectral_settings.h:5
+
+// LUT block offsets (floats) inside the context's single LUT buffer.
+constexpr size_t kLutCond = 0;
+constexpr size_t kLutDielOut = kLutCond + 32 * 32;
+constexpr size_t kLutDielIn = kLutDielOut + 32 * 32 * 32;
+constexpr size_t kLutReflOut = kLutDielIn + 32 * 32 * 32;
+constexpr size_t kLutReflIn = kLutReflOut + 32 * 32 * 32;
+constexpr size_t kLutSheen = kLutReflIn + 32 * 32 * 32;
+constexpr size_t kLutTotal = kLutSheen + 32 * 32 * 4;
+
+// Binary BVH node, 64 bytes: both children's boxes in one node so a
+// traversal step is one 64-byte fetch (two dwordx4 pairs).
+// child >= 0: inner node index; child < 0: ~(first triangle) with the
+// triangle count in the low bits of cnt.
+struct alignas(16) BvhNode {
+    float lminx, lmaxx, lminy, lmaxy;   // left child box (x, y)
+    float rminx, rmaxx, rminy, rmaxy;   // right child box (x, y)
+    float lminz, lmaxz, rminz, rmaxz;   // z extents
+    int32_t left, right;                // child references
+    uint32_t lcnt, rcnt;                // triangle counts of leaf children (0 = inner)
+};
+static_assert(sizeof(BvhNode) == 64, "BvhNode is 64 bytes");
+
+// Triangle in leaf order: vertices + original triangle index, 48 bytes.
+struct alignas(16) TriRecord {
+    float v0x, v0y, v0z; uint32_t prim;
+    float v1x, v1y, v1z; uint32_t pad0;
+    float v2x, v2y, v2z; uint32_t pad1;
+};
+static_assert(sizeof(TriRecord) == 48, "TriRecord is 48 bytes");
+
+// LightTree::PackedNode (src/core/light_tree_types.h:119-162)
+struct PackedNode {
+    float mean[3];
+    float variance;
+    uint32_t dir;
+    float sharpness;
+    float flux;
+    uint32_t core;
+};
+static_assert(sizeof(PackedNode) == 32, "PackedNode is 32 bytes");
+
+// EmissiveTriangleLightData (src/core/light_tree_types.h:40-46)
+struct EmissiveTri {
+    float v[9];
+    float uv[6];
+    uint32_t material;
+    uint32_t prim;
+};
+static_assert(sizeof(EmissiveTri) == 68, "EmissiveTri is 68 bytes");
+
+struct Material {
+    uint32_t bxdf, flags, attributes, attributeStride;
+    uint32_t normalOffset, frameOffset, texCoord1Offset, texCoord2Offset;
+    uint32_t iorSpectrum, acSpectrum, emissionSpectrum;
+    float mfRoughness, ior, alphaCutoff;
+    uint32_t textures, texCoordSets;
+    int32_t tex[7];
+    uint32_t factors;
+    float baseColor[4];
+    float metalness, roughness, transmission;
+    float sheenColor[3];
+    float sheenRoughness;
+    float emissive[3];
+};
+static_assert(sizeof(Material) == 152, "Material is 152 bytes");
+
+struct TextureDesc {
+    uint32_t width, height, depth, format, srgb;
+    uint32_t addr[3];
+    uint32_t filter, pad;
+    uint64_t offset;
+};
+static_assert(sizeof(TextureDesc) == 48, "TextureDesc is 48 bytes");
+
+}  // namespace rbh
