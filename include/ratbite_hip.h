@@ -173,6 +173,11 @@ int rbh_build_accel(rbh_ctx ctx);
  * on the host from n emissive triangles with their flux, uploads nodes,
  * light pointers, bitmasks and the triangle records. */
 int rbh_build_light_tree(rbh_ctx ctx, const rbh_emissive_tri* tris, const float* flux, uint32_t n);
+/* The same builder without a context or device (host only): fills nodes
+ * (capacity 2n), lightPointers (n) and bitmasks (n); returns the node count
+ * or a negative RBH_ERR_* code. */
+int rbh_light_tree_build_host(const rbh_emissive_tri* tris, const float* flux, uint32_t n, rbh_packed_node* nodes,
+                              uint32_t* lightPointers, uint64_t* bitmasks);
 /* Read the built tree back: nodes (<= 2n), lightPointers (n), bitmasks (n). */
 int rbh_get_light_tree(rbh_ctx ctx, rbh_packed_node* nodes, uint32_t* nnodes, uint32_t* lightPointers,
                        uint64_t* bitmasks);

@@ -1,0 +1,46 @@
+/*
+ * ratbite_hip_test.h — test-only C ABI: runs single device functions of the
+ * hot path over batches of inputs so each can be compared with a CPU
+ * restatement. Inputs and outputs are host buffers; n is the batch size.
+ * The reference has no such interface (its device functions are only
+ * reachable inside the OptiX pipeline); each entry point names the
+ * reference function it exercises.
+ */
+#ifndef RATBITE_HIP_TEST_H
+#define RATBITE_HIP_TEST_H
+#include "ratbite_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* QRNG::Sobol::sample1D/2D/3D (src/device/quasi_random.h:76-99).
+ * in: n x {sample, pixelHash, dimensionOffset, dimension, kind(1|2|3)}; out: n x 3 floats */
+int rbh_test_sobol(rbh_ctx ctx, const uint32_t* in, float* out, uint32_t n);
+/* QRNG::getPixelHash (quasi_random.h:30-37). in: n x {x, y}; out: n */
+int rbh_test_pixel_hash(rbh_ctx ctx, const uint32_t* xy, uint32_t* out, uint32_t n);
+/* SampledWavelengths::sampleVisible (src/device/spectral.h:227-241). out: n x 4 each */
+int rbh_test_wavelengths(rbh_ctx ctx, const float* u, float* lambda, float* pdf, uint32_t n);
+/* color::RGBtoSpectrum (src/device/color.h:369-381) with the uploaded basis.
+ * in: n x {r, g, b, l0, l1, l2, l3}; out: n x 4 */
+int rbh_test_rgb_to_spectrum(rbh_ctx ctx, const float* in, float* out, uint32_t n);
+/* Octohedral::encodeU32 / decodeU32 (src/core/math_util.h:199-219) */
+int rbh_test_oct(rbh_ctx ctx, const float* v, uint32_t* enc, float* dec, uint32_t n);
+/* Closest-hit / any-hit query on the built LBVH. rays: n x {o.xyz, tmin(ignored, 0), d.xyz, tmax};
+ * hits: n x {t, prim (uint bits, 0xFFFFFFFF = miss), b1, b2} */
+int rbh_test_trace(rbh_ctx ctx, const float* rays, float* hits, uint32_t n, int anyhit);
+/* traverseLightTree (src/kernels/pathtrace.cu:1002-1049) on the built tree.
+ * lights: n light indices; posNrm: n x 6; out: n pdfs */
+int rbh_test_light_pdf(rbh_ctx ctx, const uint32_t* lights, const float* posNrm, float* pdf, uint32_t n);
+/* sampleLightTree (pathtrace.cu:1050-1112). in: n x {rand, pos.xyz, nrm.xyz};
+ * typeIndex: n x 2; pdf: n */
+int rbh_test_light_sample(rbh_ctx ctx, const float* in, uint32_t* typeIndex, float* pdf, uint32_t n);
+/* BxDF callables (pathtrace.cu:464-547, 548-905) for one shading point each.
+ * Record layouts: 64-word input, 16-word output, documented in
+ * ratbite_amd/csrc/kernels/unit.hip. */
+int rbh_test_bxdf(rbh_ctx ctx, const float* in, float* out, uint32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -30,6 +30,11 @@ int orc_sobol(const uint32_t* in, float* out, uint32_t n) {
     return 0;
 }
 
+int orc_sg_erf(const float* x, float* out, uint32_t n) {
+    for (uint32_t i = 0; i < n; ++i) out[i] = sg_erf(x[i]);
+    return 0;
+}
+
 int orc_sobol_table(uint32_t* out) {
     const SobolTable& T = sobol_table();
     std::memcpy(out, T.t, sizeof(T.t));

@@ -97,6 +97,7 @@ typedef struct orc_render_stats {
 /* QRNG / spectral unit functions (n independent inputs) */
 int orc_sobol(const uint32_t* in /* n x {sample, hash, offset, dim, kind} */, float* out /* n x 3 */, uint32_t n);
 int orc_sobol_table(uint32_t* out /* 4 x 32 */);
+int orc_sg_erf(const float* x, float* out, uint32_t n);
 int orc_pixel_hash(const uint32_t* xy, uint32_t* out, uint32_t n);
 int orc_wavelengths(const float* u, float* lambda /* n x 4 */, float* pdf /* n x 4 */, uint32_t n);
 

@@ -95,6 +95,7 @@ _SIGS = {
     "rbh_upload_geometry": (c_int, [c_void_p, c_void_p, c_uint32, c_void_p, c_uint32, c_void_p, c_uint64, c_void_p, c_void_p]),
     "rbh_build_accel": (c_int, [c_void_p]),
     "rbh_build_light_tree": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32]),
+    "rbh_light_tree_build_host": (c_int, [c_void_p, c_void_p, c_uint32, c_void_p, c_void_p, c_void_p]),
     "rbh_get_light_tree": (c_int, [c_void_p, c_void_p, ctypes.POINTER(c_uint32), c_void_p, c_void_p]),
     "rbh_envmap_build": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_uint32]),
     "rbh_envmap_rebuild": (c_int, [c_void_p, c_uint32, c_int]),
@@ -164,3 +165,17 @@ class RbhError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("rbh error %d: %s" % (code, msg))
         self.code = code
+
+
+# include/ratbite_hip_test.h
+register({
+    "rbh_test_sobol": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32]),
+    "rbh_test_pixel_hash": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32]),
+    "rbh_test_wavelengths": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint32]),
+    "rbh_test_rgb_to_spectrum": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32]),
+    "rbh_test_oct": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint32]),
+    "rbh_test_trace": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_int]),
+    "rbh_test_light_pdf": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint32]),
+    "rbh_test_light_sample": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint32]),
+    "rbh_test_bxdf": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32]),
+})

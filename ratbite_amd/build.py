@@ -48,7 +48,11 @@ def _compile(src, force):
     newest_dep = max([os.path.getmtime(src)] + [os.path.getmtime(h) for h in headers()])
     if not force and os.path.exists(obj) and os.path.getmtime(obj) >= newest_dep:
         return obj, None
-    cmd = [HIPCC] + COMMON + DEVICE + ["-c", src, "-o", obj]
+    if os.sep + "host" + os.sep in src:
+        # host-only C++: no device pass, no floating-point contraction
+        cmd = [HIPCC] + COMMON + ["-ffp-contract=off", "-c", src, "-o", obj]
+    else:
+        cmd = [HIPCC] + COMMON + DEVICE + ["-c", src, "-o", obj]
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         return obj, "compile failed: %s\n%s\n%s" % (" ".join(cmd), p.stdout, p.stderr)

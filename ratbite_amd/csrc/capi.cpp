@@ -3,6 +3,8 @@
 #include "context.hpp"
 #include <cstring>
 #include <new>
+#include <vector>
+#include <algorithm>
 
 namespace rbh {
 hipError_t envmap_cdf_launch(rbh_ctx c, bool fp64);
@@ -153,8 +155,31 @@ int rbh_upload_geometry(rbh_ctx c, const float* verts, uint32_t nverts, const ui
     if (ntris > (1u << 30)) return fail(c, RBH_ERR_RANGE, "at most 2^30 triangles");
     for (uint64_t i = 0; i < 3ull * ntris; ++i)
         if (indices[i] >= nverts) return fail(c, RBH_ERR_INVALID, "index %llu out of range", (unsigned long long)i);
-    for (uint32_t i = 0; i < ntris; ++i)
+    int64_t maxLight = -1;
+    for (uint32_t i = 0; i < ntris; ++i) {
         if (triMaterial[i] >= c->nmaterials) return fail(c, RBH_ERR_INVALID, "triangle %u: material %u of %u", i, triMaterial[i], c->nmaterials);
+        if (triLight[i] != 0xFFFFFFFFu) {
+            if (triLight[i] >= (1u << 30)) return fail(c, RBH_ERR_RANGE, "triangle %u: light index above 2^30", i);
+            maxLight = std::max<int64_t>(maxLight, triLight[i]);
+        }
+    }
+    // every material's vertex attributes must lie inside the attribute buffer
+    {
+        std::vector<Material> mats(c->nmaterials);
+        RBH_HIP(c, hipMemcpy(mats.data(), c->materials.p, mats.size() * sizeof(Material), hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < c->nmaterials; ++i) {
+            const Material& m = mats[i];
+            struct { uint32_t bit, off, size; } at[] = {{1u, m.normalOffset, 8u}, {2u, m.frameOffset, 16u},
+                                                        {8u, m.texCoord1Offset, 8u}, {16u, m.texCoord2Offset, 8u}};
+            for (auto& a : at) {
+                if (!(m.attributes & a.bit)) continue;
+                const uint64_t end = (uint64_t)(nverts - 1) * m.attributeStride + a.off + a.size;
+                if (end > attribBytes || (a.off % 4) || (m.attributeStride % 4))
+                    return fail(c, RBH_ERR_INVALID, "material %u: attribute (bit %u) at offset %u, stride %u exceeds %llu attribute bytes",
+                                i, a.bit, a.off, m.attributeStride, (unsigned long long)attribBytes);
+            }
+        }
+    }
     Geometry& g = c->geom;
     RBH_ALLOC(c, g.verts, (size_t)nverts * 12);
     RBH_ALLOC(c, g.indices, (size_t)ntris * 12);
@@ -169,6 +194,7 @@ int rbh_upload_geometry(rbh_ctx c, const float* verts, uint32_t nverts, const ui
     RBH_HIP(c, hipStreamSynchronize(c->stream));
     g.nverts = nverts;
     g.ntris = ntris;
+    g.maxLight = maxLight;
     c->accel.built = false;
     return RBH_OK;
 }

@@ -59,6 +59,7 @@ struct LightTreeDev {
 
 struct Geometry {
     uint32_t nverts = 0, ntris = 0;
+    int64_t maxLight = -1;   // largest triLight index (-1: no emissive triangles)
     DevBuf verts, indices, attribs, triMaterial, triLight;
 };
 

@@ -1,0 +1,131 @@
+// LBVH traversal: closest-hit and any-hit queries replacing optixTrace /
+// optixTraverse(TERMINATE_ON_FIRST_HIT) (src/kernels/pathtrace.cu:1162-1177,
+// 1704-1720). Stack in LDS (one column per lane, so consecutive lanes hit
+// consecutive banks); ray/box slabs with a conservative far bound; the
+// watertight ray/triangle test of Woop, Benthin and Wald (JCGT 2013) with
+// per-ray shear constants.
+#pragma once
+#include "vec.hpp"
+#include "types.hpp"
+
+namespace rbh {
+namespace dev {
+
+struct RayShear {
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+};
+__device__ __forceinline__ float comp(V3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+__device__ __forceinline__ RayShear make_shear(V3 d) {
+    RayShear s;
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    s.kz = ax > ay ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+    s.kx = s.kz == 2 ? 0 : s.kz + 1;
+    s.ky = s.kx == 2 ? 0 : s.kx + 1;
+    const float dz = comp(d, s.kz);
+    if (dz < 0.0f) { const int t = s.kx; s.kx = s.ky; s.ky = t; }
+    s.Sx = comp(d, s.kx) / dz;
+    s.Sy = comp(d, s.ky) / dz;
+    s.Sz = 1.0f / dz;
+    return s;
+}
+
+struct Hit {
+    float t, b1, b2;
+    uint32_t prim;   // 0xFFFFFFFF: miss
+};
+
+__device__ __forceinline__ bool tri_hit(const RayShear& s, V3 o, const TriRecord& tr, float tmax, float& t, float& b1,
+                                        float& b2) {
+    // No FMA contraction: the 2D edge functions of a shared edge must round
+    // identically from both triangles for the test to stay watertight.
+#pragma clang fp contract(off)
+    const V3 A{tr.v0x - o.x, tr.v0y - o.y, tr.v0z - o.z};
+    const V3 B{tr.v1x - o.x, tr.v1y - o.y, tr.v1z - o.z};
+    const V3 C{tr.v2x - o.x, tr.v2y - o.y, tr.v2z - o.z};
+    const float Akz = comp(A, s.kz), Bkz = comp(B, s.kz), Ckz = comp(C, s.kz);
+    const float Ax = comp(A, s.kx) - s.Sx * Akz, Ay = comp(A, s.ky) - s.Sy * Akz;
+    const float Bx = comp(B, s.kx) - s.Sx * Bkz, By = comp(B, s.ky) - s.Sy * Bkz;
+    const float Cx = comp(C, s.kx) - s.Sx * Ckz, Cy = comp(C, s.ky) - s.Sy * Ckz;
+    float U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = (float)((double)Cx * By - (double)Cy * Bx);
+        V = (float)((double)Ax * Cy - (double)Ay * Cx);
+        W = (float)((double)Bx * Ay - (double)By * Ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    float det = U + V + W;
+    if (det == 0.0f) return false;
+    float T = U * (s.Sz * Akz) + V * (s.Sz * Bkz) + W * (s.Sz * Ckz);
+    if (det < 0.0f) { T = -T; det = -det; V = -V; W = -W; }
+    if (T < 0.0f || T > tmax * det) return false;
+    const float r = 1.0f / det;
+    t = T * r;
+    b1 = V * r;
+    b2 = W * r;
+    return true;
+}
+
+// Traverses the LBVH; `stack` points at this lane's LDS column base with a
+// stride of `stride` words. Returns the closest hit below tmax or, with ANY,
+// the first one found.
+template <bool ANY>
+__device__ __forceinline__ Hit traverse(const BvhNode* __restrict__ nodes, const TriRecord* __restrict__ tris,
+                                        uint32_t nnodes, V3 o, V3 d, float tmax, int* __restrict__ stack,
+                                        int stride) {
+    Hit h;
+    h.t = tmax;
+    h.prim = 0xFFFFFFFFu;
+    h.b1 = h.b2 = 0.0f;
+    const RayShear sh = make_shear(d);
+    const float ix = 1.0f / d.x, iy = 1.0f / d.y, iz = 1.0f / d.z;
+    int node = nnodes ? 0 : ~0;
+    int sp = 0;
+    while (true) {
+        if (node >= 0) {
+            const float4* q = reinterpret_cast<const float4*>(nodes + node);
+            const float4 a = q[0], b = q[1], c = q[2];
+            const int4 e = reinterpret_cast<const int4*>(q)[3];
+            const float best = h.t * 1.00000024f;
+            // left child
+            const float l0x = (a.x - o.x) * ix, l1x = (a.y - o.x) * ix;
+            const float l0y = (a.z - o.y) * iy, l1y = (a.w - o.y) * iy;
+            const float l0z = (c.x - o.z) * iz, l1z = (c.y - o.z) * iz;
+            const float lnear = fmaxf(fmaxf(fminf(l0x, l1x), fminf(l0y, l1y)), fmaxf(fminf(l0z, l1z), 0.0f));
+            const float lfar = fminf(fminf(fmaxf(l0x, l1x), fmaxf(l0y, l1y)), fminf(fmaxf(l0z, l1z), best));
+            // right child
+            const float r0x = (b.x - o.x) * ix, r1x = (b.y - o.x) * ix;
+            const float r0y = (b.z - o.y) * iy, r1y = (b.w - o.y) * iy;
+            const float r0z = (c.z - o.z) * iz, r1z = (c.w - o.z) * iz;
+            const float rnear = fmaxf(fmaxf(fminf(r0x, r1x), fminf(r0y, r1y)), fmaxf(fminf(r0z, r1z), 0.0f));
+            const float rfar = fminf(fminf(fmaxf(r0x, r1x), fmaxf(r0y, r1y)), fminf(fmaxf(r0z, r1z), best));
+            const bool hl = lnear <= lfar * 1.00000024f, hr = rnear <= rfar * 1.00000024f;
+            if (hl && hr) {
+                const bool lfirst = lnear <= rnear;
+                stack[sp * stride] = lfirst ? e.y : e.x;
+                ++sp;
+                node = lfirst ? e.x : e.y;
+                continue;
+            }
+            if (hl) { node = e.x; continue; }
+            if (hr) { node = e.y; continue; }
+        } else {
+            const TriRecord tr = tris[~node];
+            float t, b1, b2;
+            if (tri_hit(sh, o, tr, h.t, t, b1, b2)) {
+                h.t = t;
+                h.b1 = b1;
+                h.b2 = b2;
+                h.prim = tr.prim;
+                if (ANY) return h;
+            }
+        }
+        if (sp == 0) break;
+        --sp;
+        node = stack[sp * stride];
+    }
+    return h;
+}
+
+}  // namespace dev
+}  // namespace rbh

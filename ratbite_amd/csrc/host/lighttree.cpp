@@ -1,0 +1,353 @@
+// Host light-tree builder (the reference's LightTree::Builder,
+// src/core/light_tree.cpp:14-418, fed as RenderingInterface::buildLightTree
+// does at src/core/rendering_interface.cpp:1065-1161).
+//
+// Binned SAOH split search over 32 bins on each axis, at most 8 lights per
+// leaf, nodes in depth-first order with the left child at index+1, one
+// 64-bit path mask per light, branch attributes filled bottom-up from the
+// packed (octahedral-quantised) child attributes. The implementation-defined
+// std::nth_element partition is replaced by a stable total order on
+// (centroid[axis], light index) so the topology is deterministic.
+//
+// Compiled without floating-point contraction so its FP32 results match a
+// straightforward scalar evaluation bit for bit.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#include "../device/types.hpp"
+
+namespace rbh {
+namespace host {
+
+namespace {
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr uint32_t kMaxPerLeaf = 8, kBins = 32;
+
+struct Aabb {
+    float lo[3], hi[3];
+};
+inline Aabb aabb_zero() { return Aabb{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}}; }
+inline Aabb aabb_empty() {
+    const float inf = std::numeric_limits<float>::infinity();
+    return Aabb{{inf, inf, inf}, {-inf, -inf, -inf}};
+}
+inline void grow(Aabb& a, const Aabb& b) {
+    for (int k = 0; k < 3; ++k) {
+        a.lo[k] = std::min(a.lo[k], b.lo[k]);
+        a.hi[k] = std::max(a.hi[k], b.hi[k]);
+    }
+}
+inline float center(const Aabb& a, int k) { return a.lo[k] + (a.hi[k] - a.lo[k]) * 0.5f; }
+inline float area(const Aabb& a) {
+    const float x = a.hi[0] - a.lo[0], y = a.hi[1] - a.lo[1], z = a.hi[2] - a.lo[2];
+    return x * y * 2.0f + x * z * 2.0f + y * z * 2.0f;
+}
+inline bool valid(const Aabb& a) { return a.lo[0] <= a.hi[0] && a.lo[1] <= a.hi[1] && a.lo[2] <= a.hi[2]; }
+
+// vMF axis length <-> sharpness (math_util.h:562-587)
+inline float axis_to_sharpness(float l) { return l * (3.0f - l * l) / (1.0f - l * l); }
+inline float sharpness_to_axis(float s) {
+    const float a = s / 3.0f;
+    const float b = a * a * a;
+    const float c = std::sqrt(1.0f + 3.0f * (a * a) * (1.0f + a * a));
+    const float th = std::atan2(c, b) / 3.0f;
+    const float d = std::sin(th) * 1.7320508075688772935274463415059f - std::cos(th);
+    return (s > 0x1.0p25f) ? 1.0f : std::sqrt(1.0f + a * a) * d + a;
+}
+
+struct Attr {
+    float mean[3], var, flux, dir[3], sharp;
+};
+
+// light_tree_types.h:65-100
+Attr merge(const Attr& A, const Attr& B) {
+    Attr r{};
+    const float wa = A.flux / (A.flux + B.flux), wb = B.flux / (A.flux + B.flux);
+    const float la = sharpness_to_axis(A.sharp), lb = sharpness_to_axis(B.sharp);
+    float d[3];
+    for (int k = 0; k < 3; ++k) d[k] = wa * la * A.dir[k] + wb * lb * B.dir[k];
+    const float dl = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    for (int k = 0; k < 3; ++k) r.dir[k] = dl > 0.0001f ? d[k] / dl : 0.0f;
+    const float m[3] = {B.mean[0] - A.mean[0], B.mean[1] - A.mean[1], B.mean[2] - A.mean[2]};
+    const float m2 = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
+    for (int k = 0; k < 3; ++k) r.mean[k] = wa * A.mean[k] + wb * B.mean[k];
+    r.var = wa * A.var + wb * B.var + wa * wb * m2;
+    r.flux = A.flux + B.flux;
+    r.sharp = axis_to_sharpness(dl);
+    return r;
+}
+
+// Octahedral 16+16 encode (math_util.h:147-207) and the [0,1]-scaled decode
+// used by unpackNode (light_tree_types.h:174-175).
+uint32_t oct_pack(const float* v) {
+    const float inv = 1.0f / (std::fabs(v[0]) + std::fabs(v[1]) + std::fabs(v[2]));
+    float px = v[0] * inv, py = v[1] * inv;
+    if (v[2] <= 0.0f) {
+        const float qx = (1.0f - std::fabs(py)) * (px >= 0.0f ? 1.0f : -1.0f);
+        const float qy = (1.0f - std::fabs(px)) * (py >= 0.0f ? 1.0f : -1.0f);
+        px = qx;
+        py = qy;
+    }
+    auto q = [](float x) { return (uint32_t)(std::min(std::max(x * 0.5f + 0.5f, 0.0f), 1.0f) * 65535.99f); };
+    return (q(py) << 16) | q(px);
+}
+void oct_unpack_unit(uint32_t e, float* out) {
+    float x = (float)(e & 0xFFFFu) / 65535.0f, y = (float)(e >> 16) / 65535.0f;
+    float z = 1.0f - std::fabs(x) - std::fabs(y);
+    const float t = std::max(0.0f, -z);
+    x += x >= 0.0f ? -t : t;
+    y += y >= 0.0f ? -t : t;
+    const float n = 1.0f / std::sqrt(x * x + y * y + z * z);
+    out[0] = x * n; out[1] = y * n; out[2] = z * n;
+}
+
+struct Prim {
+    Aabb box;
+    float cone[3];
+    float cosCone;
+    float flux;
+    uint32_t id;
+};
+
+struct Bin {
+    Aabb box = aabb_zero();
+    uint32_t n = 0;
+    float flux = 0.0f;
+    float cone[3] = {0.0f, 0.0f, 0.0f};
+    float cosCone = 1.0f;
+    void add(const Aabb& b, uint32_t c, float f, const float* d) {
+        grow(box, b);
+        n += c;
+        flux += f;
+        cone[0] += d[0]; cone[1] += d[1]; cone[2] += d[2];
+    }
+};
+
+// light_tree.cpp:167-185
+float cone_union(const float* da, float ca, const float* db, float cb) {
+    if (ca == -1.0f || cb == -1.0f) return -1.0f;
+    const float cd = da[0] * db[0] + da[1] * db[1] + da[2] * db[2];
+    const float sd = std::sqrt(1.0f - cd * cd);
+    const float sb = std::sqrt(1.0f - cb * cb);
+    const float ct = cb * cd - sb * sd;
+    const float st = sb * cd + cb * sd;
+    return st > 0.0f ? std::min(cb, ct) : -1.0f;
+}
+// light_tree.cpp:187-199 (surface area x orientation measure x flux)
+float saoh(const Aabb& b, float flux, float cosT) {
+    const float ac = valid(b) ? area(b) : 0.0f;
+    const float tO = cosT != -1.0f ? std::acos(std::max(std::min(cosT, 1.0f), -1.0f)) : kPi;
+    const float tW = std::min(tO + kPi / 2.0f, kPi);
+    const float sO = std::sin(tO), cO = std::cos(tO);
+    const float oc = 2.0f * kPi * (1.0f - cO) + kPi / 2.0f * (2.0f * tW * sO - std::cos(tO - 2.0f * tW) - 2.0f * tO * sO + cO);
+    return flux * ac * oc;
+}
+
+struct Builder {
+    const EmissiveTri* tris;
+    std::vector<Prim> prims;
+    std::vector<PackedNode> nodes;
+    std::vector<uint32_t> pointers;
+    std::vector<uint64_t> masks;
+
+    Attr leaf_attr(uint32_t id) const {
+        const float* v = tris[id].v;
+        const float e1[3] = {v[3] - v[0], v[4] - v[1], v[5] - v[2]};
+        const float e2[3] = {v[6] - v[0], v[7] - v[1], v[8] - v[2]};
+        float n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+        const float inv = 1.0f / std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        Attr a{};
+        for (int k = 0; k < 3; ++k) {
+            a.mean[k] = (v[k] + v[3 + k] + v[6 + k]) / 3.0f;
+            a.dir[k] = n[k] * inv;
+        }
+        const float d11 = e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2];
+        const float d22 = e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2];
+        const float d12 = e1[0] * e2[0] + e1[1] * e2[1] + e1[2] * e2[2];
+        a.var = (d11 + d22 - d12) / 18.0f;
+        a.sharp = axis_to_sharpness(0.5f);
+        a.flux = 0.0f;
+        return a;
+    }
+
+    static void pack(PackedNode& p, const Attr& a) {
+        for (int k = 0; k < 3; ++k) p.mean[k] = a.mean[k];
+        p.variance = a.var;
+        p.dir = oct_pack(a.dir);
+        p.sharpness = a.sharp;
+        p.flux = a.flux;
+    }
+
+    // light_tree.cpp:162-394; returns axis/index, or index = UINT32_MAX for none
+    void split(uint32_t begin, uint32_t end, const Aabb& bounds, uint32_t& outAxis, uint32_t& outIndex) {
+        float best = std::numeric_limits<float>::max();
+        outAxis = outIndex = UINT32_MAX;
+        const float dims[3] = {bounds.hi[0] - bounds.lo[0], bounds.hi[1] - bounds.lo[1], bounds.hi[2] - bounds.lo[2]};
+        const uint32_t largest = dims[0] > dims[1] && dims[0] > dims[2] ? 0u : (dims[1] > dims[2] ? 1u : 2u);
+        Bin bins[kBins];
+        float costs[kBins - 1];
+        for (uint32_t ax = 0; ax < 3; ++ax) {
+            const float w = bounds.hi[ax] - bounds.lo[ax];
+            const float scale = w > std::numeric_limits<float>::min() ? (float)kBins / w : 0.0f;
+            auto binOf = [&](const Prim& p) {
+                return std::min((uint32_t)((center(p.box, (int)ax) - bounds.lo[ax]) * scale), kBins - 1);
+            };
+            for (auto& b : bins) b = Bin{};
+            for (uint32_t i = begin; i < end; ++i) bins[binOf(prims[i])].add(prims[i].box, 1, prims[i].flux, prims[i].cone);
+            for (auto& b : bins) {
+                const float l = std::sqrt(b.cone[0] * b.cone[0] + b.cone[1] * b.cone[1] + b.cone[2] * b.cone[2]);
+                if (l < std::numeric_limits<float>::min()) {
+                    b.cosCone = -1.0f;
+                } else {
+                    b.cosCone = 1.0f;
+                    b.cone[0] /= l; b.cone[1] /= l; b.cone[2] /= l;
+                }
+            }
+            for (uint32_t i = begin; i < end; ++i) {
+                Bin& b = bins[binOf(prims[i])];
+                b.cosCone = cone_union(b.cone, b.cosCone, prims[i].cone, prims[i].cosCone);
+            }
+            auto sweep_cost = [&](const Bin& tot, uint32_t j0, uint32_t j1) {
+                float cosT = -1.0f;
+                const float l = std::sqrt(tot.cone[0] * tot.cone[0] + tot.cone[1] * tot.cone[1] + tot.cone[2] * tot.cone[2]);
+                if (l > std::numeric_limits<float>::min()) {
+                    cosT = 1.0f;
+                    const float cd[3] = {tot.cone[0] / l, tot.cone[1] / l, tot.cone[2] / l};
+                    for (uint32_t j = j0; j <= j1; ++j) cosT = cone_union(cd, cosT, bins[j].cone, bins[j].cosCone);
+                }
+                return saoh(tot.box, tot.flux, cosT);
+            };
+            Bin tot;
+            for (uint32_t i = 0; i < kBins - 1; ++i) {
+                tot.add(bins[i].box, bins[i].n, bins[i].flux, bins[i].cone);
+                costs[i] = sweep_cost(tot, 0, i);
+            }
+            tot = Bin{};
+            for (uint32_t i = kBins - 1; i > 0; --i) {
+                tot.add(bins[i].box, bins[i].n, bins[i].flux, bins[i].cone);
+                costs[i - 1] += sweep_cost(tot, i, kBins - 1);
+            }
+            float axCost = std::numeric_limits<float>::max();
+            uint32_t axIndex = 0, run = begin;
+            for (uint32_t i = 0; i < kBins - 1; ++i) {
+                run += bins[i].n;
+                if (costs[i] < axCost) { axCost = costs[i]; axIndex = run; }
+            }
+            axCost *= dims[largest] / dims[ax];
+            if (axIndex == begin || axIndex == end) continue;
+            if (axCost < best) { best = axCost; outAxis = ax; outIndex = axIndex; }
+        }
+        if (outIndex == UINT32_MAX && end - begin > kMaxPerLeaf) {
+            outAxis = largest;
+            outIndex = (begin + end) / 2;
+        }
+    }
+
+    uint32_t node(uint64_t mask, uint32_t depth, uint32_t begin, uint32_t end) {
+        Aabb b = aabb_empty();
+        for (uint32_t i = begin; i < end; ++i) grow(b, prims[i].box);
+        uint32_t ax = UINT32_MAX, at = UINT32_MAX;
+        if (end - begin > kMaxPerLeaf) split(begin, end, b, ax, at);
+        const uint32_t me = (uint32_t)nodes.size();
+        nodes.push_back(PackedNode{});
+        if (at != UINT32_MAX) {
+            std::sort(prims.begin() + begin, prims.begin() + end, [ax](const Prim& p, const Prim& q) {
+                const float cp = center(p.box, (int)ax), cq = center(q.box, (int)ax);
+                return cp != cq ? cp < cq : p.id < q.id;
+            });
+            node(mask, depth + 1, begin, at);
+            const uint32_t right = node(mask | (1ull << depth), depth + 1, at, end);
+            nodes[me].core = ((1u << 31) - 1u) & right;
+            return me;
+        }
+        Attr a = leaf_attr(prims[begin].id);
+        a.flux = prims[begin].flux;
+        for (uint32_t i = begin + 1; i < end; ++i) {
+            Attr o = leaf_attr(prims[i].id);
+            o.flux = prims[i].flux;
+            a = merge(a, o);
+        }
+        pack(nodes[me], a);
+        const uint32_t cnt = end - begin;
+        nodes[me].core = (1u << 31) | ((cnt & 15u) << 27) | (begin & ((1u << 27) - 1u));
+        for (uint32_t i = begin; i < end; ++i) {
+            pointers[i] = (0u << 30) | (prims[i].id & ((1u << 30) - 1u));
+            masks[prims[i].id] = mask;
+        }
+        return me;
+    }
+
+    Attr unpacked(uint32_t i) const {
+        const PackedNode& p = nodes[i];
+        Attr a{};
+        for (int k = 0; k < 3; ++k) a.mean[k] = p.mean[k];
+        a.var = p.variance;
+        a.flux = p.flux;
+        a.sharp = p.sharpness;
+        oct_unpack_unit(p.dir, a.dir);
+        return a;
+    }
+    // light_tree.cpp:396-418
+    Attr fill(uint32_t i) {
+        if ((nodes[i].core >> 31) != 0u) return unpacked(i);
+        const Attr l = fill(i + 1);
+        const Attr r = fill(nodes[i].core);
+        const Attr a = merge(l, r);
+        pack(nodes[i], a);
+        return a;
+    }
+};
+
+}  // namespace
+
+// Returns the node count, 0 for no lights, or a negative error.
+int build_light_tree(const EmissiveTri* tris, const float* flux, uint32_t n, std::vector<PackedNode>& nodes,
+                     std::vector<uint32_t>& pointers, std::vector<uint64_t>& masks) {
+    nodes.clear();
+    pointers.clear();
+    masks.clear();
+    if (n == 0) return 0;
+    if (n > (1u << 27) - 1u + kMaxPerLeaf) return -1;
+    Builder b;
+    b.tris = tris;
+    b.prims.resize(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* v = tris[i].v;
+        Prim& p = b.prims[i];
+        const float e1[3] = {v[3] - v[0], v[4] - v[1], v[5] - v[2]};
+        const float e2[3] = {v[6] - v[0], v[7] - v[1], v[8] - v[2]};
+        float nn[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+        const float inv = 1.0f / std::sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+        for (int k = 0; k < 3; ++k) {
+            p.box.lo[k] = std::min(v[k], std::min(v[3 + k], v[6 + k]));
+            p.box.hi[k] = std::max(v[k], std::max(v[3 + k], v[6 + k]));
+            if (p.box.hi[k] <= p.box.lo[k]) {
+                p.box.lo[k] = p.box.hi[k] - std::numeric_limits<float>::epsilon();
+                p.box.hi[k] = p.box.hi[k] + std::numeric_limits<float>::epsilon();
+            }
+            p.cone[k] = nn[k] * inv;
+        }
+        p.cosCone = 1.0f;
+        p.flux = flux[i];
+        p.id = i;
+    }
+    b.pointers.assign(n, 0u);
+    b.masks.assign(n, ~0ull);
+    b.nodes.reserve(2 * (size_t)n);
+    b.node(0ull, 0u, 0u, n);
+    for (uint32_t i = 0; i < n; ++i)
+        if (b.masks[i] == ~0ull) return -2;
+    b.fill(0);
+    nodes = std::move(b.nodes);
+    pointers = std::move(b.pointers);
+    masks = std::move(b.masks);
+    return (int)nodes.size();
+}
+
+}  // namespace host
+}  // namespace rbh

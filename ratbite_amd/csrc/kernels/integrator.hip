@@ -1,0 +1,601 @@
+// Wavefront spectral path tracer replacing the OptiX megakernel
+// __raygen__main (src/kernels/pathtrace.cu:1114-1813).
+//
+// One pass renders one sample for every pixel; path state lives in
+// structure-of-arrays buffers in HBM (slot = pixel). Per bounce:
+//   k_trace    closest hit for the queued paths           (pathtrace.cu:1162-1177)
+//   k_shade    emission + MIS, surface attributes, light-tree NEE, BxDF,
+//              Russian roulette, next origin, depth caps   (pathtrace.cu:1179-1785)
+//   k_shadow   any-hit test of the deferred NEE rays; adds the NEE term of
+//              the bounce when the light is visible        (pathtrace.cu:1704-1720)
+// and after the last bounce k_accumulate adds each sample's XYZ to the FP64
+// accumulation buffer (pathtrace.cu:1789-1812). Queues are compacted with
+// wave ballots (one atomic per wave). Queue lengths stay on the device; every
+// kernel reads its count and exits early, so the host issues a fixed launch
+// sequence per pass.
+#include "../context.hpp"
+#include "../device/bxdf.hpp"
+#include "../device/lighttree.hpp"
+#include "../device/qrng.hpp"
+#include "../device/traverse.hpp"
+#include "../integrator.hpp"
+
+namespace rbh {
+
+using namespace dev;
+
+namespace {
+
+constexpr int kTraceBlock = 128;
+constexpr int kShadeBlock = 128;
+constexpr int kStack = 64;
+
+__device__ __forceinline__ uint32_t wave_push(uint32_t* __restrict__ counter, bool pred) {
+    // one atomic per wave: ballot, leader adds the popcount, lanes take
+    // consecutive slots by their rank among the active lanes
+    const unsigned long long mask = __ballot(pred);
+    const uint32_t n = (uint32_t)__popcll(mask);
+    if (n == 0) return 0xFFFFFFFFu;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, n);
+    base = __shfl(base, leader);
+    const uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    return pred ? base + rank : 0xFFFFFFFFu;
+}
+
+__device__ __forceinline__ void wave_count(unsigned long long* __restrict__ counter, uint32_t v) {
+    // sum v over the wave with one atomic
+    uint32_t s = v;
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (__lane_id() == 0 && s) atomicAdd(counter, (unsigned long long)s);
+}
+
+// ------------------------------------------------------------------ camera
+__global__ __launch_bounds__(256) void k_camera(Frame1 f) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= f.P) return;
+    const uint32_t x = i % f.W, y = i / f.W;
+    const Qrng q = qrng_at(f.sample, 0, pixel_hash(x, y));
+    const Wl wl = sample_wavelengths(q.s1(QD_WAVELENGTH));
+    const V2 fo = gaussian_offset(q.s2(QD_FILTER));
+    const float cx = (float)x + fo.x, cy = (float)y + fo.y;
+    V3 o{0.0f, 0.0f, 0.0f}, d;
+    if (f.dof) {
+        // src/device/dir_gen.h:358-374
+        const V2 ls = q.s2(QD_LENS);
+        const float xs = (2.0f * cx * f.invW - 1.0f) * f.psW;
+        const float zs = (2.0f * cy * f.invH - 1.0f) * f.psH;
+        const V3 focus = V3{xs, zs, 1.0f} * f.focus;
+        const V2 dk = disk_polar(ls);
+        const V3 lens{dk.x * f.aperture, dk.y * f.aperture, 0.0f};
+        o = f.camU * lens.x + f.camW * lens.y;
+        const V3 rd = nrm(focus - lens);
+        d = f.camU * rd.x + f.camW * rd.y + f.camV * rd.z;
+    } else {
+        // src/device/dir_gen.h:346-357
+        const float xs = 2.0f * cx * f.invW - 1.0f;
+        const float zs = 2.0f * cy * f.invH - 1.0f;
+        d = nrm(f.camV + f.camU * xs * f.psW + f.camW * zs * f.psH);
+    }
+    PathBuffers& p = f.p;
+    p.od[i] = make_float4(o.x, o.y, o.z, 1.0f);           // bxdfPdf = 1
+    p.dr[i] = make_float4(d.x, d.y, d.z, 1.0f);           // refractionScale = 1
+    p.lam[i] = make_float4(wl.lambda.v[0], wl.lambda.v[1], wl.lambda.v[2], wl.lambda.v[3]);
+    p.lpdf[i] = make_float4(wl.pdf.v[0], wl.pdf.v[1], wl.pdf.v[2], wl.pdf.v[3]);
+    p.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    p.T[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    p.nf[i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
+    p.depth[i] = 0u;
+    if (i == 0) {
+        f.counters[CNT_QUEUE0] = f.P;
+        f.counters[CNT_QUEUE1] = 0u;
+        f.counters[CNT_SHADOW] = 0u;
+    }
+}
+
+__global__ void k_reset(uint32_t* __restrict__ counters, uint32_t which) {
+    if (threadIdx.x == 0) counters[which] = 0u;
+    if (threadIdx.x == 1) counters[CNT_SHADOW] = 0u;
+}
+
+// ------------------------------------------------------------- closest hit
+__global__ __launch_bounds__(kTraceBlock) void k_trace(Frame1 f, uint32_t qsel) {
+    __shared__ int stack[kStack * kTraceBlock];
+    const uint32_t count = f.counters[qsel ? CNT_QUEUE1 : CNT_QUEUE0];
+    const uint32_t* __restrict__ queue = qsel ? f.p.q1 : f.p.q0;
+    const bool identity = f.bounce == 0;
+    uint32_t nrays = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i - threadIdx.x < count; i += gridDim.x * blockDim.x) {
+        if (i < count) {
+            const uint32_t pid = identity ? i : queue[i];
+            const float4 od = f.p.od[pid], dr = f.p.dr[pid];
+            const Hit h = traverse<false>(f.nodes, f.tris, f.nnodes, V3{od.x, od.y, od.z}, V3{dr.x, dr.y, dr.z},
+                                          3.402823466e+38f, stack + threadIdx.x, kTraceBlock);
+            f.p.hit[pid] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.prim));
+            ++nrays;
+        }
+    }
+    wave_count(f.stats + STAT_CLOSEST, nrays);
+}
+
+// ------------------------------------------------------------------ shadow
+__global__ __launch_bounds__(kTraceBlock) void k_shadow(Frame1 f) {
+    __shared__ int stack[kStack * kTraceBlock];
+    const uint32_t count = f.counters[CNT_SHADOW];
+    uint32_t nrays = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i - threadIdx.x < count; i += gridDim.x * blockDim.x) {
+        if (i < count) {
+            const uint32_t pid = f.p.sq[i];
+            const float4 so = f.p.sro[pid], sd = f.p.srd[pid];
+            const Hit h = traverse<true>(f.nodes, f.tris, f.nnodes, V3{so.x, so.y, so.z}, V3{sd.x, sd.y, sd.z}, so.w,
+                                         stack + threadIdx.x, kTraceBlock);
+            if (h.prim == 0xFFFFFFFFu) {
+                const float4 a = f.p.Lnee[pid];
+                float4 L = f.p.L[pid];
+                L.x += a.x; L.y += a.y; L.z += a.z; L.w += a.w;
+                f.p.L[pid] = L;
+            }
+            ++nrays;
+        }
+    }
+    wave_count(f.stats + STAT_SHADOW, nrays);
+}
+
+// ------------------------------------------------------------------- shade
+__device__ __forceinline__ S4 tri_emission(const Frame1& f, const EmissiveTri& tri, float ba, float bb, float bc,
+                                           const Wl& wl, const ColorTables& ct) {
+    const Material& m = f.materials[tri.material];
+    if (m.flags & 4u) return dense4(f.spectra + (size_t)m.emissionSpectrum * kDense, wl) * m.emissive[0];
+    V3 e{m.emissive[0], m.emissive[1], m.emissive[2]};
+    if (m.textures & 64u) {
+        const float u = ba * tri.uv[0] + bb * tri.uv[2] + bc * tri.uv[4];
+        const float v = ba * tri.uv[1] + bb * tri.uv[3] + bc * tri.uv[5];
+        const float4 t = tex2d(f.texDescs, f.texels, m.tex[6], u, v);
+        e = V3{e.x * t.x, e.y * t.y, e.z * t.z};
+    }
+    return ct.rgb(e, wl);
+}
+
+__device__ __forceinline__ V2 attr2(const uint8_t* __restrict__ a, uint32_t stride, uint32_t off, uint32_t vi) {
+    const float2 v = *reinterpret_cast<const float2*>(a + (size_t)stride * vi + off);
+    return V2{v.x, v.y};
+}
+
+// glm::mat3_cast columns of a unit quaternion (x, y, z, w)
+__device__ __forceinline__ void quat_cols(float4 q, V3& c0, V3& c2) {
+    const float x = q.x, y = q.y, z = q.z, w = q.w;
+    c0 = V3{1.0f - 2.0f * (y * y + z * z), 2.0f * (x * y + w * z), 2.0f * (x * z - w * y)};
+    c2 = V3{2.0f * (x * z + w * y), 2.0f * (y * z - w * x), 1.0f - 2.0f * (x * x + y * y)};
+}
+
+__global__ __launch_bounds__(kShadeBlock) void k_shade(Frame1 f, uint32_t qsel) {
+    const uint32_t count = f.counters[qsel ? CNT_QUEUE1 : CNT_QUEUE0];
+    const uint32_t* __restrict__ qin = qsel ? f.p.q1 : f.p.q0;
+    uint32_t* __restrict__ qout = qsel ? f.p.q0 : f.p.q1;
+    uint32_t* __restrict__ cout = f.counters + (qsel ? CNT_QUEUE0 : CNT_QUEUE1);
+    const bool identity = f.bounce == 0;
+    const ColorTables ct{f.colors};
+    const Luts lut{f.luts};
+    const LightTree tree{f.treeNodes, f.treePointers, f.treeMasks, f.envOn != 0};
+    const bool treeOn = f.treeNodes != nullptr;
+    uint32_t nvert = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i - threadIdx.x < count; i += gridDim.x * blockDim.x) {
+        bool alive = false, shadow = false;
+        uint32_t pid = 0;
+        if (i < count) {
+            pid = identity ? i : qin[i];
+            ++nvert;
+            const float4 od = f.p.od[pid], dr = f.p.dr[pid], nf = f.p.nf[pid], hr = f.p.hit[pid];
+            PathIO P;
+            P.rd = V3{dr.x, dr.y, dr.z};
+            P.refr = dr.w;
+            P.bxdfPdf = od.w;
+            P.flags = __float_as_uint(nf.w);
+            {
+                const float4 a = f.p.lam[pid], b = f.p.lpdf[pid];
+                P.wl.lambda = S4{{a.x, a.y, a.z, a.w}};
+                P.wl.pdf = S4{{b.x, b.y, b.z, b.w}};
+            }
+            float4 Lv = f.p.L[pid], Tv = f.p.T[pid];
+            S4 L{{Lv.x, Lv.y, Lv.z, Lv.w}};
+            P.T = S4{{Tv.x, Tv.y, Tv.z, Tv.w}};
+            V3 ro{od.x, od.y, od.z};
+            V3 lastN{nf.x, nf.y, nf.z};
+            uint32_t depth = f.p.depth[pid];
+            const uint32_t px = pid % f.W, py = pid / f.W;
+            const Qrng q = qrng_at(f.sample, depth, pixel_hash(px, py));
+
+            // ---- interaction (pathtrace.cu:86-149, 906-953)
+            const uint32_t prim = __float_as_uint(hr.w);
+            const bool isHit = prim != 0xFFFFFFFFu;
+            uint32_t ltype = LT_NONE, lindex = 0, mat = 0;
+            V3 gn{0, 0, 0}, hitPos{0, 0, 0}, hitInterp{0, 0, 0}, v0{0, 0, 0}, v1{0, 0, 0}, v2{0, 0, 0};
+            uint32_t vi0 = 0, vi1 = 0, vi2 = 0;
+            const float b1 = hr.y, b2 = hr.z;
+            bool backHit = false;
+            if (!isHit) {
+                ltype = LT_SKY;
+                P.flags &= ~PF_TRIANGULAR;
+            } else {
+                vi0 = f.indices[3 * prim]; vi1 = f.indices[3 * prim + 1]; vi2 = f.indices[3 * prim + 2];
+                v0 = ld3(f.verts + 3 * (size_t)vi0); v1 = ld3(f.verts + 3 * (size_t)vi1); v2 = ld3(f.verts + 3 * (size_t)vi2);
+                V3 g = cross(v1 - v0, v2 - v0);
+                g = g * rsqrtf(dot(g, g));
+                gn = oct_dec_u32(oct_enc_u32(g));
+                mat = f.triMaterial[prim];
+                const uint32_t tl = f.triLight[prim];
+                if (tl != 0xFFFFFFFFu) { ltype = LT_TRIANGLE; lindex = tl; }
+                P.flags |= PF_TRIANGULAR;
+                const float b0 = 1.0f - b1 - b2;
+                hitPos = V3{v0.x * b0 + v1.x * b1 + v2.x * b2, v0.y * b0 + v1.y * b1 + v2.y * b2,
+                            v0.z * b0 + v1.z * b1 + v2.z * b2};
+                hitInterp = hitPos;
+                backHit = dot(gn, P.rd) > 0.0f;
+            }
+
+            // ---- emission + MIS (pathtrace.cu:1185-1274)
+            bool finished = false;
+            if (ltype != LT_NONE) {
+                const bool prevNEE = depth != 0 && !(P.flags & PF_PREV_SPECULAR);
+                const float treePdf = prevNEE ? tree.pdf(ltype, lindex, ro, lastN) : 1.0f;
+                float lightPdf = 0.0f;
+                S4 Le = s4(0.0f);
+                bool noEm = false;
+                if (ltype == LT_TRIANGLE) {
+                    const EmissiveTri& tri = f.lights[lindex];
+                    const V3 a = ld3(tri.v), b = ld3(tri.v + 3), c = ld3(tri.v + 6);
+                    V3 n = cross(b - a, c - a);
+                    const float nl = len(n);
+                    n = n / nl;
+                    const V3 th = hitPos - ro;
+                    const float d2 = th.x * th.x + th.y * th.y + th.z * th.z;
+                    const float lcos = -dot(P.rd, n);
+                    noEm = lcos <= 0.0f;
+                    lightPdf = (1.0f / (nl / 2.0f)) * d2 / lcos;
+                    Le = tri_emission(f, tri, 1.0f - b1 - b2, b1, b2, P.wl, ct);
+                } else {
+                    if (f.envOn) {
+                        const float phi = atan2f(P.rd.y, P.rd.x);
+                        const float theta = acosf(P.rd.z);
+                        const V3 sky = env_lookup(f.envTex, f.envW, f.envH, f.envC, 0.5f - phi / (2.0f * 3.14159265358979f),
+                                                  theta / 3.14159265358979f);
+                        lightPdf = (1.0f / (4.0f * 3.14159265358979f)) * ((sky.x + sky.y + sky.z) / 3.0f) * sinf(theta) /
+                                   f.envIntegral;
+                        Le = ct.rgb(sky, P.wl);
+                    } else {
+                        noEm = true;
+                    }
+                    P.flags |= PF_FINISHED;
+                }
+                if (!noEm) {
+                    lightPdf *= treePdf;
+                    const float w = prevNEE ? power_heuristic(P.bxdfPdf, lightPdf) : 1.0f;
+                    L += P.T * Le * w;
+                }
+            }
+            if (P.flags & PF_FINISHED) finished = true;
+
+            if (!finished) {
+                // ---- surface attributes (pathtrace.cu:1279-1567)
+                const Material& m = f.materials[mat];
+                const float bw0 = 1.0f - b1 - b2, bw1 = b1, bw2 = b2;
+                V2 tc1{0, 0}, tc2{0, 0};
+                V3 normal, tangent;
+                bool flipT = false;
+                const uint8_t* ab = f.attribs;
+                const uint32_t stride = m.attributeStride;
+                if (m.attributes & 3u) {
+                    V3 sN, sT, n0, n1, n2;
+                    if (m.attributes & 2u) {
+                        const float4 q0 = *reinterpret_cast<const float4*>(ab + (size_t)stride * vi0 + m.frameOffset);
+                        const float4 q1 = *reinterpret_cast<const float4*>(ab + (size_t)stride * vi1 + m.frameOffset);
+                        const float4 q2 = *reinterpret_cast<const float4*>(ab + (size_t)stride * vi2 + m.frameOffset);
+                        flipT = signbit(q0.w);
+                        V3 t0, t1, t2;
+                        quat_cols(q0, t0, n0);
+                        quat_cols(q1, t1, n1);
+                        quat_cols(q2, t2, n2);
+                        sN = bw0 * n0 + bw1 * n1 + bw2 * n2;
+                        sT = bw0 * t0 + bw1 * t1 + bw2 * t2;
+                    } else {
+                        const V2 e0 = attr2(ab, stride, m.normalOffset, vi0), e1 = attr2(ab, stride, m.normalOffset, vi1),
+                                 e2 = attr2(ab, stride, m.normalOffset, vi2);
+                        n0 = oct_dec(e0.x, e0.y); n1 = oct_dec(e1.x, e1.y); n2 = oct_dec(e2.x, e2.y);
+                        sN = bw0 * n0 + bw1 * n1 + bw2 * n2;
+                        const float sg = copysignf(1.0f, sN.z);
+                        const float aa = -1.0f / (sg + sN.z);
+                        const float bb = sN.x * sN.y * aa;
+                        sT = V3{1.0f + sg * (sN.x * sN.x) * aa, sg * bb, -sg * sN.x};
+                    }
+                    // Hanika 2021 shadow-terminator offset (pathtrace.cu:1380-1399)
+                    const V3 nA = nrm(n0), nB = nrm(n1), nC = nrm(n2);
+                    V3 tu = hitPos - v0, tv = hitPos - v1, tw = hitPos - v2;
+                    tu = tu - fminf(0.0f, dot(tu, nA)) * nA;
+                    tv = tv - fminf(0.0f, dot(tv, nB)) * nB;
+                    tw = tw - fminf(0.0f, dot(tw, nC)) * nC;
+                    hitInterp = hitPos + bw0 * tu + bw1 * tv + bw2 * tw;
+                    normal = nrm(sN);
+                    tangent = nrm(sT);
+                } else {
+                    normal = gn;
+                    const float sg = copysignf(1.0f, gn.z);
+                    const float aa = -1.0f / (sg + gn.z);
+                    const float bb = gn.x * gn.y * aa;
+                    tangent = V3{1.0f + sg * (gn.x * gn.x) * aa, sg * bb, -sg * gn.x};
+                }
+                if (m.attributes & 8u) {
+                    const V2 t0 = attr2(ab, stride, m.texCoord1Offset, vi0), t1 = attr2(ab, stride, m.texCoord1Offset, vi1),
+                             t2 = attr2(ab, stride, m.texCoord1Offset, vi2);
+                    tc1 = V2{bw0 * t0.x + bw1 * t1.x + bw2 * t2.x, bw0 * t0.y + bw1 * t1.y + bw2 * t2.y};
+                }
+                if (m.attributes & 16u) {
+                    const V2 t0 = attr2(ab, stride, m.texCoord2Offset, vi0), t1 = attr2(ab, stride, m.texCoord2Offset, vi1),
+                             t2 = attr2(ab, stride, m.texCoord2Offset, vi2);
+                    tc2 = V2{bw0 * t0.x + bw1 * t1.x + bw2 * t2.x, bw0 * t0.y + bw1 * t1.y + bw2 * t2.y};
+                }
+                auto uvOf = [&](int slot) { return ((m.texCoordSets >> slot) & 1u) ? tc2 : tc1; };
+                const V3 bit = cross(normal, tangent) * (flipT ? -1.0f : 1.0f);
+                const Frame tbn{tangent, bit, normal};
+                V3 n{0.0f, 0.0f, 1.0f};
+                if (m.textures & 2u) {
+                    const V2 uv = uvOf(1);
+                    const float4 t = tex2d(f.texDescs, f.texels, m.tex[1], uv.x, uv.y);
+                    n = nrm(V3{t.x * 2.0f - 1.0f, t.y * 2.0f - 1.0f, t.z * 2.0f - 1.0f});
+                }
+                n = tbn.to_world(n);
+                {
+                    const float dNs = dot(n, P.rd), dNg = dot(gn, P.rd);
+                    const bool fromOut = dNs > 0.0f && dNg < 0.0f;
+                    const bool fromIn = dNs < 0.0f && dNg > 0.0f;
+                    if (fromOut || fromIn) n = nrm(n - (P.rd * (dNs + (fromOut ? 0.001f : -0.001f))));
+                }
+                V3 sB = nrm(cross(n, tangent));
+                const V3 sT2 = nrm(cross(sB, n));
+                sB = sB * (flipT ? -1.0f : 1.0f);
+                const Frame local{sT2, sB, n};
+                lastN = n;
+
+                Surface s;
+                s.sheen = false;
+                s.base = V3{1.0f, 1.0f, 1.0f};
+                s.metalness = 1.0f;
+                s.ior = 1.5f;
+                s.roughness = 1.0f;
+                s.transmission = 0.0f;
+                s.sheenColor = V3{1.0f, 1.0f, 1.0f};
+                s.sheenAlpha = 0.0f;
+                bool skipped = false;
+                {
+                    const bool bcTex = (m.textures & 1u) != 0, bcFac = (m.factors & 1u) != 0;
+                    float alphaV = 0.0f;
+                    if (bcTex) {
+                        const V2 uv = uvOf(1);  // the reference reads the normal map's set here (pathtrace.cu:1453)
+                        const float4 t = tex2d(f.texDescs, f.texels, m.tex[0], uv.x, uv.y);
+                        s.base = V3{t.x, t.y, t.z};
+                        alphaV = t.w;
+                        if (bcFac) {
+                            s.base = s.base * V3{m.baseColor[0], m.baseColor[1], m.baseColor[2]};
+                            alphaV *= m.baseColor[3];
+                        }
+                    } else if (bcFac) {
+                        s.base = V3{m.baseColor[0], m.baseColor[1], m.baseColor[2]};
+                        alphaV = m.baseColor[3];
+                    }
+                    if (m.factors & 16u) {
+                        if (alphaV < m.alphaCutoff) skipped = true;
+                    } else if (m.factors & 32u) {
+                        if (q.s1(QD_ALPHA_BLEND) > alphaV) skipped = true;
+                    }
+                    const bool metF = (m.factors & 2u) != 0, rF = (m.factors & 4u) != 0;
+                    if (m.textures & 4u) {
+                        const V2 uv = uvOf(2);
+                        const float4 t = tex2d(f.texDescs, f.texels, m.tex[2], uv.x, uv.y);
+                        s.metalness = t.z * (metF ? m.metalness : 1.0f);
+                        s.roughness = t.y * (rF ? m.roughness : 1.0f);
+                    } else {
+                        if (metF) s.metalness = m.metalness;
+                        if (rF) s.roughness = m.roughness;
+                    }
+                    const bool trF = (m.factors & 8u) != 0;
+                    if (m.textures & 8u) {
+                        const V2 uv = uvOf(3);
+                        const float4 t = tex2d(f.texDescs, f.texels, m.tex[3], uv.x, uv.y);
+                        s.transmission = t.x * (trF ? m.transmission : 1.0f);
+                    } else if (trF) {
+                        s.transmission = m.transmission;
+                    }
+                    s.ior = m.ior;
+                    if (m.flags & 2u) {
+                        s.sheen = true;
+                        const bool shcF = (m.factors & 64u) != 0;
+                        if (m.textures & 16u) {
+                            const V2 uv = uvOf(4);
+                            const float4 t = tex2d(f.texDescs, f.texels, m.tex[4], uv.x, uv.y);
+                            s.sheenColor = V3{t.x * (shcF ? m.sheenColor[0] : 1.0f), t.y * (shcF ? m.sheenColor[1] : 1.0f),
+                                              t.z * (shcF ? m.sheenColor[2] : 1.0f)};
+                        } else if (shcF) {
+                            s.sheenColor = V3{m.sheenColor[0], m.sheenColor[1], m.sheenColor[2]};
+                        }
+                        const bool shrF = (m.factors & 128u) != 0;
+                        if (m.textures & 32u) {
+                            const V2 uv = uvOf(5);
+                            const float4 t = tex2d(f.texDescs, f.texels, m.tex[5], uv.x, uv.y);
+                            const float r = t.w * (shrF ? m.sheenRoughness : 1.0f);
+                            s.sheenAlpha = r * r;
+                        } else if (shrF) {
+                            s.sheenAlpha = m.sheenRoughness;
+                        }
+                    }
+                }
+
+                // ---- NEE sample (pathtrace.cu:1569-1720) + BxDF
+                if (!skipped) {
+                    Light dl;
+                    dl.spectrum = s4(0.0f);
+                    dl.dir = V3{0, 0, 0};
+                    dl.pdf = 0.0f;
+                    dl.occluded = false;
+                    V3 lro{0, 0, 0};
+                    float dToL = 0.0f;
+                    if (!f.envOn && !treeOn) {
+                        dl.occluded = true;
+                    } else {
+                        const V3 rnd = q.s3(QD_LIGHT);
+                        uint32_t type = LT_NONE, index = 0;
+                        float treePdf = 0.0f;
+                        tree.sample(rnd.z, hitPos, lastN, type, index, treePdf);
+                        float lightPdf = 0.0f;
+                        if (type == LT_TRIANGLE) {
+                            const EmissiveTri& tri = f.lights[index];
+                            const V3 a = ld3(tri.v), b = ld3(tri.v + 3), c = ld3(tri.v + 6);
+                            V3 nl = cross(b - a, c - a);
+                            const float nlen = len(nl);
+                            nl = nl / nlen;
+                            V2 u2{rnd.x, rnd.y};
+                            if (u2.x + u2.y > 1.0f) u2 = V2{1.0f - u2.x, 1.0f - u2.y};
+                            const float ba = 1.0f - u2.x - u2.y, bb = u2.x, bc = u2.y;
+                            const V3 lp = a + (b - a) * u2.x + (c - a) * u2.y;
+                            const bool inSample = dot(lastN, lp - hitPos) < 0.0f;
+                            lro = offset_point(inSample ? hitPos : hitInterp, inSample ? -gn : gn);
+                            const V3 rl = lp - lro;
+                            const float s2 = rl.x * rl.x + rl.y * rl.y + rl.z * rl.z;
+                            dToL = sqrtf(s2);
+                            dl.dir = rl / dToL;
+                            const float lcos = -dot(nl, dl.dir);
+                            dl.occluded = lcos <= 0.0f;
+                            lightPdf = (1.0f / (nlen / 2.0f)) * s2 / lcos;
+                            dl.spectrum = tri_emission(f, tri, ba, bb, bc, P.wl, ct);
+                        } else if (type == LT_SKY) {
+                            const float W = (float)f.envW, H = (float)f.envH;
+                            const float fcx = fminf(rnd.x * (W - 1.0f), W - 1.0001f);
+                            const float fcy = fminf(rnd.y * (H - 1.0f), H - 1.0001f);
+                            const float flx = floorf(fcx), fly = floorf(fcy);
+                            const uint32_t ix = (uint32_t)flx, iy = (uint32_t)fly;
+                            const float m0 = (float)f.envMarg[iy], m1 = (float)f.envMarg[iy + 1];
+                            const size_t row = (size_t)(uint32_t)(m0 * W);
+                            const float c0 = (float)f.envCond[row + ix], c1 = (float)f.envCond[row + ix + 1];
+                            const float ux = lerpf(c0, c1, fcx - flx) * (1.0f / W);
+                            const float uy = lerpf(m0, m1, fcy - fly) * (1.0f / H);
+                            float sp, cp, st, ctt;
+                            sincosf(2.0f * 3.14159265358979f * ux, &sp, &cp);
+                            sincosf(3.14159265358979f * uy, &st, &ctt);
+                            dl.dir = V3{-st * cp, st * sp, ctt};
+                            const V3 rgb = env_lookup(f.envTex, f.envW, f.envH, f.envC, ux, uy);
+                            dl.spectrum = ct.rgb(rgb, P.wl);
+                            lightPdf = (1.0f / (4.0f * 3.14159265358979f)) * ((rgb.x + rgb.y + rgb.z) / 3.0f) * st / f.envIntegral;
+                            dToL = 3.402823466e+38f;
+                            lro = offset_point(hitInterp, backHit ? -gn : gn);
+                        } else {
+                            dl.occluded = true;
+                        }
+                        dl.pdf = lightPdf * treePdf;
+                    }
+                    S4 Lnee = s4(0.0f);
+                    if (m.bxdf == 0)
+                        conductor_bxdf(f.spectra + (size_t)m.iorSpectrum * kDense, f.spectra + (size_t)m.acSpectrum * kDense,
+                                       m.mfRoughness, dl, q, local, P, Lnee);
+                    else if (m.bxdf == 2)
+                        complex_bxdf(ct, lut, dl, q, local, s, P, Lnee);
+                    // bxdf 1: PureDielectric has an empty body in the reference
+                    if (!dl.occluded) {
+                        shadow = true;
+                        f.p.Lnee[pid] = make_float4(Lnee.v[0], Lnee.v[1], Lnee.v[2], Lnee.v[3]);
+                        f.p.sro[pid] = make_float4(lro.x, lro.y, lro.z, dToL - 0.1f);
+                        f.p.srd[pid] = make_float4(dl.dir.x, dl.dir.y, dl.dir.z, 0.0f);
+                    }
+                }
+
+                // ---- Russian roulette (pathtrace.cu:1736-1744)
+                bool rr = false;
+                {
+                    const float tMax = smax4(P.T) * P.refr;
+                    if (tMax < 1.0f && depth > 0) {
+                        float qq = fmaxf(0.0f, 1.0f - tMax);
+                        qq = qq * qq;
+                        if (q.s1(QD_ROULETTE) < qq) rr = true;
+                        else P.T = P.T * (1.0f / (1.0f - qq));
+                    }
+                }
+                if (!rr) {
+                    // ---- next origin + direction fix (pathtrace.cu:1746-1766)
+                    const bool refracted = (P.flags & PF_RAY_REFRACTED) != 0;
+                    const bool triG = (P.flags & PF_TRIANGULAR) != 0;
+                    const bool in = refracted || skipped;
+                    const V3 org = ((in == backHit) && triG) ? hitInterp : hitPos;
+                    const V3 off = in == backHit ? gn : -gn;
+                    ro = offset_point(org, off);
+                    const float dNgRo = dot(gn, P.rd);
+                    if (dNgRo > 0.0f && (backHit != refracted) && !skipped) P.rd = nrm(P.rd - (gn * (dNgRo + 0.001f)));
+                    // ---- depth caps (pathtrace.cu:1768-1779)
+                    bool cont = ++depth < f.maxDepth;
+                    if (!(P.flags & PF_CUR_SPECULAR) && !skipped)
+                        cont = (P.flags & PF_REFRACTION_HAPPENED) ? depth < f.maxTrans : depth < f.maxRefl;
+                    if (!cont) P.flags |= PF_FINISHED;
+                    // ---- updateStateFlags (pathtrace.cu:954-972)
+                    const uint32_t excl = PF_CUR_SPECULAR | PF_PREV_SPECULAR | PF_TRIANGULAR | PF_RIGHT_HANDED | PF_RAY_REFRACTED;
+                    uint32_t incl = (P.flags & PF_CUR_SPECULAR) ? PF_PREV_SPECULAR : PF_REGULARIZED;
+                    if (P.flags & PF_RAY_REFRACTED) incl |= PF_REFRACTION_HAPPENED;
+                    P.flags = (P.flags & ~excl) | incl;
+                    alive = !(P.flags & PF_FINISHED) && !(P.flags & PF_TERMINATED);
+                }
+            }
+            f.p.L[pid] = make_float4(L.v[0], L.v[1], L.v[2], L.v[3]);
+            f.p.T[pid] = make_float4(P.T.v[0], P.T.v[1], P.T.v[2], P.T.v[3]);
+            f.p.od[pid] = make_float4(ro.x, ro.y, ro.z, P.bxdfPdf);
+            f.p.dr[pid] = make_float4(P.rd.x, P.rd.y, P.rd.z, P.refr);
+            f.p.nf[pid] = make_float4(lastN.x, lastN.y, lastN.z, __uint_as_float(P.flags));
+            f.p.depth[pid] = depth;
+        }
+        const uint32_t slot = wave_push(cout, alive);
+        if (alive) qout[slot] = pid;
+        const uint32_t sslot = wave_push(f.counters + CNT_SHADOW, shadow);
+        if (shadow) f.p.sq[sslot] = pid;
+    }
+    wave_count(f.stats + STAT_VERTICES, nvert);
+}
+
+// ----------------------------------------------------------- accumulation
+__global__ __launch_bounds__(256) void k_accumulate(Frame1 f, uint32_t first) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= f.P) return;
+    const uint32_t flags = __float_as_uint(f.p.nf[i].w);
+    double4 a = first ? make_double4(0.0, 0.0, 0.0, 0.0) : f.accum[i];
+    if (!(flags & PF_TERMINATED)) {
+        const float4 Lv = f.p.L[i], la = f.p.lam[i], lp = f.p.lpdf[i];
+        Wl wl;
+        wl.lambda = S4{{la.x, la.y, la.z, la.w}};
+        wl.pdf = S4{{lp.x, lp.y, lp.z, lp.w}};
+        S4 L{{Lv.x, Lv.y, Lv.z, Lv.w}};
+        for (int k = 0; k < kNL; ++k) L.v[k] = wl.pdf.v[k] != 0.0f ? L.v[k] / wl.pdf.v[k] : 0.0f;
+        double xyz[3];
+        ColorTables{f.colors}.xyz(wl, L, xyz);
+        a.x += xyz[0];
+        a.y += xyz[1];
+        a.z += xyz[2];
+    }
+    a.w += 1.0;
+    f.accum[i] = a;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host
+hipError_t launch_pass(rbh_ctx c, Frame1 f, uint32_t depthMax, bool firstPass, int traceGrid, int shadeGrid) {
+    const int nb = (int)((f.P + 255) / 256);
+    f.bounce = 0;
+    hipLaunchKernelGGL(k_camera, dim3(nb), dim3(256), 0, c->stream, f);
+    for (uint32_t b = 0; b < depthMax; ++b) {
+        const uint32_t qsel = b & 1u;
+        f.bounce = b;
+        hipLaunchKernelGGL(k_trace, dim3(traceGrid), dim3(kTraceBlock), 0, c->stream, f, qsel);
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, c->stream, f.counters, qsel ? CNT_QUEUE0 : CNT_QUEUE1);
+        hipLaunchKernelGGL(k_shade, dim3(shadeGrid), dim3(kShadeBlock), 0, c->stream, f, qsel);
+        hipLaunchKernelGGL(k_shadow, dim3(traceGrid), dim3(kTraceBlock), 0, c->stream, f);
+    }
+    hipLaunchKernelGGL(k_accumulate, dim3(nb), dim3(256), 0, c->stream, f, firstPass ? 1u : 0u);
+    return hipGetLastError();
+}
+
+}  // namespace rbh

@@ -1,0 +1,275 @@
+// Linear BVH build on gfx950, replacing optixAccelBuild
+// (src/core/rendering_interface.cpp:733-1064):
+//   1. triangle centroid bounds (block reduction + ordered-int atomics)
+//   2. 30-bit Morton codes on the centroid grid, made unique with the
+//      triangle index in the low 32 bits of a 64-bit key
+//   3. LSD radix sort of the keys (rocPRIM via hipCUB)
+//   4. Karras (HPG 2012) hierarchy: internal node i covers a key range and
+//      splits at the highest differing bit
+//   5. bottom-up refit with per-node arrival counters
+//   6. emission of 64-byte two-child nodes (both child boxes in the parent)
+//      and of triangles in leaf order
+// Every root-to-leaf path strictly lengthens the common key prefix, so the
+// depth is at most 64 and a 64-entry traversal stack never overflows.
+#include "../context.hpp"
+#include <hipcub/hipcub.hpp>
+
+namespace rbh {
+
+namespace {
+
+constexpr int kBuildBlock = 256;
+
+__device__ __forceinline__ uint32_t f2ord(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(uint32_t u) {
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+__global__ void k_bounds_init(uint32_t* __restrict__ b) {
+    if (threadIdx.x < 3) b[threadIdx.x] = 0xFFFFFFFFu;           // min (ordered)
+    else if (threadIdx.x < 6) b[threadIdx.x] = 0u;               // max (ordered)
+    else if (threadIdx.x == 6) b[6] = 0u;                         // depth
+}
+
+__global__ __launch_bounds__(kBuildBlock) void k_centroid_bounds(const float* __restrict__ v, const uint32_t* __restrict__ idx,
+                                                                 uint32_t n, uint32_t* __restrict__ b) {
+    __shared__ float smin[3][kBuildBlock], smax[3][kBuildBlock];
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = t; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t a = idx[3 * i], bb = idx[3 * i + 1], c = idx[3 * i + 2];
+        for (int k = 0; k < 3; ++k) {
+            const float lo = fminf(fminf(v[3 * a + k], v[3 * bb + k]), v[3 * c + k]);
+            const float hi = fmaxf(fmaxf(v[3 * a + k], v[3 * bb + k]), v[3 * c + k]);
+            const float ce = 0.5f * (lo + hi);
+            mn[k] = fminf(mn[k], ce);
+            mx[k] = fmaxf(mx[k], ce);
+        }
+    }
+    for (int k = 0; k < 3; ++k) { smin[k][threadIdx.x] = mn[k]; smax[k][threadIdx.x] = mx[k]; }
+    __syncthreads();
+    for (int s = kBuildBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int k = 0; k < 3; ++k) {
+                smin[k][threadIdx.x] = fminf(smin[k][threadIdx.x], smin[k][threadIdx.x + s]);
+                smax[k][threadIdx.x] = fmaxf(smax[k][threadIdx.x], smax[k][threadIdx.x + s]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; ++k) {
+            atomicMin(&b[k], f2ord(smin[k][0]));
+            atomicMax(&b[3 + k], f2ord(smax[k][0]));
+        }
+}
+
+__device__ __forceinline__ uint32_t spread10(uint32_t x) {
+    x &= 0x3FFu;
+    x = (x | (x << 16)) & 0x030000FFu;
+    x = (x | (x << 8)) & 0x0300F00Fu;
+    x = (x | (x << 4)) & 0x030C30C3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+__global__ __launch_bounds__(kBuildBlock) void k_morton(const float* __restrict__ v, const uint32_t* __restrict__ idx,
+                                                        uint32_t n, const uint32_t* __restrict__ b,
+                                                        unsigned long long* __restrict__ keys) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float lo[3], ext[3];
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = ord2f(b[k]);
+        ext[k] = ord2f(b[3 + k]) - lo[k];
+    }
+    const uint32_t a = idx[3 * i], bb = idx[3 * i + 1], c = idx[3 * i + 2];
+    uint32_t q[3];
+    for (int k = 0; k < 3; ++k) {
+        const float l = fminf(fminf(v[3 * a + k], v[3 * bb + k]), v[3 * c + k]);
+        const float h = fmaxf(fmaxf(v[3 * a + k], v[3 * bb + k]), v[3 * c + k]);
+        const float ce = 0.5f * (l + h);
+        const float u = ext[k] > 0.0f ? (ce - lo[k]) / ext[k] : 0.5f;
+        q[k] = (uint32_t)fminf(fmaxf(u * 1024.0f, 0.0f), 1023.0f);
+    }
+    const uint32_t m = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+    keys[i] = ((unsigned long long)m << 32) | i;
+}
+
+__device__ __forceinline__ int common_prefix(const unsigned long long* __restrict__ k, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    return __clzll(k[i] ^ k[j]);
+}
+
+// Karras 2012, Figure 4. Internal nodes 0..n-2; child ids >= 0 internal,
+// < 0 leaf ~sortedIndex. parent[] holds internal parents of internal nodes
+// (slot i) and of leaves (slot n-1+j).
+__global__ __launch_bounds__(kBuildBlock) void k_karras(const unsigned long long* __restrict__ k, int n,
+                                                        int2* __restrict__ child, int* __restrict__ parent) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    const int d = (common_prefix(k, n, i, i + 1) - common_prefix(k, n, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = common_prefix(k, n, i, i - d);
+    int lmax = 2;
+    while (common_prefix(k, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (common_prefix(k, n, i, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = common_prefix(k, n, i, j);
+    int s = 0;
+    int t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (common_prefix(k, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    const int split = i + s * d + min(d, 0);
+    const int first = min(i, j), last = max(i, j);
+    const int left = (first == split) ? ~split : split;
+    const int right = (last == split + 1) ? ~(split + 1) : split + 1;
+    child[i] = make_int2(left, right);
+    if (left >= 0) parent[left] = i; else parent[n - 1 + split] = i;
+    if (right >= 0) parent[right] = i; else parent[n - 1 + split + 1] = i;
+    if (i == 0) parent[0] = -1;
+}
+
+struct Box { float lo[3], hi[3]; };
+
+__global__ __launch_bounds__(kBuildBlock) void k_refit(const float* __restrict__ v, const uint32_t* __restrict__ idx,
+                                                       const unsigned long long* __restrict__ keys, int n,
+                                                       const int2* __restrict__ child, const int* __restrict__ parent,
+                                                       float* __restrict__ ibox, float* __restrict__ lbox,
+                                                       unsigned int* __restrict__ flags, TriRecord* __restrict__ tris) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t tri = (uint32_t)(keys[j] & 0xFFFFFFFFull);
+    const uint32_t a = idx[3 * tri], b = idx[3 * tri + 1], c = idx[3 * tri + 2];
+    TriRecord r;
+    r.v0x = v[3 * a]; r.v0y = v[3 * a + 1]; r.v0z = v[3 * a + 2]; r.prim = tri;
+    r.v1x = v[3 * b]; r.v1y = v[3 * b + 1]; r.v1z = v[3 * b + 2]; r.pad0 = 0;
+    r.v2x = v[3 * c]; r.v2y = v[3 * c + 1]; r.v2z = v[3 * c + 2]; r.pad1 = 0;
+    tris[j] = r;
+    float* lb = lbox + 6 * (size_t)j;
+    lb[0] = fminf(fminf(r.v0x, r.v1x), r.v2x); lb[3] = fmaxf(fmaxf(r.v0x, r.v1x), r.v2x);
+    lb[1] = fminf(fminf(r.v0y, r.v1y), r.v2y); lb[4] = fmaxf(fmaxf(r.v0y, r.v1y), r.v2y);
+    lb[2] = fminf(fminf(r.v0z, r.v1z), r.v2z); lb[5] = fmaxf(fmaxf(r.v0z, r.v1z), r.v2z);
+    int p = n > 1 ? parent[n - 1 + j] : -1;
+    while (p >= 0) {
+        __threadfence();
+        if (atomicAdd(&flags[p], 1u) == 0u) return;  // sibling not done yet
+        __threadfence();
+        const int2 ch = child[p];
+        const float* A = ch.x >= 0 ? ibox + 6 * (size_t)ch.x : lbox + 6 * (size_t)(~ch.x);
+        const float* B = ch.y >= 0 ? ibox + 6 * (size_t)ch.y : lbox + 6 * (size_t)(~ch.y);
+        float* o = ibox + 6 * (size_t)p;
+        for (int k = 0; k < 3; ++k) {
+            o[k] = fminf(__builtin_nontemporal_load(A + k), __builtin_nontemporal_load(B + k));
+            o[3 + k] = fmaxf(__builtin_nontemporal_load(A + 3 + k), __builtin_nontemporal_load(B + 3 + k));
+        }
+        p = parent[p];
+    }
+}
+
+// Depth of every leaf (number of internal ancestors); the maximum bounds the
+// traversal stack.
+__global__ __launch_bounds__(kBuildBlock) void k_depth(int n, const int* __restrict__ parent, uint32_t* __restrict__ depthOut) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    uint32_t d = 0;
+    for (int p = n > 1 ? parent[n - 1 + j] : -1; p >= 0 && d < 1000; p = parent[p]) ++d;
+    atomicMax(depthOut, d);
+}
+
+__global__ __launch_bounds__(kBuildBlock) void k_emit(int n, const int2* __restrict__ child, const float* __restrict__ ibox,
+                                                      const float* __restrict__ lbox, BvhNode* __restrict__ nodes) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    const int2 ch = child[i];
+    const float* A = ch.x >= 0 ? ibox + 6 * (size_t)ch.x : lbox + 6 * (size_t)(~ch.x);
+    const float* B = ch.y >= 0 ? ibox + 6 * (size_t)ch.y : lbox + 6 * (size_t)(~ch.y);
+    BvhNode nd;
+    nd.lminx = A[0]; nd.lmaxx = A[3]; nd.lminy = A[1]; nd.lmaxy = A[4];
+    nd.rminx = B[0]; nd.rmaxx = B[3]; nd.rminy = B[1]; nd.rmaxy = B[4];
+    nd.lminz = A[2]; nd.lmaxz = A[5]; nd.rminz = B[2]; nd.rmaxz = B[5];
+    nd.left = ch.x; nd.right = ch.y;
+    nd.lcnt = ch.x < 0 ? 1u : 0u;
+    nd.rcnt = ch.y < 0 ? 1u : 0u;
+    nodes[i] = nd;
+}
+
+__global__ void k_zero_u32(unsigned int* __restrict__ p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+inline size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+}  // namespace
+
+int lbvh_build(rbh_ctx c) {
+    Geometry& g = c->geom;
+    Accel& acc = c->accel;
+    const int n = (int)g.ntris;
+    if (n <= 0) return fail(c, RBH_ERR_STATE, "no geometry uploaded");
+    const size_t nInt = n > 1 ? (size_t)n - 1 : 1;
+    // scratch layout
+    size_t off = 0;
+    const size_t oBounds = off; off = align_up(off + 64);
+    const size_t oKeys = off; off = align_up(off + (size_t)n * 8);
+    const size_t oKeys2 = off; off = align_up(off + (size_t)n * 8);
+    const size_t oChild = off; off = align_up(off + nInt * 8);
+    const size_t oParent = off; off = align_up(off + ((size_t)n + nInt) * 4);
+    const size_t oIbox = off; off = align_up(off + nInt * 24);
+    const size_t oLbox = off; off = align_up(off + (size_t)n * 24);
+    const size_t oFlags = off; off = align_up(off + nInt * 4);
+    size_t sortBytes = 0;
+    RBH_HIP(c, hipcub::DeviceRadixSort::SortKeys(nullptr, sortBytes, (unsigned long long*)nullptr,
+                                                 (unsigned long long*)nullptr, n, 0, 64, c->stream));
+    const size_t oSort = off; off = align_up(off + sortBytes);
+    RBH_ALLOC(c, acc.scratch, off);
+    RBH_ALLOC(c, acc.nodes, nInt * sizeof(BvhNode));
+    RBH_ALLOC(c, acc.tris, (size_t)n * sizeof(TriRecord));
+    char* s = acc.scratch.as<char>();
+    uint32_t* bounds = reinterpret_cast<uint32_t*>(s + oBounds);
+    auto* keys = reinterpret_cast<unsigned long long*>(s + oKeys);
+    auto* keys2 = reinterpret_cast<unsigned long long*>(s + oKeys2);
+    int2* child = reinterpret_cast<int2*>(s + oChild);
+    int* parent = reinterpret_cast<int*>(s + oParent);
+    float* ibox = reinterpret_cast<float*>(s + oIbox);
+    float* lbox = reinterpret_cast<float*>(s + oLbox);
+    unsigned int* flags = reinterpret_cast<unsigned int*>(s + oFlags);
+
+    const float* v = g.verts.as<float>();
+    const uint32_t* idx = g.indices.as<uint32_t>();
+    const int nb = (n + kBuildBlock - 1) / kBuildBlock;
+    RBH_HIP(c, hipEventRecord(c->ev0, c->stream));
+    hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, c->stream, bounds);
+    hipLaunchKernelGGL(k_centroid_bounds, dim3(std::min(nb, 2048)), dim3(kBuildBlock), 0, c->stream, v, idx, (uint32_t)n, bounds);
+    hipLaunchKernelGGL(k_morton, dim3(nb), dim3(kBuildBlock), 0, c->stream, v, idx, (uint32_t)n, bounds, keys);
+    RBH_HIP(c, hipcub::DeviceRadixSort::SortKeys(s + oSort, sortBytes, keys, keys2, n, 0, 64, c->stream));
+    const int nbi = (int)((nInt + kBuildBlock - 1) / kBuildBlock);
+    hipLaunchKernelGGL(k_zero_u32, dim3(nbi), dim3(kBuildBlock), 0, c->stream, flags, nInt);
+    if (n > 1) hipLaunchKernelGGL(k_karras, dim3(nbi), dim3(kBuildBlock), 0, c->stream, keys2, n, child, parent);
+    hipLaunchKernelGGL(k_refit, dim3(nb), dim3(kBuildBlock), 0, c->stream, v, idx, keys2, n, child, parent, ibox, lbox,
+                       flags, acc.tris.as<TriRecord>());
+    hipLaunchKernelGGL(k_depth, dim3(nb), dim3(kBuildBlock), 0, c->stream, n, parent, bounds + 6);
+    if (n > 1) hipLaunchKernelGGL(k_emit, dim3(nbi), dim3(kBuildBlock), 0, c->stream, n, child, ibox, lbox, acc.nodes.as<BvhNode>());
+    RBH_HIP(c, hipGetLastError());
+    RBH_HIP(c, hipEventRecord(c->ev1, c->stream));
+    RBH_HIP(c, hipStreamSynchronize(c->stream));
+    float ms = 0.0f;
+    RBH_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    uint32_t depth = 0;
+    RBH_HIP(c, hipMemcpy(&depth, bounds + 6, 4, hipMemcpyDeviceToHost));
+    c->stats.bvhMs = ms;
+    c->stats.bvhDepth = depth;
+    acc.nnodes = n > 1 ? (uint32_t)(n - 1) : 0u;
+    acc.depth = depth;
+    if (depth > 64) return fail(c, RBH_ERR_RANGE, "LBVH depth %u exceeds the 64-entry traversal stack", depth);
+    acc.built = true;
+    return RBH_OK;
+}
+
+}  // namespace rbh

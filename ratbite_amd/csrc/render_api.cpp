@@ -1,0 +1,250 @@
+// C ABI entry points of the acceleration structure, light tree and render
+// loop (include/ratbite_hip.h).
+#include <cstring>
+#include <vector>
+
+#include "context.hpp"
+#include "integrator.hpp"
+
+namespace rbh {
+int lbvh_build(rbh_ctx c);
+hipError_t launch_pass(rbh_ctx c, Frame1 f, uint32_t depthMax, bool firstPass, int traceGrid, int shadeGrid);
+hipError_t resolve_launch(rbh_ctx c, const float* colorspace, float expo, void* out);
+namespace host {
+int build_light_tree(const EmissiveTri* tris, const float* flux, uint32_t n, std::vector<PackedNode>& nodes,
+                     std::vector<uint32_t>& pointers, std::vector<uint64_t>& masks);
+}
+}  // namespace rbh
+
+using namespace rbh;
+
+namespace {
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Path-state buffers for P slots; returns the byte size and fills pointers
+// when base != nullptr.
+size_t layout_paths(char* base, size_t P, PathBuffers* pb, uint32_t** counters, unsigned long long** stats) {
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char* p = base ? base + off : nullptr;
+        off = align256(off + bytes);
+        return p;
+    };
+    float4** f4[] = {&pb->od, &pb->dr, &pb->lam, &pb->lpdf, &pb->L, &pb->T, &pb->nf, &pb->hit, &pb->Lnee, &pb->sro, &pb->srd};
+    for (float4** q : f4) *q = reinterpret_cast<float4*>(take(P * sizeof(float4)));
+    pb->depth = reinterpret_cast<uint32_t*>(take(P * 4));
+    pb->q0 = reinterpret_cast<uint32_t*>(take(P * 4));
+    pb->q1 = reinterpret_cast<uint32_t*>(take(P * 4));
+    pb->sq = reinterpret_cast<uint32_t*>(take(P * 4));
+    *counters = reinterpret_cast<uint32_t*>(take(CNT_COUNT * 4));
+    *stats = reinterpret_cast<unsigned long long*>(take(STAT_COUNT * 8));
+    return off;
+}
+
+__global__ void k_zero_stats(unsigned long long* s) {
+    if (threadIdx.x < STAT_COUNT) s[threadIdx.x] = 0ull;
+}
+
+int cu_count(int device) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || n <= 0) n = 256;
+    return n;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rbh_build_accel(rbh_ctx c) {
+    if (!c) return RBH_ERR_INVALID;
+    return lbvh_build(c);
+}
+
+int rbh_build_light_tree(rbh_ctx c, const rbh_emissive_tri* tris, const float* flux, uint32_t n) {
+    if (!c) return RBH_ERR_INVALID;
+    if (n && (!tris || !flux)) return fail(c, RBH_ERR_INVALID, "light arrays are null");
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!(flux[i] > 0.0f) || !std::isfinite(flux[i])) return fail(c, RBH_ERR_INVALID, "light %u: flux must be positive and finite", i);
+        if (tris[i].materialIndex >= c->nmaterials) return fail(c, RBH_ERR_INVALID, "light %u: material %u of %u", i, tris[i].materialIndex, c->nmaterials);
+    }
+    LightTreeDev& t = c->tree;
+    const EmissiveTri* et = reinterpret_cast<const EmissiveTri*>(tris);
+    std::vector<PackedNode> nodes;
+    const int r = host::build_light_tree(et, flux, n, nodes, t.hostPointers, t.hostMasks);
+    if (r == -1) return fail(c, RBH_ERR_RANGE, "light count %u exceeds the tree's 27-bit leaf offsets", n);
+    if (r < 0) return fail(c, RBH_ERR_INVALID, "light tree produced an invalid bitmask");
+    t.hostNodes.resize(nodes.size());
+    std::memcpy(t.hostNodes.data(), nodes.data(), nodes.size() * sizeof(PackedNode));
+    t.nnodes = (uint32_t)nodes.size();
+    t.nlights = n;
+    if (n == 0) return RBH_OK;
+    RBH_ALLOC(c, t.nodes, nodes.size() * sizeof(PackedNode));
+    RBH_ALLOC(c, t.lightPointers, (size_t)n * 4);
+    RBH_ALLOC(c, t.bitmasks, (size_t)n * 8);
+    RBH_ALLOC(c, t.tris, (size_t)n * sizeof(EmissiveTri));
+    RBH_HIP(c, hipMemcpyAsync(t.nodes.p, nodes.data(), nodes.size() * sizeof(PackedNode), hipMemcpyHostToDevice, c->stream));
+    RBH_HIP(c, hipMemcpyAsync(t.lightPointers.p, t.hostPointers.data(), (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+    RBH_HIP(c, hipMemcpyAsync(t.bitmasks.p, t.hostMasks.data(), (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+    RBH_HIP(c, hipMemcpyAsync(t.tris.p, tris, (size_t)n * sizeof(EmissiveTri), hipMemcpyHostToDevice, c->stream));
+    RBH_HIP(c, hipStreamSynchronize(c->stream));
+    return RBH_OK;
+}
+
+int rbh_light_tree_build_host(const rbh_emissive_tri* tris, const float* flux, uint32_t n, rbh_packed_node* nodes,
+                              uint32_t* lightPointers, uint64_t* bitmasks) {
+    if (n && (!tris || !flux || !nodes || !lightPointers || !bitmasks)) return RBH_ERR_INVALID;
+    std::vector<PackedNode> nv;
+    std::vector<uint32_t> lp;
+    std::vector<uint64_t> bm;
+    const int r = host::build_light_tree(reinterpret_cast<const EmissiveTri*>(tris), flux, n, nv, lp, bm);
+    if (r == -1) return RBH_ERR_RANGE;
+    if (r < 0) return RBH_ERR_INVALID;
+    if (r) {
+        std::memcpy(nodes, nv.data(), nv.size() * sizeof(PackedNode));
+        std::memcpy(lightPointers, lp.data(), lp.size() * 4);
+        std::memcpy(bitmasks, bm.data(), bm.size() * 8);
+    }
+    return r;
+}
+
+int rbh_get_light_tree(rbh_ctx c, rbh_packed_node* nodes, uint32_t* nnodes, uint32_t* lightPointers, uint64_t* bitmasks) {
+    if (!c) return RBH_ERR_INVALID;
+    const LightTreeDev& t = c->tree;
+    if (nnodes) *nnodes = t.nnodes;
+    if (nodes && t.nnodes) std::memcpy(nodes, t.hostNodes.data(), (size_t)t.nnodes * sizeof(PackedNode));
+    if (lightPointers && t.nlights) std::memcpy(lightPointers, t.hostPointers.data(), (size_t)t.nlights * 4);
+    if (bitmasks && t.nlights) std::memcpy(bitmasks, t.hostMasks.data(), (size_t)t.nlights * 8);
+    return RBH_OK;
+}
+
+int rbh_render(rbh_ctx c, const rbh_launch_params* lp) {
+    if (!c || !lp) return RBH_ERR_INVALID;
+    if (!c->accel.built) return fail(c, RBH_ERR_STATE, "rbh_build_accel has not been run on the current geometry");
+    if (!c->haveColor || !c->haveLuts) return fail(c, RBH_ERR_STATE, "colour tables and LUTs must be uploaded first");
+    if (lp->width == 0 || lp->height == 0) return fail(c, RBH_ERR_INVALID, "empty film");
+    if ((uint64_t)lp->width * lp->height > (1ull << 28)) return fail(c, RBH_ERR_RANGE, "film above 2^28 pixels");
+    if (lp->maxPathDepth == 0 || lp->maxPathDepth > 1024) return fail(c, RBH_ERR_INVALID, "maxPathDepth must be in [1, 1024]");
+    const bool treeOn = c->tree.nlights > 0;
+    if (c->geom.maxLight >= (int64_t)c->tree.nlights)
+        return fail(c, RBH_ERR_STATE, "triangle light index %lld has no light record (%u lights built)",
+                    (long long)c->geom.maxLight, c->tree.nlights);
+    RenderState& rs = c->rs;
+    const size_t P = (size_t)lp->width * lp->height;
+    PathBuffers pb{};
+    uint32_t* counters = nullptr;
+    unsigned long long* stats = nullptr;
+    const size_t bytes = layout_paths(nullptr, P, &pb, &counters, &stats);
+    RBH_ALLOC(c, rs.paths, bytes);
+    layout_paths(rs.paths.as<char>(), P, &pb, &counters, &stats);
+    const bool sizeChanged = rs.width != lp->width || rs.height != lp->height;
+    RBH_ALLOC(c, rs.accum, P * sizeof(double) * 4);
+    if (sizeChanged && lp->sampleOffset != 0) return fail(c, RBH_ERR_STATE, "sampleOffset != 0 but no accumulation of this size exists");
+    rs.width = lp->width;
+    rs.height = lp->height;
+
+    Frame1 f{};
+    f.P = (uint32_t)P;
+    f.W = lp->width;
+    f.H = lp->height;
+    f.maxDepth = std::max(1u, lp->maxPathDepth);
+    f.maxRefl = std::min(f.maxDepth, lp->maxReflectedPathDepth);
+    f.maxTrans = std::min(f.maxDepth, lp->maxTransmittedPathDepth);
+    f.invW = 1.0f / (float)lp->width;
+    f.invH = 1.0f / (float)lp->height;
+    f.psW = lp->perspectiveScaleW;
+    f.psH = lp->perspectiveScaleH;
+    f.camU = dev::V3{lp->camU[0], lp->camU[1], lp->camU[2]};
+    f.camV = dev::V3{lp->camV[0], lp->camV[1], lp->camV[2]};
+    f.camW = dev::V3{lp->camW[0], lp->camW[1], lp->camW[2]};
+    f.dof = lp->depthOfField;
+    f.aperture = lp->aperture;
+    f.focus = lp->focusDistance;
+    f.nodes = c->accel.nodes.as<BvhNode>();
+    f.tris = c->accel.tris.as<TriRecord>();
+    f.nnodes = c->accel.nnodes;
+    f.verts = c->geom.verts.as<float>();
+    f.indices = c->geom.indices.as<uint32_t>();
+    f.attribs = c->geom.attribs.as<uint8_t>();
+    f.triMaterial = c->geom.triMaterial.as<uint32_t>();
+    f.triLight = c->geom.triLight.as<uint32_t>();
+    f.materials = c->materials.as<Material>();
+    f.texDescs = c->textures.as<TextureDesc>();
+    f.texels = c->texels.as<uint8_t>();
+    f.spectra = c->spectra.as<float>();
+    f.colors = c->colorTables.as<float>();
+    f.luts = c->luts.as<float>();
+    f.treeNodes = treeOn ? c->tree.nodes.as<PackedNode>() : nullptr;
+    f.treePointers = treeOn ? c->tree.lightPointers.as<uint32_t>() : nullptr;
+    f.treeMasks = treeOn ? c->tree.bitmasks.as<uint64_t>() : nullptr;
+    f.lights = treeOn ? c->tree.tris.as<EmissiveTri>() : nullptr;
+    f.envOn = c->env.enabled ? 1u : 0u;
+    if (c->env.enabled) {
+        f.envW = (uint32_t)c->env.width;
+        f.envH = (uint32_t)c->env.height;
+        f.envC = (uint32_t)c->env.channels;
+        f.envTex = c->env.texels.as<float>();
+        f.envCond = c->env.cond.as<uint16_t>();
+        f.envMarg = c->env.marg.as<uint16_t>();
+        f.envIntegral = c->env.integral;
+    }
+    f.p = pb;
+    f.counters = counters;
+    f.stats = stats;
+    f.accum = rs.accum.as<double4>();
+
+    const int cus = cu_count(c->device);
+    const int traceGrid = cus * 16;
+    const int shadeGrid = cus * 16;
+    RBH_HIP(c, hipEventRecord(c->ev0, c->stream));
+    hipLaunchKernelGGL(k_zero_stats, dim3(1), dim3(64), 0, c->stream, stats);
+    for (uint32_t s = 0; s < lp->sampleCount; ++s) {
+        f.sample = lp->sampleOffset + s;
+        const bool first = (lp->sampleOffset == 0 && s == 0);
+        RBH_HIP(c, launch_pass(c, f, f.maxDepth, first, traceGrid, shadeGrid));
+    }
+    RBH_HIP(c, hipEventRecord(c->ev1, c->stream));
+    RBH_HIP(c, hipStreamSynchronize(c->stream));
+    float ms = 0.0f;
+    RBH_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    unsigned long long st[STAT_COUNT] = {};
+    RBH_HIP(c, hipMemcpy(st, stats, sizeof(st), hipMemcpyDeviceToHost));
+    c->stats.renderMs = ms;
+    c->stats.closestRays = st[STAT_CLOSEST];
+    c->stats.shadowRays = st[STAT_SHADOW];
+    c->stats.pathVertices = st[STAT_VERTICES];
+    c->stats.samples = (uint64_t)P * lp->sampleCount;
+    c->stats.traceLaunches = lp->sampleCount * f.maxDepth;
+    return RBH_OK;
+}
+
+int rbh_read_accum(rbh_ctx c, double* rgbw) {
+    if (!c || !rgbw) return RBH_ERR_INVALID;
+    if (!c->rs.accum.p) return fail(c, RBH_ERR_STATE, "nothing rendered");
+    RBH_HIP(c, hipStreamSynchronize(c->stream));
+    RBH_HIP(c, hipMemcpy(rgbw, c->rs.accum.p, (size_t)c->rs.width * c->rs.height * 32, hipMemcpyDeviceToHost));
+    return RBH_OK;
+}
+
+int rbh_write_accum(rbh_ctx c, const double* rgbw, uint32_t w, uint32_t h) {
+    if (!c || !rgbw || w == 0 || h == 0) return RBH_ERR_INVALID;
+    RBH_ALLOC(c, c->rs.accum, (size_t)w * h * 32);
+    RBH_HIP(c, hipMemcpyAsync(c->rs.accum.p, rgbw, (size_t)w * h * 32, hipMemcpyHostToDevice, c->stream));
+    RBH_HIP(c, hipStreamSynchronize(c->stream));
+    c->rs.width = w;
+    c->rs.height = h;
+    return RBH_OK;
+}
+
+int rbh_resolve(rbh_ctx c, const float* colorspace, float exposureScale, uint8_t* rgba8) {
+    if (!c || !colorspace || !rgba8) return RBH_ERR_INVALID;
+    if (!c->rs.accum.p) return fail(c, RBH_ERR_STATE, "nothing rendered");
+    const size_t n = (size_t)c->rs.width * c->rs.height;
+    RBH_ALLOC(c, c->rs.image, n * 4);
+    RBH_HIP(c, resolve_launch(c, colorspace, exposureScale, c->rs.image.p));
+    RBH_HIP(c, hipMemcpyAsync(rgba8, c->rs.image.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    RBH_HIP(c, hipStreamSynchronize(c->stream));
+    return RBH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,445 @@
+"""Seeded synthetic workloads for the benchmark configurations.
+
+A Scene holds flattened, camera-relative world-space geometry plus every
+table the renderer needs, as numpy arrays in the byte layouts of
+include/ratbite_hip.h. The same arrays feed the HIP library and the CPU
+oracle.
+
+  cornell(...)   256x256 Cornell box: 5 wall quads, 8 UV spheres (32x16),
+                 one emissive quad with a 4000 K blackbody spectrum, glossy-
+                 diffuse OpenPBR materials, no env map, depth 8, 16 spp.
+  spheres(...)   1920x1080 field of displaced level-5 icospheres, textured
+                 emissive triangles feeding the light tree, a log-normal
+                 env map with a sun, a 40/25/20/15 mix of glossy-diffuse /
+                 conductor (Cauchy eta, k) / translucent / sheen materials,
+                 depth 16, 64 spp.
+"""
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._lib import EMISSIVE_TRI_DTYPE, MATERIAL_DTYPE, TEXTURE_DTYPE, LaunchParams
+from .envmaps import lognormal_envmap
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TABLES = os.path.join(HERE, "data", "tables.npz")
+
+# MaterialData bitfields (src/core/material.h:19-57)
+AT_NORMAL, AT_FRAME, AT_COLOR, AT_TEX1, AT_TEX2 = 1, 2, 4, 8, 16
+TX_BASE_COLOR, TX_NORMAL, TX_MET_ROUGH, TX_TRANSMISSION, TX_SHEEN_COLOR, TX_SHEEN_ROUGH, TX_EMISSION = 1, 2, 4, 8, 16, 32, 64
+FC_BASE_COLOR, FC_METALNESS, FC_ROUGHNESS, FC_TRANSMISSION, FC_CUTOFF, FC_BLEND = 1, 2, 4, 8, 16, 32
+FC_SHEEN_COLOR, FC_SHEEN_ROUGH, FC_EMISSION = 64, 128, 256
+MF_DOUBLE_SIDED, MF_SHEEN, MF_SPECTRAL_EMISSION = 1, 2, 4
+BXDF_CONDUCTOR, BXDF_DIELECTRIC, BXDF_COMPLEX = 0, 1, 2
+NO_LIGHT = 0xFFFFFFFF
+ATTRIB_STRIDE = 16   # oct normal (2 f32) + uv0 (2 f32)
+
+
+def load_tables():
+    t = np.load(TABLES)
+    return {k: t[k] for k in t.files}
+
+
+@dataclass
+class Scene:
+    verts: np.ndarray
+    indices: np.ndarray
+    attribs: np.ndarray
+    tri_material: np.ndarray
+    tri_light: np.ndarray
+    materials: np.ndarray
+    spectra: np.ndarray
+    textures: np.ndarray
+    texels: np.ndarray
+    lights: np.ndarray
+    flux: np.ndarray
+    env: np.ndarray = None
+    camera: dict = field(default_factory=dict)
+    width: int = 256
+    height: int = 256
+    depth: int = 8
+    spp: int = 16
+    tables: dict = field(default_factory=dict)
+    name: str = ""
+
+    def launch_params(self, offset=0, count=None, width=None, height=None, depth=None, flags=0):
+        p = LaunchParams()
+        p.width = width or self.width
+        p.height = height or self.height
+        d = depth or self.depth
+        p.maxPathDepth = d
+        p.maxReflectedPathDepth = d
+        p.maxTransmittedPathDepth = d
+        p.sampleOffset = offset
+        p.sampleCount = self.spp if count is None else count
+        for k, name in (("u", "camU"), ("v", "camV"), ("w", "camW")):
+            getattr(p, name)[:] = [float(x) for x in self.camera[k]]
+        fov = self.camera.get("fov", math.radians(70.0))
+        p.perspectiveScaleW = float(np.float32(math.tan(fov * 0.5) * (p.width / p.height)))
+        p.perspectiveScaleH = float(np.float32(math.tan(fov * 0.5)))
+        p.depthOfField = 0
+        p.aperture = 0.0
+        p.focusDistance = 1.0
+        p.flags = flags
+        return p
+
+    @property
+    def ntris(self):
+        return len(self.indices)
+
+
+# ----------------------------------------------------------------- helpers
+def oct_encode(n):
+    """src/core/math_util.h:147-162 on arrays (attribute normals)."""
+    n = np.asarray(n, np.float32)
+    s = np.abs(n).sum(axis=1, keepdims=True)
+    p = n[:, :2] / s
+    neg = n[:, 2] <= 0
+    sx = np.where(p[:, 0] >= 0, 1.0, -1.0)
+    sy = np.where(p[:, 1] >= 0, 1.0, -1.0)
+    q = p.copy()
+    q[neg, 0] = (1.0 - np.abs(p[neg, 1])) * sx[neg]
+    q[neg, 1] = (1.0 - np.abs(p[neg, 0])) * sy[neg]
+    return q.astype(np.float32)
+
+
+def pack_attribs(normals, uvs):
+    n = len(normals)
+    a = np.zeros((n, 4), np.float32)
+    a[:, 0:2] = oct_encode(normals)
+    a[:, 2:4] = uvs
+    return a.view(np.uint8).reshape(-1)
+
+
+def camera_basis(pos, target, up):
+    """src/core/camera.h:33-38"""
+    v = np.asarray(target, np.float64) - np.asarray(pos, np.float64)
+    v /= np.linalg.norm(v)
+    u = np.cross(v, np.asarray(up, np.float64))
+    u /= np.linalg.norm(u)
+    w = np.cross(u, v)
+    return {"pos": np.asarray(pos, np.float64), "u": u.astype(np.float32), "v": v.astype(np.float32),
+            "w": w.astype(np.float32)}
+
+
+def material(bxdf=BXDF_COMPLEX, base=(1, 1, 1), metal=0.0, rough=0.5, ior=1.5, transmission=None,
+             sheen=None, emissive=None, emission_tex=-1, spectral_emission=None, eta_k=None):
+    m = np.zeros(1, MATERIAL_DTYPE)[0]
+    m["bxdf"] = bxdf
+    m["attributes"] = AT_NORMAL | AT_TEX1
+    m["attributeStride"] = ATTRIB_STRIDE
+    m["normalOffset"] = 0
+    m["texCoord1Offset"] = 8
+    m["ior"] = ior
+    m["tex"][:] = -1
+    m["baseColor"][:] = (base[0], base[1], base[2], 1.0)
+    m["metalness"] = metal
+    m["roughness"] = rough
+    m["mfRoughness"] = rough
+    m["factors"] = FC_BASE_COLOR | FC_METALNESS | FC_ROUGHNESS
+    if transmission is not None:
+        m["transmission"] = transmission
+        m["factors"] |= FC_TRANSMISSION
+    if sheen is not None:
+        m["flags"] |= MF_SHEEN
+        m["sheenColor"][:] = sheen[0]
+        m["sheenRoughness"] = sheen[1]
+        m["factors"] |= FC_SHEEN_COLOR | FC_SHEEN_ROUGH
+    if emissive is not None:
+        m["emissive"][:] = emissive
+        m["factors"] |= FC_EMISSION
+    if emission_tex >= 0:
+        m["textures"] |= TX_EMISSION
+        m["tex"][6] = emission_tex
+    if spectral_emission is not None:
+        m["flags"] |= MF_SPECTRAL_EMISSION
+        m["emissionSpectrum"] = spectral_emission[0]
+        m["emissive"][:] = (spectral_emission[1], 0, 0)
+    if eta_k is not None:
+        m["iorSpectrum"], m["acSpectrum"] = eta_k
+    return m
+
+
+def emissive_records(verts, indices, tri_ids, tri_material, uvs_per_tri):
+    rec = np.zeros(len(tri_ids), EMISSIVE_TRI_DTYPE)
+    v = verts[indices[tri_ids]]                  # (n, 3, 3)
+    rec["vertices"] = v.reshape(-1, 9)
+    rec["uvs"] = uvs_per_tri.reshape(-1, 6)
+    rec["materialIndex"] = tri_material[tri_ids]
+    rec["primitiveDataIndex"] = tri_ids
+    return rec
+
+
+def triangle_flux(rec, materials, tex_avg):
+    """Emitted flux of each light as src/core/scene.cpp:519-607 computes it:
+    mean emissive factor x mean textured emission over the UV footprint x
+    area x pi (spectral emitters: luminance-normalised spectrum x scale)."""
+    v = rec["vertices"].reshape(-1, 3, 3).astype(np.float64)
+    area = 0.5 * np.linalg.norm(np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0]), axis=1)
+    m = materials[rec["materialIndex"]]
+    f = m["emissive"].mean(axis=1).astype(np.float64)
+    spectral = (m["flags"] & MF_SPECTRAL_EMISSION) != 0
+    f = np.where(spectral, m["emissive"][:, 0], f)
+    has_tex = (m["textures"] & TX_EMISSION) != 0
+    tscale = np.where(has_tex, tex_avg[np.maximum(m["tex"][:, 6], 0)] if len(tex_avg) else 1.0, 1.0)
+    return (f * tscale * area * np.pi).astype(np.float32)
+
+
+def value_noise_texture(rng, size=256, cells=8):
+    g = rng.random((cells + 1, cells + 1, 3))
+    g[cells] = g[0]
+    g[:, cells] = g[:, 0]
+    t = (np.arange(size) + 0.5) / size * cells
+    i = np.floor(t).astype(int)
+    f = t - i
+    f = f * f * (3 - 2 * f)
+    a = g[i][:, i]
+    b = g[i + 1][:, i]
+    c = g[i][:, i + 1]
+    d = g[i + 1][:, i + 1]
+    fy = f[:, None, None]
+    fx = f[None, :, None]
+    val = (a * (1 - fy) + b * fy) * (1 - fx) + (c * (1 - fy) + d * fy) * fx
+    rgba = np.concatenate([val, np.ones((size, size, 1))], axis=2)
+    return (np.clip(rgba, 0, 1) * 255 + 0.5).astype(np.uint8)
+
+
+def uv_footprint_average(tex):
+    """Mean (r+g+b)*a/3 of the texels whose centres fall inside the UV
+    triangle (0,0) (1,0) (0,1), the rasterised footprint of
+    src/core/scene.cpp:571-586 for that triangle."""
+    h, w = tex.shape[:2]
+    ys, xs = np.mgrid[0:h, 0:w]
+    inside = ((xs + 0.5) / w + (ys + 0.5) / h) <= 1.0
+    t = tex.astype(np.float64) / 255.0
+    e = (t[..., 0] + t[..., 1] + t[..., 2]) * t[..., 3] / 3.0
+    return float(e[inside].mean())
+
+
+def _transform(verts, cam_pos):
+    return (np.asarray(verts, np.float64) - cam_pos).astype(np.float32)
+
+
+# ------------------------------------------------------------------ shapes
+def quad(p0, p1, p2, p3):
+    v = np.array([p0, p1, p2, p3], np.float64)
+    idx = np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
+    n = np.cross(v[1] - v[0], v[2] - v[0])
+    n /= np.linalg.norm(n)
+    uv = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float32)
+    return v, idx, np.repeat(n[None], 4, 0), uv
+
+
+def uv_sphere(center, radius, seg=32, rings=16):
+    verts, norms, uvs = [], [], []
+    for r in range(rings + 1):
+        th = math.pi * r / rings
+        for s in range(seg + 1):
+            ph = 2 * math.pi * s / seg
+            n = np.array([math.sin(th) * math.cos(ph), math.cos(th), math.sin(th) * math.sin(ph)])
+            verts.append(np.asarray(center) + radius * n)
+            norms.append(n)
+            uvs.append((s / seg, r / rings))
+    idx = []
+    for r in range(rings):
+        for s in range(seg):
+            a = r * (seg + 1) + s
+            b = a + seg + 1
+            if r != 0:
+                idx.append((a, a + 1, b))
+            if r != rings - 1:
+                idx.append((a + 1, b + 1, b))
+    return np.array(verts), np.array(idx, np.uint32), np.array(norms), np.array(uvs, np.float32)
+
+
+def icosphere(level):
+    t = (1.0 + 5 ** 0.5) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6),
+         (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+         (8, 6, 7), (9, 8, 1)]
+    verts = [np.array(p, np.float64) / np.linalg.norm(p) for p in v]
+    faces = np.array(f, np.int64)
+    for _ in range(level):
+        cache = {}
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = np.array(nf, np.int64)
+    return np.array(verts), faces.astype(np.uint32)
+
+
+def vertex_normals(verts, idx):
+    v = verts[idx]
+    fn = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])
+    n = np.zeros_like(verts)
+    for k in range(3):
+        np.add.at(n, idx[:, k], fn)
+    return n / np.maximum(np.linalg.norm(n, axis=1, keepdims=True), 1e-20)
+
+
+def random_rotation(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def _finish(parts, mats, spectra, tex_list, cam, env, width, height, depth, spp, name, tables, light_uv=None):
+    verts = np.concatenate([p[0] for p in parts])
+    norms = np.concatenate([p[2] for p in parts])
+    uvs = np.concatenate([p[3] for p in parts])
+    idx, tm, off = [], [], 0
+    for p in parts:
+        idx.append(p[1] + off)
+        tm.append(np.full(len(p[1]), p[4], np.uint32) if np.isscalar(p[4]) else p[4])
+        off += len(p[0])
+    idx = np.concatenate(idx).astype(np.uint32)
+    tri_material = np.concatenate(tm).astype(np.uint32)
+    tri_light = np.full(len(idx), NO_LIGHT, np.uint32)
+    materials = np.array(mats, MATERIAL_DTYPE)
+    em = ((materials["factors"][tri_material] & FC_EMISSION) != 0)
+    light_ids = np.nonzero(em)[0].astype(np.uint32)
+    tri_light[light_ids] = np.arange(len(light_ids), dtype=np.uint32)
+    verts_c = _transform(verts, cam["pos"])
+    if light_uv is None:
+        luv = uvs[idx[light_ids]]
+    else:
+        luv = np.broadcast_to(light_uv, (len(light_ids), 3, 2))
+    lights = emissive_records(verts_c, idx, light_ids, tri_material, np.ascontiguousarray(luv, np.float32))
+    # textures
+    descs = np.zeros(len(tex_list), TEXTURE_DTYPE)
+    blob, offs, avgs = [], 0, []
+    for i, t in enumerate(tex_list):
+        descs[i]["width"], descs[i]["height"], descs[i]["depth"] = t.shape[1], t.shape[0], 1
+        descs[i]["format"], descs[i]["srgb"], descs[i]["addr"] = 0, 1, (0, 0, 0)
+        descs[i]["filter"], descs[i]["offset"] = 1, offs
+        blob.append(t.reshape(-1))
+        offs += t.size
+        avgs.append(uv_footprint_average(t))
+    texels = np.concatenate(blob) if blob else np.zeros(0, np.uint8)
+    flux = triangle_flux(lights, materials, np.array(avgs))
+    return Scene(verts=verts_c, indices=idx, attribs=pack_attribs(norms, uvs), tri_material=tri_material,
+                 tri_light=tri_light, materials=materials,
+                 spectra=np.ascontiguousarray(np.array(spectra, np.float32).reshape(-1, 471)),
+                 textures=descs, texels=texels, lights=lights, flux=flux, env=env, camera=cam, width=width,
+                 height=height, depth=depth, spp=spp, tables=tables, name=name)
+
+
+# ----------------------------------------------------------------- configs
+def cornell(seed=0x5EED0001, width=256, height=256, spp=16, depth=8):
+    rng = np.random.default_rng(seed)
+    tables = load_tables()
+    mats, parts = [], []
+    spectra = [tables["blackbody4000"]]
+
+    def gd():
+        return material(base=rng.uniform(0.1, 0.9, 3), metal=0.0, rough=float(rng.uniform(0.05, 1.0)), ior=1.5)
+
+    walls = [
+        ((-1, 0, -1), (-1, 0, 1), (1, 0, 1), (1, 0, -1)),          # floor (faces +y)
+        ((-1, 2, -1), (1, 2, -1), (1, 2, 1), (-1, 2, 1)),          # ceiling (faces -y)
+        ((-1, 0, -1), (-1, 2, -1), (1, 2, -1), (1, 0, -1)),        # back wall (faces +z)
+        ((-1, 0, -1), (-1, 0, 1), (-1, 2, 1), (-1, 2, -1)),        # left wall (faces +x)
+        ((1, 0, -1), (1, 2, -1), (1, 2, 1), (1, 0, 1)),            # right wall (faces -x)
+    ]
+    for w in walls:
+        v, i, n, uv = quad(*w)
+        if np.dot(n[0], np.array([0.0, 1.0, 0.0]) - v.mean(axis=0)) < 0:   # face into the box
+            v, i, n, uv = quad(*w[::-1])
+        mats.append(gd())
+        parts.append((v, i, n, uv, len(mats) - 1))
+    # emissive quad just below the ceiling, facing down, blackbody 4000 K
+    v, i, n, uv = quad((-0.3, 1.98, -0.3), (0.3, 1.98, -0.3), (0.3, 1.98, 0.3), (-0.3, 1.98, 0.3))
+    mats.append(material(base=(0.8, 0.8, 0.8), metal=0.0, rough=1.0, spectral_emission=(0, 12.0)))
+    mats[-1]["factors"] |= FC_EMISSION
+    parts.append((v, i, n, uv, len(mats) - 1))
+    for _ in range(8):
+        r = float(rng.uniform(0.1, 0.3))
+        c = rng.uniform([-1 + r, r, -1 + r], [1 - r, 2 - r, 1 - r])
+        v, i, n, uv = uv_sphere(c, r)
+        mats.append(gd())
+        parts.append((v, i, n, uv, len(mats) - 1))
+    cam = camera_basis((0.0, 1.0, 3.4), (0.0, 1.0, 0.0), (0.0, 1.0, 0.0))
+    cam["fov"] = math.radians(45.0)
+    return _finish(parts, mats, spectra, [], cam, None, width, height, depth, spp, "cornell", tables)
+
+
+def cauchy_spectra(rng):
+    """Seeded Cauchy fits n = A + B / l^2, k = C + D / l^2 (l in um) at 1 nm."""
+    lam = np.arange(360, 831, dtype=np.float64) * 1e-3
+    A, B = rng.uniform(0.2, 1.5), rng.uniform(0.0, 0.15)
+    C, D = rng.uniform(1.5, 4.0), rng.uniform(0.0, 0.6)
+    return (A + B / lam ** 2).astype(np.float32), (C + D / lam ** 2).astype(np.float32)
+
+
+def spheres(seed=0x5EED0002, n_spheres=200, level=5, n_lights=10000, n_light_tex=16, width=1920, height=1080,
+            spp=64, depth=16, env_size=(2048, 1024), env_sigma=2.0):
+    rng = np.random.default_rng(seed)
+    tables = load_tables()
+    base_v, base_i = icosphere(level)
+    mats, parts, spectra = [], [], []
+    for s in range(n_spheres):
+        R = random_rotation(rng)
+        scale = float(np.exp(rng.uniform(np.log(0.2), np.log(5.0))))
+        center = np.array([rng.uniform(-60, 60), rng.uniform(-8, 25), rng.uniform(-120, -15)])
+        # radial displacement from smooth noise on the unit sphere
+        k = rng.normal(size=(4, 3))
+        ph = rng.uniform(0, 2 * np.pi, 4)
+        disp = 1.0 + 0.08 * np.sum(np.sin(base_v @ k.T * 3.0 + ph), axis=1) / 4.0
+        v = (base_v * disp[:, None]) @ R.T * scale + center
+        n = vertex_normals(v, base_i)
+        uv = np.stack([0.5 + np.arctan2(base_v[:, 2], base_v[:, 0]) / (2 * np.pi),
+                       np.arccos(np.clip(base_v[:, 1], -1, 1)) / np.pi], axis=1).astype(np.float32)
+        r = rng.random()
+        rough = float(rng.uniform(0.02, 1.0))
+        if r < 0.40:
+            m = material(base=rng.uniform(0.1, 0.9, 3), metal=0.0, rough=rough)
+        elif r < 0.65:
+            eta, kk = cauchy_spectra(rng)
+            spectra += [eta, kk]
+            m = material(bxdf=BXDF_CONDUCTOR, rough=rough, eta_k=(len(spectra) - 2, len(spectra) - 1))
+        elif r < 0.85:
+            m = material(base=rng.uniform(0.5, 1.0, 3), metal=0.0, rough=rough, ior=float(rng.uniform(1.3, 1.8)),
+                         transmission=1.0)
+        else:
+            m = material(base=rng.uniform(0.1, 0.9, 3), metal=0.0, rough=rough,
+                         sheen=(rng.uniform(0.3, 1.0, 3), float(rng.uniform(0.05, 1.0))))
+        mats.append(m)
+        parts.append((v, base_i, n, uv, len(mats) - 1))
+    # textured emissive triangles scattered through the volume
+    tex_list = [value_noise_texture(rng) for _ in range(n_light_tex)]
+    em_first = len(mats)
+    for t in range(n_light_tex):
+        mats.append(material(base=(0.8, 0.8, 0.8), rough=1.0, emissive=rng.uniform(2.0, 8.0, 3), emission_tex=t))
+    c = rng.uniform([-70, -10, -130], [70, 30, -10], size=(n_lights, 3))
+    size = np.exp(rng.uniform(np.log(0.3), np.log(1.5), n_lights))
+    lv = np.zeros((n_lights * 3, 3))
+    for k in range(3):
+        d = rng.normal(size=(n_lights, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        lv[k::3] = c + d * size[:, None]
+    li = np.arange(n_lights * 3, dtype=np.uint32).reshape(-1, 3)
+    ln = vertex_normals(lv, li)
+    luv = np.tile(np.array([[0, 0], [1, 0], [0, 1]], np.float32), (n_lights, 1))
+    lmat = (em_first + rng.integers(0, n_light_tex, n_lights)).astype(np.uint32)
+    parts.append((lv, li, ln, luv, lmat))
+    if not spectra:
+        spectra = [np.ones(471, np.float32)]
+    cam = camera_basis((0.0, 5.0, 10.0), (0.0, 4.0, -60.0), (0.0, 1.0, 0.0))
+    cam["fov"] = math.radians(60.0)
+    env = lognormal_envmap(env_size[0], env_size[1], sigma=env_sigma, seed=seed, sun=True) if env_size else None
+    return _finish(parts, mats, spectra, tex_list, cam, env, width, height, depth, spp, "spheres", tables)

@@ -1,0 +1,108 @@
+"""Light tree (src/core/light_tree.cpp, src/kernels/pathtrace.cu:974-1112).
+
+* the product's host builder (rbh_light_tree_build_host) and the oracle's
+  independent restatement produce the same topology, nodes and masks bit for
+  bit;
+* structural invariants of the reference (every bitmask reaches the leaf that
+  holds its light, <= 2n-1 nodes, <= 8 lights per leaf, flux conserved);
+* the oracle's PDF/sampling pair is pinned by invariants: the PDFs of all
+  lights sum to 1 - envImportance, the PDF reported by sampling equals the
+  traversal PDF of the sampled light, and sampling frequencies follow the PDF.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from ratbite_amd import _lib as L
+from ratbite_amd import scenes
+
+
+def _lights(n, seed):
+    sc = scenes.spheres(n_spheres=1, level=1, n_lights=n, width=8, height=8, env_size=None, seed=seed)
+    return sc.lights, sc.flux
+
+
+def product_build(lights, flux):
+    n = len(lights)
+    nodes = np.zeros(max(1, 2 * n), L.PACKED_NODE_DTYPE)
+    lp = np.zeros(max(1, n), np.uint32)
+    bm = np.zeros(max(1, n), np.uint64)
+    r = L.lib().rbh_light_tree_build_host(L.ptr(np.ascontiguousarray(lights)), L.ptr(np.ascontiguousarray(flux, np.float32)),
+                                          n, L.ptr(nodes), L.ptr(lp), L.ptr(bm))
+    assert r >= 0, r
+    return nodes[:r], lp[:n], bm[:n]
+
+
+@pytest.mark.parametrize("n,seed", [(1, 1), (7, 2), (9, 3), (100, 4), (2000, 5)])
+def test_builder_bit_exact_with_oracle(n, seed):
+    lights, flux = _lights(n, seed)
+    a = product_build(lights, flux)
+    b = O.light_tree_build(lights, flux)
+    assert len(a[0]) == len(b[0])
+    assert a[0].tobytes() == b[0].tobytes()
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+
+
+def _walk(nodes, mask):
+    i, depth = 0, 0
+    while (int(nodes[i]["coreData"]) >> 31) == 0:
+        i = int(nodes[i]["coreData"]) if (int(mask) >> depth) & 1 else i + 1
+        depth += 1
+    core = int(nodes[i]["coreData"])
+    return i, (core >> 27) & 15, core & ((1 << 27) - 1)
+
+
+def test_structural_invariants():
+    lights, flux = _lights(3000, 11)
+    nodes, lp, bm = product_build(lights, flux)
+    n = len(lights)
+    assert len(nodes) <= 2 * n - 1
+    for light in range(0, n, 7):
+        leaf, cnt, off = _walk(nodes, bm[light])
+        assert 1 <= cnt <= 8
+        assert light in set(int(x) & ((1 << 30) - 1) for x in lp[off:off + cnt])
+    assert abs(float(nodes[0]["flux"]) - float(flux.astype(np.float64).sum())) / float(flux.sum()) < 1e-4
+
+
+@pytest.mark.parametrize("env", [0, 1])
+def test_pdf_partition_of_unity(env):
+    lights, flux = _lights(500, 12)
+    nodes, lp, bm = O.light_tree_build(lights, flux)
+    n = len(lights)
+    rng = np.random.default_rng(3)
+    for _ in range(5):
+        pos = rng.uniform([-60, -8, -120], [60, 25, -10]).astype(np.float32)
+        nrm = rng.normal(size=3).astype(np.float32)
+        nrm /= np.linalg.norm(nrm)
+        pn = np.ascontiguousarray(np.tile(np.concatenate([pos, nrm]), (n, 1)), np.float32)
+        ids = np.arange(n, dtype=np.uint32)
+        pdf = np.zeros(n, np.float32)
+        O.lib().orc_light_tree_pdf(O.p(nodes), O.p(lp), O.p(bm), env, O.p(ids), O.p(pn), O.p(pdf), n)
+        total = pdf.astype(np.float64).sum()
+        assert abs(total - (1.0 - 0.3 * env)) < 1e-4
+
+
+def test_sample_pdf_consistency_and_frequencies():
+    lights, flux = _lights(40, 13)
+    nodes, lp, bm = O.light_tree_build(lights, flux)
+    n = len(lights)
+    rng = np.random.default_rng(4)
+    pos = np.array([0.0, 5.0, -50.0], np.float32)
+    nrm = np.array([0.0, 1.0, 0.0], np.float32)
+    m = 200000
+    inp = np.zeros((m, 7), np.float32)
+    inp[:, 0] = (np.arange(m) + rng.random(m)) / m     # stratified
+    inp[:, 1:4], inp[:, 4:7] = pos, nrm
+    ti = np.zeros((m, 2), np.uint32)
+    sp = np.zeros(m, np.float32)
+    O.lib().orc_light_tree_sample(O.p(nodes), O.p(lp), O.p(bm), 0, O.p(inp), O.p(ti), O.p(sp), m)
+    assert np.all(ti[:, 0] == 0)
+    ids = ti[:, 1]
+    pn = np.ascontiguousarray(np.tile(np.concatenate([pos, nrm]), (n, 1)), np.float32)
+    pdf = np.zeros(n, np.float32)
+    O.lib().orc_light_tree_pdf(O.p(nodes), O.p(lp), O.p(bm), 0, O.p(np.arange(n, dtype=np.uint32)), O.p(pn), O.p(pdf), n)
+    np.testing.assert_allclose(sp, pdf[ids], rtol=1e-5)
+    freq = np.bincount(ids, minlength=n) / m
+    np.testing.assert_allclose(freq, pdf, atol=3e-3)
