@@ -17,8 +17,13 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
           "-I" + os.path.join(HERE, "..", "include")]
+# Shading, light-tree and intersection arithmetic is evaluated as written (no
+# implicit FMA contraction), the same operation sequence the CPU restatement
+# uses: near-specular lobe PDFs, sharp spherical-Gaussian amplitudes and shared
+# triangle edges all amplify a one-ulp rounding difference. Kernels that want
+# fused multiply-adds spell them out with fmaf().
 DEVICE = ["-x", "hip", "--offload-arch=gfx950", "-munsafe-fp-atomics",
-          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-ffp-contract=fast"]
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-ffp-contract=off"]
 
 
 def sources():
