@@ -1,0 +1,165 @@
+#!/usr/bin/env python3
+"""Throughput of the spectral path-tracing hot path on one MI355X.
+
+Workload: the seeded 1920x1080 scene of 200 displaced level-5 icospheres
+(4.1M triangles), 10,000 textured emissive triangles behind the Gaussian light
+tree, a 2048x1024 log-normal env map with a sun, the four-way material mix,
+64 spp per frame and path depth 16. One step renders one full frame
+(rbh_render with sampleCount 64) into the FP64 accumulation buffer; the scene,
+LBVH, light tree and env-map tables are built once before the timed region.
+
+Printed: one JSON line with samples/s (whole frames), Mrays/s, the roofline
+of the dominant kernel measured with HIP events on the render stream, and the
+CPU baseline: the scalar C++ restatement (oracle/) on the centre 256x256 crop
+of the same frame, timed on this machine's host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per unit of work, from the reference's own data: an
+# extension ray moves a 32 B ray in and a 16 B hit out, a shadow ray 32 B in
+# and a 4 B flag out, a shaded vertex reads and writes the 136 B path state.
+BYTES_CLOSEST = 48
+BYTES_SHADOW = 36
+BYTES_VERTEX = 272
+BYTES_PIXEL_LAUNCH = 64      # FP64 RGBW read-modify-write per pixel per pass
+KERNEL_NAMES = {"trace": "k_trace", "shade": "k_shade", "shadow": "k_shadow"}
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3, help="timed frames")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed frames")
+    ap.add_argument("--spp", type=int, default=64, help="samples per pixel per frame")
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--depth", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-spp", type=int, default=32, help="spp of the CPU baseline crop")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, host cores)")
+    return ap.parse_args()
+
+
+def traffic_per_launch(kernel):
+    """HBM bytes per launch of `kernel`, from the committed PMC summary
+    profiles/<kernel>_spheres1080p_pmc.txt (a line 'traffic_bytes_per_launch: X'),
+    or None when no such measurement exists."""
+    path = os.path.join(ROOT, "profiles", "%s_spheres1080p_pmc.txt" % kernel)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
+def cpu_baseline(sc, args, tree):
+    """The oracle (scalar C++ restatement, OpenMP over 16x16 tiles) on the centre
+    256x256 crop of the frame; returns samples/s of path tracing alone."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_lib as O
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    cw, ch = min(256, sc.width), min(256, sc.height)
+    crop = ((sc.width - cw) // 2, (sc.height - ch) // 2, cw, ch)
+    lp = sc.launch_params(offset=0, count=args.cpu_spp, depth=args.depth)
+    _, st = O.render(sc, lp, threads=threads, crop=crop, oscene=O.OracleScene(sc, tree=tree))
+    v = cw * ch * args.cpu_spp / st["renderSeconds"]
+    return {"value": v, "unit": "samples/s", "cores": threads, "kind": "port",
+            "sample": "centre %dx%d crop of the same frame at %d spp, depth %d (%.1f s of path tracing; "
+                      "CPU BVH build of %.1f s excluded)" % (cw, ch, args.cpu_spp, args.depth, st["renderSeconds"],
+                                                             st["bvhSeconds"])}
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        if int(os.environ.get("RANK", "0")) == 0:
+            print("bench.py measures one GPU: the path tracer runs on a single MI355X here, "
+                  "so --gpus %d (world size %d) is not supported" % (args.gpus, world), file=sys.stderr)
+        sys.exit(2)
+
+    from ratbite_amd import _lib as L
+    from ratbite_amd import scenes
+    from ratbite_amd.renderer import Renderer
+
+    t0 = time.time()
+    sc = scenes.spheres(width=args.width, height=args.height, spp=args.spp, depth=args.depth)
+    print("scene: %d triangles, %d emissive, env %s (%.1f s)" % (len(sc.indices), len(sc.lights),
+          None if sc.env is None else sc.env.shape[:2], time.time() - t0), file=sys.stderr, flush=True)
+    r = Renderer(sc)
+    ctx = r.ctx
+    st0 = ctx.stats()
+    print("LBVH %.1f ms depth %d; env CDF %.3f ms" % (st0["bvhMs"], st0["bvhDepth"], st0["cdfMs"]),
+          file=sys.stderr, flush=True)
+
+    params = sc.launch_params(offset=0, count=args.spp, flags=L.RBH_RENDER_TIMING)
+    for _ in range(args.warmup):
+        ctx.render(params)
+    ctx.synchronize()
+
+    tot = {k: 0 for k in ("closestRays", "shadowRays", "pathVertices", "traceMs", "shadeMs", "shadowMs",
+                          "renderMs", "traceLaunches")}
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.render(params)
+        s = ctx.stats()
+        for k in tot:
+            tot[k] += s[k]
+        print("step: %.1f ms, %d closest + %d shadow rays" % (s["renderMs"], s["closestRays"], s["shadowRays"]),
+              file=sys.stderr, flush=True)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t1
+
+    pixels = sc.width * sc.height
+    samples = pixels * args.spp * args.steps
+    rays = tot["closestRays"] + tot["shadowRays"]
+    kms = {"trace": tot["traceMs"], "shade": tot["shadeMs"], "shadow": tot["shadowMs"]}
+    kbytes = {"trace": BYTES_CLOSEST * tot["closestRays"], "shade": BYTES_VERTEX * tot["pathVertices"],
+              "shadow": BYTES_SHADOW * tot["shadowRays"]}
+    dom = max(kms, key=kms.get)
+    launches = tot["traceLaunches"]
+    achieved = kbytes[dom] / (kms[dom] * 1e-3) / 1e9
+    traffic = traffic_per_launch(KERNEL_NAMES[dom])
+    path_bytes = (BYTES_CLOSEST + BYTES_SHADOW + BYTES_VERTEX) * tot["pathVertices"] + \
+        BYTES_PIXEL_LAUNCH * pixels * args.spp * args.steps
+
+    out = {
+        "metric": "1080p samples/s", "value": samples / elapsed, "unit": "samples/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene generator)",
+        "config": {"workload": "spheres 1920x1080: 200 level-5 icospheres (%d triangles), %d textured emissive "
+                               "triangles, env 2048x1024 log-normal + sun, 64 spp, depth 16" % (len(sc.indices),
+                                                                                                 len(sc.lights)),
+                   "width": sc.width, "height": sc.height, "spp": args.spp, "depth": args.depth},
+        "mrays_per_s": rays / elapsed / 1e6,
+        "rays": {"closest": tot["closestRays"], "shadow": tot["shadowRays"], "vertices": tot["pathVertices"]},
+        "kernel_ms": {KERNEL_NAMES[k]: v for k, v in kms.items()},
+        "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "avg_launch_ms": kms[dom] / max(1, launches),
+                     "algorithmic_bytes_per_launch": kbytes[dom] / max(1, launches)},
+        "path_roofline": {"achieved": path_bytes / elapsed / 1e9, "unit": "GB/s",
+                          "frac": path_bytes / elapsed / 1e9 / HBM_PEAK_GBS},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(sc, args, r.light_tree())
+        out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+    r.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -114,6 +114,7 @@ enum {
     RBH_RENDER_GUIDED = 1,        /* one-sample MIS with the guiding MLP */
     RBH_RENDER_COLLECT = 2,       /* record guiding training samples */
     RBH_RENDER_GUIDE_FP8 = 4,     /* FP8 E4M3 guiding inference */
+    RBH_RENDER_TIMING = 8,        /* per-kernel HIP-event timing into rbh_stats */
 };
 
 enum {
@@ -196,8 +197,14 @@ int rbh_envmap_disable(rbh_ctx ctx);
 /* ------------------------------------------------------------------ render
  * optixLaunch(__raygen__main) (rendering_interface.cpp:1873-1879): renders
  * sampleCount samples per pixel starting at sampleOffset into the FP64
- * accumulation buffer (re-read when sampleOffset != 0). Asynchronous. */
+ * accumulation buffer (re-read when sampleOffset != 0). Returns when the
+ * work has finished; rbh_stats_get then describes this launch. */
 int rbh_render(rbh_ctx ctx, const rbh_launch_params* p);
+/* Path slots of one wavefront pass: a pass renders max(1, slots / (W*H))
+ * samples of every pixel at once, about 200 B of device memory per slot
+ * (default 2^25). Results do not depend on it. No reference counterpart: the
+ * OptiX megakernel keeps each path's state in registers. */
+int rbh_set_path_slots(rbh_ctx ctx, uint64_t slots);
 /* Copy the FP64 (x, y, z, w) accumulation buffer (W*H*4 doubles). */
 int rbh_read_accum(rbh_ctx ctx, double* rgbw);
 int rbh_write_accum(rbh_ctx ctx, const double* rgbw, uint32_t width, uint32_t height);

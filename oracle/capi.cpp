@@ -187,6 +187,106 @@ int orc_bxdf_unit(const orc_scene* sc, const float* in, float* out, uint32_t n) 
     return 0;
 }
 
+// Single-function evaluations of the shading and light-tree math for the
+// oracle's own pins (quadrature, closed forms, identities). Records are
+// 8 floats in, 4 floats out; `fn` selects the function:
+//  0 ggx_D(m[0..2], alpha x/y [3,4])                       -> D
+//  1 ggx_G1(w[0..2], alpha [3,4])                          -> G1
+//  2 vndf_sample_cap(wo[0..2], alpha [3,4], u [5,6])       -> m
+//  3 vndf_pdf_cap(wo[0..2], alpha [3,4], m from in[5..7])  -> pdf of m
+//  4 vndf_sample_bounded (as 2)                            -> m
+//  5 vndf_pdf_bounded (as 3)                               -> pdf of m
+//  6 fresnel_complex1(cos [0], eta [1], k [2])             -> R
+//  7 f82(f0 [0..2], mu [3]) and schlick3                   -> f82 rgb, max |f82 - schlick|
+//  8 ltc_sample(wo[0..2], A [3], B [4], u [5,6])           -> wi
+//  9 ltc_eval(wo[0..2], A [3], B [4], wi from in[5..7])    -> f, pdf
+// 10 sg_integral(s [0]), sg_expm1(x [1]), sg_expm1_over_x(x [1])
+// 11 clamped cosine product integral over pi (cos [0], s [1]), upper [2], lower [3]
+// 12 sg_product(axis1 [0..2] from oct, s1 [3], axis2 from in[4..5] oct, s2 [6]) -> axis, sharp ... (see below)
+// 13 vmf_axis_length_to_sharpness(l [0]), vmf_sharpness_to_axis_length(s [1])
+// 14 gaussian_filter_sample(u [0,1])                      -> offset
+// 15 refract(v [0..2], n = +z, eta [3])                   -> t
+// 16 sample_visible(u [0]) hero lambda/pdf                 -> lambda0, pdf0
+int orc_math_unit(int fn, const float* in, float* out, uint32_t n) {
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* a = in + 8 * (size_t)i;
+        float* o = out + 4 * (size_t)i;
+        o[0] = o[1] = o[2] = o[3] = 0.0f;
+        const V3 w{a[0], a[1], a[2]};
+        const Alpha al{a[3], a[4]};
+        switch (fn) {
+            case 0: o[0] = ggx_D(ctx_m(w), al); break;
+            case 1: o[0] = ggx_G1(ctx_w(w), al); break;
+            case 2: case 4: {
+                const V3 m = fn == 2 ? vndf_sample_cap(w, al, {a[5], a[6]}) : vndf_sample_bounded(w, al, {a[5], a[6]});
+                o[0] = m.x; o[1] = m.y; o[2] = m.z;
+                break;
+            }
+            case 3: case 5: {
+                // m is passed as (theta, phi) in a[5], a[6]
+                const float st = std::sin(a[5]), ct = std::cos(a[5]);
+                const V3 m{st * std::cos(a[6]), st * std::sin(a[6]), ct};
+                const float aw = std::fabs(dot(w, m));
+                o[0] = fn == 3 ? vndf_pdf_cap(w, aw, ctx_w(w), ctx_m(m), al) : vndf_pdf_bounded(w, aw, ctx_w(w), ctx_m(m), al);
+                if (dot(w, m) <= 0.0f) o[0] = 0.0f;   // the samplers never return back-facing normals
+                break;
+            }
+            case 6: o[0] = fresnel_complex1(a[0], a[1], a[2]); break;
+            case 7: {
+                const V3 f0{a[0], a[1], a[2]};
+                const V3 f = f82(f0, a[3]), s = schlick3(f0, a[3]);
+                o[0] = f.x; o[1] = f.y; o[2] = f.z;
+                o[3] = std::max(std::fabs(f.x - s.x), std::max(std::fabs(f.y - s.y), std::fabs(f.z - s.z)));
+                break;
+            }
+            case 8: {
+                const V3 r = ltc_sample(w, a[3], a[4], {a[5], a[6]});
+                o[0] = r.x; o[1] = r.y; o[2] = r.z;
+                break;
+            }
+            case 9: {
+                const float st = std::sin(a[5]), ct = std::cos(a[5]);
+                const V3 wi{st * std::cos(a[6]), st * std::sin(a[6]), ct};
+                ltc_eval(w, wi, a[3], a[4], o[0], o[1]);
+                break;
+            }
+            case 10: o[0] = sg_integral(a[0]); o[1] = sg_expm1(a[1]); o[2] = sg_expm1_over_x(a[1]); break;
+            case 11:
+                o[0] = sg_clamped_cosine_product_integral_over_pi_2024(a[0], a[1]);
+                o[1] = sg_upper_cc_over_2pi(a[1]);
+                o[2] = sg_lower_cc_over_2pi(a[1]);
+                break;
+            case 12: {
+                const V3 a1 = normalize(V3{a[0], a[1], a[2]});
+                const float th = a[4], ph = a[5];
+                const V3 a2{std::sin(th) * std::cos(ph), std::sin(th) * std::sin(ph), std::cos(th)};
+                const SGLobe l = sg_product(a1, a[3], a2, a[6]);
+                // out: sharpness, logAmplitude, axis . a1, axis . a2
+                o[0] = l.sharpness; o[1] = l.logAmplitude; o[2] = dot(l.axis, a1); o[3] = dot(l.axis, a2);
+                break;
+            }
+            case 13: o[0] = vmf_axis_length_to_sharpness(a[0]); o[1] = vmf_sharpness_to_axis_length(a[1]); break;
+            case 14: {
+                const V2 g = gaussian_filter_sample({a[0], a[1]});
+                o[0] = g.x; o[1] = g.y;
+                break;
+            }
+            case 15: {
+                const V3 t = refract(w, V3{0.0f, 0.0f, 1.0f}, a[3]);
+                o[0] = t.x; o[1] = t.y; o[2] = t.z;
+                break;
+            }
+            case 16: {
+                const Wavelengths wl = sample_visible(a[0]);
+                o[0] = wl.lambda.v[0]; o[1] = wl.pdf.v[0]; o[2] = wl.lambda.v[1]; o[3] = wl.pdf.v[1];
+                break;
+            }
+            default: return -1;
+        }
+    }
+    return 0;
+}
+
 int orc_render(const orc_scene* sc, const orc_params* p, double* accum, int threads, orc_render_stats* stats) {
     return render(sc, p, accum, threads, stats);
 }

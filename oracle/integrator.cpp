@@ -6,6 +6,7 @@
 // (pathtrace.cu:906-953) are restated for a flattened, camera-relative scene
 // with identity instance transforms. This is both the image-level oracle and
 // the CPU throughput baseline.
+#include <chrono>
 #include "render.hpp"
 #include "rb_bvh.hpp"
 #include <vector>
@@ -484,7 +485,9 @@ int render(const orc_scene* sc, const orc_params* p, double* accum, int threads,
     Ctx C;
     C.sc = sc;
     C.p = p;
+    const auto tb0 = std::chrono::steady_clock::now();
     C.bvh.build(sc->verts, sc->indices, sc->ntris);
+    const auto tb1 = std::chrono::steady_clock::now();
     C.tree.nodes = (sc->nodes && sc->nnodes) ? (const PackedNode*)sc->nodes : nullptr;
     C.tree.lightPointers = sc->lightPointers;
     C.tree.bitmasks = sc->bitmasks;
@@ -516,7 +519,14 @@ int render(const orc_scene* sc, const orc_params* p, double* accum, int threads,
         }
         aC += st.closest; aS += st.shadow; aV += st.vertices;
     }
-    if (stats) { stats->closestRays = aC; stats->shadowRays = aS; stats->pathVertices = aV; }
+    const auto tr1 = std::chrono::steady_clock::now();
+    if (stats) {
+        stats->closestRays = aC;
+        stats->shadowRays = aS;
+        stats->pathVertices = aV;
+        stats->bvhSeconds = std::chrono::duration<double>(tb1 - tb0).count();
+        stats->renderSeconds = std::chrono::duration<double>(tr1 - tb1).count();
+    }
     return 0;
 }
 

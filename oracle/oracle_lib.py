@@ -1,7 +1,7 @@
-"""Loader of the CPU oracle (oracle/_build/liboracle.so) for the tests.
+"""TEST INFRASTRUCTURE — ctypes loader of the CPU oracle (oracle/_build/liboracle.so).
 
-The oracle is the checker: tests compare the HIP library against it. It is
-never used as the product path.
+The oracle is the checker: the tests, the smoke check and the benchmark's
+CPU-baseline leg call it. It is never used as the product path.
 """
 import ctypes
 import os
@@ -9,8 +9,7 @@ import subprocess
 
 import numpy as np
 
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_DIR = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
 
 _lib = None
@@ -135,7 +134,8 @@ class OrcParams(ctypes.Structure):
 
 
 class OrcStats(ctypes.Structure):
-    _fields_ = [("closestRays", ctypes.c_uint64), ("shadowRays", ctypes.c_uint64), ("pathVertices", ctypes.c_uint64)]
+    _fields_ = [("closestRays", ctypes.c_uint64), ("shadowRays", ctypes.c_uint64), ("pathVertices", ctypes.c_uint64),
+                ("bvhSeconds", ctypes.c_double), ("renderSeconds", ctypes.c_double)]
 
 
 def light_tree_build(lights, flux):
@@ -223,7 +223,8 @@ def render(sc, params_lp, threads=0, crop=None, accum=None, oscene=None):
     st = OrcStats()
     rc = lib().orc_render(ctypes.byref(os_.s), ctypes.byref(pr), p(accum), threads, ctypes.byref(st))
     assert rc == 0
-    return accum, {"closestRays": st.closestRays, "shadowRays": st.shadowRays, "pathVertices": st.pathVertices}
+    return accum, {"closestRays": st.closestRays, "shadowRays": st.shadowRays, "pathVertices": st.pathVertices,
+                   "bvhSeconds": st.bvhSeconds, "renderSeconds": st.renderSeconds}
 
 
 def resolve(accum, m3, exposure_scale):

@@ -92,6 +92,8 @@ typedef struct orc_params {
 
 typedef struct orc_render_stats {
     uint64_t closestRays, shadowRays, pathVertices;
+    double bvhSeconds;        /* CPU BVH build inside orc_render */
+    double renderSeconds;     /* path tracing only (after the BVH build) */
 } orc_render_stats;
 
 /* QRNG / spectral unit functions (n independent inputs) */
@@ -124,6 +126,10 @@ int orc_trace_brute(const float* verts, const uint32_t* idx, uint32_t ntris, con
 /* per-sample BxDF check: evaluates the reference BxDF callable body for
  * given inputs; see oracle/bxdf_unit.cpp for the record layouts */
 int orc_bxdf_unit(const orc_scene* sc, const float* in, float* out, uint32_t n);
+
+/* single-function evaluations for the oracle's own pins (8 floats in,
+ * 4 out per record; function table in capi.cpp) */
+int orc_math_unit(int fn, const float* in, float* out, uint32_t n);
 
 /* full CPU path tracer (the reference's __raygen__main restated) */
 int orc_render(const orc_scene* sc, const orc_params* p, double* accum /* W*H*4 */, int threads, orc_render_stats* stats);

@@ -28,6 +28,7 @@ RBH_ERR_NOMEM = -5
 RBH_RENDER_GUIDED = 1
 RBH_RENDER_COLLECT = 2
 RBH_RENDER_GUIDE_FP8 = 4
+RBH_RENDER_TIMING = 8
 RBH_CDF_FP64 = 1
 
 # Record layouts (byte-identical to include/ratbite_hip.h)
@@ -102,6 +103,7 @@ _SIGS = {
     "rbh_envmap_read": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_double)]),
     "rbh_envmap_disable": (c_int, [c_void_p]),
     "rbh_render": (c_int, [c_void_p, ctypes.POINTER(LaunchParams)]),
+    "rbh_set_path_slots": (c_int, [c_void_p, c_uint64]),
     "rbh_read_accum": (c_int, [c_void_p, c_void_p]),
     "rbh_write_accum": (c_int, [c_void_p, c_void_p, c_uint32, c_uint32]),
     "rbh_resolve": (c_int, [c_void_p, c_void_p, c_float, c_void_p]),

@@ -138,6 +138,10 @@ class Context:
         self.width, self.height = params.width, params.height
         self.call("rbh_render", ctypes.byref(params))
 
+    def set_path_slots(self, slots):
+        """Path slots of one wavefront pass (samples per pass = slots // pixels)."""
+        self.call("rbh_set_path_slots", int(slots))
+
     def read_accum(self):
         a = np.zeros((self.height, self.width, 4), np.float64)
         self.call("rbh_read_accum", L.ptr(a))

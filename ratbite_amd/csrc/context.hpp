@@ -71,6 +71,45 @@ struct Accel {
     DevBuf scratch;   // build scratch
 };
 
+// HIP-event timing of individual render-loop kernels (RBH_RENDER_TIMING):
+// one event pair per launch, recorded on the render stream around the launch.
+struct KernelTimer {
+    enum { TRACE = 0, SHADE = 1, SHADOW = 2, NCLASS = 3 };
+    std::vector<hipEvent_t> pool;   // 2 events per recorded launch
+    std::vector<int> cls;           // kernel class of launch i
+    KernelTimer() = default;
+    KernelTimer(const KernelTimer&) = delete;
+    KernelTimer& operator=(const KernelTimer&) = delete;
+    ~KernelTimer() {
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    }
+    void reset() { cls.clear(); }
+    hipError_t begin(hipStream_t s, int k) {
+        const size_t i = cls.size();
+        while (pool.size() < 2 * i + 2) {
+            hipEvent_t e = nullptr;
+            hipError_t r = hipEventCreate(&e);
+            if (r != hipSuccess) return r;
+            pool.push_back(e);
+        }
+        cls.push_back(k);
+        return hipEventRecord(pool[2 * i], s);
+    }
+    hipError_t end(hipStream_t s) { return hipEventRecord(pool[2 * cls.size() - 1], s); }
+    // Sums the durations per class; the stream must have been synchronised.
+    hipError_t collect(double ms[NCLASS], uint32_t n[NCLASS]) const {
+        for (int k = 0; k < NCLASS; ++k) { ms[k] = 0.0; n[k] = 0; }
+        for (size_t i = 0; i < cls.size(); ++i) {
+            float t = 0.0f;
+            hipError_t r = hipEventElapsedTime(&t, pool[2 * i], pool[2 * i + 1]);
+            if (r != hipSuccess) return r;
+            ms[cls[i]] += t;
+            n[cls[i]] += 1;
+        }
+        return hipSuccess;
+    }
+};
+
 struct RenderState {
     uint32_t width = 0, height = 0;
     DevBuf accum;     // W*H*4 doubles
@@ -78,6 +117,10 @@ struct RenderState {
     DevBuf queues;    // ray/shadow queues + counters
     DevBuf image;     // RGBA8 resolve output
     size_t pathCapacity = 0;
+    // path slots one pass may hold (samples per pass = slotBudget / pixels);
+    // about 200 B of HBM per slot
+    size_t slotBudget = size_t(1) << 25;
+    KernelTimer timer;
 };
 
 }  // namespace rbh

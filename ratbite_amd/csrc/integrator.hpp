@@ -9,7 +9,7 @@ namespace rbh {
 enum : uint32_t { CNT_QUEUE0 = 0, CNT_QUEUE1 = 1, CNT_SHADOW = 2, CNT_COUNT = 4 };
 enum : uint32_t { STAT_CLOSEST = 0, STAT_SHADOW = 1, STAT_VERTICES = 2, STAT_COUNT = 4 };
 
-// Structure-of-arrays path state (slot = pixel of the pass).
+// Structure-of-arrays path state, one entry per path slot of the pass.
 struct PathBuffers {
     float4* od;     // ray origin, bxdfPDF
     float4* dr;     // ray direction, refractionScale
@@ -29,8 +29,9 @@ struct PathBuffers {
 };
 
 struct Frame1 {
-    // pass
-    uint32_t P, W, H, sample;
+    // pass: S consecutive samples of all P = W*H pixels, NP = S*P path slots;
+    // slot i holds sample (sample + i / P) of pixel i % P
+    uint32_t P, W, H, sample, S, NP;
     uint32_t maxDepth, maxRefl, maxTrans;
     uint32_t bounce;
     float invW, invH, psW, psH;

@@ -1,8 +1,10 @@
 // Wavefront spectral path tracer replacing the OptiX megakernel
 // __raygen__main (src/kernels/pathtrace.cu:1114-1813).
 //
-// One pass renders one sample for every pixel; path state lives in
-// structure-of-arrays buffers in HBM (slot = pixel). Per bounce:
+// One pass renders S consecutive samples of every pixel at once (S*W*H path
+// slots, slot = sample-major pixel index) so that the long tail of a few deep
+// paths is paid once per pass rather than once per sample; path state lives in
+// structure-of-arrays buffers in HBM. Per bounce:
 //   k_trace    closest hit for the queued paths           (pathtrace.cu:1162-1177)
 //   k_shade    emission + MIS, surface attributes, light-tree NEE, BxDF,
 //              Russian roulette, next origin, depth caps   (pathtrace.cu:1179-1785)
@@ -55,9 +57,10 @@ __device__ __forceinline__ void wave_count(unsigned long long* __restrict__ coun
 // ------------------------------------------------------------------ camera
 __global__ __launch_bounds__(256) void k_camera(Frame1 f) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= f.P) return;
-    const uint32_t x = i % f.W, y = i / f.W;
-    const Qrng q = qrng_at(f.sample, 0, pixel_hash(x, y));
+    if (i >= f.NP) return;
+    const uint32_t k = i / f.P, pix = i - k * f.P;
+    const uint32_t x = pix % f.W, y = pix / f.W;
+    const Qrng q = qrng_at(f.sample + k, 0, pixel_hash(x, y));
     const Wl wl = sample_wavelengths(q.s1(QD_WAVELENGTH));
     const V2 fo = gaussian_offset(q.s2(QD_FILTER));
     const float cx = (float)x + fo.x, cy = (float)y + fo.y;
@@ -89,7 +92,7 @@ __global__ __launch_bounds__(256) void k_camera(Frame1 f) {
     p.nf[i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
     p.depth[i] = 0u;
     if (i == 0) {
-        f.counters[CNT_QUEUE0] = f.P;
+        f.counters[CNT_QUEUE0] = f.NP;
         f.counters[CNT_QUEUE1] = 0u;
         f.counters[CNT_SHADOW] = 0u;
     }
@@ -204,8 +207,9 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(Frame1 f, uint32_t qsel) 
             V3 ro{od.x, od.y, od.z};
             V3 lastN{nf.x, nf.y, nf.z};
             uint32_t depth = f.p.depth[pid];
-            const uint32_t px = pid % f.W, py = pid / f.W;
-            const Qrng q = qrng_at(f.sample, depth, pixel_hash(px, py));
+            const uint32_t sk = pid / f.P, pix = pid - sk * f.P;
+            const uint32_t px = pix % f.W, py = pix / f.W;
+            const Qrng q = qrng_at(f.sample + sk, depth, pixel_hash(px, py));
 
             // ---- interaction (pathtrace.cu:86-149, 906-953)
             const uint32_t prim = __float_as_uint(hr.w);
@@ -558,43 +562,62 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(Frame1 f, uint32_t qsel) 
 
 // ----------------------------------------------------------- accumulation
 __global__ __launch_bounds__(256) void k_accumulate(Frame1 f, uint32_t first) {
+    // per pixel, the S samples of the pass are added one at a time in sample
+    // order, so any split of a sample range into passes gives the same sums
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= f.P) return;
-    const uint32_t flags = __float_as_uint(f.p.nf[i].w);
     double4 a = first ? make_double4(0.0, 0.0, 0.0, 0.0) : f.accum[i];
-    if (!(flags & PF_TERMINATED)) {
-        const float4 Lv = f.p.L[i], la = f.p.lam[i], lp = f.p.lpdf[i];
-        Wl wl;
-        wl.lambda = S4{{la.x, la.y, la.z, la.w}};
-        wl.pdf = S4{{lp.x, lp.y, lp.z, lp.w}};
-        S4 L{{Lv.x, Lv.y, Lv.z, Lv.w}};
-        for (int k = 0; k < kNL; ++k) L.v[k] = wl.pdf.v[k] != 0.0f ? L.v[k] / wl.pdf.v[k] : 0.0f;
-        double xyz[3];
-        ColorTables{f.colors}.xyz(wl, L, xyz);
-        a.x += xyz[0];
-        a.y += xyz[1];
-        a.z += xyz[2];
+    const ColorTables ct{f.colors};
+    for (uint32_t k = 0; k < f.S; ++k) {
+        const uint32_t j = k * f.P + i;
+        const uint32_t flags = __float_as_uint(f.p.nf[j].w);
+        if (!(flags & PF_TERMINATED)) {
+            const float4 Lv = f.p.L[j], la = f.p.lam[j], lp = f.p.lpdf[j];
+            Wl wl;
+            wl.lambda = S4{{la.x, la.y, la.z, la.w}};
+            wl.pdf = S4{{lp.x, lp.y, lp.z, lp.w}};
+            S4 L{{Lv.x, Lv.y, Lv.z, Lv.w}};
+            for (int c = 0; c < kNL; ++c) L.v[c] = wl.pdf.v[c] != 0.0f ? L.v[c] / wl.pdf.v[c] : 0.0f;
+            double xyz[3];
+            ct.xyz(wl, L, xyz);
+            a.x += xyz[0];
+            a.y += xyz[1];
+            a.z += xyz[2];
+        }
+        a.w += 1.0;
     }
-    a.w += 1.0;
     f.accum[i] = a;
 }
 
 }  // namespace
 
 // ------------------------------------------------------------------ host
-hipError_t launch_pass(rbh_ctx c, Frame1 f, uint32_t depthMax, bool firstPass, int traceGrid, int shadeGrid) {
-    const int nb = (int)((f.P + 255) / 256);
+hipError_t launch_pass(rbh_ctx c, Frame1 f, uint32_t depthMax, bool firstPass, int traceGrid, int shadeGrid,
+                       KernelTimer* kt) {
+    const int nb = (int)((f.NP + 255) / 256);
+    const int nbPix = (int)((f.P + 255) / 256);
+    hipError_t e = hipSuccess;
+#define RBH_TIMED(cls, launch)                                       \
+    do {                                                             \
+        if (kt && (e = kt->begin(c->stream, cls)) != hipSuccess) return e; \
+        launch;                                                      \
+        if (kt && (e = kt->end(c->stream)) != hipSuccess) return e;  \
+    } while (0)
     f.bounce = 0;
     hipLaunchKernelGGL(k_camera, dim3(nb), dim3(256), 0, c->stream, f);
     for (uint32_t b = 0; b < depthMax; ++b) {
         const uint32_t qsel = b & 1u;
         f.bounce = b;
-        hipLaunchKernelGGL(k_trace, dim3(traceGrid), dim3(kTraceBlock), 0, c->stream, f, qsel);
+        RBH_TIMED(KernelTimer::TRACE,
+                  hipLaunchKernelGGL(k_trace, dim3(traceGrid), dim3(kTraceBlock), 0, c->stream, f, qsel));
         hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, c->stream, f.counters, qsel ? CNT_QUEUE0 : CNT_QUEUE1);
-        hipLaunchKernelGGL(k_shade, dim3(shadeGrid), dim3(kShadeBlock), 0, c->stream, f, qsel);
-        hipLaunchKernelGGL(k_shadow, dim3(traceGrid), dim3(kTraceBlock), 0, c->stream, f);
+        RBH_TIMED(KernelTimer::SHADE,
+                  hipLaunchKernelGGL(k_shade, dim3(shadeGrid), dim3(kShadeBlock), 0, c->stream, f, qsel));
+        RBH_TIMED(KernelTimer::SHADOW,
+                  hipLaunchKernelGGL(k_shadow, dim3(traceGrid), dim3(kTraceBlock), 0, c->stream, f));
     }
-    hipLaunchKernelGGL(k_accumulate, dim3(nb), dim3(256), 0, c->stream, f, firstPass ? 1u : 0u);
+#undef RBH_TIMED
+    hipLaunchKernelGGL(k_accumulate, dim3(nbPix), dim3(256), 0, c->stream, f, firstPass ? 1u : 0u);
     return hipGetLastError();
 }
 

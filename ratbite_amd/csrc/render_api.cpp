@@ -1,5 +1,6 @@
 // C ABI entry points of the acceleration structure, light tree and render
 // loop (include/ratbite_hip.h).
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -8,7 +9,8 @@
 
 namespace rbh {
 int lbvh_build(rbh_ctx c);
-hipError_t launch_pass(rbh_ctx c, Frame1 f, uint32_t depthMax, bool firstPass, int traceGrid, int shadeGrid);
+hipError_t launch_pass(rbh_ctx c, Frame1 f, uint32_t depthMax, bool firstPass, int traceGrid, int shadeGrid,
+                       KernelTimer* kt);
 hipError_t resolve_launch(rbh_ctx c, const float* colorspace, float expo, void* out);
 namespace host {
 int build_light_tree(const EmissiveTri* tris, const float* flux, uint32_t n, std::vector<PackedNode>& nodes,
@@ -125,18 +127,24 @@ int rbh_render(rbh_ctx c, const rbh_launch_params* lp) {
     if (lp->width == 0 || lp->height == 0) return fail(c, RBH_ERR_INVALID, "empty film");
     if ((uint64_t)lp->width * lp->height > (1ull << 28)) return fail(c, RBH_ERR_RANGE, "film above 2^28 pixels");
     if (lp->maxPathDepth == 0 || lp->maxPathDepth > 1024) return fail(c, RBH_ERR_INVALID, "maxPathDepth must be in [1, 1024]");
+    if (lp->flags & ~(uint32_t)RBH_RENDER_TIMING)
+        return fail(c, RBH_ERR_INVALID, "render flags 0x%x are not supported by this entry point", lp->flags);
     const bool treeOn = c->tree.nlights > 0;
     if (c->geom.maxLight >= (int64_t)c->tree.nlights)
         return fail(c, RBH_ERR_STATE, "triangle light index %lld has no light record (%u lights built)",
                     (long long)c->geom.maxLight, c->tree.nlights);
     RenderState& rs = c->rs;
     const size_t P = (size_t)lp->width * lp->height;
+    // samples per pass: as many as fit the path-slot budget (at least one)
+    const uint32_t spp = lp->sampleCount;
+    const uint32_t S = (uint32_t)std::max<size_t>(1, std::min<size_t>(spp ? spp : 1, rs.slotBudget / P));
+    const size_t NP = P * S;
     PathBuffers pb{};
     uint32_t* counters = nullptr;
     unsigned long long* stats = nullptr;
-    const size_t bytes = layout_paths(nullptr, P, &pb, &counters, &stats);
+    const size_t bytes = layout_paths(nullptr, NP, &pb, &counters, &stats);
     RBH_ALLOC(c, rs.paths, bytes);
-    layout_paths(rs.paths.as<char>(), P, &pb, &counters, &stats);
+    layout_paths(rs.paths.as<char>(), NP, &pb, &counters, &stats);
     const bool sizeChanged = rs.width != lp->width || rs.height != lp->height;
     RBH_ALLOC(c, rs.accum, P * sizeof(double) * 4);
     if (sizeChanged && lp->sampleOffset != 0) return fail(c, RBH_ERR_STATE, "sampleOffset != 0 but no accumulation of this size exists");
@@ -196,12 +204,19 @@ int rbh_render(rbh_ctx c, const rbh_launch_params* lp) {
     const int cus = cu_count(c->device);
     const int traceGrid = cus * 16;
     const int shadeGrid = cus * 16;
+    KernelTimer* kt = (lp->flags & RBH_RENDER_TIMING) ? &rs.timer : nullptr;
+    if (kt) kt->reset();
     RBH_HIP(c, hipEventRecord(c->ev0, c->stream));
     hipLaunchKernelGGL(k_zero_stats, dim3(1), dim3(64), 0, c->stream, stats);
-    for (uint32_t s = 0; s < lp->sampleCount; ++s) {
-        f.sample = lp->sampleOffset + s;
-        const bool first = (lp->sampleOffset == 0 && s == 0);
-        RBH_HIP(c, launch_pass(c, f, f.maxDepth, first, traceGrid, shadeGrid));
+    uint32_t passes = 0;
+    for (uint32_t done = 0; done < spp;) {
+        f.sample = lp->sampleOffset + done;
+        f.S = std::min(S, spp - done);
+        f.NP = (uint32_t)(P * f.S);
+        const bool first = (lp->sampleOffset == 0 && done == 0);
+        RBH_HIP(c, launch_pass(c, f, f.maxDepth, first, traceGrid, shadeGrid, kt));
+        done += f.S;
+        ++passes;
     }
     RBH_HIP(c, hipEventRecord(c->ev1, c->stream));
     RBH_HIP(c, hipStreamSynchronize(c->stream));
@@ -214,7 +229,23 @@ int rbh_render(rbh_ctx c, const rbh_launch_params* lp) {
     c->stats.shadowRays = st[STAT_SHADOW];
     c->stats.pathVertices = st[STAT_VERTICES];
     c->stats.samples = (uint64_t)P * lp->sampleCount;
-    c->stats.traceLaunches = lp->sampleCount * f.maxDepth;
+    c->stats.traceLaunches = passes * f.maxDepth;
+    c->stats.traceMs = c->stats.shadeMs = c->stats.shadowMs = 0.0;
+    if (kt) {
+        double kms[KernelTimer::NCLASS];
+        uint32_t kn[KernelTimer::NCLASS];
+        RBH_HIP(c, kt->collect(kms, kn));
+        c->stats.traceMs = kms[KernelTimer::TRACE];
+        c->stats.shadeMs = kms[KernelTimer::SHADE];
+        c->stats.shadowMs = kms[KernelTimer::SHADOW];
+    }
+    return RBH_OK;
+}
+
+int rbh_set_path_slots(rbh_ctx c, uint64_t slots) {
+    if (!c) return RBH_ERR_INVALID;
+    if (slots == 0 || slots > (1ull << 31)) return fail(c, RBH_ERR_RANGE, "path slots must be in [1, 2^31]");
+    c->rs.slotBudget = (size_t)slots;
     return RBH_OK;
 }
 

@@ -72,6 +72,22 @@ def test_progressive_equals_single_launch(ctx):
     assert np.array_equal(a, b)
 
 
+def test_pass_size_does_not_change_results(ctx):
+    # one sample per pass, three per pass and all eight in one pass: the FP64
+    # sums are added per sample in sample order, so all three agree exactly
+    sc = scenes.spheres(n_spheres=8, level=2, n_lights=100, width=40, height=24, spp=8, env_size=(128, 64))
+    r = Renderer(sc, ctx=ctx)
+    out = []
+    try:
+        for slots in (40 * 24, 3 * 40 * 24, 1 << 25):
+            ctx.set_path_slots(slots)
+            r.render(spp=8, offset=0)
+            out.append(r.accum())
+    finally:
+        ctx.set_path_slots(1 << 25)
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
+
+
 def test_resolve_matches_oracle(ctx):
     sc = scenes.cornell(width=48, height=48, spp=4)
     r = Renderer(sc, ctx=ctx)
