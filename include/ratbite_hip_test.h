@@ -26,9 +26,12 @@ int rbh_test_wavelengths(rbh_ctx ctx, const float* u, float* lambda, float* pdf,
 int rbh_test_rgb_to_spectrum(rbh_ctx ctx, const float* in, float* out, uint32_t n);
 /* Octohedral::encodeU32 / decodeU32 (src/core/math_util.h:199-219) */
 int rbh_test_oct(rbh_ctx ctx, const float* v, uint32_t* enc, float* dec, uint32_t n);
-/* Closest-hit / any-hit query on the built LBVH. rays: n x {o.xyz, tmin(ignored, 0), d.xyz, tmax};
- * hits: n x {t, prim (uint bits, 0xFFFFFFFF = miss), b1, b2} */
-int rbh_test_trace(rbh_ctx ctx, const float* rays, float* hits, uint32_t n, int anyhit);
+/* Closest-hit / any-hit query on the built LBVH (optixTrace / optixTraverse,
+ * src/kernels/pathtrace.cu:1162-1177, 1704-1720). rays: n x {o.xyz, tmin
+ * (ignored, 0), d.xyz, tmax}; hits: n x {t, prim (uint bits, 0xFFFFFFFF =
+ * miss), b1, b2}. mode bit 0: any hit; bit 1: a 2-entry LDS stack so that the
+ * global spill part of the traversal stack is exercised. */
+int rbh_test_trace(rbh_ctx ctx, const float* rays, float* hits, uint32_t n, int mode);
 /* traverseLightTree (src/kernels/pathtrace.cu:1002-1049) on the built tree.
  * lights: n light indices; posNrm: n x 6; out: n pdfs */
 int rbh_test_light_pdf(rbh_ctx ctx, const uint32_t* lights, const float* posNrm, float* pdf, uint32_t n);

@@ -116,6 +116,7 @@ struct RenderState {
     DevBuf paths;     // SoA path state
     DevBuf queues;    // ray/shadow queues + counters
     DevBuf image;     // RGBA8 resolve output
+    DevBuf spill;     // traversal-stack spill columns
     size_t pathCapacity = 0;
     // path slots one pass may hold (samples per pass = slotBudget / pixels);
     // about 200 B of HBM per slot

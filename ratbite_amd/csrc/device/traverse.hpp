@@ -66,13 +66,36 @@ __device__ __forceinline__ bool tri_hit(const RayShear& s, V3 o, const TriRecord
     return true;
 }
 
-// Traverses the LBVH; `stack` points at this lane's LDS column base with a
-// stride of `stride` words. Returns the closest hit below tmax or, with ANY,
-// the first one found.
-template <bool ANY>
+// Traversal stack of at most kMaxStack entries (the LBVH builder rejects
+// trees deeper than that). The LDS_DEPTH entries nearest the bottom live in
+// LDS, one column per lane (stride = lanes per block), so consecutive lanes
+// hit consecutive banks; deeper entries go to a per-lane spill column in
+// global memory (entry k of lane t at spill[k * spillStride + t]), which only
+// rays with many deferred subtrees touch. A small LDS part keeps more waves
+// resident per CU than a full 64-entry LDS stack would.
+constexpr int kMaxStack = 64;
+
+template <int LDS_DEPTH>
+struct TravStack {
+    int* __restrict__ lds;
+    int stride;
+    int* __restrict__ spill;
+    uint32_t spillStride;
+    __device__ __forceinline__ void put(int sp, int v) const {
+        if (LDS_DEPTH >= kMaxStack || sp < LDS_DEPTH) lds[sp * stride] = v;
+        else spill[(size_t)(sp - LDS_DEPTH) * spillStride] = v;
+    }
+    __device__ __forceinline__ int get(int sp) const {
+        if (LDS_DEPTH >= kMaxStack || sp < LDS_DEPTH) return lds[sp * stride];
+        return spill[(size_t)(sp - LDS_DEPTH) * spillStride];
+    }
+};
+
+// Traverses the LBVH. Returns the closest hit below tmax or, with ANY, the
+// first one found.
+template <bool ANY, int LDS_DEPTH>
 __device__ __forceinline__ Hit traverse(const BvhNode* __restrict__ nodes, const TriRecord* __restrict__ tris,
-                                        uint32_t nnodes, V3 o, V3 d, float tmax, int* __restrict__ stack,
-                                        int stride) {
+                                        uint32_t nnodes, V3 o, V3 d, float tmax, const TravStack<LDS_DEPTH>& stack) {
     Hit h;
     h.t = tmax;
     h.prim = 0xFFFFFFFFu;
@@ -102,7 +125,7 @@ __device__ __forceinline__ Hit traverse(const BvhNode* __restrict__ nodes, const
             const bool hl = lnear <= lfar * 1.00000024f, hr = rnear <= rfar * 1.00000024f;
             if (hl && hr) {
                 const bool lfirst = lnear <= rnear;
-                stack[sp * stride] = lfirst ? e.y : e.x;
+                stack.put(sp, lfirst ? e.y : e.x);
                 ++sp;
                 node = lfirst ? e.x : e.y;
                 continue;
@@ -122,7 +145,7 @@ __device__ __forceinline__ Hit traverse(const BvhNode* __restrict__ nodes, const
         }
         if (sp == 0) break;
         --sp;
-        node = stack[sp * stride];
+        node = stack.get(sp);
     }
     return h;
 }

@@ -6,6 +6,9 @@
 
 namespace rbh {
 
+constexpr int kTraceBlock = 128;   // lanes per block of the trace and shadow kernels
+constexpr int kShadeBlock = 128;
+
 enum : uint32_t { CNT_QUEUE0 = 0, CNT_QUEUE1 = 1, CNT_SHADOW = 2, CNT_COUNT = 4 };
 enum : uint32_t { STAT_CLOSEST = 0, STAT_SHADOW = 1, STAT_VERTICES = 2, STAT_COUNT = 4 };
 
@@ -67,6 +70,8 @@ struct Frame1 {
     float envIntegral;
     // state
     PathBuffers p;
+    int* spill;              // traversal-stack spill columns of the trace/shadow grid
+    uint32_t spillStride;    // lanes of the trace/shadow grid
     uint32_t* counters;
     unsigned long long* stats;
     double4* accum;

@@ -28,9 +28,7 @@ using namespace dev;
 
 namespace {
 
-constexpr int kTraceBlock = 128;
-constexpr int kShadeBlock = 128;
-constexpr int kStack = 64;
+constexpr int kStackLds = 16;   // traversal stack entries kept in LDS per lane
 
 __device__ __forceinline__ uint32_t wave_push(uint32_t* __restrict__ counter, bool pred) {
     // one atomic per wave: ballot, leader adds the popcount, lanes take
@@ -104,8 +102,14 @@ __global__ void k_reset(uint32_t* __restrict__ counters, uint32_t which) {
 }
 
 // ------------------------------------------------------------- closest hit
+__device__ __forceinline__ TravStack<kStackLds> lane_stack(const Frame1& f, int* lds) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    return TravStack<kStackLds>{lds + threadIdx.x, (int)blockDim.x, f.spill + t, f.spillStride};
+}
+
 __global__ __launch_bounds__(kTraceBlock) void k_trace(Frame1 f, uint32_t qsel) {
-    __shared__ int stack[kStack * kTraceBlock];
+    __shared__ int stack[kStackLds * kTraceBlock];
+    const TravStack<kStackLds> st = lane_stack(f, stack);
     const uint32_t count = f.counters[qsel ? CNT_QUEUE1 : CNT_QUEUE0];
     const uint32_t* __restrict__ queue = qsel ? f.p.q1 : f.p.q0;
     const bool identity = f.bounce == 0;
@@ -115,7 +119,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(Frame1 f, uint32_t qsel) 
             const uint32_t pid = identity ? i : queue[i];
             const float4 od = f.p.od[pid], dr = f.p.dr[pid];
             const Hit h = traverse<false>(f.nodes, f.tris, f.nnodes, V3{od.x, od.y, od.z}, V3{dr.x, dr.y, dr.z},
-                                          3.402823466e+38f, stack + threadIdx.x, kTraceBlock);
+                                          3.402823466e+38f, st);
             f.p.hit[pid] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.prim));
             ++nrays;
         }
@@ -125,7 +129,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(Frame1 f, uint32_t qsel) 
 
 // ------------------------------------------------------------------ shadow
 __global__ __launch_bounds__(kTraceBlock) void k_shadow(Frame1 f) {
-    __shared__ int stack[kStack * kTraceBlock];
+    __shared__ int stack[kStackLds * kTraceBlock];
+    const TravStack<kStackLds> st = lane_stack(f, stack);
     const uint32_t count = f.counters[CNT_SHADOW];
     uint32_t nrays = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i - threadIdx.x < count; i += gridDim.x * blockDim.x) {
@@ -133,7 +138,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(Frame1 f) {
             const uint32_t pid = f.p.sq[i];
             const float4 so = f.p.sro[pid], sd = f.p.srd[pid];
             const Hit h = traverse<true>(f.nodes, f.tris, f.nnodes, V3{so.x, so.y, so.z}, V3{sd.x, sd.y, sd.z}, so.w,
-                                         stack + threadIdx.x, kTraceBlock);
+                                         st);
             if (h.prim == 0xFFFFFFFFu) {
                 const float4 a = f.p.Lnee[pid];
                 float4 L = f.p.L[pid];
@@ -592,6 +597,9 @@ __global__ __launch_bounds__(256) void k_accumulate(Frame1 f, uint32_t first) {
 }  // namespace
 
 // ------------------------------------------------------------------ host
+// Spill entries per lane of the traversal stack beyond the LDS part.
+size_t trace_spill_words(size_t lanes) { return lanes * (size_t)(kMaxStack - kStackLds); }
+
 hipError_t launch_pass(rbh_ctx c, Frame1 f, uint32_t depthMax, bool firstPass, int traceGrid, int shadeGrid,
                        KernelTimer* kt) {
     const int nb = (int)((f.NP + 255) / 256);

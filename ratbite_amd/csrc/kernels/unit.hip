@@ -73,15 +73,16 @@ __global__ void k_oct(const float* v, uint32_t* enc, float* dec, uint32_t n) {
     dec[3 * i] = d.x; dec[3 * i + 1] = d.y; dec[3 * i + 2] = d.z;
 }
 
-template <bool ANY>
+template <bool ANY, int LDS_DEPTH>
 __global__ __launch_bounds__(128) void k_trace_test(const BvhNode* nodes, const TriRecord* tris, uint32_t nnodes,
-                                                    const float* rays, float* hits, uint32_t n) {
-    __shared__ int stack[64 * 128];
+                                                    const float* rays, float* hits, uint32_t n, int* spill,
+                                                    uint32_t spillStride) {
+    __shared__ int stack[LDS_DEPTH * 128];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const TravStack<LDS_DEPTH> st{stack + threadIdx.x, 128, spill + i, spillStride};
     const float* r = rays + 8 * (size_t)i;
-    const Hit h = traverse<ANY>(nodes, tris, nnodes, V3{r[0], r[1], r[2]}, V3{r[4], r[5], r[6]}, r[7],
-                                stack + threadIdx.x, 128);
+    const Hit h = traverse<ANY>(nodes, tris, nnodes, V3{r[0], r[1], r[2]}, V3{r[4], r[5], r[6]}, r[7], st);
     float* o = hits + 4 * (size_t)i;
     o[0] = h.prim == 0xFFFFFFFFu ? -1.0f : h.t;
     o[1] = __uint_as_float(h.prim);
@@ -255,19 +256,29 @@ int rbh_test_oct(rbh_ctx c, const float* v, uint32_t* enc, float* dec, uint32_t 
     return RBH_OK;
 }
 
-int rbh_test_trace(rbh_ctx c, const float* rays, float* hits, uint32_t n, int anyhit) {
+int rbh_test_trace(rbh_ctx c, const float* rays, float* hits, uint32_t n, int mode) {
     if (!c || !rays || !hits) return RBH_ERR_INVALID;
     if (!c->accel.built) return fail(c, RBH_ERR_STATE, "no acceleration structure");
     Staging s{c};
     void* dr = s.in(rays, (size_t)n * 32);
     void* dh = s.out((size_t)n * 16);
-    STAGE_CHECK(dr && dh);
-    if (anyhit)
-        hipLaunchKernelGGL(k_trace_test<true>, grid(n, 128), dim3(128), 0, 0, c->accel.nodes.as<BvhNode>(),
-                           c->accel.tris.as<TriRecord>(), c->accel.nnodes, (const float*)dr, (float*)dh, n);
+    const dim3 g = grid(n, 128);
+    const uint32_t lanes = g.x * 128u;
+    int* spill = (int*)s.out((size_t)lanes * kMaxStack * 4);
+    STAGE_CHECK(dr && dh && spill);
+    const BvhNode* nodes = c->accel.nodes.as<BvhNode>();
+    const TriRecord* tris = c->accel.tris.as<TriRecord>();
+    const uint32_t nn = c->accel.nnodes;
+    const bool any = (mode & 1) != 0, tiny = (mode & 2) != 0;
+    // mode bit 1: a 2-entry LDS stack, so the spill columns carry the rest
+    if (any && tiny)
+        hipLaunchKernelGGL((k_trace_test<true, 2>), g, dim3(128), 0, 0, nodes, tris, nn, (const float*)dr, (float*)dh, n, spill, lanes);
+    else if (any)
+        hipLaunchKernelGGL((k_trace_test<true, 16>), g, dim3(128), 0, 0, nodes, tris, nn, (const float*)dr, (float*)dh, n, spill, lanes);
+    else if (tiny)
+        hipLaunchKernelGGL((k_trace_test<false, 2>), g, dim3(128), 0, 0, nodes, tris, nn, (const float*)dr, (float*)dh, n, spill, lanes);
     else
-        hipLaunchKernelGGL(k_trace_test<false>, grid(n, 128), dim3(128), 0, 0, c->accel.nodes.as<BvhNode>(),
-                           c->accel.tris.as<TriRecord>(), c->accel.nnodes, (const float*)dr, (float*)dh, n);
+        hipLaunchKernelGGL((k_trace_test<false, 16>), g, dim3(128), 0, 0, nodes, tris, nn, (const float*)dr, (float*)dh, n, spill, lanes);
     FINISH(hits, dh, (size_t)n * 16);
     return RBH_OK;
 }

@@ -12,6 +12,7 @@ int lbvh_build(rbh_ctx c);
 hipError_t launch_pass(rbh_ctx c, Frame1 f, uint32_t depthMax, bool firstPass, int traceGrid, int shadeGrid,
                        KernelTimer* kt);
 hipError_t resolve_launch(rbh_ctx c, const float* colorspace, float expo, void* out);
+size_t trace_spill_words(size_t lanes);
 namespace host {
 int build_light_tree(const EmissiveTri* tris, const float* flux, uint32_t n, std::vector<PackedNode>& nodes,
                      std::vector<uint32_t>& pointers, std::vector<uint64_t>& masks);
@@ -204,6 +205,10 @@ int rbh_render(rbh_ctx c, const rbh_launch_params* lp) {
     const int cus = cu_count(c->device);
     const int traceGrid = cus * 16;
     const int shadeGrid = cus * 16;
+    const size_t lanes = (size_t)traceGrid * kTraceBlock;
+    RBH_ALLOC(c, rs.spill, trace_spill_words(lanes) * sizeof(int));
+    f.spill = rs.spill.as<int>();
+    f.spillStride = (uint32_t)lanes;
     KernelTimer* kt = (lp->flags & RBH_RENDER_TIMING) ? &rs.timer : nullptr;
     if (kt) kt->reset();
     RBH_HIP(c, hipEventRecord(c->ev0, c->stream));

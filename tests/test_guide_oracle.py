@@ -1,0 +1,116 @@
+"""Pins of the FP64 guiding-network oracle (oracle/guide_oracle.py): its
+analytic gradient against central finite differences, the vMF mixture's
+normalisation by quadrature, its sampler against its density, and that Adam on
+the loss moves the mixture towards a known target distribution."""
+import numpy as np
+import pytest
+
+import guide_oracle as G
+
+
+def _records(n, rng, target=None):
+    rec = np.zeros((n, G.REC))
+    rec[:, 0:3] = rng.random((n, 3))
+    nrm = rng.normal(size=(n, 3)); nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    wo = rng.normal(size=(n, 3)); wo /= np.linalg.norm(wo, axis=1, keepdims=True)
+    rec[:, 3:6], rec[:, 6:9] = nrm, wo
+    rec[:, 9] = rng.random(n)
+    rec[:, 10] = rng.random(n)
+    d = rng.normal(size=(n, 3)) if target is None else target(n, rng)
+    rec[:, 11:14] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rec[:, 14] = rng.exponential(size=n)
+    return rec
+
+
+def test_parameter_count():
+    assert G.n_params() == 80 * 64 + 3 * 64 * 64 + 64 * 40 + 4 * 64 + 40 == 20264
+
+
+def test_encoding_matches_direct_sin_cos():
+    rng = np.random.default_rng(0)
+    rec = _records(50, rng)
+    f = G.encode(rec)
+    for a in range(3):
+        for k in range(G.N_FREQ):
+            np.testing.assert_allclose(f[:, 24 * a + 2 * k], np.sin(np.pi * 2 ** k * rec[:, a]), atol=1e-9)
+            np.testing.assert_allclose(f[:, 24 * a + 2 * k + 1], np.cos(np.pi * 2 ** k * rec[:, a]), atol=1e-9)
+    np.testing.assert_array_equal(f[:, 72:78], rec[:, 3:9])
+
+
+def test_gradient_matches_finite_differences():
+    rng = np.random.default_rng(1)
+    ps = G.init_params(seed=2)
+    # larger output-layer weights so the lobes are sharp and the mixture uneven
+    ps[-1] = (ps[-1][0] * 3.0, rng.normal(size=40) * 0.5)
+    rec = _records(64, rng)
+    loss, grads = G.loss_grad(ps, rec)
+    flat = G.flatten(ps)
+    gflat = G.flatten(grads)
+    idx = rng.choice(len(flat), 120, replace=False)
+    idx = np.concatenate([idx, np.arange(len(flat) - 40, len(flat))])      # every output bias
+    h = 1e-6
+    for i in idx:
+        p1 = flat.copy(); p1[i] += h
+        p2 = flat.copy(); p2[i] -= h
+        l1, _ = G.loss_grad(G.unflatten(p1), rec)
+        l2, _ = G.loss_grad(G.unflatten(p2), rec)
+        fd = (l1 - l2) / (2 * h)
+        assert abs(fd - gflat[i]) <= 1e-6 + 1e-5 * abs(fd), (i, fd, gflat[i])
+
+
+def _sphere_grid(nc, nphi):
+    c = -1 + (np.arange(nc) + 0.5) * 2 / nc
+    ph = (np.arange(nphi) + 0.5) * 2 * np.pi / nphi
+    C, PH = np.meshgrid(c, ph, indexing="ij")
+    s = np.sqrt(1 - C ** 2)
+    d = np.stack([s * np.cos(PH), s * np.sin(PH), C], -1).reshape(-1, 3)
+    return d, (2 / nc) * (2 * np.pi / nphi)
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_mixture_pdf_normalised(seed):
+    rng = np.random.default_rng(seed)
+    out = rng.normal(size=(1, 40)) * 2.0
+    out[0, 3::5] = rng.uniform(-3, 5, 8)       # log kappa up to ~150
+    d, w = _sphere_grid(4000, 720)
+    q = G.pdf(np.repeat(out, len(d), 0), d)
+    assert abs(q.sum() * w - 1.0) < 2e-3
+
+
+def test_sampler_follows_pdf():
+    rng = np.random.default_rng(5)
+    out = rng.normal(size=(1, 40))
+    out[0, 3::5] = rng.uniform(0, 3, 8)
+    n = 200000
+    s = G.sample(np.repeat(out, n, 0), rng.random((n, 3)))
+    np.testing.assert_allclose(np.linalg.norm(s, axis=1), 1.0, atol=1e-12)
+    d, w = _sphere_grid(800, 360)
+    q = G.pdf(np.repeat(out, len(d), 0), d)
+    for fn in (lambda x: x[:, 0], lambda x: x[:, 2] ** 2, lambda x: x[:, 0] * x[:, 1]):
+        mc = fn(s).mean()
+        quad = np.sum(fn(d) * q) * w
+        assert abs(mc - quad) < 5 * fn(s).std() / np.sqrt(n) + 1e-3
+
+
+def test_training_moves_towards_target():
+    # target: directions from one vMF lobe around +z (kappa 20), independent of x
+    def target(n, rng):
+        k = 20.0
+        u = rng.random((n, 2))
+        cz = 1 + np.log(u[:, 0] + (1 - u[:, 0]) * np.exp(-2 * k)) / k
+        sz = np.sqrt(1 - cz * cz)
+        return np.stack([sz * np.cos(2 * np.pi * u[:, 1]), sz * np.sin(2 * np.pi * u[:, 1]), cz], 1)
+
+    rng = np.random.default_rng(6)
+    ps = G.init_params(seed=7)
+    flat = G.flatten(ps)
+    opt = G.Adam(len(flat), lr=3e-3)
+    first = None
+    for it in range(150):
+        rec = _records(256, rng, target)
+        rec[:, 14] = 1.0
+        loss, grads = G.loss_grad(G.unflatten(flat), rec)
+        first = loss if first is None else first
+        flat = opt.step(flat, G.flatten(grads))
+    # the optimum is the target's negative entropy, -log(20 / 2pi) + 1 = -0.158
+    assert first > 1.5 and loss < 0.3, (first, loss)
