@@ -111,10 +111,12 @@ typedef struct rbh_launch_params {
 } rbh_launch_params;
 
 enum {
-    RBH_RENDER_GUIDED = 1,        /* one-sample MIS with the guiding MLP */
-    RBH_RENDER_COLLECT = 2,       /* record guiding training samples */
+    RBH_RENDER_GUIDED = 1,        /* one-sample MIS with the guiding MLP (rbh_guide_configure first) */
+    RBH_RENDER_COLLECT = 2,       /* with GUIDED: record training samples (rbh_guide_train_collected) */
     RBH_RENDER_GUIDE_FP8 = 4,     /* FP8 E4M3 guiding inference */
     RBH_RENDER_TIMING = 8,        /* per-kernel HIP-event timing into rbh_stats */
+    RBH_RENDER_NO_DROP = 16,      /* an invalid BxDF sample ends the path (FINISHED) instead of
+                                     discarding the whole sample as the reference does (implied by GUIDED) */
 };
 
 enum {
@@ -134,6 +136,7 @@ typedef struct rbh_stats {
     double bvhMs;                 /* time of the last LBVH build */
     uint32_t traceLaunches;       /* closest-hit kernel launches of the last render */
     uint32_t bvhDepth;            /* max depth of the LBVH */
+    double guideMs;               /* time in guiding inference kernels */
 } rbh_stats;
 
 /* ----------------------------------------------------------------- context
@@ -212,6 +215,39 @@ int rbh_write_accum(rbh_ctx ctx, const double* rgbw, uint32_t width, uint32_t he
  * exposure scale 2^(e-7), AgX, RGBA8 out (W*H*4 bytes). */
 int rbh_resolve(rbh_ctx ctx, const float* colorspace, float exposureScale, uint8_t* rgba8);
 int rbh_stats_get(rbh_ctx ctx, rbh_stats* out);
+
+/* ------------------------------------------------------------- path guiding
+ * No reference counterpart: a learned proposal that extends the BSDF
+ * sampling of src/kernels/pathtrace.cu:548-905 through one-sample MIS.
+ * An MLP 80 -> 64 x4 (ReLU) -> 40 maps a shading point to 8 von Mises-Fisher
+ * lobes; BF16 weights with FP32 accumulation and FP32 master weights, Adam.
+ * Training records are 16 floats: position normalised to [0,1]^3 by the
+ * configured scene bounds [0:3], shading normal [3:6], outgoing direction
+ * [6:9], roughness [9], wavelength normalised to [0,1] over 360..830 nm [10],
+ * sampled incident direction [11:14], weight = radiance / sampling pdf [14].
+ * The loss is sum(weight * -log q(direction)) / sum(weight). */
+typedef struct rbh_guide_config {
+    float lr, beta1, beta2, eps;      /* Adam (1e-3, 0.9, 0.999, 1e-8) */
+    float sceneMin[3], sceneMax[3];   /* position normalisation */
+    uint32_t seed;                    /* He-uniform initialisation */
+    uint32_t recordCapacity;          /* records kept per RBH_RENDER_COLLECT render */
+} rbh_guide_config;
+int rbh_guide_configure(rbh_ctx ctx, const rbh_guide_config* cfg);
+/* FP32 master parameters, per layer W[out][in] then b[out] (20264 floats);
+ * setting them resets the Adam moments */
+int rbh_guide_set_params(rbh_ctx ctx, const float* params, uint32_t n);
+int rbh_guide_get_params(rbh_ctx ctx, float* params, uint32_t n);
+/* gradient of the loss over n records (no update) */
+int rbh_guide_gradient(rbh_ctx ctx, const float* records, uint32_t n, float* grad, float* loss);
+/* steps Adam steps over consecutive batches of the n records */
+int rbh_guide_train(rbh_ctx ctx, const float* records, uint32_t n, uint32_t batch, uint32_t steps, float* lastLoss);
+/* raw network outputs (n x 40: per lobe mean xyz, log sharpness, logit) */
+int rbh_guide_infer(rbh_ctx ctx, const float* records, uint32_t n, float* out);
+/* Training records gathered by the last RBH_RENDER_COLLECT render (count in
+ * *n; up to max copied when out is not null). */
+int rbh_guide_records(rbh_ctx ctx, float* out, uint32_t max, uint32_t* n);
+/* steps Adam steps over consecutive batches of the gathered records */
+int rbh_guide_train_collected(rbh_ctx ctx, uint32_t batch, uint32_t steps, float* lastLoss);
 
 #ifdef __cplusplus
 }

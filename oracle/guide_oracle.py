@@ -83,13 +83,26 @@ def unflatten(v):
     return ps
 
 
-def forward(ps, x):
-    """Returns the raw 40 outputs and the per-layer inputs (for backward)."""
-    acts = [x]
-    h = x
+def bf16(x):
+    """Round to the nearest bfloat16 (ties to even), returned as float64."""
+    u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def forward(ps, x, emulate_bf16=False):
+    """Returns the raw 40 outputs and the per-layer inputs (for backward).
+    With emulate_bf16 the layer inputs and weights are rounded to bfloat16 as
+    the matrix-core kernels store them (products accumulate exactly here)."""
+    q = bf16 if emulate_bf16 else (lambda a: a)
+    h = q(x)
+    acts = [h]
     for li, (W, b) in enumerate(ps):
-        z = h @ W.T + b
-        h = np.maximum(z, 0.0) if li < len(ps) - 1 else z
+        z = h @ q(W).T + b
+        if li < len(ps) - 1:
+            h = q(np.maximum(z, 0.0))
+        else:
+            h = z
         acts.append(h)
     return h, acts
 
@@ -181,23 +194,24 @@ def loss_and_dout(out, rec):
     return loss, dout.reshape(-1, N_OUT)
 
 
-def backward(ps, acts, dout):
+def backward(ps, acts, dout, emulate_bf16=False):
+    q = bf16 if emulate_bf16 else (lambda a: a)
     grads = [None] * len(ps)
-    dh = dout
+    dh = q(dout)
     for li in range(len(ps) - 1, -1, -1):
         W, b = ps[li]
         x = acts[li]
         grads[li] = (dh.T @ x, dh.sum(axis=0))
         if li > 0:
-            dh = (dh @ W) * (acts[li] > 0)
+            dh = q((dh @ q(W)) * (acts[li] > 0))
     return grads
 
 
-def loss_grad(ps, rec):
+def loss_grad(ps, rec, emulate_bf16=False):
     x = encode(rec)
-    out, acts = forward(ps, x)
+    out, acts = forward(ps, x, emulate_bf16)
     loss, dout = loss_and_dout(out, rec)
-    return loss, backward(ps, acts, dout)
+    return loss, backward(ps, acts, dout, emulate_bf16)
 
 
 class Adam:

@@ -29,6 +29,7 @@ RBH_RENDER_GUIDED = 1
 RBH_RENDER_COLLECT = 2
 RBH_RENDER_GUIDE_FP8 = 4
 RBH_RENDER_TIMING = 8
+RBH_RENDER_NO_DROP = 16
 RBH_CDF_FP64 = 1
 
 # Record layouts (byte-identical to include/ratbite_hip.h)
@@ -78,7 +79,19 @@ class Stats(ctypes.Structure):
         ("closestRays", c_uint64), ("shadowRays", c_uint64), ("pathVertices", c_uint64), ("samples", c_uint64),
         ("renderMs", c_double), ("traceMs", c_double), ("shadowMs", c_double), ("shadeMs", c_double),
         ("cdfMs", c_double), ("bvhMs", c_double), ("traceLaunches", c_uint32), ("bvhDepth", c_uint32),
+        ("guideMs", c_double),
     ]
+
+
+class GuideConfig(ctypes.Structure):
+    _fields_ = [
+        ("lr", c_float), ("beta1", c_float), ("beta2", c_float), ("eps", c_float),
+        ("sceneMin", c_float * 3), ("sceneMax", c_float * 3), ("seed", c_uint32), ("recordCapacity", c_uint32),
+    ]
+
+
+GUIDE_PARAMS = 20264
+GUIDE_REC = 16
 
 
 # name -> (restype, argtypes)
@@ -104,6 +117,14 @@ _SIGS = {
     "rbh_envmap_disable": (c_int, [c_void_p]),
     "rbh_render": (c_int, [c_void_p, ctypes.POINTER(LaunchParams)]),
     "rbh_set_path_slots": (c_int, [c_void_p, c_uint64]),
+    "rbh_guide_configure": (c_int, [c_void_p, ctypes.POINTER(GuideConfig)]),
+    "rbh_guide_set_params": (c_int, [c_void_p, c_void_p, c_uint32]),
+    "rbh_guide_get_params": (c_int, [c_void_p, c_void_p, c_uint32]),
+    "rbh_guide_gradient": (c_int, [c_void_p, c_void_p, c_uint32, c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    "rbh_guide_train": (c_int, [c_void_p, c_void_p, c_uint32, c_uint32, c_uint32, ctypes.POINTER(ctypes.c_float)]),
+    "rbh_guide_infer": (c_int, [c_void_p, c_void_p, c_uint32, c_void_p]),
+    "rbh_guide_records": (c_int, [c_void_p, c_void_p, c_uint32, ctypes.POINTER(c_uint32)]),
+    "rbh_guide_train_collected": (c_int, [c_void_p, c_uint32, c_uint32, ctypes.POINTER(ctypes.c_float)]),
     "rbh_read_accum": (c_int, [c_void_p, c_void_p]),
     "rbh_write_accum": (c_int, [c_void_p, c_void_p, c_uint32, c_uint32]),
     "rbh_resolve": (c_int, [c_void_p, c_void_p, c_float, c_void_p]),

@@ -1,6 +1,7 @@
 // C ABI entry points (include/ratbite_hip.h): context lifetime, uploads and
 // the environment-map tables. Rendering entry points live in render_api.cpp.
 #include "context.hpp"
+#include "guide.hpp"
 #include <cstring>
 #include <new>
 #include <vector>
@@ -47,6 +48,11 @@ int rbh_destroy(rbh_ctx c) {
     (void)hipStreamSynchronize(c->stream);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->guide) {
+        rbh::guide_release(*static_cast<rbh::GuideState*>(c->guide));
+        delete static_cast<rbh::GuideState*>(c->guide);
+        c->guide = nullptr;
+    }
     hipStream_t s = c->stream;
     delete c;  // DevBuf destructors free device memory
     if (s) (void)hipStreamDestroy(s);

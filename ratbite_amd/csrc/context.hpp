@@ -74,7 +74,7 @@ struct Accel {
 // HIP-event timing of individual render-loop kernels (RBH_RENDER_TIMING):
 // one event pair per launch, recorded on the render stream around the launch.
 struct KernelTimer {
-    enum { TRACE = 0, SHADE = 1, SHADOW = 2, NCLASS = 3 };
+    enum { TRACE = 0, SHADE = 1, SHADOW = 2, GUIDE = 3, NCLASS = 4 };
     std::vector<hipEvent_t> pool;   // 2 events per recorded launch
     std::vector<int> cls;           // kernel class of launch i
     KernelTimer() = default;
@@ -117,6 +117,8 @@ struct RenderState {
     DevBuf queues;    // ray/shadow queues + counters
     DevBuf image;     // RGBA8 resolve output
     DevBuf spill;     // traversal-stack spill columns
+    DevBuf gfeat;     // guided renders: network input per hit-queue entry
+    DevBuf gmix;      // guided renders: mixture per hit-queue entry
     size_t pathCapacity = 0;
     // path slots one pass may hold (samples per pass = slotBudget / pixels);
     // about 200 B of HBM per slot

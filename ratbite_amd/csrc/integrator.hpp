@@ -9,7 +9,7 @@ namespace rbh {
 constexpr int kTraceBlock = 128;   // lanes per block of the trace and shadow kernels
 constexpr int kShadeBlock = 128;
 
-enum : uint32_t { CNT_QUEUE0 = 0, CNT_QUEUE1 = 1, CNT_SHADOW = 2, CNT_COUNT = 4 };
+enum : uint32_t { CNT_QUEUE0 = 0, CNT_QUEUE1 = 1, CNT_SHADOW = 2, CNT_HIT = 3, CNT_MISS = 4, CNT_COUNT = 8 };
 enum : uint32_t { STAT_CLOSEST = 0, STAT_SHADOW = 1, STAT_VERTICES = 2, STAT_COUNT = 4 };
 
 // Structure-of-arrays path state, one entry per path slot of the pass.
@@ -26,9 +26,12 @@ struct PathBuffers {
     float4* Lnee;   // deferred NEE term
     float4* sro;    // shadow ray origin, tmax
     float4* srd;    // shadow ray direction
-    uint32_t* q0;
+    uint32_t* q0;   // paths to trace (even / odd bounces)
     uint32_t* q1;
-    uint32_t* sq;
+    uint32_t* hq;   // paths whose closest hit found a triangle (to shade)
+    uint32_t* mq;   // paths whose ray left the scene
+    uint32_t* sq;   // paths with a deferred NEE shadow ray
+    uint32_t* prec; // guided renders that collect training data: last record of the path
 };
 
 struct Frame1 {
@@ -75,6 +78,27 @@ struct Frame1 {
     uint32_t* counters;
     unsigned long long* stats;
     double4* accum;
+    // path guiding (guided renders only)
+    uint32_t guided;          // one-sample MIS with the network at ComplexSurface vertices
+    uint32_t noDrop;          // an invalid BxDF sample ends the path instead of discarding the sample
+    const __bf16* gwf;        // network weights (BF16, forward layout) and FP32 biases
+    const float* gbias;
+    float* gfeat;             // per hit-queue entry: 16-float input record
+    float* gmix;              // per hit-queue entry: 40-float mixture (world space)
+    float gmin[3], ginv[3];   // position normalisation
+    uint32_t collect;         // record training data
+    uint32_t recThreshold;    // keep a vertex when its hash is below this
+    float* grec;              // pending records: 24 floats each
+    float* gtrain;            // finished training records: 16 floats each
+    uint32_t* grecCount;
+    uint32_t grecCap;
 };
+
+// pending-record layout (floats): 0..10 input features, 11..13 sampled
+// direction, 14 1/pdf, 15 previous record of the path (uint bits),
+// 16..19 throughput after the vertex, 20..23 radiance before the next vertex
+constexpr int kPendingRec = 24;
+constexpr uint32_t kRecPending = 0x80000000u;   // radiance snapshot not taken yet
+constexpr uint32_t kNoRec = 0xFFFFFFFFu;
 
 }  // namespace rbh

@@ -5,7 +5,10 @@
 // slots, slot = sample-major pixel index) so that the long tail of a few deep
 // paths is paid once per pass rather than once per sample; path state lives in
 // structure-of-arrays buffers in HBM. Per bounce:
-//   k_trace    closest hit for the queued paths           (pathtrace.cu:1162-1177)
+//   k_trace    closest hit for the queued paths           (pathtrace.cu:1162-1177),
+//              sorting them into a hit queue and a miss queue
+//   k_miss     sky emission + MIS for the misses; the path ends
+//              (pathtrace.cu:1185-1274, sky branch)
 //   k_shade    emission + MIS, surface attributes, light-tree NEE, BxDF,
 //              Russian roulette, next origin, depth caps   (pathtrace.cu:1179-1785)
 //   k_shadow   any-hit test of the deferred NEE rays; adds the NEE term of
@@ -17,6 +20,8 @@
 // sequence per pass.
 #include "../context.hpp"
 #include "../device/bxdf.hpp"
+#include "../device/guided.hpp"
+#include "../device/mlp.hpp"
 #include "../device/lighttree.hpp"
 #include "../device/qrng.hpp"
 #include "../device/traverse.hpp"
@@ -89,16 +94,79 @@ __global__ __launch_bounds__(256) void k_camera(Frame1 f) {
     p.T[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
     p.nf[i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
     p.depth[i] = 0u;
+    if (f.collect) p.prec[i] = kNoRec;
     if (i == 0) {
         f.counters[CNT_QUEUE0] = f.NP;
         f.counters[CNT_QUEUE1] = 0u;
         f.counters[CNT_SHADOW] = 0u;
+        f.counters[CNT_HIT] = 0u;
+        f.counters[CNT_MISS] = 0u;
     }
 }
 
 __global__ void k_reset(uint32_t* __restrict__ counters, uint32_t which) {
     if (threadIdx.x == 0) counters[which] = 0u;
     if (threadIdx.x == 1) counters[CNT_SHADOW] = 0u;
+}
+
+__global__ void k_reset_hits(uint32_t* __restrict__ counters) {
+    if (threadIdx.x == 0) counters[CNT_HIT] = 0u;
+    if (threadIdx.x == 1) counters[CNT_MISS] = 0u;
+}
+
+// Guided renders that collect training data: the radiance of the path just
+// before the vertex its last recorded direction leads to.
+__device__ __forceinline__ void snapshot_radiance(const Frame1& f, uint32_t pid, float4 L) {
+    const uint32_t pr = f.p.prec[pid];
+    if (pr != kNoRec && (pr & kRecPending)) {
+        const uint32_t r = pr & ~kRecPending;
+        float* rec = f.grec + (size_t)r * kPendingRec;
+        rec[20] = L.x;
+        rec[21] = L.y;
+        rec[22] = L.z;
+        rec[23] = L.w;
+        f.p.prec[pid] = r;
+    }
+}
+
+// A path whose ray left the scene: sky emission with the MIS weight of the
+// previous vertex's BxDF sample, then the path ends (pathtrace.cu:1185-1274,
+// the sky branch; 1240-1252 for the lookup and its pdf).
+__device__ __forceinline__ void shade_miss(const Frame1& f, uint32_t pid) {
+    float4 nf = f.p.nf[pid];
+    uint32_t flags = __float_as_uint(nf.w);
+    flags &= ~PF_TRIANGULAR;
+    if (f.collect) snapshot_radiance(f, pid, f.p.L[pid]);
+    if (f.envOn) {
+        const uint32_t depth = f.p.depth[pid];
+        const float4 od = f.p.od[pid], dr = f.p.dr[pid];
+        const V3 rd{dr.x, dr.y, dr.z};
+        const LightTree tree{f.treeNodes, f.treePointers, f.treeMasks, true};
+        const bool prevNEE = depth != 0 && !(flags & PF_PREV_SPECULAR);
+        const float treePdf = prevNEE ? tree.pdf(LT_SKY, 0, V3{od.x, od.y, od.z}, V3{nf.x, nf.y, nf.z}) : 1.0f;
+        const float phi = atan2f(rd.y, rd.x);
+        const float theta = acosf(rd.z);
+        const V3 sky = env_lookup(f.envTex, f.envW, f.envH, f.envC, 0.5f - phi / (2.0f * 3.14159265358979f),
+                                  theta / 3.14159265358979f);
+        float lightPdf = (1.0f / (4.0f * 3.14159265358979f)) * ((sky.x + sky.y + sky.z) / 3.0f) * sinf(theta) /
+                         f.envIntegral;
+        const float4 la = f.p.lam[pid];
+        Wl wl;
+        wl.lambda = S4{{la.x, la.y, la.z, la.w}};
+        const S4 Le = ColorTables{f.colors}.rgb(sky, wl);
+        lightPdf *= treePdf;
+        const float w = prevNEE ? power_heuristic(od.w, lightPdf) : 1.0f;
+        const float4 Tv = f.p.T[pid];
+        float4 L = f.p.L[pid];
+        L.x += Tv.x * Le.v[0] * w;
+        L.y += Tv.y * Le.v[1] * w;
+        L.z += Tv.z * Le.v[2] * w;
+        L.w += Tv.w * Le.v[3] * w;
+        f.p.L[pid] = L;
+    }
+    flags |= PF_FINISHED;
+    nf.w = __uint_as_float(flags);
+    f.p.nf[pid] = nf;
 }
 
 // ------------------------------------------------------------- closest hit
@@ -115,16 +183,104 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(Frame1 f, uint32_t qsel) 
     const bool identity = f.bounce == 0;
     uint32_t nrays = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i - threadIdx.x < count; i += gridDim.x * blockDim.x) {
+        bool hit = false;
+        uint32_t pid = 0;
         if (i < count) {
-            const uint32_t pid = identity ? i : queue[i];
+            pid = identity ? i : queue[i];
             const float4 od = f.p.od[pid], dr = f.p.dr[pid];
             const Hit h = traverse<false>(f.nodes, f.tris, f.nnodes, V3{od.x, od.y, od.z}, V3{dr.x, dr.y, dr.z},
                                           3.402823466e+38f, st);
-            f.p.hit[pid] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.prim));
+            hit = h.prim != 0xFFFFFFFFu;
+            if (hit) f.p.hit[pid] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.prim));
             ++nrays;
         }
+        const bool live = i < count;
+        const uint32_t slot = wave_push(f.counters + CNT_HIT, live && hit);
+        if (live && hit) f.p.hq[slot] = pid;
+        const uint32_t mslot = wave_push(f.counters + CNT_MISS, live && !hit);
+        if (live && !hit) f.p.mq[mslot] = pid;
     }
     wave_count(f.stats + STAT_CLOSEST, nrays);
+}
+
+// Network input of a hit: scene-normalised position, geometric normal turned
+// towards the viewer, outgoing direction, roughness factor, hero wavelength.
+__device__ __forceinline__ void guide_input(const Frame1& f, uint32_t pid, float* __restrict__ out) {
+    const float4 hr = f.p.hit[pid], dr = f.p.dr[pid];
+    const uint32_t prim = __float_as_uint(hr.w);
+    const V3 v0 = ld3(f.verts + 3 * (size_t)f.indices[3 * prim]), v1 = ld3(f.verts + 3 * (size_t)f.indices[3 * prim + 1]),
+             v2 = ld3(f.verts + 3 * (size_t)f.indices[3 * prim + 2]);
+    const float b1 = hr.y, b2 = hr.z, b0 = 1.0f - b1 - b2;
+    const V3 hp{v0.x * b0 + v1.x * b1 + v2.x * b2, v0.y * b0 + v1.y * b1 + v2.y * b2, v0.z * b0 + v1.z * b1 + v2.z * b2};
+    V3 g = cross(v1 - v0, v2 - v0);
+    g = g * rsqrtf(dot(g, g));
+    const V3 wo{-dr.x, -dr.y, -dr.z};
+    if (dot(g, wo) < 0.0f) g = -g;
+    const Material& m = f.materials[f.triMaterial[prim]];
+    const float rough = m.bxdf == 0 ? m.mfRoughness : ((m.factors & 4u) ? m.roughness : 1.0f);
+    out[0] = clampf((hp.x - f.gmin[0]) * f.ginv[0], 0.0f, 1.0f);
+    out[1] = clampf((hp.y - f.gmin[1]) * f.ginv[1], 0.0f, 1.0f);
+    out[2] = clampf((hp.z - f.gmin[2]) * f.ginv[2], 0.0f, 1.0f);
+    out[3] = g.x; out[4] = g.y; out[5] = g.z;
+    out[6] = wo.x; out[7] = wo.y; out[8] = wo.z;
+    out[9] = rough;
+    out[10] = (f.p.lam[pid].x - 360.0f) * (1.0f / 470.0f);
+    for (int k = 11; k < 16; ++k) out[k] = 0.0f;
+}
+
+// Raw outputs -> mixture (normalised means, clamped sharpness, softmax weights)
+__device__ __forceinline__ void guide_mixture(const float* __restrict__ o, float* __restrict__ mix) {
+    float lmax = -3.4e38f;
+    for (int j = 0; j < 8; ++j) lmax = fmaxf(lmax, o[5 * j + 4]);
+    float ws = 0.0f;
+    for (int j = 0; j < 8; ++j) ws += expf(o[5 * j + 4] - lmax);
+    for (int j = 0; j < 8; ++j) {
+        const float* q = o + 5 * j;
+        const float n = fmaxf(sqrtf(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]), 1e-6f);
+        mix[5 * j + 0] = q[0] / n;
+        mix[5 * j + 1] = q[1] / n;
+        mix[5 * j + 2] = q[2] / n;
+        mix[5 * j + 3] = expf(fminf(fmaxf(q[3], guide::kLogKappaMin), guide::kLogKappaMax));
+        mix[5 * j + 4] = expf(q[4] - lmax) / ws;
+    }
+}
+
+// Network inference for every queued hit: input records into LDS, the BF16
+// MFMA forward pass, mixtures to gmix (indexed like the hit queue).
+__global__ __launch_bounds__(256) void k_guide_paths(Frame1 f) {
+    __shared__ mlp::InferLds L;
+    __shared__ float recs[mlp::kTile * guide::kRec];
+    const uint32_t count = f.counters[CNT_HIT];
+    const int wave = threadIdx.x >> 6, lane = __lane_id(), r = lane & 15, g = lane >> 4;
+    const int row0 = wave * mlp::kRows;
+    for (uint32_t tile = blockIdx.x; (size_t)tile * mlp::kTile < count; tile += gridDim.x) {
+        const uint32_t base = tile * mlp::kTile;
+        if (threadIdx.x < mlp::kTile) {
+            const uint32_t i = base + threadIdx.x;
+            float* dst = recs + threadIdx.x * guide::kRec;
+            if (i < count) {
+                guide_input(f, f.p.hq[i], dst);
+                for (int k = 0; k < guide::kRec; ++k) f.gfeat[(size_t)i * guide::kRec + k] = dst[k];
+            } else {
+                for (int k = 0; k < guide::kRec; ++k) dst[k] = 0.0f;
+            }
+        }
+        __syncthreads();
+        mlp::encode_rows(L.x0, recs, mlp::kTile, 0, row0);
+        mlp::forward_rows(L, f.gwf, f.gbias, row0);
+        __syncthreads();
+        if (g == 0) {
+            const uint32_t i = base + row0 + r;
+            if (i < count) guide_mixture(L.out + (row0 + r) * mlp::kSO, f.gmix + (size_t)i * kMixFloats);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_miss(Frame1 f) {
+    const uint32_t count = f.counters[CNT_MISS];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
+        shade_miss(f, f.p.mq[i]);
 }
 
 // ------------------------------------------------------------------ shadow
@@ -178,12 +334,13 @@ __device__ __forceinline__ void quat_cols(float4 q, V3& c0, V3& c2) {
     c2 = V3{2.0f * (x * z + w * y), 2.0f * (y * z - w * x), 1.0f - 2.0f * (x * x + y * y)};
 }
 
+template <bool GUIDED>
 __global__ __launch_bounds__(kShadeBlock) void k_shade(Frame1 f, uint32_t qsel) {
-    const uint32_t count = f.counters[qsel ? CNT_QUEUE1 : CNT_QUEUE0];
-    const uint32_t* __restrict__ qin = qsel ? f.p.q1 : f.p.q0;
+    // shades the paths whose ray hit a triangle (misses were finished by k_trace)
+    const uint32_t count = f.counters[CNT_HIT];
+    const uint32_t* __restrict__ qin = f.p.hq;
     uint32_t* __restrict__ qout = qsel ? f.p.q0 : f.p.q1;
     uint32_t* __restrict__ cout = f.counters + (qsel ? CNT_QUEUE0 : CNT_QUEUE1);
-    const bool identity = f.bounce == 0;
     const ColorTables ct{f.colors};
     const Luts lut{f.luts};
     const LightTree tree{f.treeNodes, f.treePointers, f.treeMasks, f.envOn != 0};
@@ -191,9 +348,12 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(Frame1 f, uint32_t qsel) 
     uint32_t nvert = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i - threadIdx.x < count; i += gridDim.x * blockDim.x) {
         bool alive = false, shadow = false;
+        bool recordSlot = false;      // guided + collecting: record this vertex's direction
+        V3 recDir{0.0f, 0.0f, 0.0f};
+        float recInvPdf = 0.0f;
         uint32_t pid = 0;
         if (i < count) {
-            pid = identity ? i : qin[i];
+            pid = qin[i];
             ++nvert;
             const float4 od = f.p.od[pid], dr = f.p.dr[pid], nf = f.p.nf[pid], hr = f.p.hit[pid];
             PathIO P;
@@ -207,6 +367,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(Frame1 f, uint32_t qsel) 
                 P.wl.pdf = S4{{b.x, b.y, b.z, b.w}};
             }
             float4 Lv = f.p.L[pid], Tv = f.p.T[pid];
+            if (GUIDED && f.collect) snapshot_radiance(f, pid, Lv);
             S4 L{{Lv.x, Lv.y, Lv.z, Lv.w}};
             P.T = S4{{Tv.x, Tv.y, Tv.z, Tv.w}};
             V3 ro{od.x, od.y, od.z};
@@ -218,31 +379,32 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(Frame1 f, uint32_t qsel) 
 
             // ---- interaction (pathtrace.cu:86-149, 906-953)
             const uint32_t prim = __float_as_uint(hr.w);
-            const bool isHit = prim != 0xFFFFFFFFu;
-            uint32_t ltype = LT_NONE, lindex = 0, mat = 0;
-            V3 gn{0, 0, 0}, hitPos{0, 0, 0}, hitInterp{0, 0, 0}, v0{0, 0, 0}, v1{0, 0, 0}, v2{0, 0, 0};
-            uint32_t vi0 = 0, vi1 = 0, vi2 = 0;
+            // k_trace queues only paths that hit a triangle
+            uint32_t ltype = LT_NONE, lindex = 0;
             const float b1 = hr.y, b2 = hr.z;
-            bool backHit = false;
-            if (!isHit) {
-                ltype = LT_SKY;
-                P.flags &= ~PF_TRIANGULAR;
-            } else {
-                vi0 = f.indices[3 * prim]; vi1 = f.indices[3 * prim + 1]; vi2 = f.indices[3 * prim + 2];
-                v0 = ld3(f.verts + 3 * (size_t)vi0); v1 = ld3(f.verts + 3 * (size_t)vi1); v2 = ld3(f.verts + 3 * (size_t)vi2);
+            const uint32_t vi0 = f.indices[3 * prim], vi1 = f.indices[3 * prim + 1], vi2 = f.indices[3 * prim + 2];
+            const V3 v0 = ld3(f.verts + 3 * (size_t)vi0), v1 = ld3(f.verts + 3 * (size_t)vi1),
+                     v2 = ld3(f.verts + 3 * (size_t)vi2);
+            V3 gn;
+            {
                 V3 g = cross(v1 - v0, v2 - v0);
                 g = g * rsqrtf(dot(g, g));
                 gn = oct_dec_u32(oct_enc_u32(g));
-                mat = f.triMaterial[prim];
+            }
+            const uint32_t mat = f.triMaterial[prim];
+            {
                 const uint32_t tl = f.triLight[prim];
                 if (tl != 0xFFFFFFFFu) { ltype = LT_TRIANGLE; lindex = tl; }
-                P.flags |= PF_TRIANGULAR;
+            }
+            P.flags |= PF_TRIANGULAR;
+            V3 hitPos;
+            {
                 const float b0 = 1.0f - b1 - b2;
                 hitPos = V3{v0.x * b0 + v1.x * b1 + v2.x * b2, v0.y * b0 + v1.y * b1 + v2.y * b2,
                             v0.z * b0 + v1.z * b1 + v2.z * b2};
-                hitInterp = hitPos;
-                backHit = dot(gn, P.rd) > 0.0f;
             }
+            V3 hitInterp = hitPos;
+            const bool backHit = dot(gn, P.rd) > 0.0f;
 
             // ---- emission + MIS (pathtrace.cu:1185-1274)
             bool finished = false;
@@ -264,19 +426,6 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(Frame1 f, uint32_t qsel) 
                     noEm = lcos <= 0.0f;
                     lightPdf = (1.0f / (nl / 2.0f)) * d2 / lcos;
                     Le = tri_emission(f, tri, 1.0f - b1 - b2, b1, b2, P.wl, ct);
-                } else {
-                    if (f.envOn) {
-                        const float phi = atan2f(P.rd.y, P.rd.x);
-                        const float theta = acosf(P.rd.z);
-                        const V3 sky = env_lookup(f.envTex, f.envW, f.envH, f.envC, 0.5f - phi / (2.0f * 3.14159265358979f),
-                                                  theta / 3.14159265358979f);
-                        lightPdf = (1.0f / (4.0f * 3.14159265358979f)) * ((sky.x + sky.y + sky.z) / 3.0f) * sinf(theta) /
-                                   f.envIntegral;
-                        Le = ct.rgb(sky, P.wl);
-                    } else {
-                        noEm = true;
-                    }
-                    P.flags |= PF_FINISHED;
                 }
                 if (!noEm) {
                     lightPdf *= treePdf;
@@ -502,12 +651,26 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(Frame1 f, uint32_t qsel) 
                         dl.pdf = lightPdf * treePdf;
                     }
                     S4 Lnee = s4(0.0f);
-                    if (m.bxdf == 0)
+                    const bool guideHere = GUIDED && m.bxdf == 2 && s.roughness >= kGuideMinRough &&
+                                           dot(local.n, -P.rd) > 0.0f;
+                    if (guideHere) {
+                        V3 wiw;
+                        const float pmix = complex_guided(ct, lut, dl, q, local, s, P, Lnee, f.gmix + (size_t)i * kMixFloats, wiw);
+                        if (pmix == 0.0f) {
+                            P.flags |= PF_FINISHED;
+                        } else if (f.collect && pixel_hash(pid, depth * 0x9E3779B9u + f.sample) < f.recThreshold) {
+                            recordSlot = true;
+                            recDir = wiw;
+                            recInvPdf = 1.0f / pmix;
+                        }
+                    } else if (m.bxdf == 0) {
                         conductor_bxdf(f.spectra + (size_t)m.iorSpectrum * kDense, f.spectra + (size_t)m.acSpectrum * kDense,
                                        m.mfRoughness, dl, q, local, P, Lnee);
-                    else if (m.bxdf == 2)
+                    } else if (m.bxdf == 2) {
                         complex_bxdf(ct, lut, dl, q, local, s, P, Lnee);
+                    }
                     // bxdf 1: PureDielectric has an empty body in the reference
+                    if (f.noDrop && (P.flags & PF_TERMINATED)) P.flags = (P.flags & ~PF_TERMINATED) | PF_FINISHED;
                     if (!dl.occluded) {
                         shadow = true;
                         f.p.Lnee[pid] = make_float4(Lnee.v[0], Lnee.v[1], Lnee.v[2], Lnee.v[3]);
@@ -561,8 +724,52 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(Frame1 f, uint32_t qsel) 
         if (alive) qout[slot] = pid;
         const uint32_t sslot = wave_push(f.counters + CNT_SHADOW, shadow);
         if (shadow) f.p.sq[sslot] = pid;
+        if (GUIDED && f.collect) {
+            // a recorded direction needs the path to go on (its radiance is measured later)
+            const bool rec = recordSlot && alive;
+            const uint32_t r = wave_push(f.grecCount, rec);
+            if (rec && r < f.grecCap) {
+                float* d = f.grec + (size_t)r * kPendingRec;
+                const float* in = f.gfeat + (size_t)i * guide::kRec;
+                for (int k = 0; k < 11; ++k) d[k] = in[k];
+                d[11] = recDir.x;
+                d[12] = recDir.y;
+                d[13] = recDir.z;
+                d[14] = recInvPdf;
+                d[15] = __uint_as_float(f.p.prec[pid]);
+                const float4 Tn = f.p.T[pid];
+                d[16] = Tn.x; d[17] = Tn.y; d[18] = Tn.z; d[19] = Tn.w;
+                f.p.prec[pid] = r | kRecPending;
+            }
+        }
     }
     wave_count(f.stats + STAT_VERTICES, nvert);
+}
+
+// Turns the pending records of a finished path into training records:
+// incident radiance along the recorded direction = radiance gathered after
+// the vertex / throughput after the vertex, averaged over the wavelengths,
+// times 1/pdf of the direction.
+__device__ __forceinline__ void finish_records(const Frame1& f, uint32_t slot, float4 L) {
+    uint32_t r = f.p.prec[slot];
+    for (int guard = 0; r != kNoRec && guard < 1024; ++guard) {
+        const bool pending = (r & kRecPending) != 0;
+        r &= ~kRecPending;
+        const float* d = f.grec + (size_t)r * kPendingRec;
+        float* t = f.gtrain + (size_t)r * guide::kRec;
+        for (int k = 0; k < 14; ++k) t[k] = d[k];
+        float w = 0.0f;
+        if (!pending) {
+            const float dl[4] = {L.x - d[20], L.y - d[21], L.z - d[22], L.w - d[23]};
+            float li = 0.0f;
+            for (int k = 0; k < 4; ++k) li += d[16 + k] > 0.0f ? dl[k] / d[16 + k] : 0.0f;
+            w = fmaxf(0.0f, 0.25f * li) * d[14];
+            if (!(w < 3.0e38f)) w = 0.0f;
+        }
+        t[14] = w;
+        t[15] = 0.0f;
+        r = __float_as_uint(d[15]);
+    }
 }
 
 // ----------------------------------------------------------- accumulation
@@ -575,6 +782,7 @@ __global__ __launch_bounds__(256) void k_accumulate(Frame1 f, uint32_t first) {
     const ColorTables ct{f.colors};
     for (uint32_t k = 0; k < f.S; ++k) {
         const uint32_t j = k * f.P + i;
+        if (f.collect) finish_records(f, j, f.p.L[j]);
         const uint32_t flags = __float_as_uint(f.p.nf[j].w);
         if (!(flags & PF_TERMINATED)) {
             const float4 Lv = f.p.L[j], la = f.p.lam[j], lp = f.p.lpdf[j];
@@ -602,6 +810,7 @@ size_t trace_spill_words(size_t lanes) { return lanes * (size_t)(kMaxStack - kSt
 
 hipError_t launch_pass(rbh_ctx c, Frame1 f, uint32_t depthMax, bool firstPass, int traceGrid, int shadeGrid,
                        KernelTimer* kt) {
+    const int guideGrid = traceGrid / 8 > 0 ? traceGrid / 8 : 1;
     const int nb = (int)((f.NP + 255) / 256);
     const int nbPix = (int)((f.P + 255) / 256);
     hipError_t e = hipSuccess;
@@ -616,11 +825,21 @@ hipError_t launch_pass(rbh_ctx c, Frame1 f, uint32_t depthMax, bool firstPass, i
     for (uint32_t b = 0; b < depthMax; ++b) {
         const uint32_t qsel = b & 1u;
         f.bounce = b;
+        if (b > 0) hipLaunchKernelGGL(k_reset_hits, dim3(1), dim3(64), 0, c->stream, f.counters);
         RBH_TIMED(KernelTimer::TRACE,
                   hipLaunchKernelGGL(k_trace, dim3(traceGrid), dim3(kTraceBlock), 0, c->stream, f, qsel));
-        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, c->stream, f.counters, qsel ? CNT_QUEUE0 : CNT_QUEUE1);
         RBH_TIMED(KernelTimer::SHADE,
-                  hipLaunchKernelGGL(k_shade, dim3(shadeGrid), dim3(kShadeBlock), 0, c->stream, f, qsel));
+                  hipLaunchKernelGGL(k_miss, dim3(shadeGrid), dim3(256), 0, c->stream, f));
+        if (f.guided)
+            RBH_TIMED(KernelTimer::GUIDE,
+                      hipLaunchKernelGGL(k_guide_paths, dim3(guideGrid), dim3(256), 0, c->stream, f));
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, c->stream, f.counters, qsel ? CNT_QUEUE0 : CNT_QUEUE1);
+        if (f.guided)
+            RBH_TIMED(KernelTimer::SHADE,
+                      hipLaunchKernelGGL(k_shade<true>, dim3(shadeGrid), dim3(kShadeBlock), 0, c->stream, f, qsel));
+        else
+            RBH_TIMED(KernelTimer::SHADE,
+                      hipLaunchKernelGGL(k_shade<false>, dim3(shadeGrid), dim3(kShadeBlock), 0, c->stream, f, qsel));
         RBH_TIMED(KernelTimer::SHADOW,
                   hipLaunchKernelGGL(k_shadow, dim3(traceGrid), dim3(kTraceBlock), 0, c->stream, f));
     }

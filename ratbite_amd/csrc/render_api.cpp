@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "context.hpp"
+#include "guide.hpp"
 #include "integrator.hpp"
 
 namespace rbh {
@@ -39,6 +40,9 @@ size_t layout_paths(char* base, size_t P, PathBuffers* pb, uint32_t** counters, 
     pb->depth = reinterpret_cast<uint32_t*>(take(P * 4));
     pb->q0 = reinterpret_cast<uint32_t*>(take(P * 4));
     pb->q1 = reinterpret_cast<uint32_t*>(take(P * 4));
+    pb->hq = reinterpret_cast<uint32_t*>(take(P * 4));
+    pb->mq = reinterpret_cast<uint32_t*>(take(P * 4));
+    pb->prec = reinterpret_cast<uint32_t*>(take(P * 4));
     pb->sq = reinterpret_cast<uint32_t*>(take(P * 4));
     *counters = reinterpret_cast<uint32_t*>(take(CNT_COUNT * 4));
     *stats = reinterpret_cast<unsigned long long*>(take(STAT_COUNT * 8));
@@ -128,8 +132,14 @@ int rbh_render(rbh_ctx c, const rbh_launch_params* lp) {
     if (lp->width == 0 || lp->height == 0) return fail(c, RBH_ERR_INVALID, "empty film");
     if ((uint64_t)lp->width * lp->height > (1ull << 28)) return fail(c, RBH_ERR_RANGE, "film above 2^28 pixels");
     if (lp->maxPathDepth == 0 || lp->maxPathDepth > 1024) return fail(c, RBH_ERR_INVALID, "maxPathDepth must be in [1, 1024]");
-    if (lp->flags & ~(uint32_t)RBH_RENDER_TIMING)
+    const uint32_t known = RBH_RENDER_GUIDED | RBH_RENDER_COLLECT | RBH_RENDER_TIMING | RBH_RENDER_NO_DROP;
+    if (lp->flags & ~known)
         return fail(c, RBH_ERR_INVALID, "render flags 0x%x are not supported by this entry point", lp->flags);
+    const bool guided = (lp->flags & RBH_RENDER_GUIDED) != 0;
+    const bool collect = (lp->flags & RBH_RENDER_COLLECT) != 0;
+    if (collect && !guided) return fail(c, RBH_ERR_INVALID, "RBH_RENDER_COLLECT needs RBH_RENDER_GUIDED");
+    GuideState* gst = static_cast<GuideState*>(c->guide);
+    if (guided && !(gst && gst->configured)) return fail(c, RBH_ERR_STATE, "guided render before rbh_guide_configure");
     const bool treeOn = c->tree.nlights > 0;
     if (c->geom.maxLight >= (int64_t)c->tree.nlights)
         return fail(c, RBH_ERR_STATE, "triangle light index %lld has no light record (%u lights built)",
@@ -198,6 +208,31 @@ int rbh_render(rbh_ctx c, const rbh_launch_params* lp) {
         f.envIntegral = c->env.integral;
     }
     f.p = pb;
+    f.noDrop = (guided || (lp->flags & RBH_RENDER_NO_DROP)) ? 1u : 0u;
+    if (guided) {
+        RBH_ALLOC(c, rs.gfeat, NP * 16 * sizeof(float));
+        RBH_ALLOC(c, rs.gmix, NP * 40 * sizeof(float));
+        f.guided = 1u;
+        f.gwf = static_cast<const __bf16*>(gst->wf);
+        f.gbias = static_cast<const float*>(gst->bias);
+        f.gfeat = rs.gfeat.as<float>();
+        f.gmix = rs.gmix.as<float>();
+        for (int a = 0; a < 3; ++a) {
+            f.gmin[a] = gst->sceneMin[a];
+            f.ginv[a] = gst->sceneInvExtent[a];
+        }
+        if (collect) {
+            f.collect = 1u;
+            f.grec = static_cast<float*>(gst->records);
+            f.gtrain = static_cast<float*>(gst->trainRecs);
+            f.grecCount = static_cast<uint32_t*>(gst->recCount);
+            f.grecCap = gst->recCapacity;
+            // keep about as many vertices as the buffer holds
+            const double frac = std::min(1.0, (double)gst->recCapacity / ((double)P * std::max(1u, lp->sampleCount)));
+            f.recThreshold = frac >= 1.0 ? 0xFFFFFFFFu : (uint32_t)(frac * 4294967295.0);
+            RBH_HIP(c, hipMemsetAsync(gst->recCount, 0, 4, c->stream));
+        }
+    }
     f.counters = counters;
     f.stats = stats;
     f.accum = rs.accum.as<double4>();
@@ -235,7 +270,7 @@ int rbh_render(rbh_ctx c, const rbh_launch_params* lp) {
     c->stats.pathVertices = st[STAT_VERTICES];
     c->stats.samples = (uint64_t)P * lp->sampleCount;
     c->stats.traceLaunches = passes * f.maxDepth;
-    c->stats.traceMs = c->stats.shadeMs = c->stats.shadowMs = 0.0;
+    c->stats.traceMs = c->stats.shadeMs = c->stats.shadowMs = c->stats.guideMs = 0.0;
     if (kt) {
         double kms[KernelTimer::NCLASS];
         uint32_t kn[KernelTimer::NCLASS];
@@ -243,6 +278,7 @@ int rbh_render(rbh_ctx c, const rbh_launch_params* lp) {
         c->stats.traceMs = kms[KernelTimer::TRACE];
         c->stats.shadeMs = kms[KernelTimer::SHADE];
         c->stats.shadowMs = kms[KernelTimer::SHADOW];
+        c->stats.guideMs = kms[KernelTimer::GUIDE];
     }
     return RBH_OK;
 }

@@ -7,8 +7,11 @@ tree (buildLightTree, 1065-1186) and the env-map tables (loadEnvironmentMap,
 1339-1432); then render sub-launches into the FP64 accumulation buffer and
 resolve (renderResolve, 1512-1533).
 """
+import ctypes
+
 import numpy as np
 
+from . import _lib as L
 from .context import Context
 
 
@@ -35,13 +38,44 @@ class Renderer:
     def light_tree(self):
         return self.ctx.get_light_tree(len(self.scene.lights))
 
-    def render(self, spp=None, offset=None, **kw):
+    def render(self, spp=None, offset=None, guided=False, collect=False, no_drop=False, flags=0, **kw):
         spp = self.scene.spp if spp is None else spp
         offset = self.samples if offset is None else offset
-        p = self.scene.launch_params(offset=offset, count=spp, **kw)
+        flags |= (L.RBH_RENDER_GUIDED if guided else 0) | (L.RBH_RENDER_COLLECT if collect else 0)
+        flags |= L.RBH_RENDER_NO_DROP if no_drop else 0
+        p = self.scene.launch_params(offset=offset, count=spp, flags=flags, **kw)
         self.ctx.render(p)
         self.samples = offset + spp
         return self.ctx.stats()
+
+    # ------------------------------------------------------------ guiding
+    def configure_guide(self, seed=1, lr=1e-3, capacity=1 << 22):
+        """Sets up the guiding network (random initialisation) with the
+        scene's bounding box as the position normalisation."""
+        v = np.asarray(self.scene.verts, np.float64)
+        lo, hi = v.min(axis=0), v.max(axis=0)
+        pad = 1e-3 * max(1e-6, float((hi - lo).max()))
+        cfg = L.GuideConfig()
+        cfg.lr, cfg.beta1, cfg.beta2, cfg.eps = lr, 0.9, 0.999, 1e-8
+        cfg.sceneMin[:] = [float(x) for x in lo - pad]
+        cfg.sceneMax[:] = [float(x) for x in hi + pad]
+        cfg.seed = seed
+        cfg.recordCapacity = capacity
+        self.ctx.call("rbh_guide_configure", ctypes.byref(cfg))
+
+    def train_guide(self, batch=1 << 18, steps=1):
+        loss = ctypes.c_float()
+        self.ctx.call("rbh_guide_train_collected", batch, steps, ctypes.byref(loss))
+        return loss.value
+
+    def guide_records(self, max_records=None):
+        n = ctypes.c_uint32()
+        self.ctx.call("rbh_guide_records", None, 0, ctypes.byref(n))
+        k = n.value if max_records is None else min(n.value, max_records)
+        out = np.zeros((k, L.GUIDE_REC), np.float32)
+        if k:
+            self.ctx.call("rbh_guide_records", L.ptr(out), k, ctypes.byref(n))
+        return out
 
     def accum(self):
         return self.ctx.read_accum()

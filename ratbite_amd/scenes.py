@@ -340,7 +340,10 @@ def _finish(parts, mats, spectra, tex_list, cam, env, width, height, depth, spp,
 
 
 # ----------------------------------------------------------------- configs
-def cornell(seed=0x5EED0001, width=256, height=256, spp=16, depth=8):
+def cornell(seed=0x5EED0001, width=256, height=256, spp=16, depth=8, light_up=False):
+    """Seeded Cornell box. With light_up the emitter faces the ceiling from
+    just below it, so every other surface is lit only indirectly (a scene
+    where BSDF sampling rarely finds the light's reflection)."""
     rng = np.random.default_rng(seed)
     tables = load_tables()
     mats, parts = [], []
@@ -363,8 +366,13 @@ def cornell(seed=0x5EED0001, width=256, height=256, spp=16, depth=8):
         mats.append(gd())
         parts.append((v, i, n, uv, len(mats) - 1))
     # emissive quad just below the ceiling, facing down, blackbody 4000 K
-    v, i, n, uv = quad((-0.3, 1.98, -0.3), (0.3, 1.98, -0.3), (0.3, 1.98, 0.3), (-0.3, 1.98, 0.3))
-    mats.append(material(base=(0.8, 0.8, 0.8), metal=0.0, rough=1.0, spectral_emission=(0, 12.0)))
+    if light_up:
+        v, i, n, uv = quad((-0.4, 1.9, -0.4), (-0.4, 1.9, 0.4), (0.4, 1.9, 0.4), (0.4, 1.9, -0.4))
+        power = 60.0
+    else:
+        v, i, n, uv = quad((-0.3, 1.98, -0.3), (0.3, 1.98, -0.3), (0.3, 1.98, 0.3), (-0.3, 1.98, 0.3))
+        power = 12.0
+    mats.append(material(base=(0.8, 0.8, 0.8), metal=0.0, rough=1.0, spectral_emission=(0, power)))
     mats[-1]["factors"] |= FC_EMISSION
     parts.append((v, i, n, uv, len(mats) - 1))
     for _ in range(8):

@@ -22,9 +22,10 @@ def _upload(ctx, verts, idx):
     return ctx.stats()
 
 
-def _trace(ctx, rays, anyhit):
+def _trace(ctx, rays, anyhit, tiny_stack=False):
     hits = np.zeros((len(rays), 4), np.float32)
-    ctx.check(L.lib().rbh_test_trace(ctx.h, L.ptr(rays), L.ptr(hits), len(rays), int(anyhit)))
+    mode = int(anyhit) | (2 if tiny_stack else 0)
+    ctx.check(L.lib().rbh_test_trace(ctx.h, L.ptr(rays), L.ptr(hits), len(rays), mode))
     return hits
 
 
@@ -67,6 +68,21 @@ def test_triangle_soup_closest_and_any(ctx, ntri):
     ga = _trace(ctx, rays, True)
     ba = _brute(v, idx, rays, False)
     assert np.array_equal(ga[:, 1].view(np.uint32) == 0xFFFFFFFF, ba[:, 1].view(np.uint32) == 0xFFFFFFFF)
+
+
+def test_stack_spill_gives_identical_hits(ctx):
+    # the same traversal with a 2-entry LDS stack pushes almost every deferred
+    # subtree through the global spill columns; results must not change
+    sc = scenes.spheres(n_spheres=30, level=4, n_lights=500, width=64, height=64, spp=1, env_size=None)
+    _upload(ctx, sc.verts, sc.indices)
+    rng = np.random.default_rng(12)
+    rays = _rays(rng, 30000, [-40, -5, -100], [40, 20, -20])
+    for anyhit in (False, True):
+        if anyhit:
+            rays[:, 7] = rng.uniform(1, 60, len(rays))
+        a = _trace(ctx, rays, anyhit)
+        b = _trace(ctx, rays, anyhit, tiny_stack=True)
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
 def test_watertight_shared_edges(ctx):
