@@ -208,6 +208,28 @@ int rbh_render(rbh_ctx ctx, const rbh_launch_params* p);
  * (default 2^25). Results do not depend on it. No reference counterpart: the
  * OptiX megakernel keeps each path's state in registers. */
 int rbh_set_path_slots(rbh_ctx ctx, uint64_t slots);
+/* Many scenes in one wavefront launch sequence (no reference counterpart;
+ * the reference renders one scene per optixLaunch). Each scene is a context
+ * prepared with the upload/build calls above, all on owner's device; lps[i]
+ * are its launch parameters (flags: RBH_RENDER_TIMING, RBH_RENDER_NO_DROP).
+ * Every pass holds the next samples of every unfinished scene; each scene
+ * accumulates into its own context. completionMs (optional, n entries)
+ * receives the time from the start of the call to the end of the pass in
+ * which each scene finished. Statistics go to owner. */
+int rbh_render_batch(rbh_ctx owner, const rbh_ctx* scenes, const rbh_launch_params* lps, uint32_t n,
+                     double* completionMs);
+/* Launch shape of the wavefront passes (no reference counterpart). */
+typedef struct rbh_tuning {
+    uint64_t pathSlots;           /* slots per pass (see rbh_set_path_slots) */
+    int32_t traceBlocksPerCU;     /* persistent grid of the trace / shadow kernels */
+    int32_t shadeBlocksPerCU;     /* persistent grid of the shade / miss kernels */
+    double samplesPerSecond;      /* measured by rbh_autotune */
+} rbh_tuning;
+/* Times trialSpp-sample renders of lp over a grid of slot budgets and grid
+ * sizes, keeps the fastest on the context and returns it. Overwrites the
+ * accumulation buffer. */
+int rbh_autotune(rbh_ctx ctx, const rbh_launch_params* lp, uint32_t trialSpp, rbh_tuning* best);
+int rbh_set_tuning(rbh_ctx ctx, const rbh_tuning* t);
 /* Copy the FP64 (x, y, z, w) accumulation buffer (W*H*4 doubles). */
 int rbh_read_accum(rbh_ctx ctx, double* rgbw);
 int rbh_write_accum(rbh_ctx ctx, const double* rgbw, uint32_t width, uint32_t height);
@@ -243,6 +265,9 @@ int rbh_guide_gradient(rbh_ctx ctx, const float* records, uint32_t n, float* gra
 int rbh_guide_train(rbh_ctx ctx, const float* records, uint32_t n, uint32_t batch, uint32_t steps, float* lastLoss);
 /* raw network outputs (n x 40: per lobe mean xyz, log sharpness, logit) */
 int rbh_guide_infer(rbh_ctx ctx, const float* records, uint32_t n, float* out);
+/* the same with FP8 E4M3 matrix products: per-layer weight scales and
+ * per-layer, per-wave activation scales */
+int rbh_guide_infer_fp8(rbh_ctx ctx, const float* records, uint32_t n, float* out);
 /* Training records gathered by the last RBH_RENDER_COLLECT render (count in
  * *n; up to max copied when out is not null). */
 int rbh_guide_records(rbh_ctx ctx, float* out, uint32_t max, uint32_t* n);

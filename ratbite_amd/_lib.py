@@ -83,6 +83,11 @@ class Stats(ctypes.Structure):
     ]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("pathSlots", c_uint64), ("traceBlocksPerCU", ctypes.c_int32), ("shadeBlocksPerCU", ctypes.c_int32),
+                ("samplesPerSecond", c_double)]
+
+
 class GuideConfig(ctypes.Structure):
     _fields_ = [
         ("lr", c_float), ("beta1", c_float), ("beta2", c_float), ("eps", c_float),
@@ -117,12 +122,16 @@ _SIGS = {
     "rbh_envmap_disable": (c_int, [c_void_p]),
     "rbh_render": (c_int, [c_void_p, ctypes.POINTER(LaunchParams)]),
     "rbh_set_path_slots": (c_int, [c_void_p, c_uint64]),
+    "rbh_render_batch": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_void_p]),
+    "rbh_autotune": (c_int, [c_void_p, ctypes.POINTER(LaunchParams), c_uint32, ctypes.POINTER(Tuning)]),
+    "rbh_set_tuning": (c_int, [c_void_p, ctypes.POINTER(Tuning)]),
     "rbh_guide_configure": (c_int, [c_void_p, ctypes.POINTER(GuideConfig)]),
     "rbh_guide_set_params": (c_int, [c_void_p, c_void_p, c_uint32]),
     "rbh_guide_get_params": (c_int, [c_void_p, c_void_p, c_uint32]),
     "rbh_guide_gradient": (c_int, [c_void_p, c_void_p, c_uint32, c_void_p, ctypes.POINTER(ctypes.c_float)]),
     "rbh_guide_train": (c_int, [c_void_p, c_void_p, c_uint32, c_uint32, c_uint32, ctypes.POINTER(ctypes.c_float)]),
     "rbh_guide_infer": (c_int, [c_void_p, c_void_p, c_uint32, c_void_p]),
+    "rbh_guide_infer_fp8": (c_int, [c_void_p, c_void_p, c_uint32, c_void_p]),
     "rbh_guide_records": (c_int, [c_void_p, c_void_p, c_uint32, ctypes.POINTER(c_uint32)]),
     "rbh_guide_train_collected": (c_int, [c_void_p, c_uint32, c_uint32, ctypes.POINTER(ctypes.c_float)]),
     "rbh_read_accum": (c_int, [c_void_p, c_void_p]),

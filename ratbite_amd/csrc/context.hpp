@@ -123,6 +123,9 @@ struct RenderState {
     // path slots one pass may hold (samples per pass = slotBudget / pixels);
     // about 200 B of HBM per slot
     size_t slotBudget = size_t(1) << 25;
+    int traceBlocksPerCU = 16;   // persistent-grid sizes (autotuned)
+    int shadeBlocksPerCU = 16;
+    DevBuf sceneTable;           // batched passes: SceneView per scene and pass
     KernelTimer timer;
 };
 

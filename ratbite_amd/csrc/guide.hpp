@@ -58,9 +58,8 @@ struct GuideState {
     void* trainRecs = nullptr;         // finished training records (float[cap][kRec])
     void* recCount = nullptr;          // uint32 counter
     uint32_t recCapacity = 0;
-    void* fp8w = nullptr;              // FP8 E4M3 forward weights
-    float fp8WScale[guide::kLayers] = {};
-    float fp8AScale[guide::kLayers] = {};
+    void* fp8w = nullptr;              // FP8 E4M3 forward weights (kWfTotal bytes, forward layout)
+    void* fp8s = nullptr;              // float[kLayers] weight scales
 };
 
 void guide_release(GuideState& g);

@@ -34,12 +34,13 @@ struct PathBuffers {
     uint32_t* prec; // guided renders that collect training data: last record of the path
 };
 
-struct Frame1 {
-    // pass: S consecutive samples of all P = W*H pixels, NP = S*P path slots;
-    // slot i holds sample (sample + i / P) of pixel i % P
-    uint32_t P, W, H, sample, S, NP;
+// Everything one scene contributes to a pass: S consecutive samples of its
+// P = W*H pixels occupy path slots [slot0, slot0 + S*P), slot slot0 + k*P + i
+// holding sample (sample + k) of pixel i.
+struct SceneView {
+    uint32_t P, W, H, sample, S, slot0;
     uint32_t maxDepth, maxRefl, maxTrans;
-    uint32_t bounce;
+    uint32_t first;          // the pass starts a fresh accumulation
     float invW, invH, psW, psH;
     dev::V3 camU, camV, camW;
     uint32_t dof;
@@ -71,18 +72,34 @@ struct Frame1 {
     const uint16_t* envCond;
     const uint16_t* envMarg;
     float envIntegral;
+    double4* accum;
+};
+
+// Persistent-grid sizes of the trace/shadow and shade/miss kernels (blocks)
+struct LaunchShape {
+    int traceGrid, shadeGrid;
+};
+
+struct Frame1 {
+    SceneView sc;                 // single-scene passes
+    const SceneView* scenes;      // batched passes: views in slot order
+    uint32_t nscenes;
+    uint32_t NP;                  // path slots of the pass (all scenes)
+    uint32_t maxP;                // largest pixel count of a scene in the pass
+    uint32_t bounce;
     // state
     PathBuffers p;
     int* spill;              // traversal-stack spill columns of the trace/shadow grid
     uint32_t spillStride;    // lanes of the trace/shadow grid
     uint32_t* counters;
     unsigned long long* stats;
-    double4* accum;
     // path guiding (guided renders only)
     uint32_t guided;          // one-sample MIS with the network at ComplexSurface vertices
     uint32_t noDrop;          // an invalid BxDF sample ends the path instead of discarding the sample
     const __bf16* gwf;        // network weights (BF16, forward layout) and FP32 biases
     const float* gbias;
+    const uint8_t* gwq;       // FP8 E4M3 weights and per-layer scales (nullptr: BF16 inference)
+    const float* gws;
     float* gfeat;             // per hit-queue entry: 16-float input record
     float* gmix;              // per hit-queue entry: 40-float mixture (world space)
     float gmin[3], ginv[3];   // position normalisation

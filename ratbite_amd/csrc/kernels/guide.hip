@@ -164,6 +164,7 @@ __global__ __launch_bounds__(256) void k_guide_train(const float* __restrict__ r
         __syncthreads();   // the previous tile's gradient reads are done
         encode_rows(L.x0, recs, n, base, row0);
         forward_rows(L, wf, bias, row0);
+        __syncthreads();
         // output gradient of this wave's rows (lanes 0..15, one record each)
         if (g == 0) {
             const uint32_t s = base + row0 + r;
@@ -276,21 +277,50 @@ __global__ __launch_bounds__(256) void k_guide_publish(const float* __restrict__
 }
 
 __global__ __launch_bounds__(256) void k_guide_infer(const float* __restrict__ recs, uint32_t n,
-                                                     const bf16* __restrict__ wf, const float* __restrict__ bias,
+                                                     const bf16* __restrict__ wf, const uint8_t* __restrict__ wq,
+                                                     const float* __restrict__ wqs, const float* __restrict__ bias,
                                                      float* __restrict__ out) {
+    // wq != nullptr: FP8 E4M3 weights with per-layer scales wqs
     __shared__ InferLds L;
-    const int wave = threadIdx.x >> 6, lane = __lane_id(), r = lane & 15, g = lane >> 4;
+    const int wave = threadIdx.x >> 6, lane = __lane_id();
     const int row0 = wave * kRows;
     for (uint32_t tile = blockIdx.x; (size_t)tile * kTile < n; tile += gridDim.x) {
         const uint32_t base = tile * kTile;
         encode_rows(L.x0, recs, n, base, row0);
-        forward_rows(L, wf, bias, row0);
-        // each wave copies its own rows out (no cross-wave dependency)
+        if (wq) forward_rows_fp8(L, wq, wqs, bias, row0);
+        else forward_rows(L, wf, bias, row0);
+        __syncthreads();
         for (int q = lane; q < kRows * kOut; q += 64) {
             const int rr = q / kOut, c = q % kOut;
             const uint32_t s = base + row0 + rr;
             if (s < n) out[(size_t)s * kOut + c] = L.out[(row0 + rr) * kSO + c];
         }
+        __syncthreads();
+    }
+}
+
+// FP8 copy of the weights: one block per layer finds the largest magnitude,
+// then writes E4M3 bytes of W / scale into the [n_pad][k_pad] layout.
+__global__ __launch_bounds__(256) void k_fp8_quant(const float* __restrict__ master, uint8_t* __restrict__ wq,
+                                                   float* __restrict__ scales) {
+    __shared__ float red[4];
+    const int l = blockIdx.x;
+    const float* W = master + w_off(l);
+    const int nw = fan_in(l) * fan_out(l);
+    float m = 0.0f;
+    for (int i = threadIdx.x; i < nw; i += blockDim.x) m = fmaxf(m, fabsf(W[i]));
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if (__lane_id() == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    const float amax = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    const float sc = amax > 0.0f ? amax / kFp8Max : 1.0f;
+    if (threadIdx.x == 0) scales[l] = sc;
+    const float inv = 1.0f / sc;
+    uint8_t* q = wq + wf_off(l);
+    for (int i = threadIdx.x; i < n_pad(l) * k_pad(l); i += blockDim.x) {
+        const int out = i / k_pad(l), in = i % k_pad(l);
+        const float v = (out < fan_out(l) && in < fan_in(l)) ? W[out * fan_in(l) + in] * inv : 0.0f;
+        q[i] = (uint8_t)(__builtin_amdgcn_cvt_pk_fp8_f32(v, 0.0f, 0, false) & 0xFF);
     }
 }
 
@@ -306,7 +336,7 @@ template <typename T> T* dptr(void* p) { return static_cast<T*>(p); }
 
 // ------------------------------------------------------------------ host
 void guide_release(GuideState& g) {
-    void* ptrs[] = {g.master, g.adamM, g.adamV, g.wf, g.wt, g.bias, g.partials, g.scalars, g.records, g.trainRecs, g.recCount, g.fp8w};
+    void* ptrs[] = {g.master, g.adamM, g.adamV, g.wf, g.wt, g.bias, g.partials, g.scalars, g.records, g.trainRecs, g.recCount, g.fp8w, g.fp8s};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     g = GuideState{};
@@ -352,6 +382,8 @@ int upload_master(rbh_ctx c, const float* params) {
     RBH_HIP(c, hipMemsetAsync(g.adamV, 0, kParams * 4, c->stream));
     hipLaunchKernelGGL(k_guide_publish, dim3((kParams + 255) / 256), dim3(256), 0, c->stream, dptr<float>(g.master),
                        dptr<bf16>(g.wf), dptr<bf16>(g.wt), dptr<float>(g.bias));
+    hipLaunchKernelGGL(k_fp8_quant, dim3(kLayers), dim3(256), 0, c->stream, dptr<float>(g.master), dptr<uint8_t>(g.fp8w),
+                       dptr<float>(g.fp8s));
     RBH_HIP(c, hipGetLastError());
     RBH_HIP(c, hipStreamSynchronize(c->stream));
     g.step = 0;
@@ -381,6 +413,9 @@ int run_train(rbh_ctx c, const float* drecs, uint32_t n, int mode, float* gradOu
                        blocks, mode, dptr<float>(g.master), dptr<float>(g.adamM), dptr<float>(g.adamV), g.lr, g.beta1,
                        g.beta2, g.eps, bc1, bc2, dptr<bf16>(g.wf), dptr<bf16>(g.wt), dptr<float>(g.bias),
                        gradOut);
+    if (mode == 0)
+        hipLaunchKernelGGL(k_fp8_quant, dim3(kLayers), dim3(256), 0, c->stream, dptr<float>(g.master),
+                           dptr<uint8_t>(g.fp8w), dptr<float>(g.fp8s));
     RBH_HIP(c, hipGetLastError());
     if (lossOut) {
         float tmp[4];
@@ -409,6 +444,9 @@ int rbh_guide_configure(rbh_ctx c, const rbh_guide_config* cfg) {
     RBH_HIP(c, alloc(&g.wt, kWfTotal * 2));
     RBH_HIP(c, alloc(&g.bias, kBiasPad * 4));
     RBH_HIP(c, alloc(&g.scalars, 16));
+    RBH_HIP(c, alloc(&g.fp8w, kWfTotal));
+    RBH_HIP(c, alloc(&g.fp8s, 64));
+    RBH_HIP(c, hipMemsetAsync(g.fp8w, 0, kWfTotal, c->stream));
     RBH_HIP(c, hipMemsetAsync(g.wf, 0, kWfTotal * 2, c->stream));
     RBH_HIP(c, hipMemsetAsync(g.wt, 0, kWfTotal * 2, c->stream));
     RBH_HIP(c, hipMemsetAsync(g.bias, 0, kBiasPad * 4, c->stream));
@@ -533,7 +571,7 @@ int rbh_guide_train_collected(rbh_ctx c, uint32_t batch, uint32_t steps, float* 
     return RBH_OK;
 }
 
-int rbh_guide_infer(rbh_ctx c, const float* recs, uint32_t n, float* out) {
+static int infer(rbh_ctx c, const float* recs, uint32_t n, float* out, bool fp8) {
     if (!c || !recs || !out) return RBH_ERR_INVALID;
     if (!gs(c).configured) return fail(c, RBH_ERR_STATE, "rbh_guide_configure first");
     if (n == 0) return RBH_OK;
@@ -544,11 +582,15 @@ int rbh_guide_infer(rbh_ctx c, const float* recs, uint32_t n, float* out) {
     RBH_ALLOC(c, o, (size_t)n * kOut * 4);
     const int blocks = (int)std::min<size_t>((size_t)train_blocks(c) * 2, (n + kTile - 1) / kTile);
     hipLaunchKernelGGL(k_guide_infer, dim3(blocks), dim3(256), 0, c->stream, d.as<float>(), n, dptr<bf16>(g.wf),
-                       dptr<float>(g.bias), o.as<float>());
+                       fp8 ? dptr<uint8_t>(g.fp8w) : nullptr, dptr<float>(g.fp8s), dptr<float>(g.bias), o.as<float>());
     RBH_HIP(c, hipGetLastError());
     RBH_HIP(c, hipStreamSynchronize(c->stream));
     RBH_HIP(c, hipMemcpy(out, o.p, (size_t)n * kOut * 4, hipMemcpyDeviceToHost));
     return RBH_OK;
 }
+
+int rbh_guide_infer(rbh_ctx c, const float* recs, uint32_t n, float* out) { return infer(c, recs, n, out, false); }
+
+int rbh_guide_infer_fp8(rbh_ctx c, const float* recs, uint32_t n, float* out) { return infer(c, recs, n, out, true); }
 
 }  // extern "C"

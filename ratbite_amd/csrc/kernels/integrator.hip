@@ -57,33 +57,50 @@ __device__ __forceinline__ void wave_count(unsigned long long* __restrict__ coun
     if (__lane_id() == 0 && s) atomicAdd(counter, (unsigned long long)s);
 }
 
+// The scene owning a path slot: the pass's only scene, or (batched passes)
+// the last view whose slot range starts at or before the slot.
+template <bool BATCH>
+__device__ __forceinline__ const SceneView& scene_of(const Frame1& f, uint32_t slot) {
+    if (!BATCH) return f.sc;
+    uint32_t lo = 0, hi = f.nscenes;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (f.scenes[mid].slot0 <= slot) lo = mid;
+        else hi = mid;
+    }
+    return f.scenes[lo];
+}
+
 // ------------------------------------------------------------------ camera
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_camera(Frame1 f) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= f.NP) return;
-    const uint32_t k = i / f.P, pix = i - k * f.P;
-    const uint32_t x = pix % f.W, y = pix / f.W;
-    const Qrng q = qrng_at(f.sample + k, 0, pixel_hash(x, y));
+    const SceneView& S = scene_of<BATCH>(f, i);
+    const uint32_t local = i - S.slot0;
+    const uint32_t k = local / S.P, pix = local - k * S.P;
+    const uint32_t x = pix % S.W, y = pix / S.W;
+    const Qrng q = qrng_at(S.sample + k, 0, pixel_hash(x, y));
     const Wl wl = sample_wavelengths(q.s1(QD_WAVELENGTH));
     const V2 fo = gaussian_offset(q.s2(QD_FILTER));
     const float cx = (float)x + fo.x, cy = (float)y + fo.y;
     V3 o{0.0f, 0.0f, 0.0f}, d;
-    if (f.dof) {
+    if (S.dof) {
         // src/device/dir_gen.h:358-374
         const V2 ls = q.s2(QD_LENS);
-        const float xs = (2.0f * cx * f.invW - 1.0f) * f.psW;
-        const float zs = (2.0f * cy * f.invH - 1.0f) * f.psH;
-        const V3 focus = V3{xs, zs, 1.0f} * f.focus;
+        const float xs = (2.0f * cx * S.invW - 1.0f) * S.psW;
+        const float zs = (2.0f * cy * S.invH - 1.0f) * S.psH;
+        const V3 focus = V3{xs, zs, 1.0f} * S.focus;
         const V2 dk = disk_polar(ls);
-        const V3 lens{dk.x * f.aperture, dk.y * f.aperture, 0.0f};
-        o = f.camU * lens.x + f.camW * lens.y;
+        const V3 lens{dk.x * S.aperture, dk.y * S.aperture, 0.0f};
+        o = S.camU * lens.x + S.camW * lens.y;
         const V3 rd = nrm(focus - lens);
-        d = f.camU * rd.x + f.camW * rd.y + f.camV * rd.z;
+        d = S.camU * rd.x + S.camW * rd.y + S.camV * rd.z;
     } else {
         // src/device/dir_gen.h:346-357
-        const float xs = 2.0f * cx * f.invW - 1.0f;
-        const float zs = 2.0f * cy * f.invH - 1.0f;
-        d = nrm(f.camV + f.camU * xs * f.psW + f.camW * zs * f.psH);
+        const float xs = 2.0f * cx * S.invW - 1.0f;
+        const float zs = 2.0f * cy * S.invH - 1.0f;
+        d = nrm(S.camV + S.camU * xs * S.psW + S.camW * zs * S.psH);
     }
     PathBuffers& p = f.p;
     p.od[i] = make_float4(o.x, o.y, o.z, 1.0f);           // bxdfPdf = 1
@@ -132,28 +149,28 @@ __device__ __forceinline__ void snapshot_radiance(const Frame1& f, uint32_t pid,
 // A path whose ray left the scene: sky emission with the MIS weight of the
 // previous vertex's BxDF sample, then the path ends (pathtrace.cu:1185-1274,
 // the sky branch; 1240-1252 for the lookup and its pdf).
-__device__ __forceinline__ void shade_miss(const Frame1& f, uint32_t pid) {
+__device__ __forceinline__ void shade_miss(const Frame1& f, const SceneView& S, uint32_t pid) {
     float4 nf = f.p.nf[pid];
     uint32_t flags = __float_as_uint(nf.w);
     flags &= ~PF_TRIANGULAR;
     if (f.collect) snapshot_radiance(f, pid, f.p.L[pid]);
-    if (f.envOn) {
+    if (S.envOn) {
         const uint32_t depth = f.p.depth[pid];
         const float4 od = f.p.od[pid], dr = f.p.dr[pid];
         const V3 rd{dr.x, dr.y, dr.z};
-        const LightTree tree{f.treeNodes, f.treePointers, f.treeMasks, true};
+        const LightTree tree{S.treeNodes, S.treePointers, S.treeMasks, true};
         const bool prevNEE = depth != 0 && !(flags & PF_PREV_SPECULAR);
         const float treePdf = prevNEE ? tree.pdf(LT_SKY, 0, V3{od.x, od.y, od.z}, V3{nf.x, nf.y, nf.z}) : 1.0f;
         const float phi = atan2f(rd.y, rd.x);
         const float theta = acosf(rd.z);
-        const V3 sky = env_lookup(f.envTex, f.envW, f.envH, f.envC, 0.5f - phi / (2.0f * 3.14159265358979f),
+        const V3 sky = env_lookup(S.envTex, S.envW, S.envH, S.envC, 0.5f - phi / (2.0f * 3.14159265358979f),
                                   theta / 3.14159265358979f);
         float lightPdf = (1.0f / (4.0f * 3.14159265358979f)) * ((sky.x + sky.y + sky.z) / 3.0f) * sinf(theta) /
-                         f.envIntegral;
+                         S.envIntegral;
         const float4 la = f.p.lam[pid];
         Wl wl;
         wl.lambda = S4{{la.x, la.y, la.z, la.w}};
-        const S4 Le = ColorTables{f.colors}.rgb(sky, wl);
+        const S4 Le = ColorTables{S.colors}.rgb(sky, wl);
         lightPdf *= treePdf;
         const float w = prevNEE ? power_heuristic(od.w, lightPdf) : 1.0f;
         const float4 Tv = f.p.T[pid];
@@ -175,6 +192,7 @@ __device__ __forceinline__ TravStack<kStackLds> lane_stack(const Frame1& f, int*
     return TravStack<kStackLds>{lds + threadIdx.x, (int)blockDim.x, f.spill + t, f.spillStride};
 }
 
+template <bool BATCH>
 __global__ __launch_bounds__(kTraceBlock) void k_trace(Frame1 f, uint32_t qsel) {
     __shared__ int stack[kStackLds * kTraceBlock];
     const TravStack<kStackLds> st = lane_stack(f, stack);
@@ -187,8 +205,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(Frame1 f, uint32_t qsel) 
         uint32_t pid = 0;
         if (i < count) {
             pid = identity ? i : queue[i];
+            const SceneView& S = scene_of<BATCH>(f, pid);
             const float4 od = f.p.od[pid], dr = f.p.dr[pid];
-            const Hit h = traverse<false>(f.nodes, f.tris, f.nnodes, V3{od.x, od.y, od.z}, V3{dr.x, dr.y, dr.z},
+            const Hit h = traverse<false>(S.nodes, S.tris, S.nnodes, V3{od.x, od.y, od.z}, V3{dr.x, dr.y, dr.z},
                                           3.402823466e+38f, st);
             hit = h.prim != 0xFFFFFFFFu;
             if (hit) f.p.hit[pid] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.prim));
@@ -205,18 +224,18 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(Frame1 f, uint32_t qsel) 
 
 // Network input of a hit: scene-normalised position, geometric normal turned
 // towards the viewer, outgoing direction, roughness factor, hero wavelength.
-__device__ __forceinline__ void guide_input(const Frame1& f, uint32_t pid, float* __restrict__ out) {
+__device__ __forceinline__ void guide_input(const Frame1& f, const SceneView& S, uint32_t pid, float* __restrict__ out) {
     const float4 hr = f.p.hit[pid], dr = f.p.dr[pid];
     const uint32_t prim = __float_as_uint(hr.w);
-    const V3 v0 = ld3(f.verts + 3 * (size_t)f.indices[3 * prim]), v1 = ld3(f.verts + 3 * (size_t)f.indices[3 * prim + 1]),
-             v2 = ld3(f.verts + 3 * (size_t)f.indices[3 * prim + 2]);
+    const V3 v0 = ld3(S.verts + 3 * (size_t)S.indices[3 * prim]), v1 = ld3(S.verts + 3 * (size_t)S.indices[3 * prim + 1]),
+             v2 = ld3(S.verts + 3 * (size_t)S.indices[3 * prim + 2]);
     const float b1 = hr.y, b2 = hr.z, b0 = 1.0f - b1 - b2;
     const V3 hp{v0.x * b0 + v1.x * b1 + v2.x * b2, v0.y * b0 + v1.y * b1 + v2.y * b2, v0.z * b0 + v1.z * b1 + v2.z * b2};
     V3 g = cross(v1 - v0, v2 - v0);
     g = g * rsqrtf(dot(g, g));
     const V3 wo{-dr.x, -dr.y, -dr.z};
     if (dot(g, wo) < 0.0f) g = -g;
-    const Material& m = f.materials[f.triMaterial[prim]];
+    const Material& m = S.materials[S.triMaterial[prim]];
     const float rough = m.bxdf == 0 ? m.mfRoughness : ((m.factors & 4u) ? m.roughness : 1.0f);
     out[0] = clampf((hp.x - f.gmin[0]) * f.ginv[0], 0.0f, 1.0f);
     out[1] = clampf((hp.y - f.gmin[1]) * f.ginv[1], 0.0f, 1.0f);
@@ -259,7 +278,7 @@ __global__ __launch_bounds__(256) void k_guide_paths(Frame1 f) {
             const uint32_t i = base + threadIdx.x;
             float* dst = recs + threadIdx.x * guide::kRec;
             if (i < count) {
-                guide_input(f, f.p.hq[i], dst);
+                guide_input(f, f.sc, f.p.hq[i], dst);
                 for (int k = 0; k < guide::kRec; ++k) f.gfeat[(size_t)i * guide::kRec + k] = dst[k];
             } else {
                 for (int k = 0; k < guide::kRec; ++k) dst[k] = 0.0f;
@@ -267,7 +286,8 @@ __global__ __launch_bounds__(256) void k_guide_paths(Frame1 f) {
         }
         __syncthreads();
         mlp::encode_rows(L.x0, recs, mlp::kTile, 0, row0);
-        mlp::forward_rows(L, f.gwf, f.gbias, row0);
+        if (f.gwq) mlp::forward_rows_fp8(L, f.gwq, f.gws, f.gbias, row0);
+        else mlp::forward_rows(L, f.gwf, f.gbias, row0);
         __syncthreads();
         if (g == 0) {
             const uint32_t i = base + row0 + r;
@@ -277,13 +297,17 @@ __global__ __launch_bounds__(256) void k_guide_paths(Frame1 f) {
     }
 }
 
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_miss(Frame1 f) {
     const uint32_t count = f.counters[CNT_MISS];
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
-        shade_miss(f, f.p.mq[i]);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const uint32_t pid = f.p.mq[i];
+        shade_miss(f, scene_of<BATCH>(f, pid), pid);
+    }
 }
 
 // ------------------------------------------------------------------ shadow
+template <bool BATCH>
 __global__ __launch_bounds__(kTraceBlock) void k_shadow(Frame1 f) {
     __shared__ int stack[kStackLds * kTraceBlock];
     const TravStack<kStackLds> st = lane_stack(f, stack);
@@ -292,8 +316,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(Frame1 f) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i - threadIdx.x < count; i += gridDim.x * blockDim.x) {
         if (i < count) {
             const uint32_t pid = f.p.sq[i];
+            const SceneView& S = scene_of<BATCH>(f, pid);
             const float4 so = f.p.sro[pid], sd = f.p.srd[pid];
-            const Hit h = traverse<true>(f.nodes, f.tris, f.nnodes, V3{so.x, so.y, so.z}, V3{sd.x, sd.y, sd.z}, so.w,
+            const Hit h = traverse<true>(S.nodes, S.tris, S.nnodes, V3{so.x, so.y, so.z}, V3{sd.x, sd.y, sd.z}, so.w,
                                          st);
             if (h.prim == 0xFFFFFFFFu) {
                 const float4 a = f.p.Lnee[pid];
@@ -308,15 +333,15 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(Frame1 f) {
 }
 
 // ------------------------------------------------------------------- shade
-__device__ __forceinline__ S4 tri_emission(const Frame1& f, const EmissiveTri& tri, float ba, float bb, float bc,
+__device__ __forceinline__ S4 tri_emission(const SceneView& S, const EmissiveTri& tri, float ba, float bb, float bc,
                                            const Wl& wl, const ColorTables& ct) {
-    const Material& m = f.materials[tri.material];
-    if (m.flags & 4u) return dense4(f.spectra + (size_t)m.emissionSpectrum * kDense, wl) * m.emissive[0];
+    const Material& m = S.materials[tri.material];
+    if (m.flags & 4u) return dense4(S.spectra + (size_t)m.emissionSpectrum * kDense, wl) * m.emissive[0];
     V3 e{m.emissive[0], m.emissive[1], m.emissive[2]};
     if (m.textures & 64u) {
         const float u = ba * tri.uv[0] + bb * tri.uv[2] + bc * tri.uv[4];
         const float v = ba * tri.uv[1] + bb * tri.uv[3] + bc * tri.uv[5];
-        const float4 t = tex2d(f.texDescs, f.texels, m.tex[6], u, v);
+        const float4 t = tex2d(S.texDescs, S.texels, m.tex[6], u, v);
         e = V3{e.x * t.x, e.y * t.y, e.z * t.z};
     }
     return ct.rgb(e, wl);
@@ -334,17 +359,13 @@ __device__ __forceinline__ void quat_cols(float4 q, V3& c0, V3& c2) {
     c2 = V3{2.0f * (x * z + w * y), 2.0f * (y * z - w * x), 1.0f - 2.0f * (x * x + y * y)};
 }
 
-template <bool GUIDED>
+template <bool GUIDED, bool BATCH>
 __global__ __launch_bounds__(kShadeBlock) void k_shade(Frame1 f, uint32_t qsel) {
     // shades the paths whose ray hit a triangle (misses were finished by k_trace)
     const uint32_t count = f.counters[CNT_HIT];
     const uint32_t* __restrict__ qin = f.p.hq;
     uint32_t* __restrict__ qout = qsel ? f.p.q0 : f.p.q1;
     uint32_t* __restrict__ cout = f.counters + (qsel ? CNT_QUEUE0 : CNT_QUEUE1);
-    const ColorTables ct{f.colors};
-    const Luts lut{f.luts};
-    const LightTree tree{f.treeNodes, f.treePointers, f.treeMasks, f.envOn != 0};
-    const bool treeOn = f.treeNodes != nullptr;
     uint32_t nvert = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i - threadIdx.x < count; i += gridDim.x * blockDim.x) {
         bool alive = false, shadow = false;
@@ -355,6 +376,11 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(Frame1 f, uint32_t qsel) 
         if (i < count) {
             pid = qin[i];
             ++nvert;
+            const SceneView& S = scene_of<BATCH>(f, pid);
+            const ColorTables ct{S.colors};
+            const Luts lut{S.luts};
+            const LightTree tree{S.treeNodes, S.treePointers, S.treeMasks, S.envOn != 0};
+            const bool treeOn = S.treeNodes != nullptr;
             const float4 od = f.p.od[pid], dr = f.p.dr[pid], nf = f.p.nf[pid], hr = f.p.hit[pid];
             PathIO P;
             P.rd = V3{dr.x, dr.y, dr.z};
@@ -373,27 +399,28 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(Frame1 f, uint32_t qsel) 
             V3 ro{od.x, od.y, od.z};
             V3 lastN{nf.x, nf.y, nf.z};
             uint32_t depth = f.p.depth[pid];
-            const uint32_t sk = pid / f.P, pix = pid - sk * f.P;
-            const uint32_t px = pix % f.W, py = pix / f.W;
-            const Qrng q = qrng_at(f.sample + sk, depth, pixel_hash(px, py));
+            const uint32_t local = pid - S.slot0;
+            const uint32_t sk = local / S.P, pix = local - sk * S.P;
+            const uint32_t px = pix % S.W, py = pix / S.W;
+            const Qrng q = qrng_at(S.sample + sk, depth, pixel_hash(px, py));
 
             // ---- interaction (pathtrace.cu:86-149, 906-953)
             const uint32_t prim = __float_as_uint(hr.w);
             // k_trace queues only paths that hit a triangle
             uint32_t ltype = LT_NONE, lindex = 0;
             const float b1 = hr.y, b2 = hr.z;
-            const uint32_t vi0 = f.indices[3 * prim], vi1 = f.indices[3 * prim + 1], vi2 = f.indices[3 * prim + 2];
-            const V3 v0 = ld3(f.verts + 3 * (size_t)vi0), v1 = ld3(f.verts + 3 * (size_t)vi1),
-                     v2 = ld3(f.verts + 3 * (size_t)vi2);
+            const uint32_t vi0 = S.indices[3 * prim], vi1 = S.indices[3 * prim + 1], vi2 = S.indices[3 * prim + 2];
+            const V3 v0 = ld3(S.verts + 3 * (size_t)vi0), v1 = ld3(S.verts + 3 * (size_t)vi1),
+                     v2 = ld3(S.verts + 3 * (size_t)vi2);
             V3 gn;
             {
                 V3 g = cross(v1 - v0, v2 - v0);
                 g = g * rsqrtf(dot(g, g));
                 gn = oct_dec_u32(oct_enc_u32(g));
             }
-            const uint32_t mat = f.triMaterial[prim];
+            const uint32_t mat = S.triMaterial[prim];
             {
-                const uint32_t tl = f.triLight[prim];
+                const uint32_t tl = S.triLight[prim];
                 if (tl != 0xFFFFFFFFu) { ltype = LT_TRIANGLE; lindex = tl; }
             }
             P.flags |= PF_TRIANGULAR;
@@ -415,7 +442,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(Frame1 f, uint32_t qsel) 
                 S4 Le = s4(0.0f);
                 bool noEm = false;
                 if (ltype == LT_TRIANGLE) {
-                    const EmissiveTri& tri = f.lights[lindex];
+                    const EmissiveTri& tri = S.lights[lindex];
                     const V3 a = ld3(tri.v), b = ld3(tri.v + 3), c = ld3(tri.v + 6);
                     V3 n = cross(b - a, c - a);
                     const float nl = len(n);
@@ -425,7 +452,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(Frame1 f, uint32_t qsel) 
                     const float lcos = -dot(P.rd, n);
                     noEm = lcos <= 0.0f;
                     lightPdf = (1.0f / (nl / 2.0f)) * d2 / lcos;
-                    Le = tri_emission(f, tri, 1.0f - b1 - b2, b1, b2, P.wl, ct);
+                    Le = tri_emission(S, tri, 1.0f - b1 - b2, b1, b2, P.wl, ct);
                 }
                 if (!noEm) {
                     lightPdf *= treePdf;
@@ -437,12 +464,12 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(Frame1 f, uint32_t qsel) 
 
             if (!finished) {
                 // ---- surface attributes (pathtrace.cu:1279-1567)
-                const Material& m = f.materials[mat];
+                const Material& m = S.materials[mat];
                 const float bw0 = 1.0f - b1 - b2, bw1 = b1, bw2 = b2;
                 V2 tc1{0, 0}, tc2{0, 0};
                 V3 normal, tangent;
                 bool flipT = false;
-                const uint8_t* ab = f.attribs;
+                const uint8_t* ab = S.attribs;
                 const uint32_t stride = m.attributeStride;
                 if (m.attributes & 3u) {
                     V3 sN, sT, n0, n1, n2;
@@ -499,7 +526,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(Frame1 f, uint32_t qsel) 
                 V3 n{0.0f, 0.0f, 1.0f};
                 if (m.textures & 2u) {
                     const V2 uv = uvOf(1);
-                    const float4 t = tex2d(f.texDescs, f.texels, m.tex[1], uv.x, uv.y);
+                    const float4 t = tex2d(S.texDescs, S.texels, m.tex[1], uv.x, uv.y);
                     n = nrm(V3{t.x * 2.0f - 1.0f, t.y * 2.0f - 1.0f, t.z * 2.0f - 1.0f});
                 }
                 n = tbn.to_world(n);
@@ -530,7 +557,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(Frame1 f, uint32_t qsel) 
                     float alphaV = 0.0f;
                     if (bcTex) {
                         const V2 uv = uvOf(1);  // the reference reads the normal map's set here (pathtrace.cu:1453)
-                        const float4 t = tex2d(f.texDescs, f.texels, m.tex[0], uv.x, uv.y);
+                        const float4 t = tex2d(S.texDescs, S.texels, m.tex[0], uv.x, uv.y);
                         s.base = V3{t.x, t.y, t.z};
                         alphaV = t.w;
                         if (bcFac) {
@@ -549,7 +576,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(Frame1 f, uint32_t qsel) 
                     const bool metF = (m.factors & 2u) != 0, rF = (m.factors & 4u) != 0;
                     if (m.textures & 4u) {
                         const V2 uv = uvOf(2);
-                        const float4 t = tex2d(f.texDescs, f.texels, m.tex[2], uv.x, uv.y);
+                        const float4 t = tex2d(S.texDescs, S.texels, m.tex[2], uv.x, uv.y);
                         s.metalness = t.z * (metF ? m.metalness : 1.0f);
                         s.roughness = t.y * (rF ? m.roughness : 1.0f);
                     } else {
@@ -559,7 +586,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(Frame1 f, uint32_t qsel) 
                     const bool trF = (m.factors & 8u) != 0;
                     if (m.textures & 8u) {
                         const V2 uv = uvOf(3);
-                        const float4 t = tex2d(f.texDescs, f.texels, m.tex[3], uv.x, uv.y);
+                        const float4 t = tex2d(S.texDescs, S.texels, m.tex[3], uv.x, uv.y);
                         s.transmission = t.x * (trF ? m.transmission : 1.0f);
                     } else if (trF) {
                         s.transmission = m.transmission;
@@ -570,7 +597,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(Frame1 f, uint32_t qsel) 
                         const bool shcF = (m.factors & 64u) != 0;
                         if (m.textures & 16u) {
                             const V2 uv = uvOf(4);
-                            const float4 t = tex2d(f.texDescs, f.texels, m.tex[4], uv.x, uv.y);
+                            const float4 t = tex2d(S.texDescs, S.texels, m.tex[4], uv.x, uv.y);
                             s.sheenColor = V3{t.x * (shcF ? m.sheenColor[0] : 1.0f), t.y * (shcF ? m.sheenColor[1] : 1.0f),
                                               t.z * (shcF ? m.sheenColor[2] : 1.0f)};
                         } else if (shcF) {
@@ -579,7 +606,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(Frame1 f, uint32_t qsel) 
                         const bool shrF = (m.factors & 128u) != 0;
                         if (m.textures & 32u) {
                             const V2 uv = uvOf(5);
-                            const float4 t = tex2d(f.texDescs, f.texels, m.tex[5], uv.x, uv.y);
+                            const float4 t = tex2d(S.texDescs, S.texels, m.tex[5], uv.x, uv.y);
                             const float r = t.w * (shrF ? m.sheenRoughness : 1.0f);
                             s.sheenAlpha = r * r;
                         } else if (shrF) {
@@ -597,7 +624,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(Frame1 f, uint32_t qsel) 
                     dl.occluded = false;
                     V3 lro{0, 0, 0};
                     float dToL = 0.0f;
-                    if (!f.envOn && !treeOn) {
+                    if (!S.envOn && !treeOn) {
                         dl.occluded = true;
                     } else {
                         const V3 rnd = q.s3(QD_LIGHT);
@@ -606,7 +633,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(Frame1 f, uint32_t qsel) 
                         tree.sample(rnd.z, hitPos, lastN, type, index, treePdf);
                         float lightPdf = 0.0f;
                         if (type == LT_TRIANGLE) {
-                            const EmissiveTri& tri = f.lights[index];
+                            const EmissiveTri& tri = S.lights[index];
                             const V3 a = ld3(tri.v), b = ld3(tri.v + 3), c = ld3(tri.v + 6);
                             V3 nl = cross(b - a, c - a);
                             const float nlen = len(nl);
@@ -624,25 +651,25 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(Frame1 f, uint32_t qsel) 
                             const float lcos = -dot(nl, dl.dir);
                             dl.occluded = lcos <= 0.0f;
                             lightPdf = (1.0f / (nlen / 2.0f)) * s2 / lcos;
-                            dl.spectrum = tri_emission(f, tri, ba, bb, bc, P.wl, ct);
+                            dl.spectrum = tri_emission(S, tri, ba, bb, bc, P.wl, ct);
                         } else if (type == LT_SKY) {
-                            const float W = (float)f.envW, H = (float)f.envH;
+                            const float W = (float)S.envW, H = (float)S.envH;
                             const float fcx = fminf(rnd.x * (W - 1.0f), W - 1.0001f);
                             const float fcy = fminf(rnd.y * (H - 1.0f), H - 1.0001f);
                             const float flx = floorf(fcx), fly = floorf(fcy);
                             const uint32_t ix = (uint32_t)flx, iy = (uint32_t)fly;
-                            const float m0 = (float)f.envMarg[iy], m1 = (float)f.envMarg[iy + 1];
+                            const float m0 = (float)S.envMarg[iy], m1 = (float)S.envMarg[iy + 1];
                             const size_t row = (size_t)(uint32_t)(m0 * W);
-                            const float c0 = (float)f.envCond[row + ix], c1 = (float)f.envCond[row + ix + 1];
+                            const float c0 = (float)S.envCond[row + ix], c1 = (float)S.envCond[row + ix + 1];
                             const float ux = lerpf(c0, c1, fcx - flx) * (1.0f / W);
                             const float uy = lerpf(m0, m1, fcy - fly) * (1.0f / H);
                             float sp, cp, st, ctt;
                             sincosf(2.0f * 3.14159265358979f * ux, &sp, &cp);
                             sincosf(3.14159265358979f * uy, &st, &ctt);
                             dl.dir = V3{-st * cp, st * sp, ctt};
-                            const V3 rgb = env_lookup(f.envTex, f.envW, f.envH, f.envC, ux, uy);
+                            const V3 rgb = env_lookup(S.envTex, S.envW, S.envH, S.envC, ux, uy);
                             dl.spectrum = ct.rgb(rgb, P.wl);
-                            lightPdf = (1.0f / (4.0f * 3.14159265358979f)) * ((rgb.x + rgb.y + rgb.z) / 3.0f) * st / f.envIntegral;
+                            lightPdf = (1.0f / (4.0f * 3.14159265358979f)) * ((rgb.x + rgb.y + rgb.z) / 3.0f) * st / S.envIntegral;
                             dToL = 3.402823466e+38f;
                             lro = offset_point(hitInterp, backHit ? -gn : gn);
                         } else {
@@ -658,13 +685,13 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(Frame1 f, uint32_t qsel) 
                         const float pmix = complex_guided(ct, lut, dl, q, local, s, P, Lnee, f.gmix + (size_t)i * kMixFloats, wiw);
                         if (pmix == 0.0f) {
                             P.flags |= PF_FINISHED;
-                        } else if (f.collect && pixel_hash(pid, depth * 0x9E3779B9u + f.sample) < f.recThreshold) {
+                        } else if (f.collect && pixel_hash(pid, depth * 0x9E3779B9u + S.sample) < f.recThreshold) {
                             recordSlot = true;
                             recDir = wiw;
                             recInvPdf = 1.0f / pmix;
                         }
                     } else if (m.bxdf == 0) {
-                        conductor_bxdf(f.spectra + (size_t)m.iorSpectrum * kDense, f.spectra + (size_t)m.acSpectrum * kDense,
+                        conductor_bxdf(S.spectra + (size_t)m.iorSpectrum * kDense, S.spectra + (size_t)m.acSpectrum * kDense,
                                        m.mfRoughness, dl, q, local, P, Lnee);
                     } else if (m.bxdf == 2) {
                         complex_bxdf(ct, lut, dl, q, local, s, P, Lnee);
@@ -701,9 +728,9 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(Frame1 f, uint32_t qsel) 
                     const float dNgRo = dot(gn, P.rd);
                     if (dNgRo > 0.0f && (backHit != refracted) && !skipped) P.rd = nrm(P.rd - (gn * (dNgRo + 0.001f)));
                     // ---- depth caps (pathtrace.cu:1768-1779)
-                    bool cont = ++depth < f.maxDepth;
+                    bool cont = ++depth < S.maxDepth;
                     if (!(P.flags & PF_CUR_SPECULAR) && !skipped)
-                        cont = (P.flags & PF_REFRACTION_HAPPENED) ? depth < f.maxTrans : depth < f.maxRefl;
+                        cont = (P.flags & PF_REFRACTION_HAPPENED) ? depth < S.maxTrans : depth < S.maxRefl;
                     if (!cont) P.flags |= PF_FINISHED;
                     // ---- updateStateFlags (pathtrace.cu:954-972)
                     const uint32_t excl = PF_CUR_SPECULAR | PF_PREV_SPECULAR | PF_TRIANGULAR | PF_RIGHT_HANDED | PF_RAY_REFRACTED;
@@ -773,15 +800,17 @@ __device__ __forceinline__ void finish_records(const Frame1& f, uint32_t slot, f
 }
 
 // ----------------------------------------------------------- accumulation
-__global__ __launch_bounds__(256) void k_accumulate(Frame1 f, uint32_t first) {
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_accumulate(Frame1 f) {
     // per pixel, the S samples of the pass are added one at a time in sample
     // order, so any split of a sample range into passes gives the same sums
+    const SceneView& S = BATCH ? f.scenes[blockIdx.y] : f.sc;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= f.P) return;
-    double4 a = first ? make_double4(0.0, 0.0, 0.0, 0.0) : f.accum[i];
-    const ColorTables ct{f.colors};
-    for (uint32_t k = 0; k < f.S; ++k) {
-        const uint32_t j = k * f.P + i;
+    if (i >= S.P) return;
+    double4 a = S.first ? make_double4(0.0, 0.0, 0.0, 0.0) : S.accum[i];
+    const ColorTables ct{S.colors};
+    for (uint32_t k = 0; k < S.S; ++k) {
+        const uint32_t j = S.slot0 + k * S.P + i;
         if (f.collect) finish_records(f, j, f.p.L[j]);
         const uint32_t flags = __float_as_uint(f.p.nf[j].w);
         if (!(flags & PF_TERMINATED)) {
@@ -799,7 +828,7 @@ __global__ __launch_bounds__(256) void k_accumulate(Frame1 f, uint32_t first) {
         }
         a.w += 1.0;
     }
-    f.accum[i] = a;
+    S.accum[i] = a;
 }
 
 }  // namespace
@@ -808,11 +837,14 @@ __global__ __launch_bounds__(256) void k_accumulate(Frame1 f, uint32_t first) {
 // Spill entries per lane of the traversal stack beyond the LDS part.
 size_t trace_spill_words(size_t lanes) { return lanes * (size_t)(kMaxStack - kStackLds); }
 
-hipError_t launch_pass(rbh_ctx c, Frame1 f, uint32_t depthMax, bool firstPass, int traceGrid, int shadeGrid,
-                       KernelTimer* kt) {
-    const int guideGrid = traceGrid / 8 > 0 ? traceGrid / 8 : 1;
+// One wavefront pass: camera rays for every slot, depthMax bounces of
+// trace / miss / (guide) / shade / shadow, then accumulation. f.sc (single
+// scene) or f.scenes (batched) must describe the pass's samples and slots.
+template <bool BATCH>
+static hipError_t launch_pass_t(rbh_ctx c, Frame1 f, uint32_t depthMax, const LaunchShape& ls, KernelTimer* kt) {
+    const int guideGrid = ls.traceGrid / 8 > 0 ? ls.traceGrid / 8 : 1;
     const int nb = (int)((f.NP + 255) / 256);
-    const int nbPix = (int)((f.P + 255) / 256);
+    const dim3 accGrid((f.maxP + 255) / 256, BATCH ? f.nscenes : 1u);
     hipError_t e = hipSuccess;
 #define RBH_TIMED(cls, launch)                                       \
     do {                                                             \
@@ -821,31 +853,35 @@ hipError_t launch_pass(rbh_ctx c, Frame1 f, uint32_t depthMax, bool firstPass, i
         if (kt && (e = kt->end(c->stream)) != hipSuccess) return e;  \
     } while (0)
     f.bounce = 0;
-    hipLaunchKernelGGL(k_camera, dim3(nb), dim3(256), 0, c->stream, f);
+    hipLaunchKernelGGL(k_camera<BATCH>, dim3(nb), dim3(256), 0, c->stream, f);
     for (uint32_t b = 0; b < depthMax; ++b) {
         const uint32_t qsel = b & 1u;
         f.bounce = b;
         if (b > 0) hipLaunchKernelGGL(k_reset_hits, dim3(1), dim3(64), 0, c->stream, f.counters);
         RBH_TIMED(KernelTimer::TRACE,
-                  hipLaunchKernelGGL(k_trace, dim3(traceGrid), dim3(kTraceBlock), 0, c->stream, f, qsel));
+                  hipLaunchKernelGGL(k_trace<BATCH>, dim3(ls.traceGrid), dim3(kTraceBlock), 0, c->stream, f, qsel));
         RBH_TIMED(KernelTimer::SHADE,
-                  hipLaunchKernelGGL(k_miss, dim3(shadeGrid), dim3(256), 0, c->stream, f));
-        if (f.guided)
+                  hipLaunchKernelGGL(k_miss<BATCH>, dim3(ls.shadeGrid), dim3(256), 0, c->stream, f));
+        if (!BATCH && f.guided)
             RBH_TIMED(KernelTimer::GUIDE,
                       hipLaunchKernelGGL(k_guide_paths, dim3(guideGrid), dim3(256), 0, c->stream, f));
         hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, c->stream, f.counters, qsel ? CNT_QUEUE0 : CNT_QUEUE1);
-        if (f.guided)
-            RBH_TIMED(KernelTimer::SHADE,
-                      hipLaunchKernelGGL(k_shade<true>, dim3(shadeGrid), dim3(kShadeBlock), 0, c->stream, f, qsel));
+        if (!BATCH && f.guided)
+            RBH_TIMED(KernelTimer::SHADE, hipLaunchKernelGGL((k_shade<true, false>), dim3(ls.shadeGrid),
+                                                             dim3(kShadeBlock), 0, c->stream, f, qsel));
         else
-            RBH_TIMED(KernelTimer::SHADE,
-                      hipLaunchKernelGGL(k_shade<false>, dim3(shadeGrid), dim3(kShadeBlock), 0, c->stream, f, qsel));
+            RBH_TIMED(KernelTimer::SHADE, hipLaunchKernelGGL((k_shade<false, BATCH>), dim3(ls.shadeGrid),
+                                                             dim3(kShadeBlock), 0, c->stream, f, qsel));
         RBH_TIMED(KernelTimer::SHADOW,
-                  hipLaunchKernelGGL(k_shadow, dim3(traceGrid), dim3(kTraceBlock), 0, c->stream, f));
+                  hipLaunchKernelGGL(k_shadow<BATCH>, dim3(ls.traceGrid), dim3(kTraceBlock), 0, c->stream, f));
     }
 #undef RBH_TIMED
-    hipLaunchKernelGGL(k_accumulate, dim3(nbPix), dim3(256), 0, c->stream, f, firstPass ? 1u : 0u);
+    hipLaunchKernelGGL(k_accumulate<BATCH>, accGrid, dim3(256), 0, c->stream, f);
     return hipGetLastError();
+}
+
+hipError_t launch_pass(rbh_ctx c, const Frame1& f, uint32_t depthMax, const LaunchShape& ls, KernelTimer* kt) {
+    return f.scenes ? launch_pass_t<true>(c, f, depthMax, ls, kt) : launch_pass_t<false>(c, f, depthMax, ls, kt);
 }
 
 }  // namespace rbh

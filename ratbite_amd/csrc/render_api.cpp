@@ -10,8 +10,7 @@
 
 namespace rbh {
 int lbvh_build(rbh_ctx c);
-hipError_t launch_pass(rbh_ctx c, Frame1 f, uint32_t depthMax, bool firstPass, int traceGrid, int shadeGrid,
-                       KernelTimer* kt);
+hipError_t launch_pass(rbh_ctx c, const Frame1& f, uint32_t depthMax, const LaunchShape& ls, KernelTimer* kt);
 hipError_t resolve_launch(rbh_ctx c, const float* colorspace, float expo, void* out);
 size_t trace_spill_words(size_t lanes);
 namespace host {
@@ -125,89 +124,171 @@ int rbh_get_light_tree(rbh_ctx c, rbh_packed_node* nodes, uint32_t* nnodes, uint
     return RBH_OK;
 }
 
-int rbh_render(rbh_ctx c, const rbh_launch_params* lp) {
-    if (!c || !lp) return RBH_ERR_INVALID;
+}  // extern "C"
+
+namespace {
+
+// Checks that a context holds a renderable scene for these launch parameters.
+int check_scene(rbh_ctx c, const rbh_launch_params* lp) {
     if (!c->accel.built) return fail(c, RBH_ERR_STATE, "rbh_build_accel has not been run on the current geometry");
     if (!c->haveColor || !c->haveLuts) return fail(c, RBH_ERR_STATE, "colour tables and LUTs must be uploaded first");
     if (lp->width == 0 || lp->height == 0) return fail(c, RBH_ERR_INVALID, "empty film");
     if ((uint64_t)lp->width * lp->height > (1ull << 28)) return fail(c, RBH_ERR_RANGE, "film above 2^28 pixels");
     if (lp->maxPathDepth == 0 || lp->maxPathDepth > 1024) return fail(c, RBH_ERR_INVALID, "maxPathDepth must be in [1, 1024]");
-    const uint32_t known = RBH_RENDER_GUIDED | RBH_RENDER_COLLECT | RBH_RENDER_TIMING | RBH_RENDER_NO_DROP;
-    if (lp->flags & ~known)
-        return fail(c, RBH_ERR_INVALID, "render flags 0x%x are not supported by this entry point", lp->flags);
-    const bool guided = (lp->flags & RBH_RENDER_GUIDED) != 0;
-    const bool collect = (lp->flags & RBH_RENDER_COLLECT) != 0;
-    if (collect && !guided) return fail(c, RBH_ERR_INVALID, "RBH_RENDER_COLLECT needs RBH_RENDER_GUIDED");
-    GuideState* gst = static_cast<GuideState*>(c->guide);
-    if (guided && !(gst && gst->configured)) return fail(c, RBH_ERR_STATE, "guided render before rbh_guide_configure");
-    const bool treeOn = c->tree.nlights > 0;
     if (c->geom.maxLight >= (int64_t)c->tree.nlights)
         return fail(c, RBH_ERR_STATE, "triangle light index %lld has no light record (%u lights built)",
                     (long long)c->geom.maxLight, c->tree.nlights);
+    return RBH_OK;
+}
+
+// Makes sure the accumulation buffer fits the film; a continued launch
+// (sampleOffset != 0) needs an existing accumulation of the same size.
+int prepare_accum(rbh_ctx c, const rbh_launch_params* lp) {
     RenderState& rs = c->rs;
     const size_t P = (size_t)lp->width * lp->height;
-    // samples per pass: as many as fit the path-slot budget (at least one)
-    const uint32_t spp = lp->sampleCount;
-    const uint32_t S = (uint32_t)std::max<size_t>(1, std::min<size_t>(spp ? spp : 1, rs.slotBudget / P));
-    const size_t NP = P * S;
+    const bool sizeChanged = rs.width != lp->width || rs.height != lp->height;
+    if (sizeChanged && lp->sampleOffset != 0)
+        return fail(c, RBH_ERR_STATE, "sampleOffset != 0 but no accumulation of this size exists");
+    RBH_ALLOC(c, rs.accum, P * sizeof(double) * 4);
+    rs.width = lp->width;
+    rs.height = lp->height;
+    return RBH_OK;
+}
+
+// Scene-level part of a pass view (the pass fields sample, S, slot0, first
+// are set per pass).
+void fill_view(rbh_ctx c, const rbh_launch_params* lp, SceneView& v) {
+    const bool treeOn = c->tree.nlights > 0;
+    v = SceneView{};
+    v.P = lp->width * lp->height;
+    v.W = lp->width;
+    v.H = lp->height;
+    v.maxDepth = std::max(1u, lp->maxPathDepth);
+    v.maxRefl = std::min(v.maxDepth, lp->maxReflectedPathDepth);
+    v.maxTrans = std::min(v.maxDepth, lp->maxTransmittedPathDepth);
+    v.invW = 1.0f / (float)lp->width;
+    v.invH = 1.0f / (float)lp->height;
+    v.psW = lp->perspectiveScaleW;
+    v.psH = lp->perspectiveScaleH;
+    v.camU = dev::V3{lp->camU[0], lp->camU[1], lp->camU[2]};
+    v.camV = dev::V3{lp->camV[0], lp->camV[1], lp->camV[2]};
+    v.camW = dev::V3{lp->camW[0], lp->camW[1], lp->camW[2]};
+    v.dof = lp->depthOfField;
+    v.aperture = lp->aperture;
+    v.focus = lp->focusDistance;
+    v.nodes = c->accel.nodes.as<BvhNode>();
+    v.tris = c->accel.tris.as<TriRecord>();
+    v.nnodes = c->accel.nnodes;
+    v.verts = c->geom.verts.as<float>();
+    v.indices = c->geom.indices.as<uint32_t>();
+    v.attribs = c->geom.attribs.as<uint8_t>();
+    v.triMaterial = c->geom.triMaterial.as<uint32_t>();
+    v.triLight = c->geom.triLight.as<uint32_t>();
+    v.materials = c->materials.as<Material>();
+    v.texDescs = c->textures.as<TextureDesc>();
+    v.texels = c->texels.as<uint8_t>();
+    v.spectra = c->spectra.as<float>();
+    v.colors = c->colorTables.as<float>();
+    v.luts = c->luts.as<float>();
+    v.treeNodes = treeOn ? c->tree.nodes.as<PackedNode>() : nullptr;
+    v.treePointers = treeOn ? c->tree.lightPointers.as<uint32_t>() : nullptr;
+    v.treeMasks = treeOn ? c->tree.bitmasks.as<uint64_t>() : nullptr;
+    v.lights = treeOn ? c->tree.tris.as<EmissiveTri>() : nullptr;
+    v.envOn = c->env.enabled ? 1u : 0u;
+    if (c->env.enabled) {
+        v.envW = (uint32_t)c->env.width;
+        v.envH = (uint32_t)c->env.height;
+        v.envC = (uint32_t)c->env.channels;
+        v.envTex = c->env.texels.as<float>();
+        v.envCond = c->env.cond.as<uint16_t>();
+        v.envMarg = c->env.marg.as<uint16_t>();
+        v.envIntegral = c->env.integral;
+    }
+    v.accum = c->rs.accum.as<double4>();
+}
+
+// Path buffers, counters, spill columns and launch shape of the owner context
+// for NP slots.
+int prepare_pass_state(rbh_ctx c, size_t NP, Frame1& f, LaunchShape& ls) {
+    RenderState& rs = c->rs;
     PathBuffers pb{};
     uint32_t* counters = nullptr;
     unsigned long long* stats = nullptr;
     const size_t bytes = layout_paths(nullptr, NP, &pb, &counters, &stats);
     RBH_ALLOC(c, rs.paths, bytes);
     layout_paths(rs.paths.as<char>(), NP, &pb, &counters, &stats);
-    const bool sizeChanged = rs.width != lp->width || rs.height != lp->height;
-    RBH_ALLOC(c, rs.accum, P * sizeof(double) * 4);
-    if (sizeChanged && lp->sampleOffset != 0) return fail(c, RBH_ERR_STATE, "sampleOffset != 0 but no accumulation of this size exists");
-    rs.width = lp->width;
-    rs.height = lp->height;
-
-    Frame1 f{};
-    f.P = (uint32_t)P;
-    f.W = lp->width;
-    f.H = lp->height;
-    f.maxDepth = std::max(1u, lp->maxPathDepth);
-    f.maxRefl = std::min(f.maxDepth, lp->maxReflectedPathDepth);
-    f.maxTrans = std::min(f.maxDepth, lp->maxTransmittedPathDepth);
-    f.invW = 1.0f / (float)lp->width;
-    f.invH = 1.0f / (float)lp->height;
-    f.psW = lp->perspectiveScaleW;
-    f.psH = lp->perspectiveScaleH;
-    f.camU = dev::V3{lp->camU[0], lp->camU[1], lp->camU[2]};
-    f.camV = dev::V3{lp->camV[0], lp->camV[1], lp->camV[2]};
-    f.camW = dev::V3{lp->camW[0], lp->camW[1], lp->camW[2]};
-    f.dof = lp->depthOfField;
-    f.aperture = lp->aperture;
-    f.focus = lp->focusDistance;
-    f.nodes = c->accel.nodes.as<BvhNode>();
-    f.tris = c->accel.tris.as<TriRecord>();
-    f.nnodes = c->accel.nnodes;
-    f.verts = c->geom.verts.as<float>();
-    f.indices = c->geom.indices.as<uint32_t>();
-    f.attribs = c->geom.attribs.as<uint8_t>();
-    f.triMaterial = c->geom.triMaterial.as<uint32_t>();
-    f.triLight = c->geom.triLight.as<uint32_t>();
-    f.materials = c->materials.as<Material>();
-    f.texDescs = c->textures.as<TextureDesc>();
-    f.texels = c->texels.as<uint8_t>();
-    f.spectra = c->spectra.as<float>();
-    f.colors = c->colorTables.as<float>();
-    f.luts = c->luts.as<float>();
-    f.treeNodes = treeOn ? c->tree.nodes.as<PackedNode>() : nullptr;
-    f.treePointers = treeOn ? c->tree.lightPointers.as<uint32_t>() : nullptr;
-    f.treeMasks = treeOn ? c->tree.bitmasks.as<uint64_t>() : nullptr;
-    f.lights = treeOn ? c->tree.tris.as<EmissiveTri>() : nullptr;
-    f.envOn = c->env.enabled ? 1u : 0u;
-    if (c->env.enabled) {
-        f.envW = (uint32_t)c->env.width;
-        f.envH = (uint32_t)c->env.height;
-        f.envC = (uint32_t)c->env.channels;
-        f.envTex = c->env.texels.as<float>();
-        f.envCond = c->env.cond.as<uint16_t>();
-        f.envMarg = c->env.marg.as<uint16_t>();
-        f.envIntegral = c->env.integral;
-    }
     f.p = pb;
+    f.counters = counters;
+    f.stats = stats;
+    const int cus = cu_count(c->device);
+    ls.traceGrid = std::max(1, cus * rs.traceBlocksPerCU);
+    ls.shadeGrid = std::max(1, cus * rs.shadeBlocksPerCU);
+    const size_t lanes = (size_t)ls.traceGrid * kTraceBlock;
+    RBH_ALLOC(c, rs.spill, trace_spill_words(lanes) * sizeof(int));
+    f.spill = rs.spill.as<int>();
+    f.spillStride = (uint32_t)lanes;
+    return RBH_OK;
+}
+
+int finish_stats(rbh_ctx c, const Frame1& f, KernelTimer* kt, uint64_t samples, uint32_t launches) {
+    RBH_HIP(c, hipEventRecord(c->ev1, c->stream));
+    RBH_HIP(c, hipStreamSynchronize(c->stream));
+    float ms = 0.0f;
+    RBH_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    unsigned long long st[STAT_COUNT] = {};
+    RBH_HIP(c, hipMemcpy(st, f.stats, sizeof(st), hipMemcpyDeviceToHost));
+    c->stats.renderMs = ms;
+    c->stats.closestRays = st[STAT_CLOSEST];
+    c->stats.shadowRays = st[STAT_SHADOW];
+    c->stats.pathVertices = st[STAT_VERTICES];
+    c->stats.samples = samples;
+    c->stats.traceLaunches = launches;
+    c->stats.traceMs = c->stats.shadeMs = c->stats.shadowMs = c->stats.guideMs = 0.0;
+    if (kt) {
+        double kms[KernelTimer::NCLASS];
+        uint32_t kn[KernelTimer::NCLASS];
+        RBH_HIP(c, kt->collect(kms, kn));
+        c->stats.traceMs = kms[KernelTimer::TRACE];
+        c->stats.shadeMs = kms[KernelTimer::SHADE];
+        c->stats.shadowMs = kms[KernelTimer::SHADOW];
+        c->stats.guideMs = kms[KernelTimer::GUIDE];
+    }
+    return RBH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rbh_render(rbh_ctx c, const rbh_launch_params* lp) {
+    if (!c || !lp) return RBH_ERR_INVALID;
+    int rc = check_scene(c, lp);
+    if (rc) return rc;
+    const uint32_t known = RBH_RENDER_GUIDED | RBH_RENDER_COLLECT | RBH_RENDER_GUIDE_FP8 | RBH_RENDER_TIMING |
+                           RBH_RENDER_NO_DROP;
+    if (lp->flags & ~known)
+        return fail(c, RBH_ERR_INVALID, "render flags 0x%x are not supported by this entry point", lp->flags);
+    const bool guided = (lp->flags & RBH_RENDER_GUIDED) != 0;
+    const bool collect = (lp->flags & RBH_RENDER_COLLECT) != 0;
+    if (collect && !guided) return fail(c, RBH_ERR_INVALID, "RBH_RENDER_COLLECT needs RBH_RENDER_GUIDED");
+    if ((lp->flags & RBH_RENDER_GUIDE_FP8) && !guided)
+        return fail(c, RBH_ERR_INVALID, "RBH_RENDER_GUIDE_FP8 needs RBH_RENDER_GUIDED");
+    GuideState* gst = static_cast<GuideState*>(c->guide);
+    if (guided && !(gst && gst->configured)) return fail(c, RBH_ERR_STATE, "guided render before rbh_guide_configure");
+    RenderState& rs = c->rs;
+    const size_t P = (size_t)lp->width * lp->height;
+    // samples per pass: as many as fit the path-slot budget (at least one)
+    const uint32_t spp = lp->sampleCount;
+    const uint32_t S = (uint32_t)std::max<size_t>(1, std::min<size_t>(spp ? spp : 1, rs.slotBudget / P));
+    const size_t NP = P * S;
+    rc = prepare_accum(c, lp);
+    if (rc) return rc;
+    Frame1 f{};
+    fill_view(c, lp, f.sc);
+    LaunchShape ls;
+    rc = prepare_pass_state(c, NP, f, ls);
+    if (rc) return rc;
+    f.maxP = (uint32_t)P;
     f.noDrop = (guided || (lp->flags & RBH_RENDER_NO_DROP)) ? 1u : 0u;
     if (guided) {
         RBH_ALLOC(c, rs.gfeat, NP * 16 * sizeof(float));
@@ -215,6 +296,10 @@ int rbh_render(rbh_ctx c, const rbh_launch_params* lp) {
         f.guided = 1u;
         f.gwf = static_cast<const __bf16*>(gst->wf);
         f.gbias = static_cast<const float*>(gst->bias);
+        if (lp->flags & RBH_RENDER_GUIDE_FP8) {
+            f.gwq = static_cast<const uint8_t*>(gst->fp8w);
+            f.gws = static_cast<const float*>(gst->fp8s);
+        }
         f.gfeat = rs.gfeat.as<float>();
         f.gmix = rs.gmix.as<float>();
         for (int a = 0; a < 3; ++a) {
@@ -233,53 +318,193 @@ int rbh_render(rbh_ctx c, const rbh_launch_params* lp) {
             RBH_HIP(c, hipMemsetAsync(gst->recCount, 0, 4, c->stream));
         }
     }
-    f.counters = counters;
-    f.stats = stats;
-    f.accum = rs.accum.as<double4>();
-
-    const int cus = cu_count(c->device);
-    const int traceGrid = cus * 16;
-    const int shadeGrid = cus * 16;
-    const size_t lanes = (size_t)traceGrid * kTraceBlock;
-    RBH_ALLOC(c, rs.spill, trace_spill_words(lanes) * sizeof(int));
-    f.spill = rs.spill.as<int>();
-    f.spillStride = (uint32_t)lanes;
     KernelTimer* kt = (lp->flags & RBH_RENDER_TIMING) ? &rs.timer : nullptr;
     if (kt) kt->reset();
     RBH_HIP(c, hipEventRecord(c->ev0, c->stream));
-    hipLaunchKernelGGL(k_zero_stats, dim3(1), dim3(64), 0, c->stream, stats);
+    hipLaunchKernelGGL(k_zero_stats, dim3(1), dim3(64), 0, c->stream, f.stats);
     uint32_t passes = 0;
     for (uint32_t done = 0; done < spp;) {
-        f.sample = lp->sampleOffset + done;
-        f.S = std::min(S, spp - done);
-        f.NP = (uint32_t)(P * f.S);
-        const bool first = (lp->sampleOffset == 0 && done == 0);
-        RBH_HIP(c, launch_pass(c, f, f.maxDepth, first, traceGrid, shadeGrid, kt));
-        done += f.S;
+        f.sc.sample = lp->sampleOffset + done;
+        f.sc.S = std::min(S, spp - done);
+        f.sc.slot0 = 0;
+        f.sc.first = (lp->sampleOffset == 0 && done == 0) ? 1u : 0u;
+        f.NP = (uint32_t)(P * f.sc.S);
+        RBH_HIP(c, launch_pass(c, f, f.sc.maxDepth, ls, kt));
+        done += f.sc.S;
         ++passes;
     }
-    RBH_HIP(c, hipEventRecord(c->ev1, c->stream));
-    RBH_HIP(c, hipStreamSynchronize(c->stream));
-    float ms = 0.0f;
-    RBH_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    unsigned long long st[STAT_COUNT] = {};
-    RBH_HIP(c, hipMemcpy(st, stats, sizeof(st), hipMemcpyDeviceToHost));
-    c->stats.renderMs = ms;
-    c->stats.closestRays = st[STAT_CLOSEST];
-    c->stats.shadowRays = st[STAT_SHADOW];
-    c->stats.pathVertices = st[STAT_VERTICES];
-    c->stats.samples = (uint64_t)P * lp->sampleCount;
-    c->stats.traceLaunches = passes * f.maxDepth;
-    c->stats.traceMs = c->stats.shadeMs = c->stats.shadowMs = c->stats.guideMs = 0.0;
-    if (kt) {
-        double kms[KernelTimer::NCLASS];
-        uint32_t kn[KernelTimer::NCLASS];
-        RBH_HIP(c, kt->collect(kms, kn));
-        c->stats.traceMs = kms[KernelTimer::TRACE];
-        c->stats.shadeMs = kms[KernelTimer::SHADE];
-        c->stats.shadowMs = kms[KernelTimer::SHADOW];
-        c->stats.guideMs = kms[KernelTimer::GUIDE];
+    return finish_stats(c, f, kt, (uint64_t)P * spp, passes * f.sc.maxDepth);
+}
+
+// Many scenes (one per context, all on the owner's device) in one wavefront:
+// each pass holds S samples of every unfinished scene's pixels, slots in
+// scene order. Per-scene completion times are the end of the pass in which
+// the scene's last sample finished.
+int rbh_render_batch(rbh_ctx owner, const rbh_ctx* scenes, const rbh_launch_params* lps, uint32_t n,
+                     double* completionMs) {
+    if (!owner || !scenes || !lps || n == 0) return RBH_ERR_INVALID;
+    if (n > 4096) return fail(owner, RBH_ERR_RANGE, "at most 4096 scenes per batch");
+    uint64_t totalP = 0, totalSamples = 0;
+    uint32_t depth = 1;
+    for (uint32_t s = 0; s < n; ++s) {
+        rbh_ctx c = scenes[s];
+        if (!c) return fail(owner, RBH_ERR_INVALID, "scene %u: null context", s);
+        if (c->device != owner->device) return fail(owner, RBH_ERR_INVALID, "scene %u is on another device", s);
+        if (lps[s].flags & ~(uint32_t)(RBH_RENDER_TIMING | RBH_RENDER_NO_DROP))
+            return fail(owner, RBH_ERR_INVALID, "scene %u: guided flags are not supported in batches", s);
+        const int rc = check_scene(c, &lps[s]);
+        if (rc) return fail(owner, rc, "scene %u: %s", s, c->err.c_str());
+        const int ra = prepare_accum(c, &lps[s]);
+        if (ra) return fail(owner, ra, "scene %u: %s", s, c->err.c_str());
+        totalP += (uint64_t)lps[s].width * lps[s].height;
+        totalSamples += (uint64_t)lps[s].width * lps[s].height * lps[s].sampleCount;
+        depth = std::max(depth, std::max(1u, lps[s].maxPathDepth));
     }
+    RenderState& rs = owner->rs;
+    const uint32_t S = (uint32_t)std::max<uint64_t>(1, rs.slotBudget / std::max<uint64_t>(1, totalP));
+    std::vector<SceneView> base(n);
+    for (uint32_t s = 0; s < n; ++s) {
+        fill_view(scenes[s], &lps[s], base[s]);
+        // a scene may need fewer bounces than the pass runs; its paths end at its own caps
+    }
+    Frame1 f{};
+    LaunchShape ls;
+    int rc = prepare_pass_state(owner, (size_t)totalP * S, f, ls);
+    if (rc) return rc;
+    f.noDrop = 0u;
+    for (uint32_t s = 0; s < n; ++s)
+        if (lps[s].flags & RBH_RENDER_NO_DROP) f.noDrop = 1u;
+    uint32_t maxPasses = 0;
+    for (uint32_t s = 0; s < n; ++s) maxPasses = std::max(maxPasses, (lps[s].sampleCount + S - 1) / S);
+    RBH_ALLOC(owner, rs.sceneTable, (size_t)std::max(1u, maxPasses) * n * sizeof(SceneView));
+    KernelTimer* kt = (lps[0].flags & RBH_RENDER_TIMING) ? &rs.timer : nullptr;
+    if (kt) kt->reset();
+    std::vector<uint32_t> done(n, 0u);
+    std::vector<int> finishedAt(n, -1);
+    std::vector<std::vector<SceneView>> tables;   // kept alive until the stream has consumed them
+    std::vector<hipEvent_t> passEnd;
+    RBH_HIP(owner, hipEventRecord(owner->ev0, owner->stream));
+    hipLaunchKernelGGL(k_zero_stats, dim3(1), dim3(64), 0, owner->stream, f.stats);
+    uint32_t passes = 0;
+    int err = RBH_OK;
+    while (true) {
+        std::vector<SceneView> t;
+        uint32_t slot = 0, maxP = 0;
+        for (uint32_t s = 0; s < n; ++s) {
+            if (done[s] >= lps[s].sampleCount) continue;
+            SceneView v = base[s];
+            v.sample = lps[s].sampleOffset + done[s];
+            v.S = std::min(S, lps[s].sampleCount - done[s]);
+            v.slot0 = slot;
+            v.first = (lps[s].sampleOffset == 0 && done[s] == 0) ? 1u : 0u;
+            slot += v.P * v.S;
+            maxP = std::max(maxP, v.P);
+            done[s] += v.S;
+            if (done[s] >= lps[s].sampleCount) finishedAt[s] = (int)passes;
+            t.push_back(v);
+        }
+        if (t.empty()) break;
+        tables.push_back(std::move(t));
+        const std::vector<SceneView>& tb = tables.back();
+        // each pass reads its own table slice, so uploads never overwrite a table in use
+        SceneView* dtab = rs.sceneTable.as<SceneView>() + (size_t)passes * n;
+        if (hipMemcpyAsync(dtab, tb.data(), tb.size() * sizeof(SceneView), hipMemcpyHostToDevice, owner->stream) !=
+            hipSuccess) { err = fail(owner, RBH_ERR_HIP, "scene table upload failed"); break; }
+        f.scenes = dtab;
+        f.nscenes = (uint32_t)tb.size();
+        f.NP = slot;
+        f.maxP = maxP;
+        const hipError_t e = launch_pass(owner, f, depth, ls, kt);
+        if (e != hipSuccess) { err = fail(owner, RBH_ERR_HIP, "launch failed: %s", hipGetErrorString(e)); break; }
+        hipEvent_t ev = nullptr;
+        if (completionMs && hipEventCreate(&ev) == hipSuccess) {
+            (void)hipEventRecord(ev, owner->stream);
+            passEnd.push_back(ev);
+        }
+        ++passes;
+    }
+    if (err == RBH_OK) err = finish_stats(owner, f, kt, totalSamples, passes * depth);
+    if (completionMs) {
+        for (uint32_t s = 0; s < n; ++s) {
+            completionMs[s] = -1.0;
+            const int pi = finishedAt[s];
+            if (err == RBH_OK && pi >= 0 && pi < (int)passEnd.size()) {
+                float ms = 0.0f;
+                if (hipEventElapsedTime(&ms, owner->ev0, passEnd[pi]) == hipSuccess) completionMs[s] = ms;
+            }
+        }
+    }
+    for (hipEvent_t ev : passEnd) (void)hipEventDestroy(ev);
+    return err;
+}
+
+int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_tuning* best) {
+    if (!c || !lp || !best || trialSpp == 0) return RBH_ERR_INVALID;
+    const int rc0 = check_scene(c, lp);
+    if (rc0) return rc0;
+    RenderState& rs = c->rs;
+    const size_t P = (size_t)lp->width * lp->height;
+    size_t freeB = 0, totalB = 0;
+    if (hipMemGetInfo(&freeB, &totalB) != hipSuccess) freeB = (size_t)8 << 30;
+    const size_t keep = rs.slotBudget;
+    const int keepT = rs.traceBlocksPerCU, keepS = rs.shadeBlocksPerCU;
+    rbh_launch_params t = *lp;
+    t.sampleOffset = 0;
+    t.sampleCount = trialSpp;
+    t.flags &= ~(uint32_t)(RBH_RENDER_COLLECT | RBH_RENDER_TIMING);
+    const size_t slotsList[] = {size_t(1) << 22, size_t(1) << 23, size_t(1) << 24, size_t(1) << 25, size_t(1) << 26};
+    const int gridList[] = {8, 16, 32};
+    double bestRate = -1.0;
+    rbh_tuning b{};
+    for (size_t slots : slotsList) {
+        // about 200 B of path state per slot; leave half the free memory alone
+        if (slots * 256 > freeB / 2) continue;
+        // more slots than the trial's samples need changes nothing
+        if (slots / 2 >= P * trialSpp && slots != slotsList[0]) continue;
+        for (int tg : gridList) {
+            for (int sg : gridList) {
+                rs.slotBudget = slots;
+                rs.traceBlocksPerCU = tg;
+                rs.shadeBlocksPerCU = sg;
+                const int rc = rbh_render(c, &t);
+                if (rc) {
+                    rs.slotBudget = keep;
+                    rs.traceBlocksPerCU = keepT;
+                    rs.shadeBlocksPerCU = keepS;
+                    return rc;
+                }
+                const double rate = (double)P * trialSpp / (c->stats.renderMs * 1e-3);
+                if (rate > bestRate) {
+                    bestRate = rate;
+                    b.pathSlots = slots;
+                    b.traceBlocksPerCU = tg;
+                    b.shadeBlocksPerCU = sg;
+                    b.samplesPerSecond = rate;
+                }
+            }
+        }
+    }
+    if (bestRate < 0.0) {
+        rs.slotBudget = keep;
+        rs.traceBlocksPerCU = keepT;
+        rs.shadeBlocksPerCU = keepS;
+        return fail(c, RBH_ERR_NOMEM, "no launch shape fits the free device memory");
+    }
+    rs.slotBudget = b.pathSlots;
+    rs.traceBlocksPerCU = b.traceBlocksPerCU;
+    rs.shadeBlocksPerCU = b.shadeBlocksPerCU;
+    *best = b;
+    return RBH_OK;
+}
+
+int rbh_set_tuning(rbh_ctx c, const rbh_tuning* t) {
+    if (!c || !t) return RBH_ERR_INVALID;
+    if (t->pathSlots == 0 || t->pathSlots > (1ull << 31) || t->traceBlocksPerCU < 1 || t->traceBlocksPerCU > 256 ||
+        t->shadeBlocksPerCU < 1 || t->shadeBlocksPerCU > 256)
+        return fail(c, RBH_ERR_RANGE, "tuning out of range");
+    c->rs.slotBudget = (size_t)t->pathSlots;
+    c->rs.traceBlocksPerCU = t->traceBlocksPerCU;
+    c->rs.shadeBlocksPerCU = t->shadeBlocksPerCU;
     return RBH_OK;
 }
 

@@ -48,6 +48,15 @@ class Renderer:
         self.samples = offset + spp
         return self.ctx.stats()
 
+    def autotune(self, trial_spp=4, **kw):
+        """Picks slots per pass and persistent grid sizes by timing short renders."""
+        t = L.Tuning()
+        p = self.scene.launch_params(offset=0, count=trial_spp, **kw)
+        self.ctx.call("rbh_autotune", ctypes.byref(p), trial_spp, ctypes.byref(t))
+        self.samples = 0
+        return {"pathSlots": t.pathSlots, "traceBlocksPerCU": t.traceBlocksPerCU,
+                "shadeBlocksPerCU": t.shadeBlocksPerCU, "samplesPerSecond": t.samplesPerSecond}
+
     # ------------------------------------------------------------ guiding
     def configure_guide(self, seed=1, lr=1e-3, capacity=1 << 22):
         """Sets up the guiding network (random initialisation) with the
@@ -90,3 +99,20 @@ class Renderer:
 
     def close(self):
         self.ctx.close()
+
+
+def render_batch(renderers, spp=None, flags=0):
+    """Renders several scenes (one Renderer each, same device) in one batched
+    launch sequence; the first renderer's context owns the pass state.
+    Returns the per-scene completion times in ms."""
+    n = len(renderers)
+    ctxs = (ctypes.c_void_p * n)(*[r.ctx.h for r in renderers])
+    lps = (L.LaunchParams * n)()
+    for i, r in enumerate(renderers):
+        lps[i] = r.scene.launch_params(offset=0, count=r.scene.spp if spp is None else spp, flags=flags)
+        r.ctx.width, r.ctx.height = lps[i].width, lps[i].height
+        r.samples = lps[i].sampleCount
+    done = np.zeros(n, np.float64)
+    owner = renderers[0].ctx
+    owner.call("rbh_render_batch", ctypes.cast(ctxs, ctypes.c_void_p), ctypes.cast(lps, ctypes.c_void_p), n, L.ptr(done))
+    return done

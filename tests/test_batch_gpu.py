@@ -1,0 +1,71 @@
+"""Batched multi-scene launches and the launch-shape autotuner: a scene
+rendered inside a batch accumulates exactly what it accumulates alone, and
+no launch shape changes any result."""
+import numpy as np
+import pytest
+
+from ratbite_amd import scenes
+from ratbite_amd.renderer import Renderer, render_batch
+
+pytestmark = pytest.mark.gpu
+
+
+def _scenes():
+    return [
+        scenes.cornell(width=32, height=32, spp=8),
+        scenes.spheres(n_spheres=10, level=2, n_lights=50, width=40, height=24, spp=5, env_size=(128, 64)),
+        scenes.cornell(seed=0x5EED0007, width=24, height=20, spp=11, light_up=True),
+        scenes.spheres(seed=0x5EED0009, n_spheres=4, level=1, n_lights=3, width=8, height=8, spp=3, env_size=None),
+    ]
+
+
+def test_batch_equals_individual_renders():
+    rs = [Renderer(sc) for sc in _scenes()]
+    try:
+        alone = []
+        for r in rs:
+            r.render(spp=r.scene.spp, offset=0)
+            alone.append(r.accum())
+        done = render_batch(rs)
+        print("completion ms", np.round(done, 3), flush=True)
+        assert np.all(done > 0)
+        for r, a in zip(rs, alone):
+            b = r.accum()
+            assert np.array_equal(a, b), r.scene.name
+    finally:
+        for r in rs:
+            r.close()
+
+
+def test_batch_with_small_pass_budget():
+    # a pass budget below one scene's pixel count still gives the same sums
+    rs = [Renderer(sc) for sc in _scenes()[:3]]
+    try:
+        alone = []
+        for r in rs:
+            r.render(spp=r.scene.spp, offset=0)
+            alone.append(r.accum())
+        rs[0].ctx.set_path_slots(700)
+        render_batch(rs)
+        for r, a in zip(rs, alone):
+            assert np.array_equal(a, r.accum()), r.scene.name
+    finally:
+        for r in rs:
+            r.close()
+
+
+def test_autotune_keeps_results(ctx):
+    sc = scenes.cornell(width=48, height=48, spp=8)
+    r = Renderer(sc, ctx=ctx)
+    r.render(spp=8, offset=0)
+    a = r.accum()
+    t = r.autotune(trial_spp=2)
+    print("tuning", t, flush=True)
+    assert t["samplesPerSecond"] > 0 and t["pathSlots"] > 0
+    r.render(spp=8, offset=0)
+    assert np.array_equal(a, r.accum())
+    ctx.set_path_slots(1 << 25)
+    from ratbite_amd import _lib as L
+    import ctypes
+    tt = L.Tuning(1 << 25, 16, 16, 0.0)
+    ctx.check(L.lib().rbh_set_tuning(ctx.h, ctypes.byref(tt)))

@@ -129,3 +129,23 @@ def test_adam_training_reaches_target(ctx):
     ctx.check(L.lib().rbh_guide_train(ctx.h, L.ptr(rec), n, 4096, 300, ctypes.byref(last)))
     print("loss first %.4f after 300 Adam steps %.4f" % (first.value, last.value), flush=True)
     assert first.value > 1.0 and last.value < 0.0
+
+
+def test_fp8_inference_close_to_bf16(ctx):
+    _configure(ctx)
+    p = _params(9)
+    _set(ctx, p)
+    rng = np.random.default_rng(10)
+    n = 4096
+    rec = _records(n, rng)
+    a = np.zeros((n, 40), np.float32)
+    b = np.zeros((n, 40), np.float32)
+    ctx.check(L.lib().rbh_guide_infer(ctx.h, L.ptr(rec), n, L.ptr(a)))
+    ctx.check(L.lib().rbh_guide_infer_fp8(ctx.h, L.ptr(rec), n, L.ptr(b)))
+    ps = G.unflatten(p.astype(np.float64))
+    ref, _ = G.forward(ps, G.encode(rec))
+    e16 = np.sqrt(np.mean((a - ref) ** 2)) / np.sqrt(np.mean(ref ** 2))
+    e8 = np.sqrt(np.mean((b - ref) ** 2)) / np.sqrt(np.mean(ref ** 2))
+    print("relative RMS error vs fp64: bf16 %.3g fp8 %.3g" % (e16, e8), flush=True)
+    assert e16 < 1e-2
+    assert e8 < 8e-2

@@ -68,3 +68,16 @@ def test_collect_train_and_gain(ctx):
     assert losses[-1] < losses[0]
     assert e_tr < e_untr
     assert e_tr < e_plain
+
+
+def test_fp8_guided_render_is_unbiased(ctx):
+    from ratbite_amd import _lib as L
+    sc = scenes.cornell(width=48, height=48, spp=16, light_up=True)
+    r = Renderer(sc, ctx=ctx)
+    r.configure_guide(seed=4)
+    plain = _img(r, 1024, no_drop=True)
+    g8 = _img(r, 1024, guided=True, flags=L.RBH_RENDER_GUIDE_FP8)
+    assert np.all(np.isfinite(g8))
+    mp, mg = plain.mean(axis=(0, 1)), g8.mean(axis=(0, 1))
+    print("image mean plain", mp, "guided fp8", mg, flush=True)
+    assert np.all(np.abs(mg - mp) / mp < 0.03)
