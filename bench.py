@@ -43,8 +43,14 @@ def parse():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--depth", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-spp", type=int, default=32, help="spp of the CPU baseline crop")
+    ap.add_argument("--cpu-spp", type=int, default=256, help="spp of the CPU baseline crop")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, host cores)")
+    ap.add_argument("--workload", choices=["spheres", "batch", "guided"], default="spheres",
+                    help="spheres: the 1080p frame (default); batch: 64 seeded scenes in one batched launch; "
+                         "guided: relMSE at equal time, guided vs plain")
+    ap.add_argument("--scenes", type=int, default=64, help="batch workload: number of scenes")
+    ap.add_argument("--budget-ms", type=float, default=4000.0, help="guided workload: time budget per method")
+    ap.add_argument("--ref-spp", type=int, default=2048, help="guided workload: spp of the plain reference")
     return ap.parse_args()
 
 
@@ -79,6 +85,135 @@ def cpu_baseline(sc, args, tree):
                                                              st["bvhSeconds"])}
 
 
+def batch_workload(args):
+    """64 seeded scenes of uneven size (triangles log-uniform in [1k, 500k],
+    films from 64^2 to 1024^2, 1..5000 emissive triangles, the material mix of
+    the spheres scene, 32 spp, depth 12) rendered in one batched launch
+    sequence after autotuning the launch shape; plus 64 env-map CDF builds
+    from 256x128 to 4096x2048. Reports aggregate samples/s and the p50/p99
+    per-scene completion times."""
+    import numpy as np
+    from ratbite_amd import scenes
+    from ratbite_amd.envmaps import lognormal_envmap
+    from ratbite_amd.renderer import Renderer, render_batch
+    rng = np.random.default_rng(0x5EED0003)
+    sides = [64, 128, 256, 512, 1024]
+    rs = []
+    t0 = time.time()
+    for i in range(args.scenes):
+        tris = float(np.exp(rng.uniform(np.log(1e3), np.log(5e5))))
+        level = 3 if tris > 2e5 else 2
+        per = 20 * 4 ** level
+        n_sph = max(1, int(round(tris / per)))
+        side = int(rng.choice(sides))
+        n_lights = int(np.exp(rng.uniform(0.0, np.log(5000.0))))
+        env = None if i % 2 else (256, 128)
+        sc = scenes.spheres(seed=0x5EED0003 + i, n_spheres=n_sph, level=level, n_lights=n_lights, width=side,
+                            height=side, spp=32, depth=12, env_size=env)
+        rs.append(Renderer(sc))
+    print("batch: %d scenes built in %.1f s" % (len(rs), time.time() - t0), file=sys.stderr, flush=True)
+    tuning = rs[0].autotune(trial_spp=2)
+    print("autotuned launch shape %s" % tuning, file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        render_batch(rs)
+    rs[0].ctx.synchronize()
+    t1 = time.perf_counter()
+    done = None
+    for _ in range(args.steps):
+        done = render_batch(rs)
+    rs[0].ctx.synchronize()
+    elapsed = time.perf_counter() - t1
+    samples = sum(r.scene.width * r.scene.height * r.scene.spp for r in rs) * args.steps
+    # env-map CDF builds alongside: 64 maps from 256x128 to 4096x2048
+    ctx = rs[0].ctx
+    cdf_ms = []
+    for i in range(64):
+        w = 256 << (i % 5)
+        tex = lognormal_envmap(w, w // 2, sigma=2.0, seed=1000 + i, sun=False)
+        ctx.envmap_build(tex)
+        cdf_ms.append(ctx.stats()["cdfMs"])
+    st = rs[0].ctx.stats()
+    out = {
+        "metric": "batched samples/s", "value": samples / elapsed, "unit": "samples/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded scene generator)",
+        "config": {"workload": "%d seeded scenes, 32 spp, depth 12, batched" % len(rs),
+                   "triangles": int(sum(len(r.scene.indices) for r in rs)),
+                   "pixels": int(sum(r.scene.width * r.scene.height for r in rs))},
+        "completion_ms_p50": float(np.percentile(done, 50)), "completion_ms_p99": float(np.percentile(done, 99)),
+        "mrays_per_s": (st["closestRays"] + st["shadowRays"]) / (st["renderMs"] * 1e-3) / 1e6,
+        "tuning": tuning,
+        "cdf_builds": {"count": 64, "total_ms": float(sum(cdf_ms)), "max_ms": float(max(cdf_ms))},
+    }
+    print(json.dumps(out), flush=True)
+    for r in rs:
+        r.close()
+
+
+def guided_workload(args):
+    """relMSE against a high-spp plain render at equal wall time: the plain
+    tracer renders for the budget; the guided tracer spends the same budget
+    on collecting, training and rendering. Both use the no-drop sample
+    semantics so they converge to the same image."""
+    import numpy as np
+    from ratbite_amd import scenes
+    from ratbite_amd.renderer import Renderer
+
+    def relmse(a, ref, eps=1e-2):
+        return float(np.mean((a - ref) ** 2 / (ref ** 2 + eps)))
+
+    sc = scenes.spheres(width=args.width, height=args.height, spp=16, depth=args.depth)
+    r = Renderer(sc)
+
+    def image():
+        a = r.accum()
+        return a[..., :3] / a[..., 3:4]
+
+    t0 = time.perf_counter()
+    done = 0
+    while done < args.ref_spp:
+        step = min(64, args.ref_spp - done)
+        r.render(spp=step, offset=done, no_drop=True)
+        done += step
+    ref = image()
+    print("reference: %d spp in %.1f s" % (args.ref_spp, time.perf_counter() - t0), file=sys.stderr, flush=True)
+
+    def run(method):
+        r.ctx.synchronize()
+        t = time.perf_counter()
+        spp = 0
+        curve = []
+        trained = 0
+        while True:
+            el = (time.perf_counter() - t) * 1e3
+            if el >= args.budget_ms:
+                break
+            if method == "guided" and trained < 16:
+                r.render(spp=4, offset=100000 + 4 * trained, guided=True, collect=True)
+                r.train_guide(batch=1 << 18, steps=4)
+                trained += 1
+                continue
+            r.render(spp=4, offset=spp, guided=(method == "guided"), no_drop=True)
+            spp += 4
+            curve.append(((time.perf_counter() - t) * 1e3, spp))
+        return relmse(image(), ref), spp, curve
+
+    e_plain, spp_plain, _ = run("plain")
+    r.configure_guide(seed=1, lr=1e-3, capacity=1 << 22)
+    e_guided, spp_guided, _ = run("guided")
+    out = {
+        "metric": "relMSE at equal time, guided vs plain", "value": e_guided / e_plain, "unit": "ratio",
+        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": args.budget_ms, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+bf16", "data": "synthetic (seeded scene generator)",
+        "config": {"workload": "spheres %dx%d, depth %d, %d ms per method, reference %d spp plain"
+                               % (sc.width, sc.height, args.depth, args.budget_ms, args.ref_spp)},
+        "relmse_plain": e_plain, "relmse_guided": e_guided, "spp_plain": spp_plain, "spp_guided": spp_guided,
+    }
+    print(json.dumps(out), flush=True)
+    r.close()
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -87,6 +222,10 @@ def main():
             print("bench.py measures one GPU: the path tracer runs on a single MI355X here, "
                   "so --gpus %d (world size %d) is not supported" % (args.gpus, world), file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batch":
+        return batch_workload(args)
+    if args.workload == "guided":
+        return guided_workload(args)
 
     from ratbite_amd import _lib as L
     from ratbite_amd import scenes
@@ -101,6 +240,8 @@ def main():
     st0 = ctx.stats()
     print("LBVH %.1f ms depth %d; env CDF %.3f ms" % (st0["bvhMs"], st0["bvhDepth"], st0["cdfMs"]),
           file=sys.stderr, flush=True)
+    tuning = r.autotune(trial_spp=8, depth=args.depth)
+    print("autotuned launch shape %s" % tuning, file=sys.stderr, flush=True)
 
     params = sc.launch_params(offset=0, count=args.spp, flags=L.RBH_RENDER_TIMING)
     for _ in range(args.warmup):
@@ -131,7 +272,8 @@ def main():
     launches = tot["traceLaunches"]
     achieved = kbytes[dom] / (kms[dom] * 1e-3) / 1e9
     traffic = traffic_per_launch(KERNEL_NAMES[dom])
-    path_bytes = (BYTES_CLOSEST + BYTES_SHADOW + BYTES_VERTEX) * tot["pathVertices"] + \
+    # every traced ray is a path vertex (misses are shaded by the miss kernel)
+    path_bytes = (BYTES_CLOSEST + BYTES_SHADOW + BYTES_VERTEX) * tot["closestRays"] + \
         BYTES_PIXEL_LAUNCH * pixels * args.spp * args.steps
 
     out = {
@@ -153,6 +295,7 @@ def main():
         "path_roofline": {"achieved": path_bytes / elapsed / 1e9, "unit": "GB/s",
                           "frac": path_bytes / elapsed / 1e9 / HBM_PEAK_GBS},
         "cpu_baseline": None,
+        "tuning": tuning,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, args, r.light_tree())

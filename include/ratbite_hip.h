@@ -204,9 +204,10 @@ int rbh_envmap_disable(rbh_ctx ctx);
  * work has finished; rbh_stats_get then describes this launch. */
 int rbh_render(rbh_ctx ctx, const rbh_launch_params* p);
 /* Path slots of one wavefront pass: a pass renders max(1, slots / (W*H))
- * samples of every pixel at once, about 200 B of device memory per slot
- * (default 2^25). Results do not depend on it. No reference counterpart: the
- * OptiX megakernel keeps each path's state in registers. */
+ * samples of every pixel at once, about 200 B of device memory per slot.
+ * 0 (the default) takes all samples of a launch in one pass when a quarter of
+ * the free device memory holds them. Results do not depend on it. No
+ * reference counterpart: the OptiX megakernel keeps path state in registers. */
 int rbh_set_path_slots(rbh_ctx ctx, uint64_t slots);
 /* Many scenes in one wavefront launch sequence (no reference counterpart;
  * the reference renders one scene per optixLaunch). Each scene is a context
@@ -220,9 +221,11 @@ int rbh_render_batch(rbh_ctx owner, const rbh_ctx* scenes, const rbh_launch_para
                      double* completionMs);
 /* Launch shape of the wavefront passes (no reference counterpart). */
 typedef struct rbh_tuning {
-    uint64_t pathSlots;           /* slots per pass (see rbh_set_path_slots) */
+    uint64_t pathSlots;           /* slots per pass (see rbh_set_path_slots; 0 automatic) */
     int32_t traceBlocksPerCU;     /* persistent grid of the trace / shadow kernels */
     int32_t shadeBlocksPerCU;     /* persistent grid of the shade / miss kernels */
+    int32_t useGraph;             /* replay each pass as a captured HIP graph (ignored with RBH_RENDER_TIMING) */
+    int32_t shadeWaves;           /* shade-kernel register budget: 1 (unconstrained), 3 or 4 waves per SIMD */
     double samplesPerSecond;      /* measured by rbh_autotune */
 } rbh_tuning;
 /* Times trialSpp-sample renders of lp over a grid of slot budgets and grid

@@ -121,12 +121,19 @@ struct RenderState {
     DevBuf gmix;      // guided renders: mixture per hit-queue entry
     size_t pathCapacity = 0;
     // path slots one pass may hold (samples per pass = slotBudget / pixels);
-    // about 200 B of HBM per slot
-    size_t slotBudget = size_t(1) << 25;
+    // about 200 B of HBM per slot; 0 = as many as a quarter of the free
+    // device memory holds (fewer passes pay the deep-bounce tail fewer times)
+    size_t slotBudget = 0;
     int traceBlocksPerCU = 16;   // persistent-grid sizes (autotuned)
     int shadeBlocksPerCU = 16;
+    int shadeWaves = 1;          // shade-kernel variant (1, 3, 4 waves per SIMD)
     DevBuf sceneTable;           // batched passes: SceneView per scene and pass
+    bool useGraph = false;       // replay each pass's launch sequence as a captured HIP graph
+    hipGraphExec_t graphExec = nullptr;
     KernelTimer timer;
+    ~RenderState() {
+        if (graphExec) (void)hipGraphExecDestroy(graphExec);
+    }
 };
 
 }  // namespace rbh

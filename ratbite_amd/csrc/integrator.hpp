@@ -78,6 +78,7 @@ struct SceneView {
 // Persistent-grid sizes of the trace/shadow and shade/miss kernels (blocks)
 struct LaunchShape {
     int traceGrid, shadeGrid;
+    int shadeWaves;          // register budget of the shade kernel: 1 (free), 3 or 4 waves per SIMD
 };
 
 struct Frame1 {

@@ -359,8 +359,11 @@ __device__ __forceinline__ void quat_cols(float4 q, V3& c0, V3& c2) {
     c2 = V3{2.0f * (x * z + w * y), 2.0f * (y * z - w * x), 1.0f - 2.0f * (x * x + y * y)};
 }
 
-template <bool GUIDED, bool BATCH>
-__global__ __launch_bounds__(kShadeBlock) void k_shade(Frame1 f, uint32_t qsel) {
+// WAVES: minimum resident waves per SIMD the register allocation must allow
+// (1 = unconstrained); the autotuner picks between fewer spills and more
+// latency hiding.
+template <bool GUIDED, bool BATCH, int WAVES>
+__global__ __launch_bounds__(kShadeBlock, WAVES) void k_shade(Frame1 f, uint32_t qsel) {
     // shades the paths whose ray hit a triangle (misses were finished by k_trace)
     const uint32_t count = f.counters[CNT_HIT];
     const uint32_t* __restrict__ qin = f.p.hq;
@@ -867,10 +870,16 @@ static hipError_t launch_pass_t(rbh_ctx c, Frame1 f, uint32_t depthMax, const La
                       hipLaunchKernelGGL(k_guide_paths, dim3(guideGrid), dim3(256), 0, c->stream, f));
         hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, c->stream, f.counters, qsel ? CNT_QUEUE0 : CNT_QUEUE1);
         if (!BATCH && f.guided)
-            RBH_TIMED(KernelTimer::SHADE, hipLaunchKernelGGL((k_shade<true, false>), dim3(ls.shadeGrid),
+            RBH_TIMED(KernelTimer::SHADE, hipLaunchKernelGGL((k_shade<true, false, 1>), dim3(ls.shadeGrid),
+                                                             dim3(kShadeBlock), 0, c->stream, f, qsel));
+        else if (ls.shadeWaves >= 4)
+            RBH_TIMED(KernelTimer::SHADE, hipLaunchKernelGGL((k_shade<false, BATCH, 4>), dim3(ls.shadeGrid),
+                                                             dim3(kShadeBlock), 0, c->stream, f, qsel));
+        else if (ls.shadeWaves == 3)
+            RBH_TIMED(KernelTimer::SHADE, hipLaunchKernelGGL((k_shade<false, BATCH, 3>), dim3(ls.shadeGrid),
                                                              dim3(kShadeBlock), 0, c->stream, f, qsel));
         else
-            RBH_TIMED(KernelTimer::SHADE, hipLaunchKernelGGL((k_shade<false, BATCH>), dim3(ls.shadeGrid),
+            RBH_TIMED(KernelTimer::SHADE, hipLaunchKernelGGL((k_shade<false, BATCH, 1>), dim3(ls.shadeGrid),
                                                              dim3(kShadeBlock), 0, c->stream, f, qsel));
         RBH_TIMED(KernelTimer::SHADOW,
                   hipLaunchKernelGGL(k_shadow<BATCH>, dim3(ls.traceGrid), dim3(kTraceBlock), 0, c->stream, f));

@@ -6,7 +6,9 @@
 //   3. LSD radix sort of the keys (rocPRIM via hipCUB)
 //   4. Karras (HPG 2012) hierarchy: internal node i covers a key range and
 //      splits at the highest differing bit
-//   5. bottom-up refit with per-node arrival counters
+//   5. refit level by level from the deepest internal nodes up (one launch
+//      per depth, so kernel boundaries order the box writes; no cross-XCD
+//      fences or arrival counters)
 //   6. emission of 64-byte two-child nodes (both child boxes in the parent)
 //      and of triangles in leaf order
 // Every root-to-leaf path strictly lengthens the common key prefix, so the
@@ -137,11 +139,10 @@ __global__ __launch_bounds__(kBuildBlock) void k_karras(const unsigned long long
 
 struct Box { float lo[3], hi[3]; };
 
-__global__ __launch_bounds__(kBuildBlock) void k_refit(const float* __restrict__ v, const uint32_t* __restrict__ idx,
-                                                       const unsigned long long* __restrict__ keys, int n,
-                                                       const int2* __restrict__ child, const int* __restrict__ parent,
-                                                       float* __restrict__ ibox, float* __restrict__ lbox,
-                                                       unsigned int* __restrict__ flags, TriRecord* __restrict__ tris) {
+// Triangle records in leaf (Morton) order and their boxes.
+__global__ __launch_bounds__(kBuildBlock) void k_leaves(const float* __restrict__ v, const uint32_t* __restrict__ idx,
+                                                        const unsigned long long* __restrict__ keys, int n,
+                                                        float* __restrict__ lbox, TriRecord* __restrict__ tris) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     const uint32_t tri = (uint32_t)(keys[j] & 0xFFFFFFFFull);
@@ -155,20 +156,54 @@ __global__ __launch_bounds__(kBuildBlock) void k_refit(const float* __restrict__
     lb[0] = fminf(fminf(r.v0x, r.v1x), r.v2x); lb[3] = fmaxf(fmaxf(r.v0x, r.v1x), r.v2x);
     lb[1] = fminf(fminf(r.v0y, r.v1y), r.v2y); lb[4] = fmaxf(fmaxf(r.v0y, r.v1y), r.v2y);
     lb[2] = fminf(fminf(r.v0z, r.v1z), r.v2z); lb[5] = fmaxf(fmaxf(r.v0z, r.v1z), r.v2z);
-    int p = n > 1 ? parent[n - 1 + j] : -1;
-    while (p >= 0) {
-        __threadfence();
-        if (atomicAdd(&flags[p], 1u) == 0u) return;  // sibling not done yet
-        __threadfence();
+}
+
+constexpr int kLevels = 65;   // internal-node depths 0..63 (64 collects anything deeper)
+
+// Depth of every internal node (ancestors up to the root) and a histogram.
+__global__ __launch_bounds__(kBuildBlock) void k_node_depth(int n, const int* __restrict__ parent,
+                                                            int* __restrict__ depth, int* __restrict__ count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    int d = 0;
+    for (int p = parent[i]; p >= 0 && d < kLevels - 1; p = parent[p]) ++d;
+    depth[i] = d;
+    atomicAdd(&count[d], 1);
+}
+
+__global__ void k_level_starts(const int* __restrict__ count, int* __restrict__ start, int* __restrict__ cursor) {
+    if (threadIdx.x != 0) return;
+    int acc = 0;
+    for (int d = 0; d < kLevels; ++d) {
+        start[d] = acc;
+        cursor[d] = acc;
+        acc += count[d];
+    }
+    start[kLevels] = acc;
+}
+
+__global__ __launch_bounds__(kBuildBlock) void k_node_scatter(int n, const int* __restrict__ depth,
+                                                              int* __restrict__ cursor, int* __restrict__ order) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    order[atomicAdd(&cursor[depth[i]], 1)] = i;
+}
+
+// Boxes of the internal nodes at one depth from their (deeper) children.
+__global__ __launch_bounds__(kBuildBlock) void k_refit_level(int level, const int* __restrict__ start,
+                                                             const int* __restrict__ order, const int2* __restrict__ child,
+                                                             float* __restrict__ ibox, const float* __restrict__ lbox) {
+    const int b = start[level], e = start[level + 1];
+    for (int t = b + blockIdx.x * blockDim.x + threadIdx.x; t < e; t += gridDim.x * blockDim.x) {
+        const int p = order[t];
         const int2 ch = child[p];
         const float* A = ch.x >= 0 ? ibox + 6 * (size_t)ch.x : lbox + 6 * (size_t)(~ch.x);
         const float* B = ch.y >= 0 ? ibox + 6 * (size_t)ch.y : lbox + 6 * (size_t)(~ch.y);
         float* o = ibox + 6 * (size_t)p;
         for (int k = 0; k < 3; ++k) {
-            o[k] = fminf(__builtin_nontemporal_load(A + k), __builtin_nontemporal_load(B + k));
-            o[3 + k] = fmaxf(__builtin_nontemporal_load(A + 3 + k), __builtin_nontemporal_load(B + 3 + k));
+            o[k] = fminf(A[k], B[k]);
+            o[3 + k] = fmaxf(A[3 + k], B[3 + k]);
         }
-        p = parent[p];
     }
 }
 
@@ -223,7 +258,9 @@ int lbvh_build(rbh_ctx c) {
     const size_t oParent = off; off = align_up(off + ((size_t)n + nInt) * 4);
     const size_t oIbox = off; off = align_up(off + nInt * 24);
     const size_t oLbox = off; off = align_up(off + (size_t)n * 24);
-    const size_t oFlags = off; off = align_up(off + nInt * 4);
+    const size_t oDepth = off; off = align_up(off + nInt * 4);
+    const size_t oOrder = off; off = align_up(off + nInt * 4);
+    const size_t oLevels = off; off = align_up(off + 3 * (kLevels + 1) * 4);
     size_t sortBytes = 0;
     RBH_HIP(c, hipcub::DeviceRadixSort::SortKeys(nullptr, sortBytes, (unsigned long long*)nullptr,
                                                  (unsigned long long*)nullptr, n, 0, 64, c->stream));
@@ -239,7 +276,11 @@ int lbvh_build(rbh_ctx c) {
     int* parent = reinterpret_cast<int*>(s + oParent);
     float* ibox = reinterpret_cast<float*>(s + oIbox);
     float* lbox = reinterpret_cast<float*>(s + oLbox);
-    unsigned int* flags = reinterpret_cast<unsigned int*>(s + oFlags);
+    int* ndepth = reinterpret_cast<int*>(s + oDepth);
+    int* order = reinterpret_cast<int*>(s + oOrder);
+    int* lcount = reinterpret_cast<int*>(s + oLevels);
+    int* lstart = lcount + (kLevels + 1);
+    int* lcursor = lstart + (kLevels + 1);
 
     const float* v = g.verts.as<float>();
     const uint32_t* idx = g.indices.as<uint32_t>();
@@ -250,10 +291,20 @@ int lbvh_build(rbh_ctx c) {
     hipLaunchKernelGGL(k_morton, dim3(nb), dim3(kBuildBlock), 0, c->stream, v, idx, (uint32_t)n, bounds, keys);
     RBH_HIP(c, hipcub::DeviceRadixSort::SortKeys(s + oSort, sortBytes, keys, keys2, n, 0, 64, c->stream));
     const int nbi = (int)((nInt + kBuildBlock - 1) / kBuildBlock);
-    hipLaunchKernelGGL(k_zero_u32, dim3(nbi), dim3(kBuildBlock), 0, c->stream, flags, nInt);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kBuildBlock), 0, c->stream, reinterpret_cast<unsigned int*>(lcount),
+                       (size_t)(kLevels + 1));
     if (n > 1) hipLaunchKernelGGL(k_karras, dim3(nbi), dim3(kBuildBlock), 0, c->stream, keys2, n, child, parent);
-    hipLaunchKernelGGL(k_refit, dim3(nb), dim3(kBuildBlock), 0, c->stream, v, idx, keys2, n, child, parent, ibox, lbox,
-                       flags, acc.tris.as<TriRecord>());
+    hipLaunchKernelGGL(k_leaves, dim3(nb), dim3(kBuildBlock), 0, c->stream, v, idx, keys2, n, lbox, acc.tris.as<TriRecord>());
+    if (n > 1) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0) cus = 256;
+        hipLaunchKernelGGL(k_node_depth, dim3(nbi), dim3(kBuildBlock), 0, c->stream, n, parent, ndepth, lcount);
+        hipLaunchKernelGGL(k_level_starts, dim3(1), dim3(64), 0, c->stream, lcount, lstart, lcursor);
+        hipLaunchKernelGGL(k_node_scatter, dim3(nbi), dim3(kBuildBlock), 0, c->stream, n, ndepth, lcursor, order);
+        const int lg = std::min(nbi, cus * 8);
+        for (int d = kLevels - 1; d >= 0; --d)
+            hipLaunchKernelGGL(k_refit_level, dim3(lg), dim3(kBuildBlock), 0, c->stream, d, lstart, order, child, ibox, lbox);
+    }
     hipLaunchKernelGGL(k_depth, dim3(nb), dim3(kBuildBlock), 0, c->stream, n, parent, bounds + 6);
     if (n > 1) hipLaunchKernelGGL(k_emit, dim3(nbi), dim3(kBuildBlock), 0, c->stream, n, child, ibox, lbox, acc.nodes.as<BvhNode>());
     RBH_HIP(c, hipGetLastError());

@@ -223,11 +223,57 @@ int prepare_pass_state(rbh_ctx c, size_t NP, Frame1& f, LaunchShape& ls) {
     const int cus = cu_count(c->device);
     ls.traceGrid = std::max(1, cus * rs.traceBlocksPerCU);
     ls.shadeGrid = std::max(1, cus * rs.shadeBlocksPerCU);
+    ls.shadeWaves = rs.shadeWaves;
     const size_t lanes = (size_t)ls.traceGrid * kTraceBlock;
     RBH_ALLOC(c, rs.spill, trace_spill_words(lanes) * sizeof(int));
     f.spill = rs.spill.as<int>();
     f.spillStride = (uint32_t)lanes;
     return RBH_OK;
+}
+
+// One pass: launched directly, or (useGraph, no per-kernel timing) captured
+// into a graph whose executable is updated in place for each pass (same
+// topology, new kernel arguments) and replayed. The pass has no memset nodes:
+// its counters are cleared by kernels.
+hipError_t run_pass(rbh_ctx c, const Frame1& f, uint32_t depth, const LaunchShape& ls, KernelTimer* kt) {
+    RenderState& rs = c->rs;
+    if (!rs.useGraph || kt) return launch_pass(c, f, depth, ls, kt);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    const hipError_t le = launch_pass(c, f, depth, ls, nullptr);
+    e = hipStreamEndCapture(c->stream, &g);
+    if (le != hipSuccess) { if (g) (void)hipGraphDestroy(g); return le; }
+    if (e != hipSuccess) return e;
+    bool fresh = rs.graphExec == nullptr;
+    if (!fresh) {
+        hipGraphNode_t errNode = nullptr;
+        hipGraphExecUpdateResult res;
+        if (hipGraphExecUpdate(rs.graphExec, g, &errNode, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
+            (void)hipGetLastError();
+            (void)hipGraphExecDestroy(rs.graphExec);
+            rs.graphExec = nullptr;
+            fresh = true;
+        }
+    }
+    if (fresh) {
+        e = hipGraphInstantiate(&rs.graphExec, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) { rs.graphExec = nullptr; (void)hipGraphDestroy(g); return e; }
+    }
+    (void)hipGraphDestroy(g);
+    return hipGraphLaunch(rs.graphExec, c->stream);
+}
+
+// Path slots of a pass: the context's budget, or (budget 0) all the samples
+// of the launch if a quarter of the free device memory holds them.
+size_t slot_budget(rbh_ctx c, size_t want, size_t bytesPerSlot) {
+    if (c->rs.slotBudget) return c->rs.slotBudget;
+    size_t freeB = 0, totalB = 0;
+    if (hipMemGetInfo(&freeB, &totalB) != hipSuccess) freeB = (size_t)16 << 30;
+    // memory already held by this context's path buffers is reusable
+    freeB += c->rs.paths.bytes;
+    const size_t cap = std::max<size_t>(size_t(1) << 20, freeB / 4 / bytesPerSlot);
+    return std::min<size_t>(std::min(want, cap), size_t(1) << 30);
 }
 
 int finish_stats(rbh_ctx c, const Frame1& f, KernelTimer* kt, uint64_t samples, uint32_t launches) {
@@ -279,7 +325,8 @@ int rbh_render(rbh_ctx c, const rbh_launch_params* lp) {
     const size_t P = (size_t)lp->width * lp->height;
     // samples per pass: as many as fit the path-slot budget (at least one)
     const uint32_t spp = lp->sampleCount;
-    const uint32_t S = (uint32_t)std::max<size_t>(1, std::min<size_t>(spp ? spp : 1, rs.slotBudget / P));
+    const size_t budget = slot_budget(c, P * std::max(1u, spp), guided ? 480 : 200);
+    const uint32_t S = (uint32_t)std::max<size_t>(1, std::min<size_t>(spp ? spp : 1, budget / P));
     const size_t NP = P * S;
     rc = prepare_accum(c, lp);
     if (rc) return rc;
@@ -329,7 +376,7 @@ int rbh_render(rbh_ctx c, const rbh_launch_params* lp) {
         f.sc.slot0 = 0;
         f.sc.first = (lp->sampleOffset == 0 && done == 0) ? 1u : 0u;
         f.NP = (uint32_t)(P * f.sc.S);
-        RBH_HIP(c, launch_pass(c, f, f.sc.maxDepth, ls, kt));
+        RBH_HIP(c, run_pass(c, f, f.sc.maxDepth, ls, kt));
         done += f.sc.S;
         ++passes;
     }
@@ -361,7 +408,8 @@ int rbh_render_batch(rbh_ctx owner, const rbh_ctx* scenes, const rbh_launch_para
         depth = std::max(depth, std::max(1u, lps[s].maxPathDepth));
     }
     RenderState& rs = owner->rs;
-    const uint32_t S = (uint32_t)std::max<uint64_t>(1, rs.slotBudget / std::max<uint64_t>(1, totalP));
+    const size_t budget = slot_budget(owner, (size_t)totalSamples, 200);
+    const uint32_t S = (uint32_t)std::max<uint64_t>(1, budget / std::max<uint64_t>(1, totalP));
     std::vector<SceneView> base(n);
     for (uint32_t s = 0; s < n; ++s) {
         fill_view(scenes[s], &lps[s], base[s]);
@@ -414,7 +462,7 @@ int rbh_render_batch(rbh_ctx owner, const rbh_ctx* scenes, const rbh_launch_para
         f.nscenes = (uint32_t)tb.size();
         f.NP = slot;
         f.maxP = maxP;
-        const hipError_t e = launch_pass(owner, f, depth, ls, kt);
+        const hipError_t e = run_pass(owner, f, depth, ls, kt);
         if (e != hipSuccess) { err = fail(owner, RBH_ERR_HIP, "launch failed: %s", hipGetErrorString(e)); break; }
         hipEvent_t ev = nullptr;
         if (completionMs && hipEventCreate(&ev) == hipSuccess) {
@@ -452,36 +500,65 @@ int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_
     t.sampleOffset = 0;
     t.sampleCount = trialSpp;
     t.flags &= ~(uint32_t)(RBH_RENDER_COLLECT | RBH_RENDER_TIMING);
-    const size_t slotsList[] = {size_t(1) << 22, size_t(1) << 23, size_t(1) << 24, size_t(1) << 25, size_t(1) << 26};
+    // the slot budget stays automatic (all samples of a launch in one pass
+    // when memory allows); the grids of the persistent kernels are searched
     const int gridList[] = {8, 16, 32};
     double bestRate = -1.0;
     rbh_tuning b{};
-    for (size_t slots : slotsList) {
-        // about 200 B of path state per slot; leave half the free memory alone
-        if (slots * 256 > freeB / 2) continue;
-        // more slots than the trial's samples need changes nothing
-        if (slots / 2 >= P * trialSpp && slots != slotsList[0]) continue;
-        for (int tg : gridList) {
-            for (int sg : gridList) {
-                rs.slotBudget = slots;
-                rs.traceBlocksPerCU = tg;
-                rs.shadeBlocksPerCU = sg;
-                const int rc = rbh_render(c, &t);
-                if (rc) {
-                    rs.slotBudget = keep;
-                    rs.traceBlocksPerCU = keepT;
-                    rs.shadeBlocksPerCU = keepS;
-                    return rc;
-                }
-                const double rate = (double)P * trialSpp / (c->stats.renderMs * 1e-3);
-                if (rate > bestRate) {
-                    bestRate = rate;
-                    b.pathSlots = slots;
-                    b.traceBlocksPerCU = tg;
-                    b.shadeBlocksPerCU = sg;
-                    b.samplesPerSecond = rate;
-                }
+    (void)freeB;
+    for (int tg : gridList) {
+        for (int sg : gridList) {
+            rs.slotBudget = keep;
+            rs.traceBlocksPerCU = tg;
+            rs.shadeBlocksPerCU = sg;
+            const int rc = rbh_render(c, &t);
+            if (rc) {
+                rs.traceBlocksPerCU = keepT;
+                rs.shadeBlocksPerCU = keepS;
+                return rc;
             }
+            const double rate = (double)P * trialSpp / (c->stats.renderMs * 1e-3);
+            if (rate > bestRate) {
+                bestRate = rate;
+                b.pathSlots = keep;
+                b.traceBlocksPerCU = tg;
+                b.shadeBlocksPerCU = sg;
+                b.samplesPerSecond = rate;
+            }
+        }
+    }
+    if (bestRate >= 0.0) {
+        // shade-kernel register budgets on the best grid
+        for (int w : {3, 4}) {
+            rs.traceBlocksPerCU = b.traceBlocksPerCU;
+            rs.shadeBlocksPerCU = b.shadeBlocksPerCU;
+            rs.shadeWaves = w;
+            const int rc = rbh_render(c, &t);
+            rs.shadeWaves = 1;
+            if (rc) return rc;
+            const double rate = (double)P * trialSpp / (c->stats.renderMs * 1e-3);
+            if (rate > bestRate) {
+                bestRate = rate;
+                b.samplesPerSecond = rate;
+                b.shadeWaves = w;
+            }
+        }
+    }
+    if (bestRate >= 0.0) {
+        // the best shape once more as a replayed graph
+        rs.slotBudget = b.pathSlots;
+        rs.traceBlocksPerCU = b.traceBlocksPerCU;
+        rs.shadeBlocksPerCU = b.shadeBlocksPerCU;
+        rs.useGraph = true;
+        rs.shadeWaves = b.shadeWaves ? b.shadeWaves : 1;
+        const int rc = rbh_render(c, &t);
+        rs.useGraph = false;
+        if (rc) return rc;
+        const double rate = (double)P * trialSpp / (c->stats.renderMs * 1e-3);
+        if (rate > bestRate) {
+            bestRate = rate;
+            b.samplesPerSecond = rate;
+            b.useGraph = 1;
         }
     }
     if (bestRate < 0.0) {
@@ -493,24 +570,29 @@ int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_
     rs.slotBudget = b.pathSlots;
     rs.traceBlocksPerCU = b.traceBlocksPerCU;
     rs.shadeBlocksPerCU = b.shadeBlocksPerCU;
+    rs.useGraph = b.useGraph != 0;
+    rs.shadeWaves = b.shadeWaves ? b.shadeWaves : 1;
+    if (!b.shadeWaves) b.shadeWaves = 1;
     *best = b;
     return RBH_OK;
 }
 
 int rbh_set_tuning(rbh_ctx c, const rbh_tuning* t) {
     if (!c || !t) return RBH_ERR_INVALID;
-    if (t->pathSlots == 0 || t->pathSlots > (1ull << 31) || t->traceBlocksPerCU < 1 || t->traceBlocksPerCU > 256 ||
+    if (t->pathSlots > (1ull << 30) || t->traceBlocksPerCU < 1 || t->traceBlocksPerCU > 256 ||
         t->shadeBlocksPerCU < 1 || t->shadeBlocksPerCU > 256)
         return fail(c, RBH_ERR_RANGE, "tuning out of range");
     c->rs.slotBudget = (size_t)t->pathSlots;
     c->rs.traceBlocksPerCU = t->traceBlocksPerCU;
     c->rs.shadeBlocksPerCU = t->shadeBlocksPerCU;
+    c->rs.useGraph = t->useGraph != 0;
+    c->rs.shadeWaves = (t->shadeWaves == 3 || t->shadeWaves == 4) ? t->shadeWaves : 1;
     return RBH_OK;
 }
 
 int rbh_set_path_slots(rbh_ctx c, uint64_t slots) {
     if (!c) return RBH_ERR_INVALID;
-    if (slots == 0 || slots > (1ull << 31)) return fail(c, RBH_ERR_RANGE, "path slots must be in [1, 2^31]");
+    if (slots > (1ull << 30)) return fail(c, RBH_ERR_RANGE, "path slots must be at most 2^30 (0: automatic)");
     c->rs.slotBudget = (size_t)slots;
     return RBH_OK;
 }

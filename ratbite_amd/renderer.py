@@ -55,7 +55,12 @@ class Renderer:
         self.ctx.call("rbh_autotune", ctypes.byref(p), trial_spp, ctypes.byref(t))
         self.samples = 0
         return {"pathSlots": t.pathSlots, "traceBlocksPerCU": t.traceBlocksPerCU,
-                "shadeBlocksPerCU": t.shadeBlocksPerCU, "samplesPerSecond": t.samplesPerSecond}
+                "shadeBlocksPerCU": t.shadeBlocksPerCU, "useGraph": t.useGraph, "shadeWaves": t.shadeWaves,
+                "samplesPerSecond": t.samplesPerSecond}
+
+    def set_tuning(self, path_slots=0, trace_blocks=16, shade_blocks=16, use_graph=False, shade_waves=1):
+        t = L.Tuning(path_slots, trace_blocks, shade_blocks, int(use_graph), shade_waves, 0.0)
+        self.ctx.call("rbh_set_tuning", ctypes.byref(t))
 
     # ------------------------------------------------------------ guiding
     def configure_guide(self, seed=1, lr=1e-3, capacity=1 << 22):

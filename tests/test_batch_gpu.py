@@ -47,6 +47,7 @@ def test_batch_with_small_pass_budget():
             alone.append(r.accum())
         rs[0].ctx.set_path_slots(700)
         render_batch(rs)
+        rs[0].ctx.set_path_slots(0)
         for r, a in zip(rs, alone):
             assert np.array_equal(a, r.accum()), r.scene.name
     finally:
@@ -64,8 +65,30 @@ def test_autotune_keeps_results(ctx):
     assert t["samplesPerSecond"] > 0 and t["pathSlots"] > 0
     r.render(spp=8, offset=0)
     assert np.array_equal(a, r.accum())
-    ctx.set_path_slots(1 << 25)
-    from ratbite_amd import _lib as L
-    import ctypes
-    tt = L.Tuning(1 << 25, 16, 16, 0.0)
-    ctx.check(L.lib().rbh_set_tuning(ctx.h, ctypes.byref(tt)))
+    for w in (3, 4):
+        r.set_tuning(shade_waves=w)
+        r.render(spp=8, offset=0)
+        assert np.array_equal(a, r.accum()), w
+    r.set_tuning()
+
+
+def test_graph_replays_match_direct_launches(ctx):
+    # several passes per render replay one graph executable (updated in place);
+    # eager renders and host synchronisations run between the graph renders
+    sc = scenes.spheres(n_spheres=8, level=2, n_lights=60, width=40, height=24, spp=12, env_size=(128, 64))
+    r = Renderer(sc, ctx=ctx)
+    ctx.set_path_slots(3 * 40 * 24)            # 3 samples per pass -> 4 passes
+    try:
+        r.set_tuning(path_slots=3 * 40 * 24, use_graph=False)
+        r.render(spp=12, offset=0)
+        eager = r.accum()
+        for rep in range(3):
+            r.set_tuning(path_slots=3 * 40 * 24, use_graph=True)
+            r.render(spp=12, offset=0)
+            g = r.accum()
+            assert np.array_equal(g, eager), rep
+            r.set_tuning(path_slots=3 * 40 * 24, use_graph=False)
+            r.render(spp=5, offset=0)           # eager work between replays
+            ctx.synchronize()
+    finally:
+        r.set_tuning()
