@@ -45,6 +45,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-spp", type=int, default=256, help="spp of the CPU baseline crop")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, host cores)")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (profiling runs)")
     ap.add_argument("--workload", choices=["spheres", "batch", "guided"], default="spheres",
                     help="spheres: the 1080p frame (default); batch: 64 seeded scenes in one batched launch; "
                          "guided: relMSE at equal time, guided vs plain")
@@ -240,7 +241,7 @@ def main():
     st0 = ctx.stats()
     print("LBVH %.1f ms depth %d; env CDF %.3f ms" % (st0["bvhMs"], st0["bvhDepth"], st0["cdfMs"]),
           file=sys.stderr, flush=True)
-    tuning = r.autotune(trial_spp=8, depth=args.depth)
+    tuning = None if args.no_tune else r.autotune(trial_spp=8, depth=args.depth)
     print("autotuned launch shape %s" % tuning, file=sys.stderr, flush=True)
 
     params = sc.launch_params(offset=0, count=args.spp, flags=L.RBH_RENDER_TIMING)

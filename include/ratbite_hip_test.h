@@ -30,8 +30,18 @@ int rbh_test_oct(rbh_ctx ctx, const float* v, uint32_t* enc, float* dec, uint32_
  * src/kernels/pathtrace.cu:1162-1177, 1704-1720). rays: n x {o.xyz, tmin
  * (ignored, 0), d.xyz, tmax}; hits: n x {t, prim (uint bits, 0xFFFFFFFF =
  * miss), b1, b2}. mode bit 0: any hit; bit 1: a 2-entry LDS stack so that the
- * global spill part of the traversal stack is exercised. */
+ * global spill part of the traversal stack is exercised; bit 2: traverse the
+ * binary tree instead of the four-wide one the renderer uses. */
 int rbh_test_trace(rbh_ctx ctx, const float* rays, float* hits, uint32_t n, int mode);
+/* The built LBVH (replaces optixAccelBuild, rendering_interface.cpp:733-1064):
+ * sorted 64-bit keys (Morton code << 32 | triangle), n entries; 64-byte nodes
+ * (nnodes entries); 48-byte triangle records in leaf order (n entries). Any
+ * output pointer may be null. */
+int rbh_test_lbvh_read(rbh_ctx ctx, uint64_t* keys, void* nodes, void* tris, uint32_t* nnodes);
+/* A plain device copy of `bytes` bytes, `repeats` times (no reference
+ * counterpart): calibrates HBM byte counters and measures copy bandwidth;
+ * *ms = average time per copy. */
+int rbh_test_copy(rbh_ctx ctx, uint64_t bytes, int repeats, float* ms);
 /* traverseLightTree (src/kernels/pathtrace.cu:1002-1049) on the built tree.
  * lights: n light indices; posNrm: n x 6; out: n pdfs */
 int rbh_test_light_pdf(rbh_ctx ctx, const uint32_t* lights, const float* posNrm, float* pdf, uint32_t n);

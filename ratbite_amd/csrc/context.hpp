@@ -66,9 +66,12 @@ struct Geometry {
 struct Accel {
     bool built = false;
     uint32_t nnodes = 0, depth = 0;
-    DevBuf nodes;     // BvhNode
+    DevBuf nodes;     // BvhNode (binary LBVH)
+    DevBuf nodes4;    // Bvh4Node (the binary tree collapsed four-wide; traversed)
+    uint32_t nnodes4 = 0;
     DevBuf tris;      // TriRecord in leaf order
     DevBuf scratch;   // build scratch
+    size_t keysOffset = 0;   // byte offset of the sorted 64-bit keys in scratch (kept after the build)
 };
 
 // HIP-event timing of individual render-loop kernels (RBH_RENDER_TIMING):

@@ -73,7 +73,8 @@ __device__ __forceinline__ bool tri_hit(const RayShear& s, V3 o, const TriRecord
 // global memory (entry k of lane t at spill[k * spillStride + t]), which only
 // rays with many deferred subtrees touch. A small LDS part keeps more waves
 // resident per CU than a full 64-entry LDS stack would.
-constexpr int kMaxStack = 64;
+constexpr int kMaxStack = 64;     // binary tree: depth <= 64
+constexpr int kMaxStack4 = 192;   // four-wide tree: at most 3 deferred children per level, <= 64 levels
 
 template <int LDS_DEPTH>
 struct TravStack {
@@ -82,11 +83,11 @@ struct TravStack {
     int* __restrict__ spill;
     uint32_t spillStride;
     __device__ __forceinline__ void put(int sp, int v) const {
-        if (LDS_DEPTH >= kMaxStack || sp < LDS_DEPTH) lds[sp * stride] = v;
+        if (sp < LDS_DEPTH) lds[sp * stride] = v;
         else spill[(size_t)(sp - LDS_DEPTH) * spillStride] = v;
     }
     __device__ __forceinline__ int get(int sp) const {
-        if (LDS_DEPTH >= kMaxStack || sp < LDS_DEPTH) return lds[sp * stride];
+        if (sp < LDS_DEPTH) return lds[sp * stride];
         return spill[(size_t)(sp - LDS_DEPTH) * spillStride];
     }
 };
@@ -133,6 +134,105 @@ __device__ __forceinline__ Hit traverse(const BvhNode* __restrict__ nodes, const
             if (hl) { node = e.x; continue; }
             if (hr) { node = e.y; continue; }
         } else {
+            const TriRecord tr = tris[~node];
+            float t, b1, b2;
+            if (tri_hit(sh, o, tr, h.t, t, b1, b2)) {
+                h.t = t;
+                h.b1 = b1;
+                h.b2 = b2;
+                h.prim = tr.prim;
+                if (ANY) return h;
+            }
+        }
+        if (sp == 0) break;
+        --sp;
+        node = stack.get(sp);
+    }
+    return h;
+}
+
+// Four-wide traversal: a node fetch tests four child boxes; leaf children
+// (single triangles) are intersected on the spot, inner children that
+// survive the updated distance are visited nearest first, the others pushed.
+template <bool ANY, int LDS_DEPTH>
+__device__ __forceinline__ Hit traverse4(const Bvh4Node* __restrict__ nodes, const TriRecord* __restrict__ tris,
+                                         uint32_t nnodes, V3 o, V3 d, float tmax, const TravStack<LDS_DEPTH>& stack) {
+    Hit h;
+    h.t = tmax;
+    h.prim = 0xFFFFFFFFu;
+    h.b1 = h.b2 = 0.0f;
+    const RayShear sh = make_shear(d);
+    const float ix = 1.0f / d.x, iy = 1.0f / d.y, iz = 1.0f / d.z;
+    int node = nnodes ? 0 : ~0;
+    int sp = 0;
+    while (true) {
+        if (node >= 0) {
+            const float4* q = reinterpret_cast<const float4*>(nodes + node);
+            const float4 lx = q[0], hx = q[1], ly = q[2], hy = q[3], lz = q[4], hz = q[5];
+            const int4 ch4 = reinterpret_cast<const int4*>(q)[6];
+            const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
+            const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
+            const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+            const int cha[4] = {ch4.x, ch4.y, ch4.z, ch4.w};
+            float tn[4];
+            bool in[4];
+            {
+                const float best = h.t * 1.00000024f;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const float x0 = (lxa[k] - o.x) * ix, x1 = (hxa[k] - o.x) * ix;
+                    const float y0 = (lya[k] - o.y) * iy, y1 = (hya[k] - o.y) * iy;
+                    const float z0 = (lza[k] - o.z) * iz, z1 = (hza[k] - o.z) * iz;
+                    const float tnear = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+                    const float tfar = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), best));
+                    tn[k] = tnear;
+                    in[k] = cha[k] != kBvh4Empty && tnear <= tfar * 1.00000024f;
+                }
+            }
+            // leaf children first: a hit shortens the ray for the inner ones
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (in[k] && cha[k] < 0) {
+                    const TriRecord tr = tris[~cha[k]];
+                    float t, b1, b2;
+                    if (tri_hit(sh, o, tr, h.t, t, b1, b2)) {
+                        h.t = t;
+                        h.b1 = b1;
+                        h.b2 = b2;
+                        h.prim = tr.prim;
+                        if (ANY) return h;
+                    }
+                }
+            }
+            int ids[4];
+            float ds[4];
+            int cnt = 0;
+            const float limit = h.t * 1.00000024f * 1.00000024f;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (in[k] && cha[k] >= 0 && tn[k] <= limit) {
+                    // insertion by distance
+                    int pos = cnt;
+                    while (pos > 0 && ds[pos - 1] > tn[k]) {
+                        ds[pos] = ds[pos - 1];
+                        ids[pos] = ids[pos - 1];
+                        --pos;
+                    }
+                    ds[pos] = tn[k];
+                    ids[pos] = cha[k];
+                    ++cnt;
+                }
+            }
+            if (cnt > 0) {
+                for (int k = cnt - 1; k >= 1; --k) {
+                    stack.put(sp, ids[k]);
+                    ++sp;
+                }
+                node = ids[0];
+                continue;
+            }
+        } else {
+            // a single-triangle scene: the root is the leaf
             const TriRecord tr = tris[~node];
             float t, b1, b2;
             if (tri_hit(sh, o, tr, h.t, t, b1, b2)) {

@@ -32,6 +32,18 @@ struct alignas(16) BvhNode {
 };
 static_assert(sizeof(BvhNode) == 64, "BvhNode is 64 bytes");
 
+// Four-wide BVH node, 128 bytes (one 128-byte cache line): the boxes of up to
+// four children as structure-of-arrays rows, then the child references
+// (>= 0 inner node, < 0 ~leaf index into the triangle records, kBvh4Empty
+// for an unused slot, whose box is empty: lo = +inf, hi = -inf).
+constexpr int32_t kBvh4Empty = (int32_t)0x80000000;
+struct alignas(16) Bvh4Node {
+    float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+    int32_t child[4];
+    int32_t pad[4];
+};
+static_assert(sizeof(Bvh4Node) == 128, "Bvh4Node is 128 bytes");
+
 // Triangle in leaf order: vertices + original triangle index, 48 bytes.
 struct alignas(16) TriRecord {
     float v0x, v0y, v0z; uint32_t prim;

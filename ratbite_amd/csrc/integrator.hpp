@@ -47,8 +47,9 @@ struct SceneView {
     float aperture, focus;
     // geometry
     const BvhNode* nodes;
+    const Bvh4Node* nodes4;
     const TriRecord* tris;
-    uint32_t nnodes;
+    uint32_t nnodes, nnodes4;
     const float* verts;
     const uint32_t* indices;
     const uint8_t* attribs;

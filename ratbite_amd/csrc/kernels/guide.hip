@@ -16,6 +16,8 @@
 #include <cstring>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "../context.hpp"
 #include "../guide.hpp"
 #include "../device/mlp.hpp"
@@ -232,6 +234,33 @@ __global__ __launch_bounds__(256) void k_weight_sum(const float* __restrict__ re
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s += recs[(size_t)i * kRec + 14];
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
     if (__lane_id() == 0) atomicAdd(out, s);
+}
+
+// ------------------------------------------------- collected training set
+struct PositiveWeight {
+    const float* recs;
+    __device__ __forceinline__ bool operator()(uint32_t i) const { return recs[(size_t)i * kRec + 14] > 0.0f; }
+};
+
+// out[j] = recs[idx[(j * stride) mod m]]: a fixed permutation (stride is
+// coprime with m) so that consecutive batches mix all regions of the image
+__global__ __launch_bounds__(256) void k_gather_records(const float* __restrict__ recs, const uint32_t* __restrict__ idx,
+                                                        uint32_t m, uint32_t stride, float* __restrict__ out) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < m; j += gridDim.x * blockDim.x) {
+        const uint32_t s = idx[(uint32_t)(((uint64_t)j * stride) % m)];
+        const float4* src = reinterpret_cast<const float4*>(recs + (size_t)s * kRec);
+        float4* dst = reinterpret_cast<float4*>(out + (size_t)j * kRec);
+#pragma unroll
+        for (int k = 0; k < kRec / 4; ++k) dst[k] = src[k];
+    }
+}
+
+// w = min(w, mult * mean w), the mean read from *sum
+__global__ __launch_bounds__(256) void k_clamp_weights(float* __restrict__ recs, uint32_t m, const float* __restrict__ sum,
+                                                       float mult) {
+    const float cap = mult * (*sum / (float)m);
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < m; j += gridDim.x * blockDim.x)
+        recs[(size_t)j * kRec + 14] = fminf(recs[(size_t)j * kRec + 14], cap);
 }
 
 // parameter p of the master layout -> (layer, out, in) or bias (in = -1)
@@ -551,15 +580,59 @@ int rbh_guide_records(rbh_ctx c, float* out, uint32_t max, uint32_t* n) {
     return RBH_OK;
 }
 
+// Training set of a collecting render, in the pending-record buffer (free
+// between renders): the records with a positive weight, in a fixed
+// pseudo-random order, weights clamped twice at kClampMean times their mean.
+// Clamping only changes what the network is fitted to, never the render's
+// estimator; it keeps a few one-sample radiance outliers from dominating
+// every batch.
+constexpr float kClampMean = 10.0f;
+
+static int prepare_collected(rbh_ctx c, uint32_t n, uint32_t* m) {
+    GuideState& g = gs(c);
+    const float* recs = static_cast<const float*>(g.trainRecs);
+    DevBuf idx, tmp;
+    RBH_ALLOC(c, idx, (size_t)n * 4 + 16);
+    uint32_t* cnt = idx.as<uint32_t>() + n + (4 - n % 4) % 4;
+    size_t tb = 0;
+    const hipcub::CountingInputIterator<uint32_t> it(0u);
+    RBH_HIP(c, hipcub::DeviceSelect::If(nullptr, tb, it, idx.as<uint32_t>(), cnt, (int)n, PositiveWeight{recs}, c->stream));
+    RBH_ALLOC(c, tmp, std::max<size_t>(tb, 16));
+    RBH_HIP(c, hipcub::DeviceSelect::If(tmp.p, tb, it, idx.as<uint32_t>(), cnt, (int)n, PositiveWeight{recs}, c->stream));
+    uint32_t k = 0;
+    RBH_HIP(c, hipMemcpyAsync(&k, cnt, 4, hipMemcpyDeviceToHost, c->stream));
+    RBH_HIP(c, hipStreamSynchronize(c->stream));
+    if (k == 0) return fail(c, RBH_ERR_STATE, "no collected training record has a positive weight");
+    static const uint32_t primes[] = {1000003u, 999983u, 104729u, 7919u};
+    uint32_t stride = 1u;
+    for (uint32_t p : primes)
+        if (k % p != 0u) { stride = p % k == 0u ? 1u : p; break; }
+    float* out = static_cast<float*>(g.records);
+    const int blocks = (int)std::min<uint32_t>(4096u, (k + 255u) / 256u);
+    hipLaunchKernelGGL(k_gather_records, dim3(blocks), dim3(256), 0, c->stream, recs, idx.as<uint32_t>(), k, stride, out);
+    float* sum = dptr<float>(g.scalars) + 2;
+    for (int pass = 0; pass < 2; ++pass) {
+        hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(64), 0, c->stream, sum, 1);
+        hipLaunchKernelGGL(k_weight_sum, dim3(256), dim3(256), 0, c->stream, out, k, sum);
+        hipLaunchKernelGGL(k_clamp_weights, dim3(blocks), dim3(256), 0, c->stream, out, k, sum, kClampMean);
+    }
+    RBH_HIP(c, hipGetLastError());
+    RBH_HIP(c, hipStreamSynchronize(c->stream));
+    *m = k;
+    return RBH_OK;
+}
+
 int rbh_guide_train_collected(rbh_ctx c, uint32_t batch, uint32_t steps, float* lastLoss) {
     if (!c || batch == 0) return RBH_ERR_INVALID;
     if (!gs(c).configured) return fail(c, RBH_ERR_STATE, "rbh_guide_configure first");
-    uint32_t n = 0;
-    int rc = collected(c, &n);
+    uint32_t n0 = 0, n = 0;
+    int rc = collected(c, &n0);
     if (rc) return rc;
-    if (n == 0) return fail(c, RBH_ERR_STATE, "no training records collected");
+    if (n0 == 0) return fail(c, RBH_ERR_STATE, "no training records collected");
+    rc = prepare_collected(c, n0, &n);
+    if (rc) return rc;
     float l = 0.0f;
-    const float* d = static_cast<const float*>(gs(c).trainRecs);
+    const float* d = static_cast<const float*>(gs(c).records);
     for (uint32_t s = 0; s < steps; ++s) {
         const uint32_t off = (uint32_t)(((uint64_t)s * batch) % n);
         const uint32_t len = std::min(batch, n - off);

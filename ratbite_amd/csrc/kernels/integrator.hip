@@ -207,8 +207,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(Frame1 f, uint32_t qsel) 
             pid = identity ? i : queue[i];
             const SceneView& S = scene_of<BATCH>(f, pid);
             const float4 od = f.p.od[pid], dr = f.p.dr[pid];
-            const Hit h = traverse<false>(S.nodes, S.tris, S.nnodes, V3{od.x, od.y, od.z}, V3{dr.x, dr.y, dr.z},
-                                          3.402823466e+38f, st);
+            const Hit h = traverse4<false>(S.nodes4, S.tris, S.nnodes4, V3{od.x, od.y, od.z}, V3{dr.x, dr.y, dr.z},
+                                           3.402823466e+38f, st);
             hit = h.prim != 0xFFFFFFFFu;
             if (hit) f.p.hit[pid] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.prim));
             ++nrays;
@@ -318,8 +318,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(Frame1 f) {
             const uint32_t pid = f.p.sq[i];
             const SceneView& S = scene_of<BATCH>(f, pid);
             const float4 so = f.p.sro[pid], sd = f.p.srd[pid];
-            const Hit h = traverse<true>(S.nodes, S.tris, S.nnodes, V3{so.x, so.y, so.z}, V3{sd.x, sd.y, sd.z}, so.w,
-                                         st);
+            const Hit h = traverse4<true>(S.nodes4, S.tris, S.nnodes4, V3{so.x, so.y, so.z}, V3{sd.x, sd.y, sd.z}, so.w,
+                                          st);
             if (h.prim == 0xFFFFFFFFu) {
                 const float4 a = f.p.Lnee[pid];
                 float4 L = f.p.L[pid];
@@ -838,7 +838,7 @@ __global__ __launch_bounds__(256) void k_accumulate(Frame1 f) {
 
 // ------------------------------------------------------------------ host
 // Spill entries per lane of the traversal stack beyond the LDS part.
-size_t trace_spill_words(size_t lanes) { return lanes * (size_t)(kMaxStack - kStackLds); }
+size_t trace_spill_words(size_t lanes) { return lanes * (size_t)(kMaxStack4 - kStackLds); }
 
 // One wavefront pass: camera rays for every slot, depthMax bounces of
 // trace / miss / (guide) / shade / shadow, then accumulation. f.sc (single

@@ -11,6 +11,9 @@
 //      fences or arrival counters)
 //   6. emission of 64-byte two-child nodes (both child boxes in the parent)
 //      and of triangles in leaf order
+//   7. collapse into 128-byte four-wide nodes, top down one level per launch:
+//      a four-wide node takes its binary node's children and keeps opening
+//      the inner child with the largest box surface until it has four
 // Every root-to-leaf path strictly lengthens the common key prefix, so the
 // depth is at most 64 and a 64-entry traversal stack never overflows.
 #include "../context.hpp"
@@ -234,6 +237,76 @@ __global__ __launch_bounds__(kBuildBlock) void k_emit(int n, const int2* __restr
     nodes[i] = nd;
 }
 
+__device__ __forceinline__ float half_area(const float* b) {
+    const float x = b[3] - b[0], y = b[4] - b[1], z = b[5] - b[2];
+    return x * y + y * z + z * x;
+}
+
+// One collapse level: in[] holds (binary node, four-wide index) pairs; each
+// produces its four-wide node and appends its inner children to out[].
+__global__ __launch_bounds__(kBuildBlock) void k_collapse(const int2* __restrict__ child, const float* __restrict__ ibox,
+                                                          const float* __restrict__ lbox, const int2* __restrict__ in,
+                                                          const unsigned int* __restrict__ inCount, int2* __restrict__ out,
+                                                          unsigned int* __restrict__ outCount,
+                                                          unsigned int* __restrict__ nodeCount, Bvh4Node* __restrict__ nodes4) {
+    const unsigned int cnt = *inCount;
+    for (unsigned int t = blockIdx.x * blockDim.x + threadIdx.x; t < cnt; t += gridDim.x * blockDim.x) {
+        const int2 w = in[t];
+        int slot[4];
+        int ns = 2;
+        slot[0] = child[w.x].x;
+        slot[1] = child[w.x].y;
+        while (ns < 4) {
+            int best = -1;
+            float area = -1.0f;
+            for (int k = 0; k < ns; ++k)
+                if (slot[k] >= 0) {
+                    const float a = half_area(ibox + 6 * (size_t)slot[k]);
+                    if (a > area) { area = a; best = k; }
+                }
+            if (best < 0) break;
+            const int2 ch = child[slot[best]];
+            slot[best] = ch.x;
+            slot[ns++] = ch.y;
+        }
+        Bvh4Node nd;
+        for (int k = 0; k < 4; ++k) {
+            if (k >= ns) {
+                nd.lox[k] = nd.loy[k] = nd.loz[k] = INFINITY;
+                nd.hix[k] = nd.hiy[k] = nd.hiz[k] = -INFINITY;
+                nd.child[k] = kBvh4Empty;
+                continue;
+            }
+            const int c = slot[k];
+            const float* b = c >= 0 ? ibox + 6 * (size_t)c : lbox + 6 * (size_t)(~c);
+            nd.lox[k] = b[0]; nd.loy[k] = b[1]; nd.loz[k] = b[2];
+            nd.hix[k] = b[3]; nd.hiy[k] = b[4]; nd.hiz[k] = b[5];
+            if (c >= 0) {
+                const int id = (int)atomicAdd(nodeCount, 1u);
+                nd.child[k] = id;
+                out[atomicAdd(outCount, 1u)] = make_int2(c, id);
+            } else {
+                nd.child[k] = c;
+            }
+        }
+        nd.pad[0] = nd.pad[1] = nd.pad[2] = nd.pad[3] = 0;
+        nodes4[w.y] = nd;
+    }
+}
+
+__global__ void k_collapse_init(int2* __restrict__ list, unsigned int* __restrict__ counts) {
+    if (threadIdx.x == 0) {
+        list[0] = make_int2(0, 0);   // the binary root becomes four-wide node 0
+        counts[0] = 1u;              // level list A
+        counts[1] = 0u;              // level list B
+        counts[2] = 1u;              // four-wide nodes allocated
+    }
+}
+
+__global__ void k_reset_count(unsigned int* __restrict__ c) {
+    if (threadIdx.x == 0) *c = 0u;
+}
+
 __global__ void k_zero_u32(unsigned int* __restrict__ p, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
@@ -261,6 +334,9 @@ int lbvh_build(rbh_ctx c) {
     const size_t oDepth = off; off = align_up(off + nInt * 4);
     const size_t oOrder = off; off = align_up(off + nInt * 4);
     const size_t oLevels = off; off = align_up(off + 3 * (kLevels + 1) * 4);
+    const size_t oListA = off; off = align_up(off + nInt * 8);
+    const size_t oListB = off; off = align_up(off + nInt * 8);
+    const size_t oCounts = off; off = align_up(off + 16);
     size_t sortBytes = 0;
     RBH_HIP(c, hipcub::DeviceRadixSort::SortKeys(nullptr, sortBytes, (unsigned long long*)nullptr,
                                                  (unsigned long long*)nullptr, n, 0, 64, c->stream));
@@ -269,6 +345,7 @@ int lbvh_build(rbh_ctx c) {
     RBH_ALLOC(c, acc.nodes, nInt * sizeof(BvhNode));
     RBH_ALLOC(c, acc.tris, (size_t)n * sizeof(TriRecord));
     char* s = acc.scratch.as<char>();
+    acc.keysOffset = oKeys2;
     uint32_t* bounds = reinterpret_cast<uint32_t*>(s + oBounds);
     auto* keys = reinterpret_cast<unsigned long long*>(s + oKeys);
     auto* keys2 = reinterpret_cast<unsigned long long*>(s + oKeys2);
@@ -307,6 +384,25 @@ int lbvh_build(rbh_ctx c) {
     }
     hipLaunchKernelGGL(k_depth, dim3(nb), dim3(kBuildBlock), 0, c->stream, n, parent, bounds + 6);
     if (n > 1) hipLaunchKernelGGL(k_emit, dim3(nbi), dim3(kBuildBlock), 0, c->stream, n, child, ibox, lbox, acc.nodes.as<BvhNode>());
+    RBH_ALLOC(c, acc.nodes4, nInt * sizeof(Bvh4Node));
+    if (n > 1) {
+        int2* la = reinterpret_cast<int2*>(s + oListA);
+        int2* lb = reinterpret_cast<int2*>(s + oListB);
+        unsigned int* counts = reinterpret_cast<unsigned int*>(s + oCounts);
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0) cus = 256;
+        hipLaunchKernelGGL(k_collapse_init, dim3(1), dim3(64), 0, c->stream, la, counts);
+        // each level consumes one list and fills the other; every level
+        // descends at least one binary level, so 64 levels drain the tree
+        for (int lvl = 0; lvl < 64; ++lvl) {
+            const bool even = (lvl & 1) == 0;
+            unsigned int* inC = counts + (even ? 0 : 1);
+            unsigned int* outC = counts + (even ? 1 : 0);
+            hipLaunchKernelGGL(k_reset_count, dim3(1), dim3(64), 0, c->stream, outC);
+            hipLaunchKernelGGL(k_collapse, dim3(std::min(nbi, cus * 8)), dim3(kBuildBlock), 0, c->stream, child, ibox,
+                               lbox, even ? la : lb, inC, even ? lb : la, outC, counts + 2, acc.nodes4.as<Bvh4Node>());
+        }
+    }
     RBH_HIP(c, hipGetLastError());
     RBH_HIP(c, hipEventRecord(c->ev1, c->stream));
     RBH_HIP(c, hipStreamSynchronize(c->stream));
@@ -314,6 +410,12 @@ int lbvh_build(rbh_ctx c) {
     RBH_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     uint32_t depth = 0;
     RBH_HIP(c, hipMemcpy(&depth, bounds + 6, 4, hipMemcpyDeviceToHost));
+    // counts: pending entries of both level lists, four-wide nodes allocated
+    uint32_t cnt4[3] = {0u, 0u, 0u};
+    if (n > 1) RBH_HIP(c, hipMemcpy(cnt4, s + oCounts, 12, hipMemcpyDeviceToHost));
+    if (n > 1 && (cnt4[0] != 0u || cnt4[2] > nInt))
+        return fail(c, RBH_ERR_RANGE, "four-wide collapse left %u nodes unwritten (%u allocated)", cnt4[0], cnt4[2]);
+    acc.nnodes4 = cnt4[2];
     c->stats.bvhMs = ms;
     c->stats.bvhDepth = depth;
     acc.nnodes = n > 1 ? (uint32_t)(n - 1) : 0u;

@@ -74,15 +74,16 @@ __global__ void k_oct(const float* v, uint32_t* enc, float* dec, uint32_t n) {
 }
 
 template <bool ANY, int LDS_DEPTH>
-__global__ __launch_bounds__(128) void k_trace_test(const BvhNode* nodes, const TriRecord* tris, uint32_t nnodes,
-                                                    const float* rays, float* hits, uint32_t n, int* spill,
-                                                    uint32_t spillStride) {
+__global__ __launch_bounds__(128) void k_trace_test(const BvhNode* nodes, const Bvh4Node* nodes4, const TriRecord* tris,
+                                                    uint32_t nnodes, uint32_t nnodes4, int binary, const float* rays,
+                                                    float* hits, uint32_t n, int* spill, uint32_t spillStride) {
     __shared__ int stack[LDS_DEPTH * 128];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const TravStack<LDS_DEPTH> st{stack + threadIdx.x, 128, spill + i, spillStride};
     const float* r = rays + 8 * (size_t)i;
-    const Hit h = traverse<ANY>(nodes, tris, nnodes, V3{r[0], r[1], r[2]}, V3{r[4], r[5], r[6]}, r[7], st);
+    const Hit h = binary ? traverse<ANY>(nodes, tris, nnodes, V3{r[0], r[1], r[2]}, V3{r[4], r[5], r[6]}, r[7], st)
+                         : traverse4<ANY>(nodes4, tris, nnodes4, V3{r[0], r[1], r[2]}, V3{r[4], r[5], r[6]}, r[7], st);
     float* o = hits + 4 * (size_t)i;
     o[0] = h.prim == 0xFFFFFFFFu ? -1.0f : h.t;
     o[1] = __uint_as_float(h.prim);
@@ -175,6 +176,13 @@ struct Staging {
 
 inline dim3 grid(uint32_t n, int b) { return dim3(n ? (n + b - 1) / b : 1u); }
 
+// plain streaming copy (16 B per lane per iteration): the reference point for
+// the HBM byte counters
+__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 }  // namespace
 }  // namespace rbh
 
@@ -264,22 +272,58 @@ int rbh_test_trace(rbh_ctx c, const float* rays, float* hits, uint32_t n, int mo
     void* dh = s.out((size_t)n * 16);
     const dim3 g = grid(n, 128);
     const uint32_t lanes = g.x * 128u;
-    int* spill = (int*)s.out((size_t)lanes * kMaxStack * 4);
+    int* spill = (int*)s.out((size_t)lanes * kMaxStack4 * 4);
     STAGE_CHECK(dr && dh && spill);
     const BvhNode* nodes = c->accel.nodes.as<BvhNode>();
+    const Bvh4Node* nodes4 = c->accel.nodes4.as<Bvh4Node>();
     const TriRecord* tris = c->accel.tris.as<TriRecord>();
-    const uint32_t nn = c->accel.nnodes;
+    const uint32_t nn = c->accel.nnodes, nn4 = c->accel.nnodes4;
     const bool any = (mode & 1) != 0, tiny = (mode & 2) != 0;
-    // mode bit 1: a 2-entry LDS stack, so the spill columns carry the rest
+    const int binary = (mode & 4) ? 1 : 0;
+    // mode bit 1: a 2-entry LDS stack, so the spill columns carry the rest;
+    // bit 2: the binary tree instead of the four-wide one
     if (any && tiny)
-        hipLaunchKernelGGL((k_trace_test<true, 2>), g, dim3(128), 0, 0, nodes, tris, nn, (const float*)dr, (float*)dh, n, spill, lanes);
+        hipLaunchKernelGGL((k_trace_test<true, 2>), g, dim3(128), 0, 0, nodes, nodes4, tris, nn, nn4, binary, (const float*)dr, (float*)dh, n, spill, lanes);
     else if (any)
-        hipLaunchKernelGGL((k_trace_test<true, 16>), g, dim3(128), 0, 0, nodes, tris, nn, (const float*)dr, (float*)dh, n, spill, lanes);
+        hipLaunchKernelGGL((k_trace_test<true, 16>), g, dim3(128), 0, 0, nodes, nodes4, tris, nn, nn4, binary, (const float*)dr, (float*)dh, n, spill, lanes);
     else if (tiny)
-        hipLaunchKernelGGL((k_trace_test<false, 2>), g, dim3(128), 0, 0, nodes, tris, nn, (const float*)dr, (float*)dh, n, spill, lanes);
+        hipLaunchKernelGGL((k_trace_test<false, 2>), g, dim3(128), 0, 0, nodes, nodes4, tris, nn, nn4, binary, (const float*)dr, (float*)dh, n, spill, lanes);
     else
-        hipLaunchKernelGGL((k_trace_test<false, 16>), g, dim3(128), 0, 0, nodes, tris, nn, (const float*)dr, (float*)dh, n, spill, lanes);
+        hipLaunchKernelGGL((k_trace_test<false, 16>), g, dim3(128), 0, 0, nodes, nodes4, tris, nn, nn4, binary, (const float*)dr, (float*)dh, n, spill, lanes);
     FINISH(hits, dh, (size_t)n * 16);
+    return RBH_OK;
+}
+
+int rbh_test_copy(rbh_ctx c, uint64_t bytes, int repeats, float* ms) {
+    if (!c || bytes < 16 || repeats < 1) return RBH_ERR_INVALID;
+    if (bytes > (uint64_t(1) << 34)) return fail(c, RBH_ERR_RANGE, "copy size above 16 GiB");
+    DevBuf a, b;
+    RBH_ALLOC(c, a, bytes);
+    RBH_ALLOC(c, b, bytes);
+    RBH_HIP(c, hipMemsetAsync(a.p, 1, bytes, c->stream));
+    const size_t n = bytes / 16;
+    RBH_HIP(c, hipEventRecord(c->ev0, c->stream));
+    for (int r = 0; r < repeats; ++r)
+        hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, c->stream, a.as<float4>(), b.as<float4>(), n);
+    RBH_HIP(c, hipGetLastError());
+    RBH_HIP(c, hipEventRecord(c->ev1, c->stream));
+    RBH_HIP(c, hipStreamSynchronize(c->stream));
+    float t = 0.0f;
+    RBH_HIP(c, hipEventElapsedTime(&t, c->ev0, c->ev1));
+    if (ms) *ms = t / repeats;
+    return RBH_OK;
+}
+
+int rbh_test_lbvh_read(rbh_ctx c, uint64_t* keys, void* nodes, void* tris, uint32_t* nnodes) {
+    if (!c || !nnodes) return RBH_ERR_INVALID;
+    if (!c->accel.built) return fail(c, RBH_ERR_STATE, "no acceleration structure");
+    RBH_HIP(c, hipStreamSynchronize(c->stream));
+    const size_t n = c->geom.ntris;
+    *nnodes = c->accel.nnodes;
+    if (keys) RBH_HIP(c, hipMemcpy(keys, c->accel.scratch.as<char>() + c->accel.keysOffset, n * 8, hipMemcpyDeviceToHost));
+    if (nodes && c->accel.nnodes)
+        RBH_HIP(c, hipMemcpy(nodes, c->accel.nodes.p, (size_t)c->accel.nnodes * sizeof(BvhNode), hipMemcpyDeviceToHost));
+    if (tris) RBH_HIP(c, hipMemcpy(tris, c->accel.tris.p, n * sizeof(TriRecord), hipMemcpyDeviceToHost));
     return RBH_OK;
 }
 

@@ -177,6 +177,8 @@ void fill_view(rbh_ctx c, const rbh_launch_params* lp, SceneView& v) {
     v.aperture = lp->aperture;
     v.focus = lp->focusDistance;
     v.nodes = c->accel.nodes.as<BvhNode>();
+    v.nodes4 = c->accel.nodes4.as<Bvh4Node>();
+    v.nnodes4 = c->accel.nnodes4;
     v.tris = c->accel.tris.as<TriRecord>();
     v.nnodes = c->accel.nnodes;
     v.verts = c->geom.verts.as<float>();

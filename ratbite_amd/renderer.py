@@ -38,9 +38,15 @@ class Renderer:
     def light_tree(self):
         return self.ctx.get_light_tree(len(self.scene.lights))
 
-    def render(self, spp=None, offset=None, guided=False, collect=False, no_drop=False, flags=0, **kw):
+    def render(self, spp=None, offset=None, guided=False, collect=False, no_drop=False, flags=0, fresh=False, **kw):
+        """Renders spp samples starting at Sobol index `offset`. A launch with
+        offset != 0 continues the accumulation (the reference's progressive
+        sub-launches); fresh=True clears it first, for an independent image
+        from a later part of the sample sequence."""
         spp = self.scene.spp if spp is None else spp
         offset = self.samples if offset is None else offset
+        if fresh and offset != 0:
+            self.ctx.write_accum(np.zeros((self.scene.height, self.scene.width, 4), np.float64))
         flags |= (L.RBH_RENDER_GUIDED if guided else 0) | (L.RBH_RENDER_COLLECT if collect else 0)
         flags |= L.RBH_RENDER_NO_DROP if no_drop else 0
         p = self.scene.launch_params(offset=offset, count=spp, flags=flags, **kw)

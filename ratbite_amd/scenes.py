@@ -340,10 +340,12 @@ def _finish(parts, mats, spectra, tex_list, cam, env, width, height, depth, spp,
 
 
 # ----------------------------------------------------------------- configs
-def cornell(seed=0x5EED0001, width=256, height=256, spp=16, depth=8, light_up=False):
+def cornell(seed=0x5EED0001, width=256, height=256, spp=16, depth=8, light_up=False, occluder=False):
     """Seeded Cornell box. With light_up the emitter faces the ceiling from
-    just below it, so every other surface is lit only indirectly (a scene
-    where BSDF sampling rarely finds the light's reflection)."""
+    just below it, so every other surface is lit only indirectly. With
+    occluder a plate under the ceiling light hides it from the room, which is
+    then lit only through the gap between the plate and the walls (a scene
+    where BSDF sampling rarely finds the light's path and NEE is occluded)."""
     rng = np.random.default_rng(seed)
     tables = load_tables()
     mats, parts = [], []
@@ -375,6 +377,11 @@ def cornell(seed=0x5EED0001, width=256, height=256, spp=16, depth=8, light_up=Fa
     mats.append(material(base=(0.8, 0.8, 0.8), metal=0.0, rough=1.0, spectral_emission=(0, power)))
     mats[-1]["factors"] |= FC_EMISSION
     parts.append((v, i, n, uv, len(mats) - 1))
+    if occluder:
+        # grey plate 0.12 below the light (intersection is two-sided)
+        mats.append(material(base=(0.6, 0.6, 0.6), metal=0.0, rough=0.8, ior=1.5))
+        v, i, n, uv = quad((-0.75, 1.86, -0.75), (0.75, 1.86, -0.75), (0.75, 1.86, 0.75), (-0.75, 1.86, 0.75))
+        parts.append((v, i, n, uv, len(mats) - 1))
     for _ in range(8):
         r = float(rng.uniform(0.1, 0.3))
         c = rng.uniform([-1 + r, r, -1 + r], [1 - r, 2 - r, 1 - r])

@@ -62,7 +62,8 @@ def test_autotune_keeps_results(ctx):
     a = r.accum()
     t = r.autotune(trial_spp=2)
     print("tuning", t, flush=True)
-    assert t["samplesPerSecond"] > 0 and t["pathSlots"] > 0
+    # pathSlots 0 is the automatic budget (a whole launch per pass when it fits)
+    assert t["samplesPerSecond"] > 0 and t["pathSlots"] >= 0
     r.render(spp=8, offset=0)
     assert np.array_equal(a, r.accum())
     for w in (3, 4):

@@ -12,8 +12,8 @@ from ratbite_amd.renderer import Renderer
 pytestmark = pytest.mark.gpu
 
 
-def _img(r, spp, **kw):
-    r.render(spp=spp, offset=0, **kw)
+def _img(r, spp, offset=0, **kw):
+    r.render(spp=spp, offset=offset, fresh=True, **kw)
     a = r.accum()
     return a[..., :3] / a[..., 3:4]
 
@@ -30,9 +30,7 @@ def test_guided_is_unbiased_with_untrained_network(ctx):
     r = Renderer(sc, ctx=ctx)
     r.configure_guide(seed=3)
     plain = _img(r, 2048, no_drop=True)
-    st = r.render(spp=2048, offset=0, guided=True, flags=0)
-    a = r.accum()
-    guided = a[..., :3] / a[..., 3:4]
+    guided = _img(r, 2048, guided=True)
     assert np.all(np.isfinite(guided))
     # whole-image means and 8x8 block means agree within Monte Carlo noise
     mp, mg = plain.mean(axis=(0, 1)), guided.mean(axis=(0, 1))
@@ -46,15 +44,22 @@ def test_guided_is_unbiased_with_untrained_network(ctx):
 
 
 def test_collect_train_and_gain(ctx):
-    sc = scenes.cornell(width=96, height=96, spp=16, light_up=True)
+    # a plate hides the ceiling light from the room, which is lit only through
+    # the gap between plate and walls: BSDF sampling rarely finds that path
+    sc = scenes.cornell(width=64, height=64, spp=16, occluder=True)
     r = Renderer(sc, ctx=ctx)
     r.configure_guide(seed=5, lr=2e-3, capacity=1 << 20)
-    ref = _img(r, 8192, no_drop=True)
-    plain = _img(r, 64, no_drop=True)
-    untrained = _img(r, 64, guided=True)
+    ref = _img(r, 16384, no_drop=True)
+    offsets = (0, 3000, 6000, 9000)
+
+    def err(**kw):
+        return float(np.mean([relmse(_img(r, 128, offset=o, **kw), ref) for o in offsets]))
+
+    e_plain = err(no_drop=True)
+    e_untr = err(guided=True)
     losses = []
-    for it in range(12):
-        r.render(spp=8, offset=1000 + 8 * it, guided=True, collect=True)
+    for it in range(24):
+        r.render(spp=8, offset=100000 + 8 * it, guided=True, collect=True)
         if it == 0:
             recs = r.guide_records()
             assert len(recs) > 1000
@@ -63,12 +68,11 @@ def test_collect_train_and_gain(ctx):
             assert np.allclose(np.linalg.norm(d, axis=1), 1.0, atol=1e-4)
             assert np.all((recs[:, 0:3] >= 0) & (recs[:, 0:3] <= 1))
         losses.append(r.train_guide(batch=1 << 16, steps=20))
-    trained = _img(r, 64, guided=True)
-    e_plain, e_untr, e_tr = relmse(plain, ref), relmse(untrained, ref), relmse(trained, ref)
+    e_tr = err(guided=True)
     print("losses", np.round(losses, 3), flush=True)
-    print("relMSE at 64 spp vs 8192-spp plain: plain %.4g, guided untrained %.4g, guided trained %.4g"
-          % (e_plain, e_untr, e_tr), flush=True)
-    assert losses[-1] < losses[0]
+    print("mean relMSE at 128 spp (4 renders) vs 16384-spp plain: plain %.4g, guided untrained %.4g, "
+          "guided trained %.4g" % (e_plain, e_untr, e_tr), flush=True)
+    assert np.mean(losses[-4:]) < np.mean(losses[:4])
     assert e_tr < e_untr
     assert e_tr < e_plain
 

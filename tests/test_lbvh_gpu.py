@@ -22,9 +22,9 @@ def _upload(ctx, verts, idx):
     return ctx.stats()
 
 
-def _trace(ctx, rays, anyhit, tiny_stack=False):
+def _trace(ctx, rays, anyhit, tiny_stack=False, binary=False):
     hits = np.zeros((len(rays), 4), np.float32)
-    mode = int(anyhit) | (2 if tiny_stack else 0)
+    mode = int(anyhit) | (2 if tiny_stack else 0) | (4 if binary else 0)
     ctx.check(L.lib().rbh_test_trace(ctx.h, L.ptr(rays), L.ptr(hits), len(rays), mode))
     return hits
 
@@ -85,6 +85,28 @@ def test_stack_spill_gives_identical_hits(ctx):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
+def test_four_wide_tree_matches_binary(ctx):
+    # the renderer traverses the four-wide collapse of the binary LBVH: the
+    # closest t is the minimum over the same triangle tests, so it is
+    # bit-identical; only exact ties between triangles may name another one
+    sc = scenes.spheres(n_spheres=30, level=4, n_lights=500, width=64, height=64, spp=1, env_size=None)
+    _upload(ctx, sc.verts, sc.indices)
+    rng = np.random.default_rng(21)
+    rays = _rays(rng, 30000, [-40, -5, -100], [40, 20, -20])
+    a = _trace(ctx, rays, False)
+    b = _trace(ctx, rays, False, binary=True)
+    assert np.array_equal(a[:, 0].view(np.uint32), b[:, 0].view(np.uint32))
+    ap, bp = a[:, 1].view(np.uint32), b[:, 1].view(np.uint32)
+    assert np.array_equal(ap == 0xFFFFFFFF, bp == 0xFFFFFFFF)
+    assert (ap == bp).mean() > 0.999
+    a2 = _trace(ctx, rays, False, tiny_stack=True)
+    assert np.array_equal(a.view(np.uint32), a2.view(np.uint32))
+    rays[:, 7] = rng.uniform(1, 60, len(rays))
+    ea = _trace(ctx, rays, True)[:, 1].view(np.uint32) == 0xFFFFFFFF
+    eb = _trace(ctx, rays, True, binary=True)[:, 1].view(np.uint32) == 0xFFFFFFFF
+    assert np.array_equal(ea, eb)
+
+
 def test_watertight_shared_edges(ctx):
     # a closed icosphere: rays from inside must always hit; rays aimed at
     # vertices and edge midpoints must not slip through
@@ -114,3 +136,80 @@ def test_spheres_scene_depth_and_hits(ctx):
     assert np.array_equal(gp == 0xFFFFFFFF, bp == 0xFFFFFFFF)
     hit = bp != 0xFFFFFFFF
     np.testing.assert_allclose(g[hit, 0], b[hit, 0], rtol=1e-5, atol=1e-5)
+
+
+NODE_DTYPE = np.dtype([(k, "f4") for k in ("lminx", "lmaxx", "lminy", "lmaxy", "rminx", "rmaxx", "rminy", "rmaxy",
+                                           "lminz", "lmaxz", "rminz", "rmaxz")] +
+                      [("left", "i4"), ("right", "i4"), ("lcnt", "u4"), ("rcnt", "u4")])
+TRI_DTYPE = np.dtype([("v0", "f4", 3), ("prim", "u4"), ("v1", "f4", 3), ("pad0", "u4"), ("v2", "f4", 3), ("pad1", "u4")])
+
+
+def _spread10(x):
+    x = x.astype(np.uint64) & 0x3FF
+    x = (x | (x << 16)) & 0x030000FF
+    x = (x | (x << 8)) & 0x0300F00F
+    x = (x | (x << 4)) & 0x030C30C3
+    x = (x | (x << 2)) & 0x09249249
+    return x
+
+
+def _read_lbvh(ctx, n):
+    import ctypes
+    keys = np.zeros(n, np.uint64)
+    nodes = np.zeros(max(1, n - 1), NODE_DTYPE)
+    tris = np.zeros(n, TRI_DTYPE)
+    nn = ctypes.c_uint32()
+    ctx.check(L.lib().rbh_test_lbvh_read(ctx.h, L.ptr(keys), L.ptr(nodes), L.ptr(tris), ctypes.byref(nn)))
+    return keys, nodes[:nn.value], tris
+
+
+@pytest.mark.parametrize("which", ["soup", "spheres"])
+def test_morton_keys_bit_exact_and_tree_invariants(ctx, which):
+    if which == "soup":
+        rng = np.random.default_rng(21)
+        ntri = 3000
+        c = rng.uniform(-5, 5, (ntri, 3))
+        v = (c[:, None, :] + rng.normal(scale=0.5, size=(ntri, 3, 3))).reshape(-1, 3).astype(np.float32)
+        idx = np.arange(3 * ntri, dtype=np.uint32).reshape(-1, 3)
+    else:
+        sc = scenes.spheres(n_spheres=12, level=3, n_lights=20, width=8, height=8, spp=1, env_size=None)
+        v, idx = sc.verts.astype(np.float32), sc.indices.astype(np.uint32)
+    _upload(ctx, v, idx)
+    n = len(idx)
+    keys, nodes, tris = _read_lbvh(ctx, n)
+    # Morton codes: 10 bits per axis of the triangle-centroid grid, FP32 as the kernel computes them
+    tv = v[idx]                                            # n x 3 x 3
+    lo, hi = tv.min(axis=1), tv.max(axis=1)
+    ce = (np.float32(0.5) * (lo + hi)).astype(np.float32)
+    cmin, cmax = ce.min(axis=0), ce.max(axis=0)
+    ext = (cmax - cmin).astype(np.float32)
+    u = np.where(ext > 0, (ce - cmin) / np.where(ext > 0, ext, 1), np.float32(0.5)).astype(np.float32)
+    q = np.clip(u * np.float32(1024.0), 0, 1023).astype(np.uint32)
+    m = (_spread10(q[:, 0]) << 2) | (_spread10(q[:, 1]) << 1) | _spread10(q[:, 2])
+    ref = np.sort((m << 32) | np.arange(n, dtype=np.uint64))
+    assert np.array_equal(keys, ref)
+    # leaf order and triangle records
+    assert np.array_equal(tris["prim"], (keys & 0xFFFFFFFF).astype(np.uint32))
+    assert np.array_equal(tris["v0"], v[idx[tris["prim"], 0]]) and np.array_equal(tris["v2"], v[idx[tris["prim"], 2]])
+    # topology: every internal node but the root and every leaf is a child exactly once
+    assert len(nodes) == n - 1
+    ch = np.concatenate([nodes["left"], nodes["right"]])
+    internal, leaves = ch[ch >= 0], ~ch[ch < 0]
+    assert np.array_equal(np.sort(internal), np.arange(1, n - 1))
+    assert np.array_equal(np.sort(leaves), np.arange(n))
+    # boxes: a child box equals the union of the child's own boxes; leaf boxes bound their triangle
+    def child_box(i, side):
+        p = "l" if side == 0 else "r"
+        return np.array([nodes[i][p + "minx"], nodes[i][p + "miny"], nodes[i][p + "minz"],
+                         nodes[i][p + "maxx"], nodes[i][p + "maxy"], nodes[i][p + "maxz"]])
+    rng = np.random.default_rng(3)
+    for i in rng.choice(n - 1, min(400, n - 1), replace=False):
+        for side, c in ((0, nodes[i]["left"]), (1, nodes[i]["right"])):
+            b = child_box(i, side)
+            if c >= 0:
+                u0, u1 = child_box(c, 0), child_box(c, 1)
+                assert np.array_equal(b[:3], np.minimum(u0[:3], u1[:3])) and np.array_equal(b[3:], np.maximum(u0[3:], u1[3:]))
+            else:
+                t = tris[~c]
+                tv3 = np.stack([t["v0"], t["v1"], t["v2"]])
+                assert np.array_equal(b[:3], tv3.min(axis=0)) and np.array_equal(b[3:], tv3.max(axis=0))
