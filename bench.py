@@ -46,6 +46,9 @@ def parse():
     ap.add_argument("--cpu-spp", type=int, default=256, help="spp of the CPU baseline crop")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, host cores)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (profiling runs)")
+    ap.add_argument("--copy-mib", type=int, default=1024,
+                    help="size of the device-to-device copy timed first (the measured bandwidth ceiling; "
+                         "its k_copy dispatch also scales PMC byte counters)")
     ap.add_argument("--workload", choices=["spheres", "batch", "guided"], default="spheres",
                     help="spheres: the 1080p frame (default); batch: 64 seeded scenes in one batched launch; "
                          "guided: relMSE at equal time, guided vs plain")
@@ -243,6 +246,15 @@ def main():
           file=sys.stderr, flush=True)
     tuning = None if args.no_tune else r.autotune(trial_spp=8, depth=args.depth)
     print("autotuned launch shape %s" % tuning, file=sys.stderr, flush=True)
+    copy_gbs = None
+    if args.copy_mib > 0:
+        import ctypes
+        ms = ctypes.c_float()
+        nbytes = args.copy_mib << 20
+        ctx.check(L.lib().rbh_test_copy(ctx.h, nbytes, 5, ctypes.byref(ms)))
+        copy_gbs = 2.0 * nbytes / (ms.value * 1e-3) / 1e9
+        print("device copy %d MiB: %.3f ms, %.0f GB/s read+write" % (args.copy_mib, ms.value, copy_gbs),
+              file=sys.stderr, flush=True)
 
     params = sc.launch_params(offset=0, count=args.spp, flags=L.RBH_RENDER_TIMING)
     for _ in range(args.warmup):
@@ -292,7 +304,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "avg_launch_ms": kms[dom] / max(1, launches),
-                     "algorithmic_bytes_per_launch": kbytes[dom] / max(1, launches)},
+                     "algorithmic_bytes_per_launch": kbytes[dom] / max(1, launches),
+                     "copy_gbs_measured": copy_gbs},
         "path_roofline": {"achieved": path_bytes / elapsed / 1e9, "unit": "GB/s",
                           "frac": path_bytes / elapsed / 1e9 / HBM_PEAK_GBS},
         "cpu_baseline": None,

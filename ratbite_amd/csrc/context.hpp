@@ -130,6 +130,7 @@ struct RenderState {
     int traceBlocksPerCU = 16;   // persistent-grid sizes (autotuned)
     int shadeBlocksPerCU = 16;
     int shadeWaves = 1;          // shade-kernel variant (1, 3, 4 waves per SIMD)
+    int traceFetch = 1;          // trace/shadow lanes fetch a new ray as soon as theirs completes
     DevBuf sceneTable;           // batched passes: SceneView per scene and pass
     bool useGraph = false;       // replay each pass's launch sequence as a captured HIP graph
     hipGraphExec_t graphExec = nullptr;

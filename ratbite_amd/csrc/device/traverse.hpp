@@ -154,100 +154,130 @@ __device__ __forceinline__ Hit traverse(const BvhNode* __restrict__ nodes, const
 // Four-wide traversal: a node fetch tests four child boxes; leaf children
 // (single triangles) are intersected on the spot, inner children that
 // survive the updated distance are visited nearest first, the others pushed.
+// Four-wide traversal as a resumable state machine, so that a persistent
+// kernel can give a lane a new ray as soon as its previous one finishes
+// while the other lanes of the wave keep traversing.
+struct TravRay {
+    V3 o, d;
+    float ix, iy, iz;
+    RayShear sh;
+    Hit h;
+    int node;   // next node to visit (~i: the root is triangle i)
+    int sp;     // entries on the traversal stack
+};
+
+__device__ __forceinline__ void trav_init(TravRay& r, V3 o, V3 d, float tmax, uint32_t nnodes) {
+    r.o = o;
+    r.d = d;
+    r.ix = 1.0f / d.x;
+    r.iy = 1.0f / d.y;
+    r.iz = 1.0f / d.z;
+    r.sh = make_shear(d);
+    r.h.t = tmax;
+    r.h.prim = 0xFFFFFFFFu;
+    r.h.b1 = r.h.b2 = 0.0f;
+    r.node = nnodes ? 0 : ~0;
+    r.sp = 0;
+}
+
+// Visits r.node: tests its triangle children (nearest hit so far shortens
+// the ray), descends into the nearest box hit and defers the others,
+// nearest last on the stack. Returns true when the query is complete.
+template <bool ANY, int LDS_DEPTH>
+__device__ __forceinline__ bool trav_step(const Bvh4Node* __restrict__ nodes, const TriRecord* __restrict__ tris,
+                                          TravRay& r, const TravStack<LDS_DEPTH>& stack) {
+    if (r.node < 0) {
+        // a single-triangle scene: the root is the leaf
+        const TriRecord tr = tris[~r.node];
+        float t, b1, b2;
+        if (tri_hit(r.sh, r.o, tr, r.h.t, t, b1, b2)) {
+            r.h.t = t;
+            r.h.b1 = b1;
+            r.h.b2 = b2;
+            r.h.prim = tr.prim;
+        }
+        return true;
+    }
+    const float4* q = reinterpret_cast<const float4*>(nodes + r.node);
+    const float4 lx = q[0], hx = q[1], ly = q[2], hy = q[3], lz = q[4], hz = q[5];
+    const int4 ch4 = reinterpret_cast<const int4*>(q)[6];
+    const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
+    const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
+    const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+    const int cha[4] = {ch4.x, ch4.y, ch4.z, ch4.w};
+    float tn[4];
+    bool in[4];
+    {
+        const float best = r.h.t * 1.00000024f;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float x0 = (lxa[k] - r.o.x) * r.ix, x1 = (hxa[k] - r.o.x) * r.ix;
+            const float y0 = (lya[k] - r.o.y) * r.iy, y1 = (hya[k] - r.o.y) * r.iy;
+            const float z0 = (lza[k] - r.o.z) * r.iz, z1 = (hza[k] - r.o.z) * r.iz;
+            const float tnear = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+            const float tfar = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), best));
+            tn[k] = tnear;
+            in[k] = cha[k] != kBvh4Empty && tnear <= tfar * 1.00000024f;
+        }
+    }
+    // triangle children first: a hit shortens the ray for the boxes
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (in[k] && cha[k] < 0) {
+            const TriRecord tr = tris[~cha[k]];
+            float t, b1, b2;
+            if (tri_hit(r.sh, r.o, tr, r.h.t, t, b1, b2)) {
+                r.h.t = t;
+                r.h.b1 = b1;
+                r.h.b2 = b2;
+                r.h.prim = tr.prim;
+                if (ANY) return true;
+            }
+        }
+    }
+    int ids[4];
+    float ds[4];
+    int cnt = 0;
+    const float limit = r.h.t * 1.00000024f * 1.00000024f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (in[k] && cha[k] >= 0 && tn[k] <= limit) {
+            // insertion by distance
+            int pos = cnt;
+            while (pos > 0 && ds[pos - 1] > tn[k]) {
+                ds[pos] = ds[pos - 1];
+                ids[pos] = ids[pos - 1];
+                --pos;
+            }
+            ds[pos] = tn[k];
+            ids[pos] = cha[k];
+            ++cnt;
+        }
+    }
+    if (cnt > 0) {
+        for (int k = cnt - 1; k >= 1; --k) {
+            stack.put(r.sp, ids[k]);
+            ++r.sp;
+        }
+        r.node = ids[0];
+        return false;
+    }
+    if (r.sp == 0) return true;
+    --r.sp;
+    r.node = stack.get(r.sp);
+    return false;
+}
+
+// Whole query in one call: the closest hit below tmax or, with ANY, the
+// first one found.
 template <bool ANY, int LDS_DEPTH>
 __device__ __forceinline__ Hit traverse4(const Bvh4Node* __restrict__ nodes, const TriRecord* __restrict__ tris,
                                          uint32_t nnodes, V3 o, V3 d, float tmax, const TravStack<LDS_DEPTH>& stack) {
-    Hit h;
-    h.t = tmax;
-    h.prim = 0xFFFFFFFFu;
-    h.b1 = h.b2 = 0.0f;
-    const RayShear sh = make_shear(d);
-    const float ix = 1.0f / d.x, iy = 1.0f / d.y, iz = 1.0f / d.z;
-    int node = nnodes ? 0 : ~0;
-    int sp = 0;
-    while (true) {
-        if (node >= 0) {
-            const float4* q = reinterpret_cast<const float4*>(nodes + node);
-            const float4 lx = q[0], hx = q[1], ly = q[2], hy = q[3], lz = q[4], hz = q[5];
-            const int4 ch4 = reinterpret_cast<const int4*>(q)[6];
-            const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
-            const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
-            const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
-            const int cha[4] = {ch4.x, ch4.y, ch4.z, ch4.w};
-            float tn[4];
-            bool in[4];
-            {
-                const float best = h.t * 1.00000024f;
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const float x0 = (lxa[k] - o.x) * ix, x1 = (hxa[k] - o.x) * ix;
-                    const float y0 = (lya[k] - o.y) * iy, y1 = (hya[k] - o.y) * iy;
-                    const float z0 = (lza[k] - o.z) * iz, z1 = (hza[k] - o.z) * iz;
-                    const float tnear = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
-                    const float tfar = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), best));
-                    tn[k] = tnear;
-                    in[k] = cha[k] != kBvh4Empty && tnear <= tfar * 1.00000024f;
-                }
-            }
-            // leaf children first: a hit shortens the ray for the inner ones
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                if (in[k] && cha[k] < 0) {
-                    const TriRecord tr = tris[~cha[k]];
-                    float t, b1, b2;
-                    if (tri_hit(sh, o, tr, h.t, t, b1, b2)) {
-                        h.t = t;
-                        h.b1 = b1;
-                        h.b2 = b2;
-                        h.prim = tr.prim;
-                        if (ANY) return h;
-                    }
-                }
-            }
-            int ids[4];
-            float ds[4];
-            int cnt = 0;
-            const float limit = h.t * 1.00000024f * 1.00000024f;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                if (in[k] && cha[k] >= 0 && tn[k] <= limit) {
-                    // insertion by distance
-                    int pos = cnt;
-                    while (pos > 0 && ds[pos - 1] > tn[k]) {
-                        ds[pos] = ds[pos - 1];
-                        ids[pos] = ids[pos - 1];
-                        --pos;
-                    }
-                    ds[pos] = tn[k];
-                    ids[pos] = cha[k];
-                    ++cnt;
-                }
-            }
-            if (cnt > 0) {
-                for (int k = cnt - 1; k >= 1; --k) {
-                    stack.put(sp, ids[k]);
-                    ++sp;
-                }
-                node = ids[0];
-                continue;
-            }
-        } else {
-            // a single-triangle scene: the root is the leaf
-            const TriRecord tr = tris[~node];
-            float t, b1, b2;
-            if (tri_hit(sh, o, tr, h.t, t, b1, b2)) {
-                h.t = t;
-                h.b1 = b1;
-                h.b2 = b2;
-                h.prim = tr.prim;
-                if (ANY) return h;
-            }
-        }
-        if (sp == 0) break;
-        --sp;
-        node = stack.get(sp);
+    TravRay r;
+    trav_init(r, o, d, tmax, nnodes);
+    while (!trav_step<ANY>(nodes, tris, r, stack)) {
     }
-    return h;
+    return r.h;
 }
 
 }  // namespace dev

@@ -9,7 +9,11 @@ namespace rbh {
 constexpr int kTraceBlock = 128;   // lanes per block of the trace and shadow kernels
 constexpr int kShadeBlock = 128;
 
-enum : uint32_t { CNT_QUEUE0 = 0, CNT_QUEUE1 = 1, CNT_SHADOW = 2, CNT_HIT = 3, CNT_MISS = 4, CNT_COUNT = 8 };
+enum : uint32_t {
+    CNT_QUEUE0 = 0, CNT_QUEUE1 = 1, CNT_SHADOW = 2, CNT_HIT = 3, CNT_MISS = 4,
+    CNT_TRACE_HEAD = 5, CNT_SHADOW_HEAD = 6,   // work-fetch positions of the trace and shadow queues
+    CNT_COUNT = 8
+};
 enum : uint32_t { STAT_CLOSEST = 0, STAT_SHADOW = 1, STAT_VERTICES = 2, STAT_COUNT = 4 };
 
 // Structure-of-arrays path state, one entry per path slot of the pass.
@@ -80,6 +84,7 @@ struct SceneView {
 struct LaunchShape {
     int traceGrid, shadeGrid;
     int shadeWaves;          // register budget of the shade kernel: 1 (free), 3 or 4 waves per SIMD
+    int traceFetch;          // trace/shadow lanes fetch a new ray when theirs completes (1) or wave by wave (0)
 };
 
 struct Frame1 {

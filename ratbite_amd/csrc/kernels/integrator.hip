@@ -118,17 +118,45 @@ __global__ __launch_bounds__(256) void k_camera(Frame1 f) {
         f.counters[CNT_SHADOW] = 0u;
         f.counters[CNT_HIT] = 0u;
         f.counters[CNT_MISS] = 0u;
+        f.counters[CNT_TRACE_HEAD] = 0u;
+        f.counters[CNT_SHADOW_HEAD] = 0u;
     }
 }
 
 __global__ void k_reset(uint32_t* __restrict__ counters, uint32_t which) {
     if (threadIdx.x == 0) counters[which] = 0u;
     if (threadIdx.x == 1) counters[CNT_SHADOW] = 0u;
+    if (threadIdx.x == 2) counters[CNT_SHADOW_HEAD] = 0u;
 }
 
 __global__ void k_reset_hits(uint32_t* __restrict__ counters) {
     if (threadIdx.x == 0) counters[CNT_HIT] = 0u;
     if (threadIdx.x == 1) counters[CNT_MISS] = 0u;
+    if (threadIdx.x == 2) counters[CNT_TRACE_HEAD] = 0u;
+}
+
+// Work fetch of the persistent trace and shadow kernels: when at least
+// kRefill lanes of a wave are idle (or all of them), the wave claims that
+// many queue entries with one atomic on *head; an idle lane whose entry lies
+// beyond count has drained the queue. Returns the claimed entry or ~0.
+constexpr int kRefill = 16;
+__device__ __forceinline__ uint32_t wave_fetch(uint32_t* __restrict__ head, uint32_t count, bool idle, bool anyBusy,
+                                               bool& exhausted) {
+    const unsigned long long mask = __ballot(idle);
+    const uint32_t n = (uint32_t)__popcll(mask);
+    if (n == 0 || (anyBusy && n < (uint32_t)kRefill)) return 0xFFFFFFFFu;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(head, n);
+    base = __shfl(base, leader);
+    if (!idle) return 0xFFFFFFFFu;
+    const uint32_t idx = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    if (idx >= count) {
+        exhausted = true;
+        return 0xFFFFFFFFu;
+    }
+    return idx;
 }
 
 // Guided renders that collect training data: the radiance of the path just
@@ -192,7 +220,7 @@ __device__ __forceinline__ TravStack<kStackLds> lane_stack(const Frame1& f, int*
     return TravStack<kStackLds>{lds + threadIdx.x, (int)blockDim.x, f.spill + t, f.spillStride};
 }
 
-template <bool BATCH>
+template <bool BATCH, bool FETCH>
 __global__ __launch_bounds__(kTraceBlock) void k_trace(Frame1 f, uint32_t qsel) {
     __shared__ int stack[kStackLds * kTraceBlock];
     const TravStack<kStackLds> st = lane_stack(f, stack);
@@ -200,24 +228,57 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(Frame1 f, uint32_t qsel) 
     const uint32_t* __restrict__ queue = qsel ? f.p.q1 : f.p.q0;
     const bool identity = f.bounce == 0;
     uint32_t nrays = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i - threadIdx.x < count; i += gridDim.x * blockDim.x) {
-        bool hit = false;
+    if (FETCH) {
+        // lanes take a new ray as soon as theirs completes
+        TravRay ray;
+        const Bvh4Node* nodes = nullptr;
+        const TriRecord* tris = nullptr;
         uint32_t pid = 0;
-        if (i < count) {
-            pid = identity ? i : queue[i];
-            const SceneView& S = scene_of<BATCH>(f, pid);
-            const float4 od = f.p.od[pid], dr = f.p.dr[pid];
-            const Hit h = traverse4<false>(S.nodes4, S.tris, S.nnodes4, V3{od.x, od.y, od.z}, V3{dr.x, dr.y, dr.z},
-                                           3.402823466e+38f, st);
-            hit = h.prim != 0xFFFFFFFFu;
-            if (hit) f.p.hit[pid] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.prim));
-            ++nrays;
+        bool active = false, exhausted = false;
+        while (true) {
+            const bool anyBusy = __ballot(active) != 0ull;
+            const uint32_t idx = wave_fetch(f.counters + CNT_TRACE_HEAD, count, !active && !exhausted, anyBusy, exhausted);
+            if (idx != 0xFFFFFFFFu) {
+                pid = identity ? idx : queue[idx];
+                const SceneView& S = scene_of<BATCH>(f, pid);
+                nodes = S.nodes4;
+                tris = S.tris;
+                const float4 od = f.p.od[pid], dr = f.p.dr[pid];
+                trav_init(ray, V3{od.x, od.y, od.z}, V3{dr.x, dr.y, dr.z}, 3.402823466e+38f, S.nnodes4);
+                active = true;
+                ++nrays;
+            }
+            if (__ballot(active) == 0ull) break;   // every lane found the queue drained
+            bool done = false;
+            if (active) done = trav_step<false>(nodes, tris, ray, st);
+            const bool hit = done && ray.h.prim != 0xFFFFFFFFu;
+            if (hit) f.p.hit[pid] = make_float4(ray.h.t, ray.h.b1, ray.h.b2, __uint_as_float(ray.h.prim));
+            const uint32_t slot = wave_push(f.counters + CNT_HIT, hit);
+            if (hit) f.p.hq[slot] = pid;
+            const uint32_t mslot = wave_push(f.counters + CNT_MISS, done && !hit);
+            if (done && !hit) f.p.mq[mslot] = pid;
+            if (done) active = false;
         }
-        const bool live = i < count;
-        const uint32_t slot = wave_push(f.counters + CNT_HIT, live && hit);
-        if (live && hit) f.p.hq[slot] = pid;
-        const uint32_t mslot = wave_push(f.counters + CNT_MISS, live && !hit);
-        if (live && !hit) f.p.mq[mslot] = pid;
+    } else {
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i - threadIdx.x < count; i += gridDim.x * blockDim.x) {
+            bool hit = false;
+            uint32_t pid = 0;
+            if (i < count) {
+                pid = identity ? i : queue[i];
+                const SceneView& S = scene_of<BATCH>(f, pid);
+                const float4 od = f.p.od[pid], dr = f.p.dr[pid];
+                const Hit h = traverse4<false>(S.nodes4, S.tris, S.nnodes4, V3{od.x, od.y, od.z},
+                                               V3{dr.x, dr.y, dr.z}, 3.402823466e+38f, st);
+                hit = h.prim != 0xFFFFFFFFu;
+                if (hit) f.p.hit[pid] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.prim));
+                ++nrays;
+            }
+            const bool live = i < count;
+            const uint32_t slot = wave_push(f.counters + CNT_HIT, live && hit);
+            if (live && hit) f.p.hq[slot] = pid;
+            const uint32_t mslot = wave_push(f.counters + CNT_MISS, live && !hit);
+            if (live && !hit) f.p.mq[mslot] = pid;
+        }
     }
     wave_count(f.stats + STAT_CLOSEST, nrays);
 }
@@ -307,26 +368,55 @@ __global__ __launch_bounds__(256) void k_miss(Frame1 f) {
 }
 
 // ------------------------------------------------------------------ shadow
-template <bool BATCH>
+__device__ __forceinline__ void add_nee(const Frame1& f, uint32_t pid) {
+    const float4 a = f.p.Lnee[pid];
+    float4 L = f.p.L[pid];
+    L.x += a.x; L.y += a.y; L.z += a.z; L.w += a.w;
+    f.p.L[pid] = L;
+}
+
+template <bool BATCH, bool FETCH>
 __global__ __launch_bounds__(kTraceBlock) void k_shadow(Frame1 f) {
     __shared__ int stack[kStackLds * kTraceBlock];
     const TravStack<kStackLds> st = lane_stack(f, stack);
     const uint32_t count = f.counters[CNT_SHADOW];
     uint32_t nrays = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i - threadIdx.x < count; i += gridDim.x * blockDim.x) {
-        if (i < count) {
-            const uint32_t pid = f.p.sq[i];
-            const SceneView& S = scene_of<BATCH>(f, pid);
-            const float4 so = f.p.sro[pid], sd = f.p.srd[pid];
-            const Hit h = traverse4<true>(S.nodes4, S.tris, S.nnodes4, V3{so.x, so.y, so.z}, V3{sd.x, sd.y, sd.z}, so.w,
-                                          st);
-            if (h.prim == 0xFFFFFFFFu) {
-                const float4 a = f.p.Lnee[pid];
-                float4 L = f.p.L[pid];
-                L.x += a.x; L.y += a.y; L.z += a.z; L.w += a.w;
-                f.p.L[pid] = L;
+    if (FETCH) {
+        TravRay ray;
+        const Bvh4Node* nodes = nullptr;
+        const TriRecord* tris = nullptr;
+        uint32_t pid = 0;
+        bool active = false, exhausted = false;
+        while (true) {
+            const bool anyBusy = __ballot(active) != 0ull;
+            const uint32_t idx = wave_fetch(f.counters + CNT_SHADOW_HEAD, count, !active && !exhausted, anyBusy, exhausted);
+            if (idx != 0xFFFFFFFFu) {
+                pid = f.p.sq[idx];
+                const SceneView& S = scene_of<BATCH>(f, pid);
+                nodes = S.nodes4;
+                tris = S.tris;
+                const float4 so = f.p.sro[pid], sd = f.p.srd[pid];
+                trav_init(ray, V3{so.x, so.y, so.z}, V3{sd.x, sd.y, sd.z}, so.w, S.nnodes4);
+                active = true;
+                ++nrays;
             }
-            ++nrays;
+            if (__ballot(active) == 0ull) break;
+            if (active && trav_step<true>(nodes, tris, ray, st)) {
+                if (ray.h.prim == 0xFFFFFFFFu) add_nee(f, pid);
+                active = false;
+            }
+        }
+    } else {
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i - threadIdx.x < count; i += gridDim.x * blockDim.x) {
+            if (i < count) {
+                const uint32_t pid = f.p.sq[i];
+                const SceneView& S = scene_of<BATCH>(f, pid);
+                const float4 so = f.p.sro[pid], sd = f.p.srd[pid];
+                const Hit h = traverse4<true>(S.nodes4, S.tris, S.nnodes4, V3{so.x, so.y, so.z},
+                                              V3{sd.x, sd.y, sd.z}, so.w, st);
+                if (h.prim == 0xFFFFFFFFu) add_nee(f, pid);
+                ++nrays;
+            }
         }
     }
     wave_count(f.stats + STAT_SHADOW, nrays);
@@ -862,7 +952,10 @@ static hipError_t launch_pass_t(rbh_ctx c, Frame1 f, uint32_t depthMax, const La
         f.bounce = b;
         if (b > 0) hipLaunchKernelGGL(k_reset_hits, dim3(1), dim3(64), 0, c->stream, f.counters);
         RBH_TIMED(KernelTimer::TRACE,
-                  hipLaunchKernelGGL(k_trace<BATCH>, dim3(ls.traceGrid), dim3(kTraceBlock), 0, c->stream, f, qsel));
+                  if (ls.traceFetch) hipLaunchKernelGGL((k_trace<BATCH, true>), dim3(ls.traceGrid), dim3(kTraceBlock), 0,
+                                                        c->stream, f, qsel);
+                  else hipLaunchKernelGGL((k_trace<BATCH, false>), dim3(ls.traceGrid), dim3(kTraceBlock), 0, c->stream,
+                                          f, qsel));
         RBH_TIMED(KernelTimer::SHADE,
                   hipLaunchKernelGGL(k_miss<BATCH>, dim3(ls.shadeGrid), dim3(256), 0, c->stream, f));
         if (!BATCH && f.guided)
@@ -882,7 +975,10 @@ static hipError_t launch_pass_t(rbh_ctx c, Frame1 f, uint32_t depthMax, const La
             RBH_TIMED(KernelTimer::SHADE, hipLaunchKernelGGL((k_shade<false, BATCH, 1>), dim3(ls.shadeGrid),
                                                              dim3(kShadeBlock), 0, c->stream, f, qsel));
         RBH_TIMED(KernelTimer::SHADOW,
-                  hipLaunchKernelGGL(k_shadow<BATCH>, dim3(ls.traceGrid), dim3(kTraceBlock), 0, c->stream, f));
+                  if (ls.traceFetch) hipLaunchKernelGGL((k_shadow<BATCH, true>), dim3(ls.traceGrid), dim3(kTraceBlock), 0,
+                                                        c->stream, f);
+                  else hipLaunchKernelGGL((k_shadow<BATCH, false>), dim3(ls.traceGrid), dim3(kTraceBlock), 0, c->stream,
+                                          f));
     }
 #undef RBH_TIMED
     hipLaunchKernelGGL(k_accumulate<BATCH>, accGrid, dim3(256), 0, c->stream, f);

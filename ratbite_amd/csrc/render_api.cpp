@@ -226,6 +226,7 @@ int prepare_pass_state(rbh_ctx c, size_t NP, Frame1& f, LaunchShape& ls) {
     ls.traceGrid = std::max(1, cus * rs.traceBlocksPerCU);
     ls.shadeGrid = std::max(1, cus * rs.shadeBlocksPerCU);
     ls.shadeWaves = rs.shadeWaves;
+    ls.traceFetch = rs.traceFetch;
     const size_t lanes = (size_t)ls.traceGrid * kTraceBlock;
     RBH_ALLOC(c, rs.spill, trace_spill_words(lanes) * sizeof(int));
     f.spill = rs.spill.as<int>();
@@ -508,6 +509,9 @@ int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_
     double bestRate = -1.0;
     rbh_tuning b{};
     (void)freeB;
+    const int keepF = rs.traceFetch;
+    rs.traceFetch = 1;
+    b.traceFetch = 1;
     for (int tg : gridList) {
         for (int sg : gridList) {
             rs.slotBudget = keep;
@@ -517,6 +521,7 @@ int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_
             if (rc) {
                 rs.traceBlocksPerCU = keepT;
                 rs.shadeBlocksPerCU = keepS;
+                rs.traceFetch = keepF;
                 return rc;
             }
             const double rate = (double)P * trialSpp / (c->stats.renderMs * 1e-3);
@@ -547,7 +552,24 @@ int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_
         }
     }
     if (bestRate >= 0.0) {
+        // wave-synchronous ray batches instead of per-lane work fetch
+        rs.traceBlocksPerCU = b.traceBlocksPerCU;
+        rs.shadeBlocksPerCU = b.shadeBlocksPerCU;
+        rs.shadeWaves = b.shadeWaves ? b.shadeWaves : 1;
+        rs.traceFetch = 0;
+        const int rc = rbh_render(c, &t);
+        rs.traceFetch = 1;
+        if (rc) return rc;
+        const double rate = (double)P * trialSpp / (c->stats.renderMs * 1e-3);
+        if (rate > bestRate) {
+            bestRate = rate;
+            b.samplesPerSecond = rate;
+            b.traceFetch = 0;
+        }
+    }
+    if (bestRate >= 0.0) {
         // the best shape once more as a replayed graph
+        rs.traceFetch = b.traceFetch;
         rs.slotBudget = b.pathSlots;
         rs.traceBlocksPerCU = b.traceBlocksPerCU;
         rs.shadeBlocksPerCU = b.shadeBlocksPerCU;
@@ -567,6 +589,7 @@ int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_
         rs.slotBudget = keep;
         rs.traceBlocksPerCU = keepT;
         rs.shadeBlocksPerCU = keepS;
+        rs.traceFetch = keepF;
         return fail(c, RBH_ERR_NOMEM, "no launch shape fits the free device memory");
     }
     rs.slotBudget = b.pathSlots;
@@ -574,6 +597,7 @@ int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_
     rs.shadeBlocksPerCU = b.shadeBlocksPerCU;
     rs.useGraph = b.useGraph != 0;
     rs.shadeWaves = b.shadeWaves ? b.shadeWaves : 1;
+    rs.traceFetch = b.traceFetch;
     if (!b.shadeWaves) b.shadeWaves = 1;
     *best = b;
     return RBH_OK;
@@ -589,6 +613,7 @@ int rbh_set_tuning(rbh_ctx c, const rbh_tuning* t) {
     c->rs.shadeBlocksPerCU = t->shadeBlocksPerCU;
     c->rs.useGraph = t->useGraph != 0;
     c->rs.shadeWaves = (t->shadeWaves == 3 || t->shadeWaves == 4) ? t->shadeWaves : 1;
+    c->rs.traceFetch = t->traceFetch != 0;
     return RBH_OK;
 }
 

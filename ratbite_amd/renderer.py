@@ -62,10 +62,11 @@ class Renderer:
         self.samples = 0
         return {"pathSlots": t.pathSlots, "traceBlocksPerCU": t.traceBlocksPerCU,
                 "shadeBlocksPerCU": t.shadeBlocksPerCU, "useGraph": t.useGraph, "shadeWaves": t.shadeWaves,
-                "samplesPerSecond": t.samplesPerSecond}
+                "traceFetch": t.traceFetch, "samplesPerSecond": t.samplesPerSecond}
 
-    def set_tuning(self, path_slots=0, trace_blocks=16, shade_blocks=16, use_graph=False, shade_waves=1):
-        t = L.Tuning(path_slots, trace_blocks, shade_blocks, int(use_graph), shade_waves, 0.0)
+    def set_tuning(self, path_slots=0, trace_blocks=16, shade_blocks=16, use_graph=False, shade_waves=1,
+                   trace_fetch=1):
+        t = L.Tuning(path_slots, trace_blocks, shade_blocks, int(use_graph), shade_waves, int(trace_fetch), 0.0)
         self.ctx.call("rbh_set_tuning", ctypes.byref(t))
 
     # ------------------------------------------------------------ guiding
@@ -107,6 +108,12 @@ class Renderer:
     def image(self, exposure=0.0):
         m = np.asarray(self.scene.tables["xyz_to_srgb"], np.float32)
         return self.ctx.resolve(np.ascontiguousarray(m.T), 2.0 ** (exposure - 7.0))
+
+    def save_png(self, path, exposure=0.0):
+        """Resolves the accumulation and writes it as a PNG, bottom film row
+        last (src/core/ui.cpp:894-912)."""
+        from .export import write_png
+        write_png(path, self.image(exposure))
 
     def close(self):
         self.ctx.close()

@@ -340,19 +340,29 @@ def _finish(parts, mats, spectra, tex_list, cam, env, width, height, depth, spp,
 
 
 # ----------------------------------------------------------------- configs
-def cornell(seed=0x5EED0001, width=256, height=256, spp=16, depth=8, light_up=False, occluder=False):
+def cornell(seed=0x5EED0001, width=256, height=256, spp=16, depth=8, light_up=False, occluder=False, bounce=False):
     """Seeded Cornell box. With light_up the emitter faces the ceiling from
     just below it, so every other surface is lit only indirectly. With
     occluder a plate under the ceiling light hides it from the room, which is
     then lit only through the gap between the plate and the walls (a scene
-    where BSDF sampling rarely finds the light's path and NEE is occluded)."""
+    where BSDF sampling rarely finds the light's path and NEE is occluded).
+    With bounce a small emitter 0.5 in front of the back wall faces it: the
+    room is lit by the bright patch it throws on the wall, which NEE cannot
+    target and BSDF sampling finds only by chance, while the patch's own NEE
+    is well conditioned (no emitter within 0.4 of a lit surface); the spheres
+    stay in the lower half of the box, out of the emitter's view, and every
+    surface is at least 0.4 rough, so that no caustic path adds its own
+    variance."""
     rng = np.random.default_rng(seed)
     tables = load_tables()
     mats, parts = [], []
     spectra = [tables["blackbody4000"]]
 
     def gd():
-        return material(base=rng.uniform(0.1, 0.9, 3), metal=0.0, rough=float(rng.uniform(0.05, 1.0)), ior=1.5)
+        rough = float(rng.uniform(0.05, 1.0))
+        if bounce:
+            rough = 0.4 + 0.6 * rough   # no sharp highlights: the variance is in the indirect light
+        return material(base=rng.uniform(0.1, 0.9, 3), metal=0.0, rough=rough, ior=1.5)
 
     walls = [
         ((-1, 0, -1), (-1, 0, 1), (1, 0, 1), (1, 0, -1)),          # floor (faces +y)
@@ -368,7 +378,10 @@ def cornell(seed=0x5EED0001, width=256, height=256, spp=16, depth=8, light_up=Fa
         mats.append(gd())
         parts.append((v, i, n, uv, len(mats) - 1))
     # emissive quad just below the ceiling, facing down, blackbody 4000 K
-    if light_up:
+    if bounce:
+        v, i, n, uv = quad((-0.2, 1.4, -0.5), (-0.2, 1.8, -0.5), (0.2, 1.8, -0.5), (0.2, 1.4, -0.5))
+        power = 60.0
+    elif light_up:
         v, i, n, uv = quad((-0.4, 1.9, -0.4), (-0.4, 1.9, 0.4), (0.4, 1.9, 0.4), (0.4, 1.9, -0.4))
         power = 60.0
     else:
@@ -385,6 +398,10 @@ def cornell(seed=0x5EED0001, width=256, height=256, spp=16, depth=8, light_up=Fa
     for _ in range(8):
         r = float(rng.uniform(0.1, 0.3))
         c = rng.uniform([-1 + r, r, -1 + r], [1 - r, 2 - r, 1 - r])
+        if bounce:
+            # lower half, behind the emitter's plane: no sphere sees its face
+            c[1] = min(c[1], 1.0)
+            c[2] = max(c[2], -0.4 + r)
         v, i, n, uv = uv_sphere(c, r)
         mats.append(gd())
         parts.append((v, i, n, uv, len(mats) - 1))

@@ -70,6 +70,10 @@ def test_autotune_keeps_results(ctx):
         r.set_tuning(shade_waves=w)
         r.render(spp=8, offset=0)
         assert np.array_equal(a, r.accum()), w
+    # wave-synchronous ray batches instead of per-lane work fetch
+    r.set_tuning(trace_fetch=0)
+    r.render(spp=8, offset=0)
+    assert np.array_equal(a, r.accum())
     r.set_tuning()
 
 
