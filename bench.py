@@ -45,7 +45,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-spp", type=int, default=256, help="spp of the CPU baseline crop")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, host cores)")
-    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (profiling runs)")
+    ap.add_argument("--tune", action="store_true",
+                    help="autotune the launch shape first (short 8-spp renders) instead of the library default, "
+                         "which was picked by a sweep of full frames of this workload")
     ap.add_argument("--copy-mib", type=int, default=1024,
                     help="size of the device-to-device copy timed first (the measured bandwidth ceiling; "
                          "its k_copy dispatch also scales PMC byte counters)")
@@ -244,8 +246,9 @@ def main():
     st0 = ctx.stats()
     print("LBVH %.1f ms depth %d; env CDF %.3f ms" % (st0["bvhMs"], st0["bvhDepth"], st0["cdfMs"]),
           file=sys.stderr, flush=True)
-    tuning = None if args.no_tune else r.autotune(trial_spp=8, depth=args.depth)
-    print("autotuned launch shape %s" % tuning, file=sys.stderr, flush=True)
+    tuning = r.autotune(trial_spp=8, depth=args.depth) if args.tune else r.tuning()
+    print("launch shape (%s) %s" % ("autotuned" if args.tune else "library default", tuning), file=sys.stderr,
+          flush=True)
     copy_gbs = None
     if args.copy_mib > 0:
         import ctypes

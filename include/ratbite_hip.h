@@ -234,6 +234,8 @@ typedef struct rbh_tuning {
  * accumulation buffer. */
 int rbh_autotune(rbh_ctx ctx, const rbh_launch_params* lp, uint32_t trialSpp, rbh_tuning* best);
 int rbh_set_tuning(rbh_ctx ctx, const rbh_tuning* t);
+/* The launch shape the context renders with (samplesPerSecond 0). */
+int rbh_get_tuning(rbh_ctx ctx, rbh_tuning* t);
 /* Copy the FP64 (x, y, z, w) accumulation buffer (W*H*4 doubles). */
 int rbh_read_accum(rbh_ctx ctx, double* rgbw);
 int rbh_write_accum(rbh_ctx ctx, const double* rgbw, uint32_t width, uint32_t height);

@@ -126,6 +126,7 @@ _SIGS = {
     "rbh_render_batch": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_void_p]),
     "rbh_autotune": (c_int, [c_void_p, ctypes.POINTER(LaunchParams), c_uint32, ctypes.POINTER(Tuning)]),
     "rbh_set_tuning": (c_int, [c_void_p, ctypes.POINTER(Tuning)]),
+    "rbh_get_tuning": (c_int, [c_void_p, ctypes.POINTER(Tuning)]),
     "rbh_guide_configure": (c_int, [c_void_p, ctypes.POINTER(GuideConfig)]),
     "rbh_guide_set_params": (c_int, [c_void_p, c_void_p, c_uint32]),
     "rbh_guide_get_params": (c_int, [c_void_p, c_void_p, c_uint32]),

@@ -127,10 +127,13 @@ struct RenderState {
     // about 200 B of HBM per slot; 0 = as many as a quarter of the free
     // device memory holds (fewer passes pay the deep-bounce tail fewer times)
     size_t slotBudget = 0;
-    int traceBlocksPerCU = 16;   // persistent-grid sizes (autotuned)
-    int shadeBlocksPerCU = 16;
-    int shadeWaves = 1;          // shade-kernel variant (1, 3, 4 waves per SIMD)
-    int traceFetch = 1;          // trace/shadow lanes fetch a new ray as soon as theirs completes
+    // launch shape: the fastest of a sweep of full 1080p 64-spp frames of the
+    // 4.1M-triangle spheres scene (persistent grids capped at one lane per
+    // path slot); rbh_autotune / rbh_set_tuning replace it per context
+    int traceBlocksPerCU = 64;
+    int shadeBlocksPerCU = 64;
+    int shadeWaves = 3;          // shade-kernel variant (1, 3, 4 waves per SIMD)
+    int traceFetch = 0;          // 1: trace/shadow lanes fetch a new ray as soon as theirs completes
     DevBuf sceneTable;           // batched passes: SceneView per scene and pass
     bool useGraph = false;       // replay each pass's launch sequence as a captured HIP graph
     hipGraphExec_t graphExec = nullptr;

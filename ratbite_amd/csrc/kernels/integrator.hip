@@ -220,9 +220,37 @@ __device__ __forceinline__ TravStack<kStackLds> lane_stack(const Frame1& f, int*
     return TravStack<kStackLds>{lds + threadIdx.x, (int)blockDim.x, f.spill + t, f.spillStride};
 }
 
+// Wave-private staging of queue pushes: finished rays are appended to an
+// LDS list of the wave and written out 64 or more at a time with one atomic,
+// instead of one atomic per wave for every traversal step in which some lane
+// finished.
+struct WaveList {
+    uint32_t* buf;   // 128 entries of this wave
+    uint32_t n;      // entries held (wave-uniform)
+    __device__ __forceinline__ void add(bool pred, uint32_t v) {
+        const unsigned long long mask = __ballot(pred);
+        const uint32_t rank = (uint32_t)__popcll(mask & ((1ull << __lane_id()) - 1ull));
+        if (pred) buf[n + rank] = v;
+        n += (uint32_t)__popcll(mask);
+    }
+    // writes the held entries to queue[*counter ...] when at least `min` are held
+    __device__ __forceinline__ void flush(uint32_t* __restrict__ counter, uint32_t* __restrict__ queue, uint32_t min) {
+        if (n == 0 || n < min) return;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");   // the lanes' LDS writes are visible
+        const int lane = __lane_id();
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(counter, n);
+        base = __shfl(base, 0);
+        for (uint32_t k = (uint32_t)lane; k < n; k += 64u) queue[base + k] = buf[k];
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");   // reads done before the list is refilled
+        n = 0;
+    }
+};
+
 template <bool BATCH, bool FETCH>
 __global__ __launch_bounds__(kTraceBlock) void k_trace(Frame1 f, uint32_t qsel) {
     __shared__ int stack[kStackLds * kTraceBlock];
+    __shared__ uint32_t staged[FETCH ? 2 * kTraceBlock * 2 : 1];   // per wave: 128 hits, 128 misses
     const TravStack<kStackLds> st = lane_stack(f, stack);
     const uint32_t count = f.counters[qsel ? CNT_QUEUE1 : CNT_QUEUE0];
     const uint32_t* __restrict__ queue = qsel ? f.p.q1 : f.p.q0;
@@ -235,6 +263,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(Frame1 f, uint32_t qsel) 
         const TriRecord* tris = nullptr;
         uint32_t pid = 0;
         bool active = false, exhausted = false;
+        const uint32_t wave = threadIdx.x >> 6;
+        WaveList hits{staged + wave * 256u, 0u}, misses{staged + wave * 256u + 128u, 0u};
         while (true) {
             const bool anyBusy = __ballot(active) != 0ull;
             const uint32_t idx = wave_fetch(f.counters + CNT_TRACE_HEAD, count, !active && !exhausted, anyBusy, exhausted);
@@ -253,12 +283,14 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(Frame1 f, uint32_t qsel) 
             if (active) done = trav_step<false>(nodes, tris, ray, st);
             const bool hit = done && ray.h.prim != 0xFFFFFFFFu;
             if (hit) f.p.hit[pid] = make_float4(ray.h.t, ray.h.b1, ray.h.b2, __uint_as_float(ray.h.prim));
-            const uint32_t slot = wave_push(f.counters + CNT_HIT, hit);
-            if (hit) f.p.hq[slot] = pid;
-            const uint32_t mslot = wave_push(f.counters + CNT_MISS, done && !hit);
-            if (done && !hit) f.p.mq[mslot] = pid;
+            hits.add(hit, pid);
+            misses.add(done && !hit, pid);
+            hits.flush(f.counters + CNT_HIT, f.p.hq, 64u);
+            misses.flush(f.counters + CNT_MISS, f.p.mq, 64u);
             if (done) active = false;
         }
+        hits.flush(f.counters + CNT_HIT, f.p.hq, 1u);
+        misses.flush(f.counters + CNT_MISS, f.p.mq, 1u);
     } else {
         for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i - threadIdx.x < count; i += gridDim.x * blockDim.x) {
             bool hit = false;

@@ -223,8 +223,9 @@ int prepare_pass_state(rbh_ctx c, size_t NP, Frame1& f, LaunchShape& ls) {
     f.counters = counters;
     f.stats = stats;
     const int cus = cu_count(c->device);
-    ls.traceGrid = std::max(1, cus * rs.traceBlocksPerCU);
-    ls.shadeGrid = std::max(1, cus * rs.shadeBlocksPerCU);
+    // persistent grids, never more lanes than the pass has path slots
+    ls.traceGrid = (int)std::max<size_t>(1, std::min<size_t>((size_t)cus * rs.traceBlocksPerCU, (NP + kTraceBlock - 1) / kTraceBlock));
+    ls.shadeGrid = (int)std::max<size_t>(1, std::min<size_t>((size_t)cus * rs.shadeBlocksPerCU, (NP + kShadeBlock - 1) / kShadeBlock));
     ls.shadeWaves = rs.shadeWaves;
     ls.traceFetch = rs.traceFetch;
     const size_t lanes = (size_t)ls.traceGrid * kTraceBlock;
@@ -505,13 +506,14 @@ int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_
     t.flags &= ~(uint32_t)(RBH_RENDER_COLLECT | RBH_RENDER_TIMING);
     // the slot budget stays automatic (all samples of a launch in one pass
     // when memory allows); the grids of the persistent kernels are searched
-    const int gridList[] = {8, 16, 32};
+    const int gridList[] = {8, 16, 32, 64};
     double bestRate = -1.0;
     rbh_tuning b{};
     (void)freeB;
-    const int keepF = rs.traceFetch;
-    rs.traceFetch = 1;
-    b.traceFetch = 1;
+    const int keepF = rs.traceFetch, keepW = rs.shadeWaves;
+    rs.traceFetch = 0;
+    rs.shadeWaves = 1;
+    b.traceFetch = 0;
     for (int tg : gridList) {
         for (int sg : gridList) {
             rs.slotBudget = keep;
@@ -522,6 +524,7 @@ int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_
                 rs.traceBlocksPerCU = keepT;
                 rs.shadeBlocksPerCU = keepS;
                 rs.traceFetch = keepF;
+                rs.shadeWaves = keepW;
                 return rc;
             }
             const double rate = (double)P * trialSpp / (c->stats.renderMs * 1e-3);
@@ -552,19 +555,19 @@ int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_
         }
     }
     if (bestRate >= 0.0) {
-        // wave-synchronous ray batches instead of per-lane work fetch
+        // per-lane work fetch instead of wave-synchronous ray batches
         rs.traceBlocksPerCU = b.traceBlocksPerCU;
         rs.shadeBlocksPerCU = b.shadeBlocksPerCU;
         rs.shadeWaves = b.shadeWaves ? b.shadeWaves : 1;
-        rs.traceFetch = 0;
-        const int rc = rbh_render(c, &t);
         rs.traceFetch = 1;
+        const int rc = rbh_render(c, &t);
+        rs.traceFetch = 0;
         if (rc) return rc;
         const double rate = (double)P * trialSpp / (c->stats.renderMs * 1e-3);
         if (rate > bestRate) {
             bestRate = rate;
             b.samplesPerSecond = rate;
-            b.traceFetch = 0;
+            b.traceFetch = 1;
         }
     }
     if (bestRate >= 0.0) {
@@ -590,6 +593,7 @@ int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_
         rs.traceBlocksPerCU = keepT;
         rs.shadeBlocksPerCU = keepS;
         rs.traceFetch = keepF;
+        rs.shadeWaves = keepW;
         return fail(c, RBH_ERR_NOMEM, "no launch shape fits the free device memory");
     }
     rs.slotBudget = b.pathSlots;
@@ -614,6 +618,18 @@ int rbh_set_tuning(rbh_ctx c, const rbh_tuning* t) {
     c->rs.useGraph = t->useGraph != 0;
     c->rs.shadeWaves = (t->shadeWaves == 3 || t->shadeWaves == 4) ? t->shadeWaves : 1;
     c->rs.traceFetch = t->traceFetch != 0;
+    return RBH_OK;
+}
+
+int rbh_get_tuning(rbh_ctx c, rbh_tuning* t) {
+    if (!c || !t) return RBH_ERR_INVALID;
+    *t = rbh_tuning{};
+    t->pathSlots = c->rs.slotBudget;
+    t->traceBlocksPerCU = c->rs.traceBlocksPerCU;
+    t->shadeBlocksPerCU = c->rs.shadeBlocksPerCU;
+    t->useGraph = c->rs.useGraph ? 1 : 0;
+    t->shadeWaves = c->rs.shadeWaves;
+    t->traceFetch = c->rs.traceFetch;
     return RBH_OK;
 }
 

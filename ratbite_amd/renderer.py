@@ -64,8 +64,17 @@ class Renderer:
                 "shadeBlocksPerCU": t.shadeBlocksPerCU, "useGraph": t.useGraph, "shadeWaves": t.shadeWaves,
                 "traceFetch": t.traceFetch, "samplesPerSecond": t.samplesPerSecond}
 
-    def set_tuning(self, path_slots=0, trace_blocks=16, shade_blocks=16, use_graph=False, shade_waves=1,
-                   trace_fetch=1):
+    def tuning(self):
+        """The launch shape this context renders with."""
+        t = L.Tuning()
+        self.ctx.call("rbh_get_tuning", ctypes.byref(t))
+        return {"pathSlots": t.pathSlots, "traceBlocksPerCU": t.traceBlocksPerCU,
+                "shadeBlocksPerCU": t.shadeBlocksPerCU, "useGraph": t.useGraph, "shadeWaves": t.shadeWaves,
+                "traceFetch": t.traceFetch}
+
+    def set_tuning(self, path_slots=0, trace_blocks=64, shade_blocks=64, use_graph=False, shade_waves=3,
+                   trace_fetch=0):
+        """Sets the launch shape; the defaults are the library's own."""
         t = L.Tuning(path_slots, trace_blocks, shade_blocks, int(use_graph), shade_waves, int(trace_fetch), 0.0)
         self.ctx.call("rbh_set_tuning", ctypes.byref(t))
 

@@ -66,12 +66,12 @@ def test_autotune_keeps_results(ctx):
     assert t["samplesPerSecond"] > 0 and t["pathSlots"] >= 0
     r.render(spp=8, offset=0)
     assert np.array_equal(a, r.accum())
-    for w in (3, 4):
+    for w in (1, 3, 4):
         r.set_tuning(shade_waves=w)
         r.render(spp=8, offset=0)
         assert np.array_equal(a, r.accum()), w
-    # wave-synchronous ray batches instead of per-lane work fetch
-    r.set_tuning(trace_fetch=0)
+    # per-lane work fetch instead of wave-synchronous ray batches
+    r.set_tuning(trace_fetch=1)
     r.render(spp=8, offset=0)
     assert np.array_equal(a, r.accum())
     r.set_tuning()
