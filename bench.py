@@ -61,15 +61,17 @@ def parse():
 
 
 def traffic_per_launch(kernel):
-    """HBM bytes per launch of `kernel`, from the committed PMC summary
-    profiles/<kernel>_spheres1080p_pmc.txt (a line 'traffic_bytes_per_launch: X'),
-    or None when no such measurement exists."""
-    path = os.path.join(ROOT, "profiles", "%s_spheres1080p_pmc.txt" % kernel)
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE of one frame
+    of this workload, scaled by the counters' reading of a plain copy), from
+    the committed PMC summary profiles/spheres1080p_pmc.txt (a line
+    '<kernel> traffic_bytes_per_launch: X'), or None when it holds none."""
+    path = os.path.join(ROOT, "profiles", "spheres1080p_pmc.txt")
     if not os.path.exists(path):
         return None
+    key = "%s traffic_bytes_per_launch:" % kernel
     with open(path) as f:
         for line in f:
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(key):
                 return float(line.split(":", 1)[1])
     return None
 
@@ -308,6 +310,7 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "avg_launch_ms": kms[dom] / max(1, launches),
                      "algorithmic_bytes_per_launch": kbytes[dom] / max(1, launches),
+                     "traffic_gbs": None if traffic is None else traffic / (kms[dom] / max(1, launches) * 1e-3) / 1e9,
                      "copy_gbs_measured": copy_gbs},
         "path_roofline": {"achieved": path_bytes / elapsed / 1e9, "unit": "GB/s",
                           "frac": path_bytes / elapsed / 1e9 / HBM_PEAK_GBS},
