@@ -55,8 +55,14 @@ def parse():
                     help="spheres: the 1080p frame (default); batch: 64 seeded scenes in one batched launch; "
                          "guided: relMSE at equal time, guided vs plain")
     ap.add_argument("--scenes", type=int, default=64, help="batch workload: number of scenes")
-    ap.add_argument("--budget-ms", type=float, default=4000.0, help="guided workload: time budget per method")
-    ap.add_argument("--ref-spp", type=int, default=2048, help="guided workload: spp of the plain reference")
+    ap.add_argument("--budget-ms", type=float, default=1000.0, help="guided workload: time budget per method")
+    ap.add_argument("--ref-spp", type=int, default=8192, help="guided workload: spp of the plain reference")
+    ap.add_argument("--scene", choices=["spheres", "bounce"], default="spheres", help="guided workload: scene")
+    ap.add_argument("--chunk-spp", type=int, default=4, help="guided workload: spp per render call")
+    ap.add_argument("--train-iters", type=int, default=8, help="guided workload: collect+train rounds")
+    ap.add_argument("--train-steps", type=int, default=8, help="guided workload: Adam steps per round")
+    ap.add_argument("--train-batch", type=int, default=1 << 18, help="guided workload: records per Adam step")
+    ap.add_argument("--train-lr", type=float, default=2e-3, help="guided workload: Adam learning rate")
     return ap.parse_args()
 
 
@@ -159,20 +165,74 @@ def batch_workload(args):
         r.close()
 
 
+FLOP_FORWARD = 2 * (80 * 64 + 3 * 64 * 64 + 64 * 40)   # 39,936 per query
+FLOP_TRAIN = 3 * FLOP_FORWARD                           # forward + backward data + weight gradients
+MFMA_PEAK_TFLOPS = {"bf16": 2500.0, "fp8": 5000.0}
+
+
+def mlp_rates(r, chunk):
+    """MFMA rates of the guiding network: 32 Adam steps on 2^18 synthetic
+    records (staged once), and inference inside timed guided renders (BF16
+    and FP8; one query per shaded vertex)."""
+    import ctypes
+    import numpy as np
+    from ratbite_amd import _lib as L
+    rng = np.random.default_rng(7)
+    n, steps = 1 << 18, 32
+    rec = np.zeros((n, L.GUIDE_REC), np.float32)
+    rec[:, 0:3] = rng.uniform(0, 1, (n, 3))
+    for a, b in ((3, 6), (6, 9), (11, 14)):
+        v = rng.normal(size=(n, 3))
+        rec[:, a:b] = v / np.linalg.norm(v, axis=1, keepdims=True)
+    rec[:, 9] = rng.uniform(0.05, 1, n)
+    rec[:, 10] = rng.uniform(0, 1, n)
+    rec[:, 14] = rng.exponential(1.0, n)
+    loss = ctypes.c_float()
+    r.ctx.call("rbh_guide_train", L.ptr(rec), n, n, 1, ctypes.byref(loss))     # warm-up
+    r.ctx.synchronize()
+    t = time.perf_counter()
+    r.ctx.call("rbh_guide_train", L.ptr(rec), n, n, steps, ctypes.byref(loss))
+    train_s = time.perf_counter() - t
+    out = {"train_step_ms": train_s * 1e3 / steps, "train_records_per_step": n,
+           "train_tflops": steps * n * FLOP_TRAIN / train_s / 1e12}
+    out["train_frac"] = out["train_tflops"] / MFMA_PEAK_TFLOPS["bf16"]
+    for name, flag in (("bf16", 0), ("fp8", L.RBH_RENDER_GUIDE_FP8)):
+        st = r.render(spp=chunk, offset=1 << 22, guided=True, no_drop=True, flags=flag | L.RBH_RENDER_TIMING)
+        q = st["pathVertices"]
+        tf = q * FLOP_FORWARD / (st["guideMs"] * 1e-3) / 1e12
+        out["infer_%s" % name] = {"queries": q, "ms": st["guideMs"], "tflops": tf,
+                                  "frac": tf / MFMA_PEAK_TFLOPS[name]}
+    return out
+
+
 def guided_workload(args):
-    """relMSE against a high-spp plain render at equal wall time: the plain
-    tracer renders for the budget; the guided tracer spends the same budget
-    on collecting, training and rendering. Both use the no-drop sample
-    semantics so they converge to the same image."""
+    """Equal-time study of the learned guide: the plain tracer renders for the
+    budget; the guided tracer spends the same wall time on collecting
+    training records, training and rendering. The collecting renders are
+    guided renders of the image's own sample range, so their samples count;
+    only the Adam steps are overhead. relMSE (raw, and without the
+    0.1% largest per-pixel errors) against a plain render of --ref-spp
+    samples is taken at eight checkpoints of the budget with the clock
+    paused, giving the final ratio and the time at which the guided tracer
+    reaches the plain tracer's final error. Both sides use the no-drop sample
+    semantics, so they converge to the same image. --scene spheres is the
+    1080p workload's scene (at --width x --height), --scene bounce the
+    Cornell box lit by the patch a small emitter throws on the back wall."""
     import numpy as np
     from ratbite_amd import scenes
     from ratbite_amd.renderer import Renderer
 
-    def relmse(a, ref, eps=1e-2):
-        return float(np.mean((a - ref) ** 2 / (ref ** 2 + eps)))
+    def errs(a, ref, eps=1e-2):
+        e = ((a - ref) ** 2 / (ref ** 2 + eps)).mean(axis=2).ravel()
+        k = max(1, int(np.ceil(1e-3 * e.size)))
+        return float(e.mean()), float(np.sort(e)[:-k].mean())
 
-    sc = scenes.spheres(width=args.width, height=args.height, spp=16, depth=args.depth)
+    if args.scene == "bounce":
+        sc = scenes.cornell(width=args.width, height=args.height, spp=16, bounce=True)
+    else:
+        sc = scenes.spheres(width=args.width, height=args.height, spp=16, depth=args.depth)
     r = Renderer(sc)
+    chunk = args.chunk_spp
 
     def image():
         a = r.accum()
@@ -181,42 +241,65 @@ def guided_workload(args):
     t0 = time.perf_counter()
     done = 0
     while done < args.ref_spp:
-        step = min(64, args.ref_spp - done)
-        r.render(spp=step, offset=done, no_drop=True)
+        step = min(256, args.ref_spp - done)
+        r.render(spp=step, offset=1 << 24 | done, no_drop=True, fresh=(done == 0))
         done += step
     ref = image()
     print("reference: %d spp in %.1f s" % (args.ref_spp, time.perf_counter() - t0), file=sys.stderr, flush=True)
 
     def run(method):
+        checkpoints = [args.budget_ms * k / 8.0 for k in range(1, 9)]
+        spent, spp, trained, curve = 0.0, 0, 0, []
+        train_ms = 0.0
         r.ctx.synchronize()
-        t = time.perf_counter()
-        spp = 0
-        curve = []
-        trained = 0
-        while True:
-            el = (time.perf_counter() - t) * 1e3
-            if el >= args.budget_ms:
-                break
-            if method == "guided" and trained < 16:
-                r.render(spp=4, offset=100000 + 4 * trained, guided=True, collect=True)
-                r.train_guide(batch=1 << 18, steps=4)
+        while checkpoints:
+            t = time.perf_counter()
+            if method == "guided" and trained < args.train_iters:
+                # collecting renders are guided renders too (unbiased for any
+                # state of the network), so their samples join the image
+                r.render(spp=chunk, offset=spp, guided=True, collect=True)
+                spp += chunk
+                t1 = time.perf_counter()
+                r.train_guide(batch=args.train_batch, steps=args.train_steps)
                 trained += 1
-                continue
-            r.render(spp=4, offset=spp, guided=(method == "guided"), no_drop=True)
-            spp += 4
-            curve.append(((time.perf_counter() - t) * 1e3, spp))
-        return relmse(image(), ref), spp, curve
+                train_ms += (time.perf_counter() - t1) * 1e3
+            else:
+                r.render(spp=chunk, offset=spp, guided=(method == "guided"), no_drop=True)
+                spp += chunk
+            spent += (time.perf_counter() - t) * 1e3
+            while checkpoints and spent >= checkpoints[0]:
+                checkpoints.pop(0)
+                curve.append((spent, spp) + (errs(image(), ref) if spp else (float("inf"), float("inf"))))
+        return curve, train_ms
 
-    e_plain, spp_plain, _ = run("plain")
-    r.configure_guide(seed=1, lr=1e-3, capacity=1 << 22)
-    e_guided, spp_guided, _ = run("guided")
+    plain, _ = run("plain")
+    r.configure_guide(seed=1, lr=args.train_lr, capacity=1 << 22)
+    guided, train_ms = run("guided")
+
+    def reach(curve, target, idx):
+        for c in curve:
+            if c[idx] <= target:
+                return c[0]
+        return None
+
+    fp, fg = plain[-1], guided[-1]
+    t_raw, t_trim = reach(guided, fp[2], 2), reach(guided, fp[3], 3)
+    mlp = mlp_rates(r, chunk)
     out = {
-        "metric": "relMSE at equal time, guided vs plain", "value": e_guided / e_plain, "unit": "ratio",
+        "metric": "relMSE at equal time, guided vs plain", "value": fg[2] / fp[2], "unit": "ratio",
         "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": args.budget_ms, "higher_is_better": False,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+bf16", "data": "synthetic (seeded scene generator)",
-        "config": {"workload": "spheres %dx%d, depth %d, %d ms per method, reference %d spp plain"
-                               % (sc.width, sc.height, args.depth, args.budget_ms, args.ref_spp)},
-        "relmse_plain": e_plain, "relmse_guided": e_guided, "spp_plain": spp_plain, "spp_guided": spp_guided,
+        "config": {"workload": "%s %dx%d, %d ms per method, reference %d spp plain, guide trained %d x (%d spp "
+                               "collect + %d Adam steps of %d records)"
+                               % (args.scene, sc.width, sc.height, args.budget_ms, args.ref_spp, args.train_iters,
+                                  chunk, args.train_steps, args.train_batch)},
+        "relmse_plain": fp[2], "relmse_guided": fg[2], "relmse_trimmed_plain": fp[3], "relmse_trimmed_guided": fg[3],
+        "ratio_trimmed": fg[3] / fp[3], "spp_plain": fp[1], "spp_guided": fg[1], "guided_training_ms": train_ms,
+        "time_to_plain_relmse": None if t_raw is None else t_raw / args.budget_ms,
+        "time_to_plain_relmse_trimmed": None if t_trim is None else t_trim / args.budget_ms,
+        "curve_plain": [[round(c[0], 1), c[1], c[2], c[3]] for c in plain],
+        "curve_guided": [[round(c[0], 1), c[1], c[2], c[3]] for c in guided],
+        "mlp": mlp,
     }
     print(json.dumps(out), flush=True)
     r.close()

@@ -137,6 +137,8 @@ typedef struct rbh_stats {
     uint32_t traceLaunches;       /* closest-hit kernel launches of the last render */
     uint32_t bvhDepth;            /* max depth of the LBVH */
     double guideMs;               /* time in guiding inference kernels */
+    uint64_t nonFiniteSamples;    /* samples whose radiance was not finite (added as black; the reference would
+                                     accumulate the NaN/inf into the pixel) */
 } rbh_stats;
 
 /* ----------------------------------------------------------------- context

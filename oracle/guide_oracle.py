@@ -7,16 +7,21 @@ CPU oracle its HIP kernels are checked against. It defines:
   * the 80-wide input: a 12-frequency sin/cos encoding of the scene-normalised
     position (72), the shading normal (3), the outgoing direction (3), the
     roughness (1) and the normalised wavelength (1);
-  * an MLP 80 -> 64 -> 64 -> 64 -> 64 -> 40 with ReLU on the hidden layers;
+  * an MLP 80 -> 64 -> 64 -> 64 -> 64 -> 41 with ReLU on the hidden layers;
   * the output as 8 von Mises-Fisher lobes, 5 numbers each: a mean direction
-    (normalised), log sharpness (clamped) and a mixture logit (softmax);
-  * the loss of one training record, weight * -log q(w_i | x), whose batch
-    mean divided by the mean weight is minimised (a KL divergence between the
-    radiance-weighted direction density and the mixture);
+    (normalised), log sharpness (clamped) and a mixture logit (softmax), and
+    a selection logit c: the renderer draws from the lobes with probability
+    a = SEL_MIN + (SEL_MAX - SEL_MIN) sigmoid(c), otherwise from the BSDF;
+  * the loss of one training record, weight * -log(a q(d) + (1 - a) p_bsdf(d)),
+    whose batch sum divided by the weight sum is minimised: a KL divergence
+    between the density of the sampled contribution (BSDF x cosine x incident
+    radiance, the record weight being that product over the sampling pdf) and
+    the one-sample MIS density the renderer actually samples;
   * its analytic gradient and Adam.
 
 Record layout (16 floats): position[0:3] in [0,1]^3, normal[3:6], wo[6:9],
-roughness[9], wavelength in [0,1][10], sampled direction[11:14], weight[14].
+roughness[9], wavelength in [0,1][10], sampled direction[11:14], weight[14],
+BSDF-sampler density of the direction[15].
 """
 import numpy as np
 
@@ -25,7 +30,9 @@ N_IN = 80
 N_IN_PAD = 96          # K padded to a multiple of 32 for the MFMA tiles
 WIDTH = 64
 N_LOBES = 8
-N_OUT = 5 * N_LOBES    # 40
+N_MIX = 5 * N_LOBES    # 40 lobe outputs
+N_OUT = N_MIX + 1      # and the selection logit
+SEL_MIN, SEL_MAX = 0.1, 0.9
 N_OUT_PAD = 48
 LAYERS = [(N_IN, WIDTH), (WIDTH, WIDTH), (WIDTH, WIDTH), (WIDTH, WIDTH), (WIDTH, N_OUT)]
 LOG_KAPPA_MIN, LOG_KAPPA_MAX = -3.0, 7.0
@@ -91,7 +98,7 @@ def bf16(x):
 
 
 def forward(ps, x, emulate_bf16=False):
-    """Returns the raw 40 outputs and the per-layer inputs (for backward).
+    """Returns the raw 41 outputs and the per-layer inputs (for backward).
     With emulate_bf16 the layer inputs and weights are rounded to bfloat16 as
     the matrix-core kernels store them (products accumulate exactly here)."""
     q = bf16 if emulate_bf16 else (lambda a: a)
@@ -109,7 +116,7 @@ def forward(ps, x, emulate_bf16=False):
 
 def mixture(out):
     """Raw outputs -> (mu (n,8,3), kappa (n,8), weights (n,8), pieces for backward)."""
-    o = out.reshape(-1, N_LOBES, 5)
+    o = np.asarray(out)[:, :N_MIX].reshape(-1, N_LOBES, 5)
     raw = o[..., 0:3]
     nrm = np.maximum(np.linalg.norm(raw, axis=-1, keepdims=True), 1e-6)
     mu = raw / nrm
@@ -120,6 +127,12 @@ def mixture(out):
     w = np.exp(lg)
     w /= w.sum(axis=1, keepdims=True)
     return mu, kappa, w, nrm
+
+
+def selection(out):
+    """Probability of drawing from the lobes rather than the BSDF."""
+    c = np.asarray(out, np.float64)[:, N_MIX]
+    return SEL_MIN + (SEL_MAX - SEL_MIN) / (1.0 + np.exp(-c))
 
 
 def log_norm(kappa):
@@ -162,20 +175,26 @@ def frame(n):
 
 
 def loss_and_dout(out, rec):
-    """Batch loss sum(w * -log q) / sum(w) and its gradient w.r.t. the raw outputs."""
+    """Batch loss sum(w * -log(a q + (1 - a) p_bsdf)) / sum(w) and its
+    gradient w.r.t. the raw outputs."""
     rec = np.asarray(rec, np.float64)
     d = rec[:, 11:14]
     wt = rec[:, 14]
+    pb = rec[:, 15]
     W = max(wt.sum(), 1e-30)
-    o = out.reshape(-1, N_LOBES, 5)
+    o = np.asarray(out)[:, :N_MIX].reshape(-1, N_LOBES, 5)
     mu, kappa, w, nrm = mixture(out)
     dot = np.einsum("nkc,nc->nk", mu, d)
     lc = np.log(w) + log_norm(kappa) + kappa * (dot - 1.0)
     m = lc.max(axis=1, keepdims=True)
     lse = (m + np.log(np.exp(lc - m).sum(axis=1, keepdims=True)))[:, 0]
-    loss = float(np.sum(wt * -lse) / W)
-    g = np.exp(lc - lse[:, None])                      # responsibilities
-    s = (wt / W)[:, None]
+    q = np.exp(lse)
+    sig = 1.0 / (1.0 + np.exp(-np.asarray(out, np.float64)[:, N_MIX]))
+    a = SEL_MIN + (SEL_MAX - SEL_MIN) * sig
+    P = np.maximum(a * q + (1.0 - a) * pb, 1e-30)
+    loss = float(np.sum(wt * -np.log(P)) / W)
+    g = np.exp(lc - lse[:, None])                      # responsibilities within the lobes
+    s = (wt / W * a * q / P)[:, None]                  # share of the lobes in P
     dlogit = -s * (g - w)
     # d(-log q)/d kappa = -g (1/kappa - 2/(e^{2k}-1) + dot - 1); kappa = exp(lk)
     dlnorm = 1.0 / kappa - 2.0 / np.expm1(2.0 * kappa)
@@ -187,11 +206,15 @@ def loss_and_dout(out, rec):
     dmu = -(s * g * kappa)[..., None] * d[:, None, :]
     proj = dmu - mu * np.sum(dmu * mu, axis=-1, keepdims=True)
     draw = proj / nrm
-    dout = np.zeros_like(o)
-    dout[..., 0:3] = draw
-    dout[..., 3] = dlk
-    dout[..., 4] = dlogit
-    return loss, dout.reshape(-1, N_OUT)
+    dout = np.zeros((len(rec), N_OUT))
+    dl = dout[:, :N_MIX].reshape(-1, N_LOBES, 5)
+    dl[..., 0:3] = draw
+    dl[..., 3] = dlk
+    dl[..., 4] = dlogit
+    dout[:, :N_MIX] = dl.reshape(-1, N_MIX)
+    # selection: d(-log P)/da = -(q - p_bsdf) / P, da/dc = (SEL_MAX - SEL_MIN) sig (1 - sig)
+    dout[:, N_MIX] = -(wt / W) * (q - pb) / P * (SEL_MAX - SEL_MIN) * sig * (1.0 - sig)
+    return loss, dout
 
 
 def backward(ps, acts, dout, emulate_bf16=False):

@@ -471,7 +471,11 @@ void render_pixel(const Ctx& C, uint32_t px, uint32_t py, double res[4], Stats& 
             for (int i = 0; i < NL; ++i) path.L.v[i] = path.wl.pdf.v[i] != 0.0f ? path.L.v[i] / path.wl.pdf.v[i] : 0.0f;
             double xyz[3];
             to_xyz(C.cA, C.cB, C.cC, path.wl, path.L, xyz);
-            res[0] += xyz[0]; res[1] += xyz[1]; res[2] += xyz[2]; res[3] += 1.0;
+            // a non-finite sample is added as black (the product path does the same and counts it)
+            if (std::isfinite(xyz[0]) && std::isfinite(xyz[1]) && std::isfinite(xyz[2])) {
+                res[0] += xyz[0]; res[1] += xyz[1]; res[2] += xyz[2];
+            }
+            res[3] += 1.0;
         } else {
             res[3] += 1.0;
         }

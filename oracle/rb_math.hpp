@@ -529,8 +529,14 @@ inline V2 gaussian_filter_sample(V2 uv) {
     float m = sigma * std::sqrt(-2.0f * std::log(u));
     return {m * std::cos(2.0f * PI * uv.y), m * std::sin(2.0f * PI * uv.y)};
 }
-// src/device/mis.h:21-28
-inline float power_heuristic(float f, float g) { float f2 = f * f, g2 = g * g; return f2 / (f2 + g2); }
+// src/device/mis.h:14-21, with PBRT-v4's guard against an infinite f^2
+// (weight 1 instead of inf/inf = NaN; the caller then divides by the
+// infinite pdf and the sample contributes 0)
+inline float power_heuristic(float f, float g) {
+    float f2 = f * f, g2 = g * g;
+    if (std::isinf(f2)) return 1.0f;
+    return f2 / (f2 + g2);
+}
 
 // ------------------------------------------------------------ microfacet --
 // src/device/surface.h:125-136

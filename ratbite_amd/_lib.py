@@ -79,7 +79,7 @@ class Stats(ctypes.Structure):
         ("closestRays", c_uint64), ("shadowRays", c_uint64), ("pathVertices", c_uint64), ("samples", c_uint64),
         ("renderMs", c_double), ("traceMs", c_double), ("shadowMs", c_double), ("shadeMs", c_double),
         ("cdfMs", c_double), ("bvhMs", c_double), ("traceLaunches", c_uint32), ("bvhDepth", c_uint32),
-        ("guideMs", c_double),
+        ("guideMs", c_double), ("nonFiniteSamples", c_uint64),
     ]
 
 
@@ -96,7 +96,7 @@ class GuideConfig(ctypes.Structure):
     ]
 
 
-GUIDE_PARAMS = 20264
+GUIDE_PARAMS = 20329
 GUIDE_REC = 16
 
 

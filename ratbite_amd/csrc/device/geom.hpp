@@ -79,8 +79,13 @@ __device__ __forceinline__ V2 gaussian_offset(V2 u) {
     return {m * c, m * s};
 }
 
+// src/device/mis.h:14-21 (nf = ng = 1), with PBRT-v4's guard: an infinite
+// f^2 (a light sample seen exactly edge-on has an infinite solid-angle pdf)
+// gives weight 1 instead of inf/inf = NaN, so the sample contributes
+// spectrum * f * cos / inf = 0 rather than a NaN that would poison its pixel.
 __device__ __forceinline__ float power_heuristic(float f, float g) {
     const float f2 = f * f, g2 = g * g;
+    if (isinf(f2)) return 1.0f;
     return f2 / (f2 + g2);
 }
 

@@ -6,8 +6,9 @@
 // ComplexSurface lobe sum for a given direction pair (the expression its NEE
 // branch evaluates, pathtrace.cu:660-790).
 //
-// Estimator: with probability kGuideAlpha draw w from the mixture, otherwise
-// from the BSDF sampler; weight f(w)|cos| / (a p_guide(w) + (1 - a) p_bsdf(w)).
+// Estimator: with probability a (the network's selection output, in
+// [0.1, 0.9]) draw w from the mixture, otherwise from the BSDF sampler;
+// weight f(w)|cos| / (a p_guide(w) + (1 - a) p_bsdf(w)).
 // Both densities are exact, so the estimator is unbiased for any network
 // output. The BSDF sampler picks its lobe with probabilities that depend on
 // wo only (Schlick Fresnel at wo instead of at the sampled microfacet), which
@@ -18,9 +19,9 @@
 namespace rbh {
 namespace dev {
 
-constexpr float kGuideAlpha = 0.5f;        // selection probability of the learned proposal
 constexpr float kGuideMinRough = 0.05f;    // smoother vertices keep the reference sampler
-constexpr int kMixFloats = 40;              // 8 lobes x (mean xyz, sharpness, weight), world space
+constexpr int kMixFloats = 41;              // 8 lobes x (mean xyz, sharpness, weight), world space,
+                                            // then the selection probability of the lobes
 constexpr uint32_t QD_GUIDE_DIR = 11;       // quasi-random dimension of the mixture sample
 
 // --------------------------------------------------------------- mixture
@@ -191,24 +192,26 @@ namespace dev {
 // Guided ComplexSurface vertex (wo on the front side of the shading frame):
 // NEE with the MIS weight of the combined density, then the one-sample MIS
 // direction. Returns the combined density of the chosen direction (0: the
-// path ends). `mix` is the vertex's mixture in world space.
+// path ends) and, in pbOut, the BSDF sampler's density of it. `mix` is the
+// vertex's mixture in world space followed by its selection probability.
 __device__ __forceinline__ float complex_guided(const ColorTables& ct, const Luts& lut, const Light& dl, const Qrng& q,
                                                 const Frame& fr, const Surface& s, PathIO& p, S4& Lnee,
-                                                const float* __restrict__ mix, V3& wiWorld) {
+                                                const float* __restrict__ mix, V3& wiWorld, float& pbOut) {
     const V3 wo = fr.to_local(-p.rd);
     const CxCtx c = cx_setup(lut, s, wo);
+    const float alpha = mix[kMixFloats - 1];
     if (!dl.occluded) {
         const V3 li = fr.to_local(dl.dir);
         S4 fNee, fSamp;
         float pb;
         cx_eval(c, ct, s, p.wl, wo, li, fNee, fSamp, pb);
-        const float pmix = kGuideAlpha * vmf_mix_pdf(mix, dl.dir) + (1.0f - kGuideAlpha) * pb;
+        const float pmix = alpha * vmf_mix_pdf(mix, dl.dir) + (1.0f - alpha) * pb;
         const float mis = power_heuristic(dl.pdf, pmix);
         Lnee += dl.spectrum * p.T * fabsf(li.z) * mis * fNee / dl.pdf;
     }
     V3 wi;
     bool ok;
-    if (q.s1(QD_GUIDE) < kGuideAlpha) {
+    if (q.s1(QD_GUIDE) < alpha) {
         wiWorld = vmf_mix_sample(mix, q.s3(QD_GUIDE_DIR));
         wi = fr.to_local(wiWorld);
         ok = true;
@@ -220,7 +223,8 @@ __device__ __forceinline__ float complex_guided(const ColorTables& ct, const Lut
     S4 fNee, fSamp;
     float pb;
     cx_eval(c, ct, s, p.wl, wo, wi, fNee, fSamp, pb);
-    const float pmix = kGuideAlpha * vmf_mix_pdf(mix, wiWorld) + (1.0f - kGuideAlpha) * pb;
+    const float pmix = alpha * vmf_mix_pdf(mix, wiWorld) + (1.0f - alpha) * pb;
+    pbOut = pb;
     if (!(pmix > 0.0f) || isinf(pmix)) return 0.0f;
     p.T *= fSamp * (fabsf(wi.z) / pmix);
     p.bxdfPdf = pmix;

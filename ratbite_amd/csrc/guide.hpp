@@ -1,10 +1,12 @@
 // Path-guiding network: sizes, parameter layout and device state.
 //
-// An MLP 80 -> 64 -> 64 -> 64 -> 64 -> 40 (ReLU on the hidden layers) maps a
+// An MLP 80 -> 64 -> 64 -> 64 -> 64 -> 41 (ReLU on the hidden layers) maps a
 // shading point (12-frequency sin/cos encoding of the scene-normalised
 // position, shading normal, outgoing direction, roughness, normalised
 // wavelength) to 8 von Mises-Fisher lobes (mean direction, log sharpness,
-// mixture logit). It has no counterpart in the reference; it extends the
+// mixture logit) and a selection logit c: a renderer draws from the lobes
+// with probability kSelMin + (kSelMax - kSelMin) sigmoid(c), otherwise from
+// the BSDF. It has no counterpart in the reference; it extends the
 // BSDF sampling of src/kernels/pathtrace.cu:548-905 through one-sample MIS.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -13,8 +15,9 @@
 namespace rbh {
 namespace guide {
 
-constexpr int kIn = 80, kWidth = 64, kOut = 40, kLobes = 8, kRec = 16, kLayers = 5, kFreq = 12;
+constexpr int kIn = 80, kWidth = 64, kLobes = 8, kMix = 5 * kLobes, kOut = kMix + 1, kRec = 16, kLayers = 5, kFreq = 12;
 constexpr float kLogKappaMin = -3.0f, kLogKappaMax = 7.0f;
+constexpr float kSelMin = 0.1f, kSelMax = 0.9f;   // range of the learned selection probability
 
 // per layer: true fan-in / fan-out and the MFMA-padded sizes
 __host__ __device__ constexpr int fan_in(int l) { return l == 0 ? kIn : kWidth; }
@@ -27,8 +30,8 @@ __host__ __device__ constexpr int w_off(int l) {
     return l == 0 ? 0 : w_off(l - 1) + fan_in(l - 1) * fan_out(l - 1) + fan_out(l - 1);
 }
 __host__ __device__ constexpr int b_off(int l) { return w_off(l) + fan_in(l) * fan_out(l); }
-constexpr int kParams = b_off(kLayers - 1) + kOut;   // 20264
-static_assert(kParams == 20264, "parameter count");
+constexpr int kParams = b_off(kLayers - 1) + kOut;   // 20329
+static_assert(kParams == 20329, "parameter count");
 
 // BF16 device copies: forward [n_pad][k_pad] and transposed [k_pad][n_pad]
 __host__ __device__ constexpr int wf_off(int l) { return l == 0 ? 0 : wf_off(l - 1) + n_pad(l - 1) * k_pad(l - 1); }

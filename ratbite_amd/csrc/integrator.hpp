@@ -14,7 +14,7 @@ enum : uint32_t {
     CNT_TRACE_HEAD = 5, CNT_SHADOW_HEAD = 6,   // work-fetch positions of the trace and shadow queues
     CNT_COUNT = 8
 };
-enum : uint32_t { STAT_CLOSEST = 0, STAT_SHADOW = 1, STAT_VERTICES = 2, STAT_COUNT = 4 };
+enum : uint32_t { STAT_CLOSEST = 0, STAT_SHADOW = 1, STAT_VERTICES = 2, STAT_NONFINITE = 3, STAT_COUNT = 4 };
 
 // Structure-of-arrays path state, one entry per path slot of the pass.
 struct PathBuffers {
@@ -108,7 +108,7 @@ struct Frame1 {
     const uint8_t* gwq;       // FP8 E4M3 weights and per-layer scales (nullptr: BF16 inference)
     const float* gws;
     float* gfeat;             // per hit-queue entry: 16-float input record
-    float* gmix;              // per hit-queue entry: 40-float mixture (world space)
+    float* gmix;              // per hit-queue entry: 41-float mixture (world space) + selection
     float gmin[3], ginv[3];   // position normalisation
     uint32_t collect;         // record training data
     uint32_t recThreshold;    // keep a vertex when its hash is below this
@@ -119,8 +119,9 @@ struct Frame1 {
 };
 
 // pending-record layout (floats): 0..10 input features, 11..13 sampled
-// direction, 14 1/pdf, 15 previous record of the path (uint bits),
-// 16..19 throughput after the vertex, 20..23 radiance before the next vertex
+// direction, 14 BSDF-sampler density of the direction, 15 previous record of
+// the path (uint bits), 16..19 throughput before the vertex, 20..23 radiance
+// before the next vertex
 constexpr int kPendingRec = 24;
 constexpr uint32_t kRecPending = 0x80000000u;   // radiance snapshot not taken yet
 constexpr uint32_t kNoRec = 0xFFFFFFFFu;

@@ -50,11 +50,13 @@ __device__ __forceinline__ LobeView lobe_at(const float* o, int j) {
     return v;
 }
 
-// Loss of one record; writes d(loss)/d(outputs) * weight / weightSum as bf16
-// into dst[0..63] (zeros past the 40 outputs). The lobes are re-derived from
-// the outputs in each pass to keep few values live.
+// Loss of one record, weight * -log(a q(d) + (1 - a) p_bsdf(d)) with q the
+// lobe mixture and a the selection probability; writes
+// d(loss)/d(outputs) * weight / weightSum as bf16 into dst[0..63] (zeros past
+// the 41 outputs). The lobes are re-derived from the outputs in each pass to
+// keep few values live.
 __device__ float record_grad(const float* __restrict__ o, const float* __restrict__ rec, float invWsum, bf16* dst) {
-    const float dx = rec[11], dy = rec[12], dz = rec[13], wt = rec[14];
+    const float dx = rec[11], dy = rec[12], dz = rec[13], wt = rec[14], pb = rec[15];
     float lmax = -3.4e38f;
     for (int j = 0; j < kLobes; ++j) lmax = fmaxf(lmax, o[5 * j + 4]);
     float wsum = 0.0f;
@@ -72,8 +74,13 @@ __device__ float record_grad(const float* __restrict__ o, const float* __restric
         const float lc = (o[5 * j + 4] - lw0) + vmf_log_norm(v.kappa) + v.kappa * (v.mx * dx + v.my * dy + v.mz * dz - 1.0f);
         se += expf(lc - cmax);
     }
-    const float lse = cmax + logf(se);
-    const float s = wt * invWsum;
+    const float lse = cmax + logf(se);              // log q
+    const float q = expf(lse);
+    const float sig = 1.0f / (1.0f + expf(-o[kMix]));
+    const float a = kSelMin + (kSelMax - kSelMin) * sig;
+    const float P = fmaxf(a * q + (1.0f - a) * pb, 1e-30f);
+    const float s0 = wt * invWsum;
+    const float s = s0 * (a * q / P);               // the lobes' share of the combined density
     for (int j = 0; j < kLobes; ++j) {
         const LobeView v = lobe_at(o, j);
         const float dot = v.mx * dx + v.my * dy + v.mz * dz;
@@ -92,8 +99,10 @@ __device__ float record_grad(const float* __restrict__ o, const float* __restric
         dst[5 * j + 3] = (bf16)(inside ? dk * v.kappa : 0.0f);
         dst[5 * j + 4] = (bf16)(-s * (gj - w));
     }
+    // selection: d(-log P)/da = -(q - p_bsdf) / P, da/dc = (kSelMax - kSelMin) sig (1 - sig)
+    dst[kMix] = (bf16)(-s0 * (q - pb) / P * (kSelMax - kSelMin) * sig * (1.0f - sig));
     for (int c2 = kOut; c2 < 64; ++c2) dst[c2] = (bf16)0.0f;
-    return wt * -lse;
+    return wt * -logf(P);
 }
 
 // ------------------------------------------------------------- backward

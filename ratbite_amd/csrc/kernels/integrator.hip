@@ -340,7 +340,8 @@ __device__ __forceinline__ void guide_input(const Frame1& f, const SceneView& S,
     for (int k = 11; k < 16; ++k) out[k] = 0.0f;
 }
 
-// Raw outputs -> mixture (normalised means, clamped sharpness, softmax weights)
+// Raw outputs -> mixture (normalised means, clamped sharpness, softmax
+// weights) and selection probability
 __device__ __forceinline__ void guide_mixture(const float* __restrict__ o, float* __restrict__ mix) {
     float lmax = -3.4e38f;
     for (int j = 0; j < 8; ++j) lmax = fmaxf(lmax, o[5 * j + 4]);
@@ -355,6 +356,7 @@ __device__ __forceinline__ void guide_mixture(const float* __restrict__ o, float
         mix[5 * j + 3] = expf(fminf(fmaxf(q[3], guide::kLogKappaMin), guide::kLogKappaMax));
         mix[5 * j + 4] = expf(q[4] - lmax) / ws;
     }
+    mix[guide::kMix] = guide::kSelMin + (guide::kSelMax - guide::kSelMin) / (1.0f + expf(-o[guide::kMix]));
 }
 
 // Network inference for every queued hit: input records into LDS, the BF16
@@ -496,7 +498,8 @@ __global__ __launch_bounds__(kShadeBlock, WAVES) void k_shade(Frame1 f, uint32_t
         bool alive = false, shadow = false;
         bool recordSlot = false;      // guided + collecting: record this vertex's direction
         V3 recDir{0.0f, 0.0f, 0.0f};
-        float recInvPdf = 0.0f;
+        float recPb = 0.0f;           // BSDF sampler's density of the recorded direction
+        S4 recT = s4(0.0f);           // throughput before the recorded vertex
         uint32_t pid = 0;
         if (i < count) {
             pid = qin[i];
@@ -807,13 +810,17 @@ __global__ __launch_bounds__(kShadeBlock, WAVES) void k_shade(Frame1 f, uint32_t
                                            dot(local.n, -P.rd) > 0.0f;
                     if (guideHere) {
                         V3 wiw;
-                        const float pmix = complex_guided(ct, lut, dl, q, local, s, P, Lnee, f.gmix + (size_t)i * kMixFloats, wiw);
+                        float pb = 0.0f;
+                        const S4 Tb = P.T;
+                        const float pmix = complex_guided(ct, lut, dl, q, local, s, P, Lnee, f.gmix + (size_t)i * kMixFloats,
+                                                          wiw, pb);
                         if (pmix == 0.0f) {
                             P.flags |= PF_FINISHED;
                         } else if (f.collect && pixel_hash(pid, depth * 0x9E3779B9u + S.sample) < f.recThreshold) {
                             recordSlot = true;
                             recDir = wiw;
-                            recInvPdf = 1.0f / pmix;
+                            recPb = pb;
+                            recT = Tb;
                         }
                     } else if (m.bxdf == 0) {
                         conductor_bxdf(S.spectra + (size_t)m.iorSpectrum * kDense, S.spectra + (size_t)m.acSpectrum * kDense,
@@ -887,10 +894,9 @@ __global__ __launch_bounds__(kShadeBlock, WAVES) void k_shade(Frame1 f, uint32_t
                 d[11] = recDir.x;
                 d[12] = recDir.y;
                 d[13] = recDir.z;
-                d[14] = recInvPdf;
+                d[14] = recPb;
                 d[15] = __uint_as_float(f.p.prec[pid]);
-                const float4 Tn = f.p.T[pid];
-                d[16] = Tn.x; d[17] = Tn.y; d[18] = Tn.z; d[19] = Tn.w;
+                d[16] = recT.v[0]; d[17] = recT.v[1]; d[18] = recT.v[2]; d[19] = recT.v[3];
                 f.p.prec[pid] = r | kRecPending;
             }
         }
@@ -898,10 +904,11 @@ __global__ __launch_bounds__(kShadeBlock, WAVES) void k_shade(Frame1 f, uint32_t
     wave_count(f.stats + STAT_VERTICES, nvert);
 }
 
-// Turns the pending records of a finished path into training records:
-// incident radiance along the recorded direction = radiance gathered after
-// the vertex / throughput after the vertex, averaged over the wavelengths,
-// times 1/pdf of the direction.
+// Turns the pending records of a finished path into training records. The
+// weight is the sampled contribution over its density, f |cos| L_i / pdf =
+// radiance gathered after the vertex / throughput before it, averaged over
+// the wavelengths; the BSDF sampler's density of the direction goes along for
+// the one-sample MIS density the loss fits.
 __device__ __forceinline__ void finish_records(const Frame1& f, uint32_t slot, float4 L) {
     uint32_t r = f.p.prec[slot];
     for (int guard = 0; r != kNoRec && guard < 1024; ++guard) {
@@ -915,11 +922,11 @@ __device__ __forceinline__ void finish_records(const Frame1& f, uint32_t slot, f
             const float dl[4] = {L.x - d[20], L.y - d[21], L.z - d[22], L.w - d[23]};
             float li = 0.0f;
             for (int k = 0; k < 4; ++k) li += d[16 + k] > 0.0f ? dl[k] / d[16 + k] : 0.0f;
-            w = fmaxf(0.0f, 0.25f * li) * d[14];
+            w = fmaxf(0.0f, 0.25f * li);
             if (!(w < 3.0e38f)) w = 0.0f;
         }
         t[14] = w;
-        t[15] = 0.0f;
+        t[15] = d[14];
         r = __float_as_uint(d[15]);
     }
 }
@@ -934,6 +941,7 @@ __global__ __launch_bounds__(256) void k_accumulate(Frame1 f) {
     if (i >= S.P) return;
     double4 a = S.first ? make_double4(0.0, 0.0, 0.0, 0.0) : S.accum[i];
     const ColorTables ct{S.colors};
+    uint32_t bad = 0;
     for (uint32_t k = 0; k < S.S; ++k) {
         const uint32_t j = S.slot0 + k * S.P + i;
         if (f.collect) finish_records(f, j, f.p.L[j]);
@@ -947,13 +955,18 @@ __global__ __launch_bounds__(256) void k_accumulate(Frame1 f) {
             for (int c = 0; c < kNL; ++c) L.v[c] = wl.pdf.v[c] != 0.0f ? L.v[c] / wl.pdf.v[c] : 0.0f;
             double xyz[3];
             ct.xyz(wl, L, xyz);
-            a.x += xyz[0];
-            a.y += xyz[1];
-            a.z += xyz[2];
+            if (isfinite(xyz[0]) && isfinite(xyz[1]) && isfinite(xyz[2])) {
+                a.x += xyz[0];
+                a.y += xyz[1];
+                a.z += xyz[2];
+            } else {
+                ++bad;   // counted and added as a black sample instead of poisoning the pixel
+            }
         }
         a.w += 1.0;
     }
     S.accum[i] = a;
+    if (bad) atomicAdd(f.stats + STAT_NONFINITE, (unsigned long long)bad);
 }
 
 }  // namespace

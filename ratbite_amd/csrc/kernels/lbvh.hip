@@ -8,7 +8,8 @@
 //      splits at the highest differing bit
 //   5. refit level by level from the deepest internal nodes up (one launch
 //      per depth, so kernel boundaries order the box writes; no cross-XCD
-//      fences or arrival counters)
+//      fences or arrival counters); node depths by pointer jumping, the
+//      depth order by a radix sort of (depth, node) pairs
 //   6. emission of 64-byte two-child nodes (both child boxes in the parent)
 //      and of triangles in leaf order
 //   7. collapse into 128-byte four-wide nodes, top down one level per launch:
@@ -164,35 +165,49 @@ __global__ __launch_bounds__(kBuildBlock) void k_leaves(const float* __restrict_
 constexpr int kLevels = 65;   // internal-node depths 0..63 (64 collects anything deeper)
 
 // Depth of every internal node (ancestors up to the root) and a histogram.
-__global__ __launch_bounds__(kBuildBlock) void k_node_depth(int n, const int* __restrict__ parent,
-                                                            int* __restrict__ depth, int* __restrict__ count) {
+// Depth of every internal node by pointer jumping over the parent links:
+// (ancestor, distance) pairs, each round adds the ancestor's distance and
+// jumps to the ancestor's ancestor, so ceil(log2(64)) + 1 = 7 rounds reach the
+// root from depth 64 (deeper trees are rejected after the build).
+constexpr int kJumpRounds = 7;
+__global__ __launch_bounds__(kBuildBlock) void k_jump_init(int nInt, const int* __restrict__ parent,
+                                                           int2* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n - 1) return;
-    int d = 0;
-    for (int p = parent[i]; p >= 0 && d < kLevels - 1; p = parent[p]) ++d;
-    depth[i] = d;
-    atomicAdd(&count[d], 1);
+    if (i >= nInt) return;
+    const int p = parent[i];
+    out[i] = make_int2(p, p >= 0 ? 1 : 0);
 }
-
-__global__ void k_level_starts(const int* __restrict__ count, int* __restrict__ start, int* __restrict__ cursor) {
-    if (threadIdx.x != 0) return;
-    int acc = 0;
-    for (int d = 0; d < kLevels; ++d) {
-        start[d] = acc;
-        cursor[d] = acc;
-        acc += count[d];
+__global__ __launch_bounds__(kBuildBlock) void k_jump(int nInt, const int2* __restrict__ in, int2* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nInt) return;
+    const int2 a = in[i];
+    if (a.x < 0) {
+        out[i] = a;
+        return;
     }
-    start[kLevels] = acc;
+    const int2 b = in[a.x];
+    out[i] = make_int2(b.x, a.y + b.y);
 }
-
-__global__ __launch_bounds__(kBuildBlock) void k_node_scatter(int n, const int* __restrict__ depth,
-                                                              int* __restrict__ cursor, int* __restrict__ order) {
+// sort keys: the depth (capped at kLevels - 1) and the node index
+__global__ __launch_bounds__(kBuildBlock) void k_depth_keys(int nInt, const int2* __restrict__ jumped,
+                                                            unsigned int* __restrict__ key, int* __restrict__ id) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n - 1) return;
-    order[atomicAdd(&cursor[depth[i]], 1)] = i;
+    if (i >= nInt) return;
+    key[i] = (unsigned int)min(jumped[i].y, kLevels - 1);
+    id[i] = i;
 }
-
-// Boxes of the internal nodes at one depth from their (deeper) children.
+// start[d] = first position of depth d in the depth-sorted order (d <= kLevels)
+__global__ void k_level_bounds(const unsigned int* __restrict__ sortedKey, int nInt, int* __restrict__ start) {
+    const int d = threadIdx.x;
+    if (d > kLevels) return;
+    int lo = 0, hi = nInt;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((int)sortedKey[mid] < d) lo = mid + 1;
+        else hi = mid;
+    }
+    start[d] = lo;
+}
 __global__ __launch_bounds__(kBuildBlock) void k_refit_level(int level, const int* __restrict__ start,
                                                              const int* __restrict__ order, const int2* __restrict__ child,
                                                              float* __restrict__ ibox, const float* __restrict__ lbox) {
@@ -307,10 +322,6 @@ __global__ void k_reset_count(unsigned int* __restrict__ c) {
     if (threadIdx.x == 0) *c = 0u;
 }
 
-__global__ void k_zero_u32(unsigned int* __restrict__ p, size_t n) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0u;
-}
 
 inline size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -337,9 +348,12 @@ int lbvh_build(rbh_ctx c) {
     const size_t oListA = off; off = align_up(off + nInt * 8);
     const size_t oListB = off; off = align_up(off + nInt * 8);
     const size_t oCounts = off; off = align_up(off + 16);
-    size_t sortBytes = 0;
+    size_t sortBytes = 0, levelSortBytes = 0;
     RBH_HIP(c, hipcub::DeviceRadixSort::SortKeys(nullptr, sortBytes, (unsigned long long*)nullptr,
                                                  (unsigned long long*)nullptr, n, 0, 64, c->stream));
+    RBH_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, levelSortBytes, (unsigned int*)nullptr, (unsigned int*)nullptr,
+                                                  (int*)nullptr, (int*)nullptr, (int)nInt, 0, 7, c->stream));
+    sortBytes = std::max(sortBytes, levelSortBytes);
     const size_t oSort = off; off = align_up(off + sortBytes);
     RBH_ALLOC(c, acc.scratch, off);
     RBH_ALLOC(c, acc.nodes, nInt * sizeof(BvhNode));
@@ -355,9 +369,7 @@ int lbvh_build(rbh_ctx c) {
     float* lbox = reinterpret_cast<float*>(s + oLbox);
     int* ndepth = reinterpret_cast<int*>(s + oDepth);
     int* order = reinterpret_cast<int*>(s + oOrder);
-    int* lcount = reinterpret_cast<int*>(s + oLevels);
-    int* lstart = lcount + (kLevels + 1);
-    int* lcursor = lstart + (kLevels + 1);
+    int* lstart = reinterpret_cast<int*>(s + oLevels);
 
     const float* v = g.verts.as<float>();
     const uint32_t* idx = g.indices.as<uint32_t>();
@@ -368,16 +380,26 @@ int lbvh_build(rbh_ctx c) {
     hipLaunchKernelGGL(k_morton, dim3(nb), dim3(kBuildBlock), 0, c->stream, v, idx, (uint32_t)n, bounds, keys);
     RBH_HIP(c, hipcub::DeviceRadixSort::SortKeys(s + oSort, sortBytes, keys, keys2, n, 0, 64, c->stream));
     const int nbi = (int)((nInt + kBuildBlock - 1) / kBuildBlock);
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kBuildBlock), 0, c->stream, reinterpret_cast<unsigned int*>(lcount),
-                       (size_t)(kLevels + 1));
     if (n > 1) hipLaunchKernelGGL(k_karras, dim3(nbi), dim3(kBuildBlock), 0, c->stream, keys2, n, child, parent);
     hipLaunchKernelGGL(k_leaves, dim3(nb), dim3(kBuildBlock), 0, c->stream, v, idx, keys2, n, lbox, acc.tris.as<TriRecord>());
     if (n > 1) {
         int cus = 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0) cus = 256;
-        hipLaunchKernelGGL(k_node_depth, dim3(nbi), dim3(kBuildBlock), 0, c->stream, n, parent, ndepth, lcount);
-        hipLaunchKernelGGL(k_level_starts, dim3(1), dim3(64), 0, c->stream, lcount, lstart, lcursor);
-        hipLaunchKernelGGL(k_node_scatter, dim3(nbi), dim3(kBuildBlock), 0, c->stream, n, ndepth, lcursor, order);
+        // internal nodes in depth order (list buffers are free until the collapse)
+        int2* ja = reinterpret_cast<int2*>(s + oListA);
+        int2* jb = reinterpret_cast<int2*>(s + oListB);
+        hipLaunchKernelGGL(k_jump_init, dim3(nbi), dim3(kBuildBlock), 0, c->stream, (int)nInt, parent, ja);
+        for (int r = 0; r < kJumpRounds; ++r) {
+            hipLaunchKernelGGL(k_jump, dim3(nbi), dim3(kBuildBlock), 0, c->stream, (int)nInt, ja, jb);
+            std::swap(ja, jb);
+        }
+        unsigned int* dkey = reinterpret_cast<unsigned int*>(ndepth);
+        int* ids = reinterpret_cast<int*>(jb);                       // free after the last round
+        unsigned int* dsorted = reinterpret_cast<unsigned int*>(ids + nInt);
+        hipLaunchKernelGGL(k_depth_keys, dim3(nbi), dim3(kBuildBlock), 0, c->stream, (int)nInt, ja, dkey, ids);
+        RBH_HIP(c, hipcub::DeviceRadixSort::SortPairs(s + oSort, sortBytes, dkey, dsorted, ids, order, (int)nInt, 0, 7,
+                                                      c->stream));
+        hipLaunchKernelGGL(k_level_bounds, dim3(1), dim3(128), 0, c->stream, dsorted, (int)nInt, lstart);
         const int lg = std::min(nbi, cus * 8);
         for (int d = kLevels - 1; d >= 0; --d)
             hipLaunchKernelGGL(k_refit_level, dim3(lg), dim3(kBuildBlock), 0, c->stream, d, lstart, order, child, ibox, lbox);

@@ -291,6 +291,7 @@ int finish_stats(rbh_ctx c, const Frame1& f, KernelTimer* kt, uint64_t samples, 
     c->stats.closestRays = st[STAT_CLOSEST];
     c->stats.shadowRays = st[STAT_SHADOW];
     c->stats.pathVertices = st[STAT_VERTICES];
+    c->stats.nonFiniteSamples = st[STAT_NONFINITE];
     c->stats.samples = samples;
     c->stats.traceLaunches = launches;
     c->stats.traceMs = c->stats.shadeMs = c->stats.shadowMs = c->stats.guideMs = 0.0;
@@ -343,7 +344,7 @@ int rbh_render(rbh_ctx c, const rbh_launch_params* lp) {
     f.noDrop = (guided || (lp->flags & RBH_RENDER_NO_DROP)) ? 1u : 0u;
     if (guided) {
         RBH_ALLOC(c, rs.gfeat, NP * 16 * sizeof(float));
-        RBH_ALLOC(c, rs.gmix, NP * 40 * sizeof(float));
+        RBH_ALLOC(c, rs.gmix, NP * (guide::kOut) * sizeof(float));   // lobes + selection per hit
         f.guided = 1u;
         f.gwf = static_cast<const __bf16*>(gst->wf);
         f.gbias = static_cast<const float*>(gst->bias);

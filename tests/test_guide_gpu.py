@@ -34,6 +34,7 @@ def _records(n, rng, sharp=False):
     d = rng.normal(size=(n, 3)); d /= np.linalg.norm(d, axis=1, keepdims=True)
     rec[:, 11:14] = d
     rec[:, 14] = rng.exponential(size=n)
+    rec[:, 15] = rng.uniform(0.0, 0.6, n) * (rng.random(n) < 0.8)   # BSDF-sampler density
     return rec
 
 
@@ -65,7 +66,7 @@ def test_forward_matches_oracle(ctx, n):
     _set(ctx, p)
     rng = np.random.default_rng(n)
     rec = _records(n, rng)
-    out = np.zeros((n, 40), np.float32)
+    out = np.zeros((n, G.N_OUT), np.float32)
     ctx.check(L.lib().rbh_guide_infer(ctx.h, L.ptr(rec), n, L.ptr(out)))
     ps = G.unflatten(p.astype(np.float64))
     ref, _ = G.forward(ps, G.encode(rec), emulate_bf16=True)
@@ -113,8 +114,9 @@ def test_gradient_matches_oracle(ctx):
 
 
 def test_adam_training_reaches_target(ctx):
-    # directions from one vMF lobe (kappa 20) around +z, uniform weights: the
-    # optimum loss is the target's negative entropy -log(20/2pi) + 1 = -0.158
+    # directions from one vMF lobe (kappa 20) around +z, uniform weights, no
+    # BSDF density: the optimum loss is the target's negative entropy
+    # -log(20/2pi) + 1 = -0.158 plus -log 0.9 (the largest selection) = -0.053
     _configure(ctx, seed=7, lr=3e-3)
     rng = np.random.default_rng(8)
     n = 1 << 16
@@ -125,12 +127,19 @@ def test_adam_training_reaches_target(ctx):
     sz = np.sqrt(1 - cz * cz)
     rec[:, 11:14] = np.stack([sz * np.cos(2 * np.pi * u[:, 1]), sz * np.sin(2 * np.pi * u[:, 1]), cz], 1)
     rec[:, 14] = 1.0
+    rec[:, 15] = 0.0
     first = ctypes.c_float()
     ctx.check(L.lib().rbh_guide_train(ctx.h, L.ptr(rec), n, 4096, 1, ctypes.byref(first)))
     last = ctypes.c_float()
     ctx.check(L.lib().rbh_guide_train(ctx.h, L.ptr(rec), n, 4096, 300, ctypes.byref(last)))
-    print("loss first %.4f after 300 Adam steps %.4f" % (first.value, last.value), flush=True)
-    assert first.value > 1.0 and last.value < 0.0
+    out = np.zeros((256, G.N_OUT), np.float32)
+    ctx.check(L.lib().rbh_guide_infer(ctx.h, L.ptr(rec[:256]), 256, L.ptr(out)))
+    sel = G.selection(out.astype(np.float64))
+    print("loss first %.4f after 300 Adam steps %.4f, selection %.3f" % (first.value, last.value, sel.mean()),
+          flush=True)
+    assert first.value > 1.0 and last.value < 0.1
+    # without a BSDF density every record pushes the selection to its maximum
+    assert sel.min() > 0.8
 
 
 def test_fp8_inference_close_to_bf16(ctx):
@@ -140,8 +149,8 @@ def test_fp8_inference_close_to_bf16(ctx):
     rng = np.random.default_rng(10)
     n = 4096
     rec = _records(n, rng)
-    a = np.zeros((n, 40), np.float32)
-    b = np.zeros((n, 40), np.float32)
+    a = np.zeros((n, G.N_OUT), np.float32)
+    b = np.zeros((n, G.N_OUT), np.float32)
     ctx.check(L.lib().rbh_guide_infer(ctx.h, L.ptr(rec), n, L.ptr(a)))
     ctx.check(L.lib().rbh_guide_infer_fp8(ctx.h, L.ptr(rec), n, L.ptr(b)))
     ps = G.unflatten(p.astype(np.float64))

@@ -1,7 +1,8 @@
 """Pins of the FP64 guiding-network oracle (oracle/guide_oracle.py): its
 analytic gradient against central finite differences, the vMF mixture's
-normalisation by quadrature, its sampler against its density, and that Adam on
-the loss moves the mixture towards a known target distribution."""
+normalisation by quadrature, its sampler against its density, that the loss
+over the selection probability is smallest at the true mixing fraction, and
+that Adam on the loss moves the mixture towards a known target distribution."""
 import numpy as np
 import pytest
 
@@ -19,11 +20,13 @@ def _records(n, rng, target=None):
     d = rng.normal(size=(n, 3)) if target is None else target(n, rng)
     rec[:, 11:14] = d / np.linalg.norm(d, axis=1, keepdims=True)
     rec[:, 14] = rng.exponential(size=n)
+    # BSDF-sampler density of the direction: zero for some (outside its support)
+    rec[:, 15] = rng.uniform(0.0, 0.6, n) * (rng.random(n) < 0.8)
     return rec
 
 
 def test_parameter_count():
-    assert G.n_params() == 80 * 64 + 3 * 64 * 64 + 64 * 40 + 4 * 64 + 40 == 20264
+    assert G.n_params() == 80 * 64 + 3 * 64 * 64 + 64 * 41 + 4 * 64 + 41 == 20329
 
 
 def test_encoding_matches_direct_sin_cos():
@@ -41,13 +44,13 @@ def test_gradient_matches_finite_differences():
     rng = np.random.default_rng(1)
     ps = G.init_params(seed=2)
     # larger output-layer weights so the lobes are sharp and the mixture uneven
-    ps[-1] = (ps[-1][0] * 3.0, rng.normal(size=40) * 0.5)
+    ps[-1] = (ps[-1][0] * 3.0, rng.normal(size=41) * 0.5)
     rec = _records(64, rng)
     loss, grads = G.loss_grad(ps, rec)
     flat = G.flatten(ps)
     gflat = G.flatten(grads)
     idx = rng.choice(len(flat), 120, replace=False)
-    idx = np.concatenate([idx, np.arange(len(flat) - 40, len(flat))])      # every output bias
+    idx = np.concatenate([idx, np.arange(len(flat) - 41, len(flat))])      # every output bias
     h = 1e-6
     for i in idx:
         p1 = flat.copy(); p1[i] += h
@@ -70,8 +73,8 @@ def _sphere_grid(nc, nphi):
 @pytest.mark.parametrize("seed", [3, 4])
 def test_mixture_pdf_normalised(seed):
     rng = np.random.default_rng(seed)
-    out = rng.normal(size=(1, 40)) * 2.0
-    out[0, 3::5] = rng.uniform(-3, 5, 8)       # log kappa up to ~150
+    out = rng.normal(size=(1, 41)) * 2.0
+    out[0, 3:40:5] = rng.uniform(-3, 5, 8)     # log kappa up to ~150
     d, w = _sphere_grid(4000, 720)
     q = G.pdf(np.repeat(out, len(d), 0), d)
     assert abs(q.sum() * w - 1.0) < 2e-3
@@ -79,8 +82,8 @@ def test_mixture_pdf_normalised(seed):
 
 def test_sampler_follows_pdf():
     rng = np.random.default_rng(5)
-    out = rng.normal(size=(1, 40))
-    out[0, 3::5] = rng.uniform(0, 3, 8)
+    out = rng.normal(size=(1, 41))
+    out[0, 3:40:5] = rng.uniform(0, 3, 8)
     n = 200000
     s = G.sample(np.repeat(out, n, 0), rng.random((n, 3)))
     np.testing.assert_allclose(np.linalg.norm(s, axis=1), 1.0, atol=1e-12)
@@ -90,6 +93,36 @@ def test_sampler_follows_pdf():
         mc = fn(s).mean()
         quad = np.sum(fn(d) * q) * w
         assert abs(mc - quad) < 5 * fn(s).std() / np.sqrt(n) + 1e-3
+
+
+@pytest.mark.parametrize("beta", [0.3, 0.7])
+def test_selection_loss_is_smallest_at_the_mixing_fraction(beta):
+    # directions drawn with probability beta from fixed lobes, otherwise from a
+    # uniform "BSDF" density: the expected loss over the selection probability
+    # is minimised at a = beta (its derivative, E[(q - p) / P], vanishes there)
+    rng = np.random.default_rng(int(beta * 10))
+    out = rng.normal(size=(1, 41))
+    out[0, 3:40:5] = rng.uniform(1, 3, 8)
+    n = 200000
+    pick = rng.random(n) < beta
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d[pick] = G.sample(np.repeat(out, pick.sum(), 0), rng.random((pick.sum(), 3)))
+    rec = np.zeros((n, G.REC))
+    rec[:, 11:14] = d
+    rec[:, 14] = 1.0
+    rec[:, 15] = 1.0 / (4.0 * np.pi)
+    res = {}
+    for a in (beta - 0.08, beta, beta + 0.08):
+        o = np.repeat(out, n, 0)
+        o[:, 40] = -np.log((G.SEL_MAX - G.SEL_MIN) / (a - G.SEL_MIN) - 1.0)
+        np.testing.assert_allclose(G.selection(o[:1]), a, rtol=1e-12)
+        loss, dout = G.loss_and_dout(o, rec)
+        res[a] = (loss, dout[:, 40].sum())
+    lo, mid, hi = (res[a] for a in sorted(res))
+    assert mid[0] < lo[0] and mid[0] < hi[0]
+    assert lo[1] < 0.0 < hi[1]            # d loss / d logit changes sign across beta
+    assert abs(mid[1]) < 0.1 * min(abs(lo[1]), abs(hi[1]))
 
 
 def test_training_moves_towards_target():
@@ -109,8 +142,10 @@ def test_training_moves_towards_target():
     for it in range(150):
         rec = _records(256, rng, target)
         rec[:, 14] = 1.0
+        rec[:, 15] = 0.0
         loss, grads = G.loss_grad(G.unflatten(flat), rec)
         first = loss if first is None else first
         flat = opt.step(flat, G.flatten(grads))
-    # the optimum is the target's negative entropy, -log(20 / 2pi) + 1 = -0.158
+    # with no BSDF density the optimum is the target's negative entropy,
+    # -log(20 / 2pi) + 1 = -0.158, plus -log of the largest selection 0.9
     assert first > 1.5 and loss < 0.3, (first, loss)
