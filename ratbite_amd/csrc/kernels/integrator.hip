@@ -71,6 +71,30 @@ __device__ __forceinline__ const SceneView& scene_of(const Frame1& f, uint32_t s
     return f.scenes[lo];
 }
 
+// Pixel of a scene-local slot index: bands of 8 rows, each band in blocks of
+// 8 columns (the last block of a band as wide as what is left), a block row
+// by row. A 64-lane wave of consecutive slots then covers an 8x8 tile
+// instead of a 64x1 strip, so its rays share BVH nodes and textures. A
+// bijection of [0, W*H) for any film size.
+__device__ __forceinline__ void slot_pixel(const SceneView& S, uint32_t pix, uint32_t& x, uint32_t& y) {
+    const uint32_t band = pix / (8u * S.W);
+    const uint32_t y0 = band * 8u;
+    const uint32_t h = min(8u, S.H - y0);
+    const uint32_t r = pix - band * 8u * S.W;     // < h * W
+    const uint32_t full = S.W / 8u;                // blocks 8 columns wide
+    const uint32_t xb = r / (8u * h);
+    if (xb < full) {
+        const uint32_t w = r - xb * 8u * h;
+        y = y0 + w / 8u;
+        x = xb * 8u + w % 8u;
+    } else {
+        const uint32_t wl = S.W - full * 8u;      // width of the band's last block (> 0 here)
+        const uint32_t w = r - full * 8u * h;
+        y = y0 + w / wl;
+        x = full * 8u + w % wl;
+    }
+}
+
 // ------------------------------------------------------------------ camera
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_camera(Frame1 f) {
@@ -79,7 +103,8 @@ __global__ __launch_bounds__(256) void k_camera(Frame1 f) {
     const SceneView& S = scene_of<BATCH>(f, i);
     const uint32_t local = i - S.slot0;
     const uint32_t k = local / S.P, pix = local - k * S.P;
-    const uint32_t x = pix % S.W, y = pix / S.W;
+    uint32_t x, y;
+    slot_pixel(S, pix, x, y);
     const Qrng q = qrng_at(S.sample + k, 0, pixel_hash(x, y));
     const Wl wl = sample_wavelengths(q.s1(QD_WAVELENGTH));
     const V2 fo = gaussian_offset(q.s2(QD_FILTER));
@@ -529,7 +554,8 @@ __global__ __launch_bounds__(kShadeBlock, WAVES) void k_shade(Frame1 f, uint32_t
             uint32_t depth = f.p.depth[pid];
             const uint32_t local = pid - S.slot0;
             const uint32_t sk = local / S.P, pix = local - sk * S.P;
-            const uint32_t px = pix % S.W, py = pix / S.W;
+            uint32_t px, py;
+            slot_pixel(S, pix, px, py);
             const Qrng q = qrng_at(S.sample + sk, depth, pixel_hash(px, py));
 
             // ---- interaction (pathtrace.cu:86-149, 906-953)
@@ -937,9 +963,12 @@ __global__ __launch_bounds__(256) void k_accumulate(Frame1 f) {
     // per pixel, the S samples of the pass are added one at a time in sample
     // order, so any split of a sample range into passes gives the same sums
     const SceneView& S = BATCH ? f.scenes[blockIdx.y] : f.sc;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;   // scene-local slot of sample 0
     if (i >= S.P) return;
-    double4 a = S.first ? make_double4(0.0, 0.0, 0.0, 0.0) : S.accum[i];
+    uint32_t px, py;
+    slot_pixel(S, i, px, py);
+    const uint32_t pixel = py * S.W + px;
+    double4 a = S.first ? make_double4(0.0, 0.0, 0.0, 0.0) : S.accum[pixel];
     const ColorTables ct{S.colors};
     uint32_t bad = 0;
     for (uint32_t k = 0; k < S.S; ++k) {
@@ -965,7 +994,7 @@ __global__ __launch_bounds__(256) void k_accumulate(Frame1 f) {
         }
         a.w += 1.0;
     }
-    S.accum[i] = a;
+    S.accum[pixel] = a;
     if (bad) atomicAdd(f.stats + STAT_NONFINITE, (unsigned long long)bad);
 }
 

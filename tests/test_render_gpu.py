@@ -48,6 +48,19 @@ def test_cornell_matches_oracle(ctx):
     assert abs(st["closestRays"] - ost["closestRays"]) / ost["closestRays"] < 0.02
 
 
+@pytest.mark.parametrize("size", [(37, 21), (5, 3), (8, 13)])
+def test_odd_film_sizes_match_oracle(ctx, size):
+    # consecutive path slots cover 8x8 pixel tiles, with partial tiles at the
+    # right and bottom edges: every pixel must still get its own samples
+    sc = scenes.cornell(width=size[0], height=size[1], spp=8)
+    g, o, st, ost = _compare(sc, ctx, 8)
+    close = np.abs(g - o) <= 1e-4 * np.maximum(np.abs(o), 1e-3)
+    frac = close.all(axis=2).mean()
+    print("cornell %dx%d: pixels within 1e-4: %.4f" % (size[0], size[1], frac), flush=True)
+    assert frac > 0.5
+    assert relmse(g, o) < 5e-2
+
+
 def test_spheres_matches_oracle(ctx):
     sc = scenes.spheres(n_spheres=12, level=3, n_lights=400, width=80, height=48, spp=8, env_size=(256, 128))
     g, o, st, ost = _compare(sc, ctx, 8)
