@@ -134,6 +134,7 @@ struct RenderState {
     int shadeBlocksPerCU = 64;
     int shadeWaves = 3;          // shade-kernel variant (1, 3, 4 waves per SIMD)
     int traceFetch = 0;          // 1: trace/shadow lanes fetch a new ray as soon as theirs completes
+    int traceWaves = 1;          // trace/shadow register budget (1, 7, 8 waves per SIMD)
     DevBuf sceneTable;           // batched passes: SceneView per scene and pass
     bool useGraph = false;       // replay each pass's launch sequence as a captured HIP graph
     hipGraphExec_t graphExec = nullptr;

@@ -85,6 +85,7 @@ struct LaunchShape {
     int traceGrid, shadeGrid;
     int shadeWaves;          // register budget of the shade kernel: 1 (free), 3 or 4 waves per SIMD
     int traceFetch;          // trace/shadow lanes fetch a new ray when theirs completes (1) or wave by wave (0)
+    int traceWaves;          // register budget of the wave-synchronous trace/shadow kernels: 1 (free), 7 or 8
 };
 
 struct Frame1 {

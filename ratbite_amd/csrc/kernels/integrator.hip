@@ -272,8 +272,10 @@ struct WaveList {
     }
 };
 
-template <bool BATCH, bool FETCH>
-__global__ __launch_bounds__(kTraceBlock) void k_trace(Frame1 f, uint32_t qsel) {
+// WAVES: minimum resident waves per SIMD the register allocation must allow
+// (1 = unconstrained), traded by the launch shape against spills
+template <bool BATCH, bool FETCH, int WAVES>
+__global__ __launch_bounds__(kTraceBlock, WAVES) void k_trace(Frame1 f, uint32_t qsel) {
     __shared__ int stack[kStackLds * kTraceBlock];
     __shared__ uint32_t staged[FETCH ? 2 * kTraceBlock * 2 : 1];   // per wave: 128 hits, 128 misses
     const TravStack<kStackLds> st = lane_stack(f, stack);
@@ -434,8 +436,8 @@ __device__ __forceinline__ void add_nee(const Frame1& f, uint32_t pid) {
     f.p.L[pid] = L;
 }
 
-template <bool BATCH, bool FETCH>
-__global__ __launch_bounds__(kTraceBlock) void k_shadow(Frame1 f) {
+template <bool BATCH, bool FETCH, int WAVES>
+__global__ __launch_bounds__(kTraceBlock, WAVES) void k_shadow(Frame1 f) {
     __shared__ int stack[kStackLds * kTraceBlock];
     const TravStack<kStackLds> st = lane_stack(f, stack);
     const uint32_t count = f.counters[CNT_SHADOW];
@@ -1004,6 +1006,24 @@ __global__ __launch_bounds__(256) void k_accumulate(Frame1 f) {
 // Spill entries per lane of the traversal stack beyond the LDS part.
 size_t trace_spill_words(size_t lanes) { return lanes * (size_t)(kMaxStack4 - kStackLds); }
 
+template <bool BATCH>
+static void launch_trace(rbh_ctx c, const Frame1& f, uint32_t qsel, const LaunchShape& ls) {
+    const dim3 g(ls.traceGrid), b(kTraceBlock);
+    if (ls.traceFetch) hipLaunchKernelGGL((k_trace<BATCH, true, 1>), g, b, 0, c->stream, f, qsel);
+    else if (ls.traceWaves >= 8) hipLaunchKernelGGL((k_trace<BATCH, false, 8>), g, b, 0, c->stream, f, qsel);
+    else if (ls.traceWaves == 7) hipLaunchKernelGGL((k_trace<BATCH, false, 7>), g, b, 0, c->stream, f, qsel);
+    else hipLaunchKernelGGL((k_trace<BATCH, false, 1>), g, b, 0, c->stream, f, qsel);
+}
+
+template <bool BATCH>
+static void launch_shadow(rbh_ctx c, const Frame1& f, const LaunchShape& ls) {
+    const dim3 g(ls.traceGrid), b(kTraceBlock);
+    if (ls.traceFetch) hipLaunchKernelGGL((k_shadow<BATCH, true, 1>), g, b, 0, c->stream, f);
+    else if (ls.traceWaves >= 8) hipLaunchKernelGGL((k_shadow<BATCH, false, 8>), g, b, 0, c->stream, f);
+    else if (ls.traceWaves == 7) hipLaunchKernelGGL((k_shadow<BATCH, false, 7>), g, b, 0, c->stream, f);
+    else hipLaunchKernelGGL((k_shadow<BATCH, false, 1>), g, b, 0, c->stream, f);
+}
+
 // One wavefront pass: camera rays for every slot, depthMax bounces of
 // trace / miss / (guide) / shade / shadow, then accumulation. f.sc (single
 // scene) or f.scenes (batched) must describe the pass's samples and slots.
@@ -1025,11 +1045,7 @@ static hipError_t launch_pass_t(rbh_ctx c, Frame1 f, uint32_t depthMax, const La
         const uint32_t qsel = b & 1u;
         f.bounce = b;
         if (b > 0) hipLaunchKernelGGL(k_reset_hits, dim3(1), dim3(64), 0, c->stream, f.counters);
-        RBH_TIMED(KernelTimer::TRACE,
-                  if (ls.traceFetch) hipLaunchKernelGGL((k_trace<BATCH, true>), dim3(ls.traceGrid), dim3(kTraceBlock), 0,
-                                                        c->stream, f, qsel);
-                  else hipLaunchKernelGGL((k_trace<BATCH, false>), dim3(ls.traceGrid), dim3(kTraceBlock), 0, c->stream,
-                                          f, qsel));
+        RBH_TIMED(KernelTimer::TRACE, launch_trace<BATCH>(c, f, qsel, ls));
         RBH_TIMED(KernelTimer::SHADE,
                   hipLaunchKernelGGL(k_miss<BATCH>, dim3(ls.shadeGrid), dim3(256), 0, c->stream, f));
         if (!BATCH && f.guided)
@@ -1048,11 +1064,7 @@ static hipError_t launch_pass_t(rbh_ctx c, Frame1 f, uint32_t depthMax, const La
         else
             RBH_TIMED(KernelTimer::SHADE, hipLaunchKernelGGL((k_shade<false, BATCH, 1>), dim3(ls.shadeGrid),
                                                              dim3(kShadeBlock), 0, c->stream, f, qsel));
-        RBH_TIMED(KernelTimer::SHADOW,
-                  if (ls.traceFetch) hipLaunchKernelGGL((k_shadow<BATCH, true>), dim3(ls.traceGrid), dim3(kTraceBlock), 0,
-                                                        c->stream, f);
-                  else hipLaunchKernelGGL((k_shadow<BATCH, false>), dim3(ls.traceGrid), dim3(kTraceBlock), 0, c->stream,
-                                          f));
+        RBH_TIMED(KernelTimer::SHADOW, launch_shadow<BATCH>(c, f, ls));
     }
 #undef RBH_TIMED
     hipLaunchKernelGGL(k_accumulate<BATCH>, accGrid, dim3(256), 0, c->stream, f);

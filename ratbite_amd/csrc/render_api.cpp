@@ -228,6 +228,7 @@ int prepare_pass_state(rbh_ctx c, size_t NP, Frame1& f, LaunchShape& ls) {
     ls.shadeGrid = (int)std::max<size_t>(1, std::min<size_t>((size_t)cus * rs.shadeBlocksPerCU, (NP + kShadeBlock - 1) / kShadeBlock));
     ls.shadeWaves = rs.shadeWaves;
     ls.traceFetch = rs.traceFetch;
+    ls.traceWaves = rs.traceWaves;
     const size_t lanes = (size_t)ls.traceGrid * kTraceBlock;
     RBH_ALLOC(c, rs.spill, trace_spill_words(lanes) * sizeof(int));
     f.spill = rs.spill.as<int>();
@@ -511,10 +512,12 @@ int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_
     double bestRate = -1.0;
     rbh_tuning b{};
     (void)freeB;
-    const int keepF = rs.traceFetch, keepW = rs.shadeWaves;
+    const int keepF = rs.traceFetch, keepW = rs.shadeWaves, keepTW = rs.traceWaves;
     rs.traceFetch = 0;
     rs.shadeWaves = 1;
+    rs.traceWaves = 1;
     b.traceFetch = 0;
+    b.traceWaves = 1;
     for (int tg : gridList) {
         for (int sg : gridList) {
             rs.slotBudget = keep;
@@ -526,6 +529,7 @@ int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_
                 rs.shadeBlocksPerCU = keepS;
                 rs.traceFetch = keepF;
                 rs.shadeWaves = keepW;
+                rs.traceWaves = keepTW;
                 return rc;
             }
             const double rate = (double)P * trialSpp / (c->stats.renderMs * 1e-3);
@@ -556,10 +560,29 @@ int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_
         }
     }
     if (bestRate >= 0.0) {
+        // trace/shadow register budgets on the best grid
+        for (int w : {7, 8}) {
+            rs.traceBlocksPerCU = b.traceBlocksPerCU;
+            rs.shadeBlocksPerCU = b.shadeBlocksPerCU;
+            rs.shadeWaves = b.shadeWaves ? b.shadeWaves : 1;
+            rs.traceWaves = w;
+            const int rc = rbh_render(c, &t);
+            rs.traceWaves = 1;
+            if (rc) return rc;
+            const double rate = (double)P * trialSpp / (c->stats.renderMs * 1e-3);
+            if (rate > bestRate) {
+                bestRate = rate;
+                b.samplesPerSecond = rate;
+                b.traceWaves = w;
+            }
+        }
+    }
+    if (bestRate >= 0.0) {
         // per-lane work fetch instead of wave-synchronous ray batches
         rs.traceBlocksPerCU = b.traceBlocksPerCU;
         rs.shadeBlocksPerCU = b.shadeBlocksPerCU;
         rs.shadeWaves = b.shadeWaves ? b.shadeWaves : 1;
+        rs.traceWaves = b.traceWaves;
         rs.traceFetch = 1;
         const int rc = rbh_render(c, &t);
         rs.traceFetch = 0;
@@ -574,6 +597,7 @@ int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_
     if (bestRate >= 0.0) {
         // the best shape once more as a replayed graph
         rs.traceFetch = b.traceFetch;
+        rs.traceWaves = b.traceWaves;
         rs.slotBudget = b.pathSlots;
         rs.traceBlocksPerCU = b.traceBlocksPerCU;
         rs.shadeBlocksPerCU = b.shadeBlocksPerCU;
@@ -595,6 +619,7 @@ int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_
         rs.shadeBlocksPerCU = keepS;
         rs.traceFetch = keepF;
         rs.shadeWaves = keepW;
+        rs.traceWaves = keepTW;
         return fail(c, RBH_ERR_NOMEM, "no launch shape fits the free device memory");
     }
     rs.slotBudget = b.pathSlots;
@@ -603,6 +628,7 @@ int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_
     rs.useGraph = b.useGraph != 0;
     rs.shadeWaves = b.shadeWaves ? b.shadeWaves : 1;
     rs.traceFetch = b.traceFetch;
+    rs.traceWaves = b.traceWaves;
     if (!b.shadeWaves) b.shadeWaves = 1;
     *best = b;
     return RBH_OK;
@@ -619,6 +645,7 @@ int rbh_set_tuning(rbh_ctx c, const rbh_tuning* t) {
     c->rs.useGraph = t->useGraph != 0;
     c->rs.shadeWaves = (t->shadeWaves == 3 || t->shadeWaves == 4) ? t->shadeWaves : 1;
     c->rs.traceFetch = t->traceFetch != 0;
+    c->rs.traceWaves = (t->traceWaves == 7 || t->traceWaves == 8) ? t->traceWaves : 1;
     return RBH_OK;
 }
 
@@ -631,6 +658,7 @@ int rbh_get_tuning(rbh_ctx c, rbh_tuning* t) {
     t->useGraph = c->rs.useGraph ? 1 : 0;
     t->shadeWaves = c->rs.shadeWaves;
     t->traceFetch = c->rs.traceFetch;
+    t->traceWaves = c->rs.traceWaves;
     return RBH_OK;
 }
 

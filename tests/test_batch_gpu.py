@@ -70,6 +70,10 @@ def test_autotune_keeps_results(ctx):
         r.set_tuning(shade_waves=w)
         r.render(spp=8, offset=0)
         assert np.array_equal(a, r.accum()), w
+    for w in (7, 8):
+        r.set_tuning(trace_waves=w)
+        r.render(spp=8, offset=0)
+        assert np.array_equal(a, r.accum()), w
     # per-lane work fetch instead of wave-synchronous ray batches
     r.set_tuning(trace_fetch=1)
     r.render(spp=8, offset=0)
