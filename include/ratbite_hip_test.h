@@ -31,7 +31,8 @@ int rbh_test_oct(rbh_ctx ctx, const float* v, uint32_t* enc, float* dec, uint32_
  * (ignored, 0), d.xyz, tmax}; hits: n x {t, prim (uint bits, 0xFFFFFFFF =
  * miss), b1, b2}. mode bit 0: any hit; bit 1: a 2-entry LDS stack so that the
  * global spill part of the traversal stack is exercised; bit 2: traverse the
- * binary tree instead of the four-wide one the renderer uses. */
+ * binary tree instead of the four-wide one the renderer uses; bit 3: the
+ * four-wide nodes quantised to 64 bytes. */
 int rbh_test_trace(rbh_ctx ctx, const float* rays, float* hits, uint32_t n, int mode);
 /* The built LBVH (replaces optixAccelBuild, rendering_interface.cpp:733-1064):
  * sorted 64-bit keys (Morton code << 32 | triangle), n entries; 64-byte nodes

@@ -86,6 +86,7 @@ class Stats(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("pathSlots", c_uint64), ("traceBlocksPerCU", ctypes.c_int32), ("shadeBlocksPerCU", ctypes.c_int32),
                 ("useGraph", ctypes.c_int32), ("shadeWaves", ctypes.c_int32), ("traceFetch", ctypes.c_int32), ("traceWaves", ctypes.c_int32),
+                ("quantNodes", ctypes.c_int32),
                 ("samplesPerSecond", c_double)]
 
 

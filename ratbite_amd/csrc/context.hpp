@@ -68,6 +68,7 @@ struct Accel {
     uint32_t nnodes = 0, depth = 0;
     DevBuf nodes;     // BvhNode (binary LBVH)
     DevBuf nodes4;    // Bvh4Node (the binary tree collapsed four-wide; traversed)
+    DevBuf nodes4q;   // the four-wide nodes quantised to 64 bytes (Bvh4QNode)
     uint32_t nnodes4 = 0;
     DevBuf tris;      // TriRecord in leaf order
     DevBuf scratch;   // build scratch
@@ -134,7 +135,8 @@ struct RenderState {
     int shadeBlocksPerCU = 64;
     int shadeWaves = 3;          // shade-kernel variant (1, 3, 4 waves per SIMD)
     int traceFetch = 0;          // 1: trace/shadow lanes fetch a new ray as soon as theirs completes
-    int traceWaves = 1;          // trace/shadow register budget (1, 7, 8 waves per SIMD)
+    int traceWaves = 7;          // trace/shadow register budget (1, 7, 8 waves per SIMD)
+    int quantNodes = 0;          // traverse the 64-byte quantised four-wide nodes
     DevBuf sceneTable;           // batched passes: SceneView per scene and pass
     bool useGraph = false;       // replay each pass's launch sequence as a captured HIP graph
     hipGraphExec_t graphExec = nullptr;

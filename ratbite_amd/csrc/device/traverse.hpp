@@ -180,11 +180,47 @@ __device__ __forceinline__ void trav_init(TravRay& r, V3 o, V3 d, float tmax, ui
     r.sp = 0;
 }
 
+// Child boxes and references of one four-wide node.
+struct Box4 {
+    float lx[4], hx[4], ly[4], hy[4], lz[4], hz[4];
+    int ch[4];
+};
+__device__ __forceinline__ void load_node(const Bvh4Node* __restrict__ nodes, int i, Box4& b) {
+    const float4* q = reinterpret_cast<const float4*>(nodes + i);
+    const float4 lx = q[0], hx = q[1], ly = q[2], hy = q[3], lz = q[4], hz = q[5];
+    const int4 ch4 = reinterpret_cast<const int4*>(q)[6];
+    b.lx[0] = lx.x; b.lx[1] = lx.y; b.lx[2] = lx.z; b.lx[3] = lx.w;
+    b.hx[0] = hx.x; b.hx[1] = hx.y; b.hx[2] = hx.z; b.hx[3] = hx.w;
+    b.ly[0] = ly.x; b.ly[1] = ly.y; b.ly[2] = ly.z; b.ly[3] = ly.w;
+    b.hy[0] = hy.x; b.hy[1] = hy.y; b.hy[2] = hy.z; b.hy[3] = hy.w;
+    b.lz[0] = lz.x; b.lz[1] = lz.y; b.lz[2] = lz.z; b.lz[3] = lz.w;
+    b.hz[0] = hz.x; b.hz[1] = hz.y; b.hz[2] = hz.z; b.hz[3] = hz.w;
+    b.ch[0] = ch4.x; b.ch[1] = ch4.y; b.ch[2] = ch4.z; b.ch[3] = ch4.w;
+}
+__device__ __forceinline__ void load_node(const Bvh4QNode* __restrict__ nodes, int i, Box4& b) {
+    const uint4* q = reinterpret_cast<const uint4*>(nodes + i);
+    const uint4 h = q[0], c = q[1], a = q[2];
+    const uint2 e = reinterpret_cast<const uint2*>(q)[6];
+    const float ox = __uint_as_float(h.x), oy = __uint_as_float(h.y), oz = __uint_as_float(h.z);
+    const float sx = qnode_step(h.w & 0xFFu), sy = qnode_step((h.w >> 8) & 0xFFu), sz = qnode_step((h.w >> 16) & 0xFFu);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        b.lx[k] = qnode_decode(ox, (a.x >> (8 * k)) & 0xFFu, sx);
+        b.ly[k] = qnode_decode(oy, (a.y >> (8 * k)) & 0xFFu, sy);
+        b.lz[k] = qnode_decode(oz, (a.z >> (8 * k)) & 0xFFu, sz);
+        b.hx[k] = qnode_decode(ox, (a.w >> (8 * k)) & 0xFFu, sx);
+        b.hy[k] = qnode_decode(oy, (e.x >> (8 * k)) & 0xFFu, sy);
+        b.hz[k] = qnode_decode(oz, (e.y >> (8 * k)) & 0xFFu, sz);
+    }
+    b.ch[0] = (int)c.x; b.ch[1] = (int)c.y; b.ch[2] = (int)c.z; b.ch[3] = (int)c.w;
+}
+
 // Visits r.node: tests its triangle children (nearest hit so far shortens
 // the ray), descends into the nearest box hit and defers the others,
 // nearest last on the stack. Returns true when the query is complete.
-template <bool ANY, int LDS_DEPTH>
-__device__ __forceinline__ bool trav_step(const Bvh4Node* __restrict__ nodes, const TriRecord* __restrict__ tris,
+// NODE: Bvh4Node or its 64-byte quantised form Bvh4QNode.
+template <bool ANY, int LDS_DEPTH, typename NODE>
+__device__ __forceinline__ bool trav_step(const NODE* __restrict__ nodes, const TriRecord* __restrict__ tris,
                                           TravRay& r, const TravStack<LDS_DEPTH>& stack) {
     if (r.node < 0) {
         // a single-triangle scene: the root is the leaf
@@ -198,13 +234,12 @@ __device__ __forceinline__ bool trav_step(const Bvh4Node* __restrict__ nodes, co
         }
         return true;
     }
-    const float4* q = reinterpret_cast<const float4*>(nodes + r.node);
-    const float4 lx = q[0], hx = q[1], ly = q[2], hy = q[3], lz = q[4], hz = q[5];
-    const int4 ch4 = reinterpret_cast<const int4*>(q)[6];
-    const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
-    const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
-    const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
-    const int cha[4] = {ch4.x, ch4.y, ch4.z, ch4.w};
+    Box4 bx;
+    load_node(nodes, r.node, bx);
+    const float* lxa = bx.lx; const float* hxa = bx.hx;
+    const float* lya = bx.ly; const float* hya = bx.hy;
+    const float* lza = bx.lz; const float* hza = bx.hz;
+    const int* cha = bx.ch;
     float tn[4];
     bool in[4];
     {
@@ -270,8 +305,8 @@ __device__ __forceinline__ bool trav_step(const Bvh4Node* __restrict__ nodes, co
 
 // Whole query in one call: the closest hit below tmax or, with ANY, the
 // first one found.
-template <bool ANY, int LDS_DEPTH>
-__device__ __forceinline__ Hit traverse4(const Bvh4Node* __restrict__ nodes, const TriRecord* __restrict__ tris,
+template <bool ANY, int LDS_DEPTH, typename NODE>
+__device__ __forceinline__ Hit traverse4(const NODE* __restrict__ nodes, const TriRecord* __restrict__ tris,
                                          uint32_t nnodes, V3 o, V3 d, float tmax, const TravStack<LDS_DEPTH>& stack) {
     TravRay r;
     trav_init(r, o, d, tmax, nnodes);

@@ -44,6 +44,27 @@ struct alignas(16) Bvh4Node {
 };
 static_assert(sizeof(Bvh4Node) == 128, "Bvh4Node is 128 bytes");
 
+// The same node quantised to 64 bytes: child boxes as 8-bit offsets from the
+// node's box corner o in power-of-two steps s per axis, decoded as
+// o + q * s (q * s is exact; one rounding in the add, the same in the builder
+// and in traversal, so the builder can make every decoded box contain its
+// child: lower bounds rounded down, upper bounds up).
+struct alignas(64) Bvh4QNode {
+    float ox, oy, oz;
+    uint32_t exps;                      // biased exponents of s: x bits 0-7, y 8-15, z 16-23
+    int32_t child[4];                   // as in Bvh4Node
+    uint32_t lox, loy, loz, hix;        // byte k = child k
+    uint32_t hiy, hiz;
+    uint32_t pad[2];
+};
+static_assert(sizeof(Bvh4QNode) == 64, "Bvh4QNode is 64 bytes");
+__host__ __device__ inline float qnode_step(uint32_t expByte) {
+    union { uint32_t u; float f; } v;
+    v.u = expByte << 23;
+    return v.f;
+}
+__host__ __device__ inline float qnode_decode(float o, uint32_t q, float s) { return o + (float)q * s; }
+
 // Triangle in leaf order: vertices + original triangle index, 48 bytes.
 struct alignas(16) TriRecord {
     float v0x, v0y, v0z; uint32_t prim;

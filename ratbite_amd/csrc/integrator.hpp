@@ -52,6 +52,7 @@ struct SceneView {
     // geometry
     const BvhNode* nodes;
     const Bvh4Node* nodes4;
+    const Bvh4QNode* nodes4q;
     const TriRecord* tris;
     uint32_t nnodes, nnodes4;
     const float* verts;
@@ -86,6 +87,7 @@ struct LaunchShape {
     int shadeWaves;          // register budget of the shade kernel: 1 (free), 3 or 4 waves per SIMD
     int traceFetch;          // trace/shadow lanes fetch a new ray when theirs completes (1) or wave by wave (0)
     int traceWaves;          // register budget of the wave-synchronous trace/shadow kernels: 1 (free), 7 or 8
+    int quantNodes;          // wave-synchronous trace/shadow kernels read the 64-byte quantised nodes
 };
 
 struct Frame1 {

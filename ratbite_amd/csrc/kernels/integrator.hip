@@ -272,9 +272,17 @@ struct WaveList {
     }
 };
 
+// The four-wide nodes a kernel traverses: 128-byte or quantised 64-byte
+template <bool QN>
+__device__ __forceinline__ auto scene_nodes(const SceneView& S) {
+    if constexpr (QN) return S.nodes4q;
+    else return S.nodes4;
+}
+
 // WAVES: minimum resident waves per SIMD the register allocation must allow
-// (1 = unconstrained), traded by the launch shape against spills
-template <bool BATCH, bool FETCH, int WAVES>
+// (1 = unconstrained), traded by the launch shape against spills; QN: the
+// quantised 64-byte nodes
+template <bool BATCH, bool FETCH, int WAVES, bool QN>
 __global__ __launch_bounds__(kTraceBlock, WAVES) void k_trace(Frame1 f, uint32_t qsel) {
     __shared__ int stack[kStackLds * kTraceBlock];
     __shared__ uint32_t staged[FETCH ? 2 * kTraceBlock * 2 : 1];   // per wave: 128 hits, 128 misses
@@ -326,7 +334,7 @@ __global__ __launch_bounds__(kTraceBlock, WAVES) void k_trace(Frame1 f, uint32_t
                 pid = identity ? i : queue[i];
                 const SceneView& S = scene_of<BATCH>(f, pid);
                 const float4 od = f.p.od[pid], dr = f.p.dr[pid];
-                const Hit h = traverse4<false>(S.nodes4, S.tris, S.nnodes4, V3{od.x, od.y, od.z},
+                const Hit h = traverse4<false>(scene_nodes<QN>(S), S.tris, S.nnodes4, V3{od.x, od.y, od.z},
                                                V3{dr.x, dr.y, dr.z}, 3.402823466e+38f, st);
                 hit = h.prim != 0xFFFFFFFFu;
                 if (hit) f.p.hit[pid] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.prim));
@@ -436,7 +444,7 @@ __device__ __forceinline__ void add_nee(const Frame1& f, uint32_t pid) {
     f.p.L[pid] = L;
 }
 
-template <bool BATCH, bool FETCH, int WAVES>
+template <bool BATCH, bool FETCH, int WAVES, bool QN>
 __global__ __launch_bounds__(kTraceBlock, WAVES) void k_shadow(Frame1 f) {
     __shared__ int stack[kStackLds * kTraceBlock];
     const TravStack<kStackLds> st = lane_stack(f, stack);
@@ -473,7 +481,7 @@ __global__ __launch_bounds__(kTraceBlock, WAVES) void k_shadow(Frame1 f) {
                 const uint32_t pid = f.p.sq[i];
                 const SceneView& S = scene_of<BATCH>(f, pid);
                 const float4 so = f.p.sro[pid], sd = f.p.srd[pid];
-                const Hit h = traverse4<true>(S.nodes4, S.tris, S.nnodes4, V3{so.x, so.y, so.z},
+                const Hit h = traverse4<true>(scene_nodes<QN>(S), S.tris, S.nnodes4, V3{so.x, so.y, so.z},
                                               V3{sd.x, sd.y, sd.z}, so.w, st);
                 if (h.prim == 0xFFFFFFFFu) add_nee(f, pid);
                 ++nrays;
@@ -1006,22 +1014,36 @@ __global__ __launch_bounds__(256) void k_accumulate(Frame1 f) {
 // Spill entries per lane of the traversal stack beyond the LDS part.
 size_t trace_spill_words(size_t lanes) { return lanes * (size_t)(kMaxStack4 - kStackLds); }
 
+template <bool BATCH, bool QN>
+static void launch_trace_q(rbh_ctx c, const Frame1& f, uint32_t qsel, const LaunchShape& ls) {
+    const dim3 g(ls.traceGrid), b(kTraceBlock);
+    if (ls.traceWaves >= 8) hipLaunchKernelGGL((k_trace<BATCH, false, 8, QN>), g, b, 0, c->stream, f, qsel);
+    else if (ls.traceWaves == 7) hipLaunchKernelGGL((k_trace<BATCH, false, 7, QN>), g, b, 0, c->stream, f, qsel);
+    else hipLaunchKernelGGL((k_trace<BATCH, false, 1, QN>), g, b, 0, c->stream, f, qsel);
+}
+
 template <bool BATCH>
 static void launch_trace(rbh_ctx c, const Frame1& f, uint32_t qsel, const LaunchShape& ls) {
+    if (ls.traceFetch)
+        hipLaunchKernelGGL((k_trace<BATCH, true, 1, false>), dim3(ls.traceGrid), dim3(kTraceBlock), 0, c->stream, f, qsel);
+    else if (ls.quantNodes) launch_trace_q<BATCH, true>(c, f, qsel, ls);
+    else launch_trace_q<BATCH, false>(c, f, qsel, ls);
+}
+
+template <bool BATCH, bool QN>
+static void launch_shadow_q(rbh_ctx c, const Frame1& f, const LaunchShape& ls) {
     const dim3 g(ls.traceGrid), b(kTraceBlock);
-    if (ls.traceFetch) hipLaunchKernelGGL((k_trace<BATCH, true, 1>), g, b, 0, c->stream, f, qsel);
-    else if (ls.traceWaves >= 8) hipLaunchKernelGGL((k_trace<BATCH, false, 8>), g, b, 0, c->stream, f, qsel);
-    else if (ls.traceWaves == 7) hipLaunchKernelGGL((k_trace<BATCH, false, 7>), g, b, 0, c->stream, f, qsel);
-    else hipLaunchKernelGGL((k_trace<BATCH, false, 1>), g, b, 0, c->stream, f, qsel);
+    if (ls.traceWaves >= 8) hipLaunchKernelGGL((k_shadow<BATCH, false, 8, QN>), g, b, 0, c->stream, f);
+    else if (ls.traceWaves == 7) hipLaunchKernelGGL((k_shadow<BATCH, false, 7, QN>), g, b, 0, c->stream, f);
+    else hipLaunchKernelGGL((k_shadow<BATCH, false, 1, QN>), g, b, 0, c->stream, f);
 }
 
 template <bool BATCH>
 static void launch_shadow(rbh_ctx c, const Frame1& f, const LaunchShape& ls) {
-    const dim3 g(ls.traceGrid), b(kTraceBlock);
-    if (ls.traceFetch) hipLaunchKernelGGL((k_shadow<BATCH, true, 1>), g, b, 0, c->stream, f);
-    else if (ls.traceWaves >= 8) hipLaunchKernelGGL((k_shadow<BATCH, false, 8>), g, b, 0, c->stream, f);
-    else if (ls.traceWaves == 7) hipLaunchKernelGGL((k_shadow<BATCH, false, 7>), g, b, 0, c->stream, f);
-    else hipLaunchKernelGGL((k_shadow<BATCH, false, 1>), g, b, 0, c->stream, f);
+    if (ls.traceFetch)
+        hipLaunchKernelGGL((k_shadow<BATCH, true, 1, false>), dim3(ls.traceGrid), dim3(kTraceBlock), 0, c->stream, f);
+    else if (ls.quantNodes) launch_shadow_q<BATCH, true>(c, f, ls);
+    else launch_shadow_q<BATCH, false>(c, f, ls);
 }
 
 // One wavefront pass: camera rays for every slot, depthMax bounces of

@@ -257,13 +257,62 @@ __device__ __forceinline__ float half_area(const float* b) {
     return x * y + y * z + z * x;
 }
 
+// Biased exponent of the smallest power-of-two step s (s >= 2^-126) with
+// lo + 255 s >= hi in decode arithmetic.
+__device__ __forceinline__ uint32_t qnode_exponent(float lo, float hi) {
+    const float ext = hi - lo;
+    if (!(ext > 0.0f)) return 1u;
+    uint32_t e = (__float_as_uint(ext * (1.0f / 255.0f)) >> 23) & 0xFFu;
+    if (e == 0u) e = 1u;
+    while (e < 254u && qnode_decode(lo, 255u, qnode_step(e)) < hi) ++e;
+    return e;
+}
+// conservative 8-bit bounds of [clo, chi] on the grid lo + q s
+__device__ __forceinline__ void qnode_bounds(float lo, float s, float clo, float chi, uint32_t& qlo, uint32_t& qhi) {
+    qlo = (uint32_t)fminf(fmaxf(floorf((clo - lo) / s), 0.0f), 255.0f);
+    while (qlo > 0u && qnode_decode(lo, qlo, s) > clo) --qlo;
+    qhi = (uint32_t)fminf(fmaxf(ceilf((chi - lo) / s), 0.0f), 255.0f);
+    while (qhi < 255u && qnode_decode(lo, qhi, s) < chi) ++qhi;
+}
+
+// Quantised copy of a four-wide node (empty slots get zero bytes; their
+// child reference masks them in traversal).
+__device__ __forceinline__ Bvh4QNode quantise_node(const Bvh4Node& nd) {
+    float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (int k = 0; k < 4; ++k) {
+        if (nd.child[k] == kBvh4Empty) continue;
+        b[0] = fminf(b[0], nd.lox[k]); b[1] = fminf(b[1], nd.loy[k]); b[2] = fminf(b[2], nd.loz[k]);
+        b[3] = fmaxf(b[3], nd.hix[k]); b[4] = fmaxf(b[4], nd.hiy[k]); b[5] = fmaxf(b[5], nd.hiz[k]);
+    }
+    Bvh4QNode q;
+    q.ox = b[0]; q.oy = b[1]; q.oz = b[2];
+    const uint32_t ex = qnode_exponent(b[0], b[3]), ey = qnode_exponent(b[1], b[4]), ez = qnode_exponent(b[2], b[5]);
+    q.exps = ex | (ey << 8) | (ez << 16);
+    const float sx = qnode_step(ex), sy = qnode_step(ey), sz = qnode_step(ez);
+    q.lox = q.loy = q.loz = q.hix = q.hiy = q.hiz = 0u;
+    for (int k = 0; k < 4; ++k) {
+        q.child[k] = nd.child[k];
+        if (nd.child[k] == kBvh4Empty) continue;
+        uint32_t l, h;
+        qnode_bounds(b[0], sx, nd.lox[k], nd.hix[k], l, h);
+        q.lox |= l << (8 * k); q.hix |= h << (8 * k);
+        qnode_bounds(b[1], sy, nd.loy[k], nd.hiy[k], l, h);
+        q.loy |= l << (8 * k); q.hiy |= h << (8 * k);
+        qnode_bounds(b[2], sz, nd.loz[k], nd.hiz[k], l, h);
+        q.loz |= l << (8 * k); q.hiz |= h << (8 * k);
+    }
+    q.pad[0] = q.pad[1] = 0u;
+    return q;
+}
+
 // One collapse level: in[] holds (binary node, four-wide index) pairs; each
 // produces its four-wide node and appends its inner children to out[].
 __global__ __launch_bounds__(kBuildBlock) void k_collapse(const int2* __restrict__ child, const float* __restrict__ ibox,
                                                           const float* __restrict__ lbox, const int2* __restrict__ in,
                                                           const unsigned int* __restrict__ inCount, int2* __restrict__ out,
                                                           unsigned int* __restrict__ outCount,
-                                                          unsigned int* __restrict__ nodeCount, Bvh4Node* __restrict__ nodes4) {
+                                                          unsigned int* __restrict__ nodeCount, Bvh4Node* __restrict__ nodes4,
+                                                          Bvh4QNode* __restrict__ nodes4q) {
     const unsigned int cnt = *inCount;
     for (unsigned int t = blockIdx.x * blockDim.x + threadIdx.x; t < cnt; t += gridDim.x * blockDim.x) {
         const int2 w = in[t];
@@ -306,6 +355,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_collapse(const int2* __restrict
         }
         nd.pad[0] = nd.pad[1] = nd.pad[2] = nd.pad[3] = 0;
         nodes4[w.y] = nd;
+        nodes4q[w.y] = quantise_node(nd);
     }
 }
 
@@ -407,6 +457,7 @@ int lbvh_build(rbh_ctx c) {
     hipLaunchKernelGGL(k_depth, dim3(nb), dim3(kBuildBlock), 0, c->stream, n, parent, bounds + 6);
     if (n > 1) hipLaunchKernelGGL(k_emit, dim3(nbi), dim3(kBuildBlock), 0, c->stream, n, child, ibox, lbox, acc.nodes.as<BvhNode>());
     RBH_ALLOC(c, acc.nodes4, nInt * sizeof(Bvh4Node));
+    RBH_ALLOC(c, acc.nodes4q, nInt * sizeof(Bvh4QNode));
     if (n > 1) {
         int2* la = reinterpret_cast<int2*>(s + oListA);
         int2* lb = reinterpret_cast<int2*>(s + oListB);
@@ -422,7 +473,8 @@ int lbvh_build(rbh_ctx c) {
             unsigned int* outC = counts + (even ? 1 : 0);
             hipLaunchKernelGGL(k_reset_count, dim3(1), dim3(64), 0, c->stream, outC);
             hipLaunchKernelGGL(k_collapse, dim3(std::min(nbi, cus * 8)), dim3(kBuildBlock), 0, c->stream, child, ibox,
-                               lbox, even ? la : lb, inC, even ? lb : la, outC, counts + 2, acc.nodes4.as<Bvh4Node>());
+                               lbox, even ? la : lb, inC, even ? lb : la, outC, counts + 2, acc.nodes4.as<Bvh4Node>(),
+                               acc.nodes4q.as<Bvh4QNode>());
         }
     }
     RBH_HIP(c, hipGetLastError());

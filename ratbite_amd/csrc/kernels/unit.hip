@@ -74,16 +74,19 @@ __global__ void k_oct(const float* v, uint32_t* enc, float* dec, uint32_t n) {
 }
 
 template <bool ANY, int LDS_DEPTH>
-__global__ __launch_bounds__(128) void k_trace_test(const BvhNode* nodes, const Bvh4Node* nodes4, const TriRecord* tris,
-                                                    uint32_t nnodes, uint32_t nnodes4, int binary, const float* rays,
-                                                    float* hits, uint32_t n, int* spill, uint32_t spillStride) {
+__global__ __launch_bounds__(128) void k_trace_test(const BvhNode* nodes, const Bvh4Node* nodes4,
+                                                    const Bvh4QNode* nodes4q, const TriRecord* tris, uint32_t nnodes,
+                                                    uint32_t nnodes4, int tree, const float* rays, float* hits,
+                                                    uint32_t n, int* spill, uint32_t spillStride) {
     __shared__ int stack[LDS_DEPTH * 128];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const TravStack<LDS_DEPTH> st{stack + threadIdx.x, 128, spill + i, spillStride};
     const float* r = rays + 8 * (size_t)i;
-    const Hit h = binary ? traverse<ANY>(nodes, tris, nnodes, V3{r[0], r[1], r[2]}, V3{r[4], r[5], r[6]}, r[7], st)
-                         : traverse4<ANY>(nodes4, tris, nnodes4, V3{r[0], r[1], r[2]}, V3{r[4], r[5], r[6]}, r[7], st);
+    const V3 ro{r[0], r[1], r[2]}, rd{r[4], r[5], r[6]};
+    const Hit h = tree == 1 ? traverse<ANY>(nodes, tris, nnodes, ro, rd, r[7], st)
+                : tree == 2 ? traverse4<ANY>(nodes4q, tris, nnodes4, ro, rd, r[7], st)
+                            : traverse4<ANY>(nodes4, tris, nnodes4, ro, rd, r[7], st);
     float* o = hits + 4 * (size_t)i;
     o[0] = h.prim == 0xFFFFFFFFu ? -1.0f : h.t;
     o[1] = __uint_as_float(h.prim);
@@ -285,20 +288,22 @@ int rbh_test_trace(rbh_ctx c, const float* rays, float* hits, uint32_t n, int mo
     STAGE_CHECK(dr && dh && spill);
     const BvhNode* nodes = c->accel.nodes.as<BvhNode>();
     const Bvh4Node* nodes4 = c->accel.nodes4.as<Bvh4Node>();
+    const Bvh4QNode* nodes4q = c->accel.nodes4q.as<Bvh4QNode>();
     const TriRecord* tris = c->accel.tris.as<TriRecord>();
     const uint32_t nn = c->accel.nnodes, nn4 = c->accel.nnodes4;
     const bool any = (mode & 1) != 0, tiny = (mode & 2) != 0;
-    const int binary = (mode & 4) ? 1 : 0;
+    const int binary = (mode & 4) ? 1 : ((mode & 8) ? 2 : 0);
     // mode bit 1: a 2-entry LDS stack, so the spill columns carry the rest;
-    // bit 2: the binary tree instead of the four-wide one
+    // bit 2: the binary tree instead of the four-wide one; bit 3: the
+    // four-wide nodes quantised to 64 bytes
     if (any && tiny)
-        hipLaunchKernelGGL((k_trace_test<true, 2>), g, dim3(128), 0, 0, nodes, nodes4, tris, nn, nn4, binary, (const float*)dr, (float*)dh, n, spill, lanes);
+        hipLaunchKernelGGL((k_trace_test<true, 2>), g, dim3(128), 0, 0, nodes, nodes4, nodes4q, tris, nn, nn4, binary, (const float*)dr, (float*)dh, n, spill, lanes);
     else if (any)
-        hipLaunchKernelGGL((k_trace_test<true, 16>), g, dim3(128), 0, 0, nodes, nodes4, tris, nn, nn4, binary, (const float*)dr, (float*)dh, n, spill, lanes);
+        hipLaunchKernelGGL((k_trace_test<true, 16>), g, dim3(128), 0, 0, nodes, nodes4, nodes4q, tris, nn, nn4, binary, (const float*)dr, (float*)dh, n, spill, lanes);
     else if (tiny)
-        hipLaunchKernelGGL((k_trace_test<false, 2>), g, dim3(128), 0, 0, nodes, nodes4, tris, nn, nn4, binary, (const float*)dr, (float*)dh, n, spill, lanes);
+        hipLaunchKernelGGL((k_trace_test<false, 2>), g, dim3(128), 0, 0, nodes, nodes4, nodes4q, tris, nn, nn4, binary, (const float*)dr, (float*)dh, n, spill, lanes);
     else
-        hipLaunchKernelGGL((k_trace_test<false, 16>), g, dim3(128), 0, 0, nodes, nodes4, tris, nn, nn4, binary, (const float*)dr, (float*)dh, n, spill, lanes);
+        hipLaunchKernelGGL((k_trace_test<false, 16>), g, dim3(128), 0, 0, nodes, nodes4, nodes4q, tris, nn, nn4, binary, (const float*)dr, (float*)dh, n, spill, lanes);
     FINISH(hits, dh, (size_t)n * 16);
     return RBH_OK;
 }

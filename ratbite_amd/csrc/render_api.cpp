@@ -178,6 +178,7 @@ void fill_view(rbh_ctx c, const rbh_launch_params* lp, SceneView& v) {
     v.focus = lp->focusDistance;
     v.nodes = c->accel.nodes.as<BvhNode>();
     v.nodes4 = c->accel.nodes4.as<Bvh4Node>();
+    v.nodes4q = c->accel.nodes4q.as<Bvh4QNode>();
     v.nnodes4 = c->accel.nnodes4;
     v.tris = c->accel.tris.as<TriRecord>();
     v.nnodes = c->accel.nnodes;
@@ -229,6 +230,7 @@ int prepare_pass_state(rbh_ctx c, size_t NP, Frame1& f, LaunchShape& ls) {
     ls.shadeWaves = rs.shadeWaves;
     ls.traceFetch = rs.traceFetch;
     ls.traceWaves = rs.traceWaves;
+    ls.quantNodes = rs.quantNodes;
     const size_t lanes = (size_t)ls.traceGrid * kTraceBlock;
     RBH_ALLOC(c, rs.spill, trace_spill_words(lanes) * sizeof(int));
     f.spill = rs.spill.as<int>();
@@ -512,7 +514,9 @@ int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_
     double bestRate = -1.0;
     rbh_tuning b{};
     (void)freeB;
-    const int keepF = rs.traceFetch, keepW = rs.shadeWaves, keepTW = rs.traceWaves;
+    const int keepF = rs.traceFetch, keepW = rs.shadeWaves, keepTW = rs.traceWaves, keepQ = rs.quantNodes;
+    rs.quantNodes = 0;
+    b.quantNodes = 0;
     rs.traceFetch = 0;
     rs.shadeWaves = 1;
     rs.traceWaves = 1;
@@ -530,6 +534,7 @@ int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_
                 rs.traceFetch = keepF;
                 rs.shadeWaves = keepW;
                 rs.traceWaves = keepTW;
+                rs.quantNodes = keepQ;
                 return rc;
             }
             const double rate = (double)P * trialSpp / (c->stats.renderMs * 1e-3);
@@ -578,6 +583,23 @@ int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_
         }
     }
     if (bestRate >= 0.0) {
+        // the 64-byte quantised nodes
+        rs.traceBlocksPerCU = b.traceBlocksPerCU;
+        rs.shadeBlocksPerCU = b.shadeBlocksPerCU;
+        rs.shadeWaves = b.shadeWaves ? b.shadeWaves : 1;
+        rs.traceWaves = b.traceWaves;
+        rs.quantNodes = 1;
+        const int rc = rbh_render(c, &t);
+        rs.quantNodes = 0;
+        if (rc) return rc;
+        const double rate = (double)P * trialSpp / (c->stats.renderMs * 1e-3);
+        if (rate > bestRate) {
+            bestRate = rate;
+            b.samplesPerSecond = rate;
+            b.quantNodes = 1;
+        }
+    }
+    if (bestRate >= 0.0) {
         // per-lane work fetch instead of wave-synchronous ray batches
         rs.traceBlocksPerCU = b.traceBlocksPerCU;
         rs.shadeBlocksPerCU = b.shadeBlocksPerCU;
@@ -598,6 +620,7 @@ int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_
         // the best shape once more as a replayed graph
         rs.traceFetch = b.traceFetch;
         rs.traceWaves = b.traceWaves;
+        rs.quantNodes = b.quantNodes;
         rs.slotBudget = b.pathSlots;
         rs.traceBlocksPerCU = b.traceBlocksPerCU;
         rs.shadeBlocksPerCU = b.shadeBlocksPerCU;
@@ -620,6 +643,7 @@ int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_
         rs.traceFetch = keepF;
         rs.shadeWaves = keepW;
         rs.traceWaves = keepTW;
+        rs.quantNodes = keepQ;
         return fail(c, RBH_ERR_NOMEM, "no launch shape fits the free device memory");
     }
     rs.slotBudget = b.pathSlots;
@@ -629,6 +653,7 @@ int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_
     rs.shadeWaves = b.shadeWaves ? b.shadeWaves : 1;
     rs.traceFetch = b.traceFetch;
     rs.traceWaves = b.traceWaves;
+    rs.quantNodes = b.quantNodes;
     if (!b.shadeWaves) b.shadeWaves = 1;
     *best = b;
     return RBH_OK;
@@ -646,6 +671,7 @@ int rbh_set_tuning(rbh_ctx c, const rbh_tuning* t) {
     c->rs.shadeWaves = (t->shadeWaves == 3 || t->shadeWaves == 4) ? t->shadeWaves : 1;
     c->rs.traceFetch = t->traceFetch != 0;
     c->rs.traceWaves = (t->traceWaves == 7 || t->traceWaves == 8) ? t->traceWaves : 1;
+    c->rs.quantNodes = t->quantNodes != 0;
     return RBH_OK;
 }
 
@@ -659,6 +685,7 @@ int rbh_get_tuning(rbh_ctx c, rbh_tuning* t) {
     t->shadeWaves = c->rs.shadeWaves;
     t->traceFetch = c->rs.traceFetch;
     t->traceWaves = c->rs.traceWaves;
+    t->quantNodes = c->rs.quantNodes;
     return RBH_OK;
 }
 

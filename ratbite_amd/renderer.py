@@ -62,7 +62,8 @@ class Renderer:
         self.samples = 0
         return {"pathSlots": t.pathSlots, "traceBlocksPerCU": t.traceBlocksPerCU,
                 "shadeBlocksPerCU": t.shadeBlocksPerCU, "useGraph": t.useGraph, "shadeWaves": t.shadeWaves,
-                "traceFetch": t.traceFetch, "traceWaves": t.traceWaves, "samplesPerSecond": t.samplesPerSecond}
+                "traceFetch": t.traceFetch, "traceWaves": t.traceWaves, "quantNodes": t.quantNodes,
+                "samplesPerSecond": t.samplesPerSecond}
 
     def tuning(self):
         """The launch shape this context renders with."""
@@ -70,13 +71,13 @@ class Renderer:
         self.ctx.call("rbh_get_tuning", ctypes.byref(t))
         return {"pathSlots": t.pathSlots, "traceBlocksPerCU": t.traceBlocksPerCU,
                 "shadeBlocksPerCU": t.shadeBlocksPerCU, "useGraph": t.useGraph, "shadeWaves": t.shadeWaves,
-                "traceFetch": t.traceFetch, "traceWaves": t.traceWaves}
+                "traceFetch": t.traceFetch, "traceWaves": t.traceWaves, "quantNodes": t.quantNodes}
 
     def set_tuning(self, path_slots=0, trace_blocks=64, shade_blocks=64, use_graph=False, shade_waves=3,
-                   trace_fetch=0, trace_waves=1):
+                   trace_fetch=0, trace_waves=7, quant_nodes=False):
         """Sets the launch shape; the defaults are the library's own."""
         t = L.Tuning(path_slots, trace_blocks, shade_blocks, int(use_graph), shade_waves, int(trace_fetch),
-                     trace_waves, 0.0)
+                     trace_waves, int(quant_nodes), 0.0)
         self.ctx.call("rbh_set_tuning", ctypes.byref(t))
 
     # ------------------------------------------------------------ guiding

@@ -70,10 +70,16 @@ def test_autotune_keeps_results(ctx):
         r.set_tuning(shade_waves=w)
         r.render(spp=8, offset=0)
         assert np.array_equal(a, r.accum()), w
-    for w in (7, 8):
+    for w in (1, 7, 8):
         r.set_tuning(trace_waves=w)
         r.render(spp=8, offset=0)
         assert np.array_equal(a, r.accum()), w
+        # quantised nodes find the same closest t; only a ray through a shared
+        # edge (both triangles at the same t) may report the other triangle
+        r.set_tuning(trace_waves=w, quant_nodes=True)
+        r.render(spp=8, offset=0)
+        same = (a == r.accum()).all(axis=2)
+        assert same.mean() > 0.998, (w, same.mean())
     # per-lane work fetch instead of wave-synchronous ray batches
     r.set_tuning(trace_fetch=1)
     r.render(spp=8, offset=0)

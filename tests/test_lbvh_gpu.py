@@ -22,9 +22,9 @@ def _upload(ctx, verts, idx):
     return ctx.stats()
 
 
-def _trace(ctx, rays, anyhit, tiny_stack=False, binary=False):
+def _trace(ctx, rays, anyhit, tiny_stack=False, binary=False, quant=False):
     hits = np.zeros((len(rays), 4), np.float32)
-    mode = int(anyhit) | (2 if tiny_stack else 0) | (4 if binary else 0)
+    mode = int(anyhit) | (2 if tiny_stack else 0) | (4 if binary else 0) | (8 if quant else 0)
     ctx.check(L.lib().rbh_test_trace(ctx.h, L.ptr(rays), L.ptr(hits), len(rays), mode))
     return hits
 
@@ -104,6 +104,34 @@ def test_four_wide_tree_matches_binary(ctx):
     rays[:, 7] = rng.uniform(1, 60, len(rays))
     ea = _trace(ctx, rays, True)[:, 1].view(np.uint32) == 0xFFFFFFFF
     eb = _trace(ctx, rays, True, binary=True)[:, 1].view(np.uint32) == 0xFFFFFFFF
+    assert np.array_equal(ea, eb)
+
+
+@pytest.mark.parametrize("which", ["soup", "spheres"])
+def test_quantised_nodes_match_full_precision(ctx, which):
+    # the 64-byte nodes hold conservative 8-bit child boxes: they prune less
+    # but never a triangle the ray hits before its current best, so the
+    # closest t is bit-identical and the any-hit occlusion the same
+    rng = np.random.default_rng(31)
+    if which == "soup":
+        c = rng.uniform(-5, 5, (3000, 3))
+        v = (c[:, None, :] + rng.normal(scale=0.5, size=(3000, 3, 3))).reshape(-1, 3).astype(np.float32)
+        idx = np.arange(9000, dtype=np.uint32).reshape(-1, 3)
+        _upload(ctx, v, idx)
+        rays = _rays(rng, 30000, -6, 6)
+    else:
+        sc = scenes.spheres(n_spheres=30, level=4, n_lights=500, width=64, height=64, spp=1, env_size=None)
+        _upload(ctx, sc.verts, sc.indices)
+        rays = _rays(rng, 30000, [-40, -5, -100], [40, 20, -20])
+    a = _trace(ctx, rays, False)
+    b = _trace(ctx, rays, False, quant=True)
+    assert np.array_equal(a[:, 0].view(np.uint32), b[:, 0].view(np.uint32))
+    assert (a[:, 1].view(np.uint32) == b[:, 1].view(np.uint32)).mean() > 0.999
+    b2 = _trace(ctx, rays, False, tiny_stack=True, quant=True)
+    assert np.array_equal(b.view(np.uint32), b2.view(np.uint32))
+    rays[:, 7] = rng.uniform(0.5, 30, len(rays))
+    ea = _trace(ctx, rays, True)[:, 1].view(np.uint32) == 0xFFFFFFFF
+    eb = _trace(ctx, rays, True, quant=True)[:, 1].view(np.uint32) == 0xFFFFFFFF
     assert np.array_equal(ea, eb)
 
 
