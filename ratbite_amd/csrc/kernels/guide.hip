@@ -20,7 +20,7 @@
 
 #include "../context.hpp"
 #include "../guide.hpp"
-#include "../device/mlp.hpp"
+#include "../device/mlp_wave.hpp"
 
 namespace rbh {
 
@@ -314,27 +314,15 @@ __global__ __launch_bounds__(256) void k_guide_publish(const float* __restrict__
     if (p < kParams) publish(p, master[p], wf, wt, bias);
 }
 
-__global__ __launch_bounds__(256) void k_guide_infer(const float* __restrict__ recs, uint32_t n,
-                                                     const bf16* __restrict__ wf, const uint8_t* __restrict__ wq,
-                                                     const float* __restrict__ wqs, const float* __restrict__ bias,
-                                                     float* __restrict__ out) {
-    // wq != nullptr: FP8 E4M3 weights with per-layer scales wqs
-    __shared__ InferLds L;
-    const int wave = threadIdx.x >> 6, lane = __lane_id();
-    const int row0 = wave * kRows;
-    for (uint32_t tile = blockIdx.x; (size_t)tile * kTile < n; tile += gridDim.x) {
-        const uint32_t base = tile * kTile;
-        encode_rows(L.x0, recs, n, base, row0);
-        if (wq) forward_rows_fp8(L, wq, wqs, bias, row0);
-        else forward_rows(L, wf, bias, row0);
-        __syncthreads();
-        for (int q = lane; q < kRows * kOut; q += 64) {
-            const int rr = q / kOut, c = q % kOut;
-            const uint32_t s = base + row0 + rr;
-            if (s < n) out[(size_t)s * kOut + c] = L.out[(row0 + rr) * kSO + c];
-        }
-        __syncthreads();
-    }
+// inference of staged records (raw outputs), the renderer's wave-resident
+// kernel body; FP8: E4M3 weights with per-layer scales
+template <bool FP8>
+__global__ __launch_bounds__(256, 2) void k_guide_infer(const float* __restrict__ recs, uint32_t n,
+                                                        const bf16* __restrict__ wf, const uint8_t* __restrict__ wq,
+                                                        const float* __restrict__ wqs, const float* __restrict__ bias,
+                                                        float* __restrict__ out) {
+    __shared__ WaveLds L[4];
+    wave_infer<FP8, false>(L[threadIdx.x >> 6], recs, n, wf, wq, wqs, bias, out);
 }
 
 // FP8 copy of the weights: one block per layer finds the largest magnitude,
@@ -662,9 +650,14 @@ static int infer(rbh_ctx c, const float* recs, uint32_t n, float* out, bool fp8)
     int rc = stage_records(c, recs, n, d);
     if (rc) return rc;
     RBH_ALLOC(c, o, (size_t)n * kOut * 4);
+    // four waves of 16 records per block
     const int blocks = (int)std::min<size_t>((size_t)train_blocks(c) * 2, (n + kTile - 1) / kTile);
-    hipLaunchKernelGGL(k_guide_infer, dim3(blocks), dim3(256), 0, c->stream, d.as<float>(), n, dptr<bf16>(g.wf),
-                       fp8 ? dptr<uint8_t>(g.fp8w) : nullptr, dptr<float>(g.fp8s), dptr<float>(g.bias), o.as<float>());
+    if (fp8)
+        hipLaunchKernelGGL((k_guide_infer<true>), dim3(blocks), dim3(256), 0, c->stream, d.as<float>(), n,
+                           dptr<bf16>(g.wf), dptr<uint8_t>(g.fp8w), dptr<float>(g.fp8s), dptr<float>(g.bias), o.as<float>());
+    else
+        hipLaunchKernelGGL((k_guide_infer<false>), dim3(blocks), dim3(256), 0, c->stream, d.as<float>(), n,
+                           dptr<bf16>(g.wf), nullptr, dptr<float>(g.fp8s), dptr<float>(g.bias), o.as<float>());
     RBH_HIP(c, hipGetLastError());
     RBH_HIP(c, hipStreamSynchronize(c->stream));
     RBH_HIP(c, hipMemcpy(out, o.p, (size_t)n * kOut * 4, hipMemcpyDeviceToHost));

@@ -21,7 +21,7 @@
 #include "../context.hpp"
 #include "../device/bxdf.hpp"
 #include "../device/guided.hpp"
-#include "../device/mlp.hpp"
+#include "../device/mlp_wave.hpp"
 #include "../device/lighttree.hpp"
 #include "../device/qrng.hpp"
 #include "../device/traverse.hpp"
@@ -375,56 +375,20 @@ __device__ __forceinline__ void guide_input(const Frame1& f, const SceneView& S,
     for (int k = 11; k < 16; ++k) out[k] = 0.0f;
 }
 
-// Raw outputs -> mixture (normalised means, clamped sharpness, softmax
-// weights) and selection probability
-__device__ __forceinline__ void guide_mixture(const float* __restrict__ o, float* __restrict__ mix) {
-    float lmax = -3.4e38f;
-    for (int j = 0; j < 8; ++j) lmax = fmaxf(lmax, o[5 * j + 4]);
-    float ws = 0.0f;
-    for (int j = 0; j < 8; ++j) ws += expf(o[5 * j + 4] - lmax);
-    for (int j = 0; j < 8; ++j) {
-        const float* q = o + 5 * j;
-        const float n = fmaxf(sqrtf(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]), 1e-6f);
-        mix[5 * j + 0] = q[0] / n;
-        mix[5 * j + 1] = q[1] / n;
-        mix[5 * j + 2] = q[2] / n;
-        mix[5 * j + 3] = expf(fminf(fmaxf(q[3], guide::kLogKappaMin), guide::kLogKappaMax));
-        mix[5 * j + 4] = expf(q[4] - lmax) / ws;
-    }
-    mix[guide::kMix] = guide::kSelMin + (guide::kSelMax - guide::kSelMin) / (1.0f + expf(-o[guide::kMix]));
+// Network input of every queued hit, one thread per hit (the gathers run
+// fully parallel), kept for the training records of the vertex.
+__global__ __launch_bounds__(256) void k_guide_features(Frame1 f) {
+    const uint32_t count = f.counters[CNT_HIT];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
+        guide_input(f, f.sc, f.p.hq[i], f.gfeat + (size_t)i * guide::kRec);
 }
 
-// Network inference for every queued hit: input records into LDS, the BF16
-// MFMA forward pass, mixtures to gmix (indexed like the hit queue).
-__global__ __launch_bounds__(256) void k_guide_paths(Frame1 f) {
-    __shared__ mlp::InferLds L;
-    __shared__ float recs[mlp::kTile * guide::kRec];
-    const uint32_t count = f.counters[CNT_HIT];
-    const int wave = threadIdx.x >> 6, lane = __lane_id(), r = lane & 15, g = lane >> 4;
-    const int row0 = wave * mlp::kRows;
-    for (uint32_t tile = blockIdx.x; (size_t)tile * mlp::kTile < count; tile += gridDim.x) {
-        const uint32_t base = tile * mlp::kTile;
-        if (threadIdx.x < mlp::kTile) {
-            const uint32_t i = base + threadIdx.x;
-            float* dst = recs + threadIdx.x * guide::kRec;
-            if (i < count) {
-                guide_input(f, f.sc, f.p.hq[i], dst);
-                for (int k = 0; k < guide::kRec; ++k) f.gfeat[(size_t)i * guide::kRec + k] = dst[k];
-            } else {
-                for (int k = 0; k < guide::kRec; ++k) dst[k] = 0.0f;
-            }
-        }
-        __syncthreads();
-        mlp::encode_rows(L.x0, recs, mlp::kTile, 0, row0);
-        if (f.gwq) mlp::forward_rows_fp8(L, f.gwq, f.gws, f.gbias, row0);
-        else mlp::forward_rows(L, f.gwf, f.gbias, row0);
-        __syncthreads();
-        if (g == 0) {
-            const uint32_t i = base + row0 + r;
-            if (i < count) guide_mixture(L.out + (row0 + r) * mlp::kSO, f.gmix + (size_t)i * kMixFloats);
-        }
-        __syncthreads();
-    }
+// Mixture of every queued hit: wave-resident MLP inference over the
+// feature records (BF16 or FP8 E4M3 weights).
+template <bool FP8>
+__global__ __launch_bounds__(256, 2) void k_guide_mlp(Frame1 f) {
+    __shared__ mlp::WaveLds L[4];
+    mlp::wave_infer<FP8, true>(L[threadIdx.x >> 6], f.gfeat, f.counters[CNT_HIT], f.gwf, f.gwq, f.gws, f.gbias, f.gmix);
 }
 
 template <bool BATCH>
@@ -1051,7 +1015,13 @@ static void launch_shadow(rbh_ctx c, const Frame1& f, const LaunchShape& ls) {
 // scene) or f.scenes (batched) must describe the pass's samples and slots.
 template <bool BATCH>
 static hipError_t launch_pass_t(rbh_ctx c, Frame1 f, uint32_t depthMax, const LaunchShape& ls, KernelTimer* kt) {
-    const int guideGrid = ls.traceGrid / 8 > 0 ? ls.traceGrid / 8 : 1;
+    // guiding: feature gather over all hit slots, then resident blocks of four
+    // MLP waves, as many per CU as the register budget allows (BF16 weights
+    // in registers: 2 waves per SIMD, FP8: 3)
+    const int nbGuide = (int)std::min<uint32_t>((f.NP + 255) / 256, 65535u * 16u);
+    int cus = 256;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0) cus = 256;
+    const int mlpGrid = (f.gwq ? 3 : 2) * cus;
     const int nb = (int)((f.NP + 255) / 256);
     const dim3 accGrid((f.maxP + 255) / 256, BATCH ? f.nscenes : 1u);
     hipError_t e = hipSuccess;
@@ -1070,9 +1040,13 @@ static hipError_t launch_pass_t(rbh_ctx c, Frame1 f, uint32_t depthMax, const La
         RBH_TIMED(KernelTimer::TRACE, launch_trace<BATCH>(c, f, qsel, ls));
         RBH_TIMED(KernelTimer::SHADE,
                   hipLaunchKernelGGL(k_miss<BATCH>, dim3(ls.shadeGrid), dim3(256), 0, c->stream, f));
-        if (!BATCH && f.guided)
+        if (!BATCH && f.guided) {
             RBH_TIMED(KernelTimer::GUIDE,
-                      hipLaunchKernelGGL(k_guide_paths, dim3(guideGrid), dim3(256), 0, c->stream, f));
+                      hipLaunchKernelGGL(k_guide_features, dim3(nbGuide), dim3(256), 0, c->stream, f));
+            RBH_TIMED(KernelTimer::GUIDE,
+                      if (f.gwq) hipLaunchKernelGGL((k_guide_mlp<true>), dim3(mlpGrid), dim3(256), 0, c->stream, f);
+                      else hipLaunchKernelGGL((k_guide_mlp<false>), dim3(mlpGrid), dim3(256), 0, c->stream, f));
+        }
         hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, c->stream, f.counters, qsel ? CNT_QUEUE0 : CNT_QUEUE1);
         if (!BATCH && f.guided)
             RBH_TIMED(KernelTimer::SHADE, hipLaunchKernelGGL((k_shade<true, false, 1>), dim3(ls.shadeGrid),
