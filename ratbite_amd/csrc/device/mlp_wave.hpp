@@ -32,36 +32,38 @@ struct WaveLds {
 // ------------------------------------------------------------------ BF16
 struct WaveWeightsBf16 {
     bf16x8 w0[4][3], w1[4][2], w2[4][2], w3[4][2], w4[3][2];
-    float b0[4], b1[4], b2[4], b3[4], b4[3];
+    const float* bias;   // padded per-layer biases (read at use, L1-resident)
 };
 
 template <int NT, int KB>
-__device__ __forceinline__ void load_layer(const bf16* __restrict__ wf, const float* __restrict__ bias, int l,
-                                           bf16x8 (&w)[NT][KB], float (&b)[NT]) {
+__device__ __forceinline__ void load_layer(const bf16* __restrict__ wf, int l, bf16x8 (&w)[NT][KB]) {
     const int lane = __lane_id(), r = lane & 15, g = lane >> 4;
 #pragma unroll
     for (int ct = 0; ct < NT; ++ct) {
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb)
             w[ct][kb] = *reinterpret_cast<const bf16x8*>(wf + wf_off(l) + (ct * 16 + r) * k_pad(l) + kb * 32 + 8 * g);
-        b[ct] = bias[bp_off(l) + ct * 16 + r];
     }
 }
 
 __device__ __forceinline__ void load_weights(WaveWeightsBf16& W, const bf16* __restrict__ wf,
                                              const float* __restrict__ bias) {
-    load_layer(wf, bias, 0, W.w0, W.b0);
-    load_layer(wf, bias, 1, W.w1, W.b1);
-    load_layer(wf, bias, 2, W.w2, W.b2);
-    load_layer(wf, bias, 3, W.w3, W.b3);
-    load_layer(wf, bias, 4, W.w4, W.b4);
+    load_layer(wf, 0, W.w0);
+    load_layer(wf, 1, W.w1);
+    load_layer(wf, 2, W.w2);
+    load_layer(wf, 3, W.w3);
+    load_layer(wf, 4, W.w4);
+    W.bias = bias;
 }
 
 // Y = act(X W^T + b) for the wave's 16 rows; X rows in LDS with stride sx
 template <int NT, int KB, bool RELU, bool F32OUT>
-__device__ __forceinline__ void wave_layer(const bf16* X, int sx, const bf16x8 (&w)[NT][KB], const float (&b)[NT],
-                                           void* Y, int sy) {
+__device__ __forceinline__ void wave_layer(const bf16* X, int sx, const bf16x8 (&w)[NT][KB],
+                                           const float* __restrict__ bl, void* Y, int sy) {
     const int lane = __lane_id(), r = lane & 15, g = lane >> 4;
+    float b[NT];
+#pragma unroll
+    for (int ct = 0; ct < NT; ++ct) b[ct] = bl[ct * 16 + r];
     bf16x8 a[KB];
 #pragma unroll
     for (int kb = 0; kb < KB; ++kb) a[kb] = *reinterpret_cast<const bf16x8*>(X + r * sx + kb * 32 + 8 * g);
@@ -87,11 +89,11 @@ __device__ __forceinline__ void wave_layer(const bf16* X, int sx, const bf16x8 (
 }
 
 __device__ __forceinline__ void wave_forward(const WaveWeightsBf16& W, WaveLds& L) {
-    wave_layer<4, 3, true, false>(L.x0, kSX0, W.w0, W.b0, L.h[0], kSH);
-    wave_layer<4, 2, true, false>(L.h[0], kSH, W.w1, W.b1, L.h[1], kSH);
-    wave_layer<4, 2, true, false>(L.h[1], kSH, W.w2, W.b2, L.h[0], kSH);
-    wave_layer<4, 2, true, false>(L.h[0], kSH, W.w3, W.b3, L.h[1], kSH);
-    wave_layer<3, 2, false, true>(L.h[1], kSH, W.w4, W.b4, L.out, kSO);
+    wave_layer<4, 3, true, false>(L.x0, kSX0, W.w0, W.bias + bp_off(0), L.h[0], kSH);
+    wave_layer<4, 2, true, false>(L.h[0], kSH, W.w1, W.bias + bp_off(1), L.h[1], kSH);
+    wave_layer<4, 2, true, false>(L.h[1], kSH, W.w2, W.bias + bp_off(2), L.h[0], kSH);
+    wave_layer<4, 2, true, false>(L.h[0], kSH, W.w3, W.bias + bp_off(3), L.h[1], kSH);
+    wave_layer<3, 2, false, true>(L.h[1], kSH, W.w4, W.bias + bp_off(4), L.out, kSO);
 }
 
 // ------------------------------------------------------------------- FP8
@@ -173,35 +175,84 @@ __device__ __forceinline__ void wave_forward(const WaveWeightsFp8& W, WaveLds& L
     wave_layer_fp8<3, 2, false, true>(L.h[1], kSH, W.w4, W.s[4], W.b4, L.out, kSO);
 }
 
-// encodes records [base, base + 16) of recs (rows past n are zero) into x0
-__device__ __forceinline__ void wave_encode(bf16* x0, const float* __restrict__ recs, uint32_t n, uint32_t base) {
+// The record of this lane's row (lane & 15) in registers, loaded with four
+// 16-byte reads issued together (zeros past n).
+struct WaveRec {
+    float4 q[4];
+};
+__device__ __forceinline__ WaveRec load_rec(const float* __restrict__ recs, uint32_t n, uint32_t base) {
+    WaveRec w;
+    const uint32_t s = base + (__lane_id() & 15);
+    const float4* p = reinterpret_cast<const float4*>(recs + (size_t)s * kRec);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w.q[k] = s < n ? p[k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return w;
+}
+
+// encodes the wave's 16 records into x0: the features of enc_feature, each
+// sin/cos pair from one sincospif; lane (r, g) writes pairs g, g + 4, ... of
+// record r and features 72 + g, 76 + g, ...
+__device__ __forceinline__ void wave_encode(bf16* x0, const WaveRec& w, bool live) {
     const int lane = __lane_id(), r = lane & 15, g = lane >> 4;
-    const uint32_t s = base + r;
-    const float* rec = recs + (size_t)s * kRec;
-    for (int f = g; f < 96; f += 4) {
-        const float v = (s < n && f < kIn) ? enc_feature(rec, f) : 0.0f;
-        x0[r * kSX0 + f] = (bf16)v;
+    const float pos[3] = {w.q[0].x, w.q[0].y, w.q[0].z};
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        const int p = g + 4 * j;          // pair index: axis j / 3, frequency p - 12 (j / 3)
+        const int k = p - 12 * (j / 3);
+        float sn = 0.0f, cs = 0.0f;
+        if (live) {
+            const float x = pos[j / 3] * (float)(1 << k);   // exact power-of-two scaling
+            const float rr = x - 2.0f * floorf(x * 0.5f);   // exact reduction modulo 2
+            sincospif(rr, &sn, &cs);
+        }
+        x0[r * kSX0 + 2 * p] = (bf16)sn;
+        x0[r * kSX0 + 2 * p + 1] = (bf16)cs;
     }
+    // features 72..79: normal (rec 3..5), wo (6..8), roughness (9), wavelength (10)
+    const float d[8] = {w.q[0].w, w.q[1].x, w.q[1].y, w.q[1].z, w.q[1].w, w.q[2].x, w.q[2].y, w.q[2].z};
+    const float d0 = g == 0 ? d[0] : g == 1 ? d[1] : g == 2 ? d[2] : d[3];
+    const float d1 = g == 0 ? d[4] : g == 1 ? d[5] : g == 2 ? d[6] : d[7];
+    x0[r * kSX0 + 72 + g] = (bf16)(live ? d0 : 0.0f);
+    x0[r * kSX0 + 76 + g] = (bf16)(live ? d1 : 0.0f);
+#pragma unroll
+    for (int f = 80; f < 96; f += 4) x0[r * kSX0 + f + g] = (bf16)0.0f;
     wave_sync();
 }
 
-// raw outputs -> mixture (normalised means, clamped sharpness, softmax
-// weights) followed by the selection probability of the lobes
-__device__ __forceinline__ void to_mixture(const float* __restrict__ o, float* __restrict__ mix) {
-    float lmax = -3.4e38f;
-    for (int j = 0; j < kLobes; ++j) lmax = fmaxf(lmax, o[5 * j + 4]);
-    float ws = 0.0f;
-    for (int j = 0; j < kLobes; ++j) ws += expf(o[5 * j + 4] - lmax);
-    for (int j = 0; j < kLobes; ++j) {
+// The mixture of the wave's 16 records from L.out, all 64 lanes busy: lane
+// (r, g) converts lobes 2g and 2g + 1 of record r; the softmax maximum and
+// sum run over the record's four lanes (shuffles at distance 16 and 32).
+__device__ __forceinline__ void wave_mixture(const float* __restrict__ o, float* __restrict__ mix, bool live) {
+    const int g = __lane_id() >> 4;
+    float lg[2], m = -3.4e38f;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        lg[h] = o[5 * (2 * g + h) + 4];
+        m = fmaxf(m, lg[h]);
+    }
+    m = fmaxf(m, __shfl_xor(m, 16));
+    m = fmaxf(m, __shfl_xor(m, 32));
+    float e[2], sum = 0.0f;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        e[h] = expf(lg[h] - m);
+        sum += e[h];
+    }
+    sum += __shfl_xor(sum, 16);
+    sum += __shfl_xor(sum, 32);
+    if (!live) return;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int j = 2 * g + h;
         const float* q = o + 5 * j;
         const float n = fmaxf(sqrtf(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]), 1e-6f);
         mix[5 * j + 0] = q[0] / n;
         mix[5 * j + 1] = q[1] / n;
         mix[5 * j + 2] = q[2] / n;
         mix[5 * j + 3] = expf(fminf(fmaxf(q[3], kLogKappaMin), kLogKappaMax));
-        mix[5 * j + 4] = expf(q[4] - lmax) / ws;
+        mix[5 * j + 4] = e[h] / sum;
     }
-    mix[kMix] = kSelMin + (kSelMax - kSelMin) / (1.0f + expf(-o[kMix]));
+    if (g == 0) mix[kMix] = kSelMin + (kSelMax - kSelMin) / (1.0f + expf(-o[kMix]));
 }
 
 // Inference over records [0, n) (n read from *nDev when given): MIX writes
@@ -211,18 +262,23 @@ __device__ __forceinline__ void wave_infer(WaveLds& L, const float* __restrict__
                                            const bf16* __restrict__ wf, const uint8_t* __restrict__ wq,
                                            const float* __restrict__ ws, const float* __restrict__ bias,
                                            float* __restrict__ out) {
-    typename std::conditional<FP8, WaveWeightsFp8, WaveWeightsBf16>::type W;
-    if constexpr (FP8) load_weights(W, wq, ws, bias);
-    else load_weights(W, wf, bias);
     const int lane = __lane_id();
     const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    if ((size_t)wave * kRows >= n) return;   // no tile for this wave: skip the weight loads
+    typename std::conditional<FP8, WaveWeightsFp8, WaveWeightsBf16>::type W;
+    if constexpr (FP8) load_weights(W, wq, ws, bias);
+    else load_weights(W, wf, bias);
+    WaveRec next = load_rec(recs, n, wave * kRows);
     for (uint32_t t = wave; (size_t)t * kRows < n; t += waves) {
         const uint32_t base = t * kRows;
-        wave_encode(L.x0, recs, n, base);
+        const WaveRec cur = next;
+        next = load_rec(recs, n, base + waves * kRows);   // prefetch the wave's next tile
+        wave_encode(L.x0, cur, base + (lane & 15) < n);
         wave_forward(W, L);
         if (MIX) {
-            if (lane < kRows && base + lane < n) to_mixture(L.out + lane * kSO, out + (size_t)(base + lane) * kOut);
+            const int r = lane & 15;
+            wave_mixture(L.out + r * kSO, out + (size_t)(base + r) * kOut, base + r < n);
         } else {
             for (int q = lane; q < kRows * kOut; q += 64) {
                 const int rr = q / kOut, c = q % kOut;

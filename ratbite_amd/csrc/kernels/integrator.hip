@@ -1016,12 +1016,12 @@ static void launch_shadow(rbh_ctx c, const Frame1& f, const LaunchShape& ls) {
 template <bool BATCH>
 static hipError_t launch_pass_t(rbh_ctx c, Frame1 f, uint32_t depthMax, const LaunchShape& ls, KernelTimer* kt) {
     // guiding: feature gather over all hit slots, then resident blocks of four
-    // MLP waves, as many per CU as the register budget allows (BF16 weights
-    // in registers: 2 waves per SIMD, FP8: 3)
+    // MLP waves, as many per CU as the register budget allows (weights and
+    // the prefetched next records in registers: 2 waves per SIMD)
     const int nbGuide = (int)std::min<uint32_t>((f.NP + 255) / 256, 65535u * 16u);
     int cus = 256;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0) cus = 256;
-    const int mlpGrid = (f.gwq ? 3 : 2) * cus;
+    const int mlpGrid = 2 * cus;
     const int nb = (int)((f.NP + 255) / 256);
     const dim3 accGrid((f.maxP + 255) / 256, BATCH ? f.nscenes : 1u);
     hipError_t e = hipSuccess;
@@ -1048,7 +1048,10 @@ static hipError_t launch_pass_t(rbh_ctx c, Frame1 f, uint32_t depthMax, const La
                       else hipLaunchKernelGGL((k_guide_mlp<false>), dim3(mlpGrid), dim3(256), 0, c->stream, f));
         }
         hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, c->stream, f.counters, qsel ? CNT_QUEUE0 : CNT_QUEUE1);
-        if (!BATCH && f.guided)
+        if (!BATCH && f.guided && ls.shadeWaves >= 3)
+            RBH_TIMED(KernelTimer::SHADE, hipLaunchKernelGGL((k_shade<true, false, 3>), dim3(ls.shadeGrid),
+                                                             dim3(kShadeBlock), 0, c->stream, f, qsel));
+        else if (!BATCH && f.guided)
             RBH_TIMED(KernelTimer::SHADE, hipLaunchKernelGGL((k_shade<true, false, 1>), dim3(ls.shadeGrid),
                                                              dim3(kShadeBlock), 0, c->stream, f, qsel));
         else if (ls.shadeWaves >= 4)
