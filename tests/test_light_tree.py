@@ -106,3 +106,45 @@ def test_sample_pdf_consistency_and_frequencies():
     np.testing.assert_allclose(sp, pdf[ids], rtol=1e-5)
     freq = np.bincount(ids, minlength=n) / m
     np.testing.assert_allclose(freq, pdf, atol=3e-3)
+
+
+def _fp64_setup(n_lights=2000, n=20000, seed=4):
+    import light_tree64 as T
+    lights, flux = _lights(n_lights, 5)
+    nodes, lp, bm = O.light_tree_build(lights, flux)
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform([-60, -8, -120], [60, 25, -10], (n, 3)).astype(np.float32)
+    nrm = rng.normal(size=(n, 3)).astype(np.float32)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    pn = np.ascontiguousarray(np.concatenate([pos, nrm], 1), np.float32)
+    ids = rng.integers(0, len(lights), n).astype(np.uint32)
+    return T, nodes, lp, bm, pn, ids, len(lights)
+
+
+def fp32_error_summary(p32, p64):
+    """Relative error of FP32 light pdfs against FP64: median, 99th
+    percentile over pdfs >= 1e-4, and the mean weighted by the FP64 pdf (the
+    error a sampled light sees on average)."""
+    rel = np.abs(p32.astype(np.float64) - p64) / p64
+    big = p64 >= 1e-4
+    return float(np.median(rel)), float(np.quantile(rel[big], 0.99)), float(np.sum(p64 * rel) / np.sum(p64))
+
+
+def test_fp64_light_pdf_partition_and_fp32_error():
+    # the FP64 restatement (exact erf/expm1) is a distribution over all lights
+    # at any shading point, and the FP32 oracle (the reference's formulation,
+    # erfc = 1 - erf from an FP32 fit) stays close to it where lights are
+    # likely; unlikely lights of a node much dimmer than its sibling carry
+    # large relative errors (cancellation in 1 - erf), harmless to the
+    # estimator, which samples and weights with the same FP32 pdf
+    T, nodes, lp, bm, pn, ids, nl = _fp64_setup()
+    all_ids = np.arange(nl, dtype=np.uint32)
+    for k in range(3):
+        total = T.pdf(nodes, bm, all_ids, np.tile(pn[k], (nl, 1))).sum()
+        assert abs(total - 1.0) < 1e-9, total
+    p32 = np.zeros(len(ids), np.float32)
+    O.lib().orc_light_tree_pdf(O.p(nodes), O.p(lp), O.p(bm), 0, O.p(ids), O.p(pn), O.p(p32), len(ids))
+    p64 = T.pdf(nodes, bm, ids, pn)
+    med, q99, wmean = fp32_error_summary(p32, p64)
+    print("fp32 oracle vs fp64: median %.2g, q99 (pdf >= 1e-4) %.2g, pdf-weighted mean %.2g" % (med, q99, wmean))
+    assert med < 1e-5 and q99 < 5e-3 and wmean < 2e-4

@@ -122,6 +122,29 @@ def test_light_tree_pdf_and_sample(ctx, spheres_ctx):
     np.testing.assert_allclose(sp[same], osp[same], rtol=1e-5, atol=1e-30)
 
 
+def test_light_tree_pdf_against_fp64(ctx, spheres_ctx):
+    # the GPU's FP32 pdfs against the FP64 restatement (exact erf/expm1): the
+    # same error profile as the FP32 oracle (tests/test_light_tree.py)
+    import light_tree64 as T
+    from test_light_tree import fp32_error_summary
+    sc, r = spheres_ctx
+    nodes, lp, bm = r.light_tree()
+    rng = np.random.default_rng(14)
+    n = 20000
+    pos = rng.uniform([-50, -5, -100], [50, 20, -10], (n, 3)).astype(np.float32)
+    nrm = rng.normal(size=(n, 3)).astype(np.float32)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    pn = np.ascontiguousarray(np.concatenate([pos, nrm], 1), np.float32)
+    lights = rng.integers(0, len(sc.lights), n).astype(np.uint32)
+    pdf = np.zeros(n, np.float32)
+    _call(ctx, "rbh_test_light_pdf", L.ptr(lights), L.ptr(pn), L.ptr(pdf), n)
+    p64 = T.pdf(nodes, bm, lights, pn, env=sc.env is not None)
+    med, q99, wmean = fp32_error_summary(pdf, p64)
+    print("gpu light pdf vs fp64: median %.2g, q99 (pdf >= 1e-4) %.2g, pdf-weighted mean %.2g" % (med, q99, wmean),
+          flush=True)
+    assert med < 1e-5 and q99 < 5e-3 and wmean < 2e-4
+
+
 def _bxdf_records(sc, n, rng):
     rec = np.zeros((n, 64), np.float32)
     u = rec.view(np.uint32)
