@@ -160,29 +160,42 @@ __global__ void k_reset_hits(uint32_t* __restrict__ counters) {
     if (threadIdx.x == 2) counters[CNT_TRACE_HEAD] = 0u;
 }
 
-// Work fetch of the persistent trace and shadow kernels: when at least
-// kRefill lanes of a wave are idle (or all of them), the wave claims that
-// many queue entries with one atomic on *head; an idle lane whose entry lies
-// beyond count has drained the queue. Returns the claimed entry or ~0.
+// Work fetch of the persistent trace and shadow kernels. A wave claims
+// chunks of kChunk consecutive queue entries with one atomic on *head and
+// hands them to its idle lanes whenever at least kRefill lanes are idle (or
+// all of them), so a lane takes a new ray as soon as its query completes
+// while the wave's rays stay a contiguous, coherent stretch of the queue.
 constexpr int kRefill = 16;
-__device__ __forceinline__ uint32_t wave_fetch(uint32_t* __restrict__ head, uint32_t count, bool idle, bool anyBusy,
-                                               bool& exhausted) {
-    const unsigned long long mask = __ballot(idle);
-    const uint32_t n = (uint32_t)__popcll(mask);
-    if (n == 0 || (anyBusy && n < (uint32_t)kRefill)) return 0xFFFFFFFFu;
-    const int lane = __lane_id();
-    const int leader = __ffsll((long long)mask) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(head, n);
-    base = __shfl(base, leader);
-    if (!idle) return 0xFFFFFFFFu;
-    const uint32_t idx = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-    if (idx >= count) {
-        exhausted = true;
-        return 0xFFFFFFFFu;
+constexpr uint32_t kChunk = 256;
+struct WaveChunk {
+    uint32_t next = 0, end = 0;   // wave-uniform: unclaimed entries [next, end) of the wave's chunk
+    bool drained = false;         // the queue has no further chunk
+    // Returns this lane's new entry or ~0. Every lane of the wave calls it.
+    __device__ __forceinline__ uint32_t fetch(uint32_t* __restrict__ head, uint32_t count, bool idle, bool anyBusy) {
+        const unsigned long long mask = __ballot(idle);
+        const uint32_t n = (uint32_t)__popcll(mask);
+        if (n == 0 || (anyBusy && n < (uint32_t)kRefill)) return 0xFFFFFFFFu;
+        if (next >= end && !drained) {
+            const int lane = __lane_id();
+            const int leader = __ffsll((long long)mask) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(head, kChunk);
+            base = __shfl(base, leader);
+            if (base >= count) {
+                drained = true;
+            } else {
+                next = base;
+                end = min(base + kChunk, count);
+            }
+        }
+        if (next >= end) return 0xFFFFFFFFu;
+        const uint32_t idx = next + (uint32_t)__popcll(mask & ((1ull << __lane_id()) - 1ull));
+        next = min(next + n, end);
+        return idle && idx < end ? idx : 0xFFFFFFFFu;
     }
-    return idx;
-}
+    // no lane active, nothing left to hand out
+    __device__ __forceinline__ bool done(bool anyActive) const { return !anyActive && drained && next >= end; }
+};
 
 // Guided renders that collect training data: the radiance of the path just
 // before the vertex its last recorded direction leads to.
@@ -297,12 +310,13 @@ __global__ __launch_bounds__(kTraceBlock, WAVES) void k_trace(Frame1 f, uint32_t
         const Bvh4Node* nodes = nullptr;
         const TriRecord* tris = nullptr;
         uint32_t pid = 0;
-        bool active = false, exhausted = false;
+        bool active = false;
+        WaveChunk chunk;
         const uint32_t wave = threadIdx.x >> 6;
         WaveList hits{staged + wave * 256u, 0u}, misses{staged + wave * 256u + 128u, 0u};
         while (true) {
             const bool anyBusy = __ballot(active) != 0ull;
-            const uint32_t idx = wave_fetch(f.counters + CNT_TRACE_HEAD, count, !active && !exhausted, anyBusy, exhausted);
+            const uint32_t idx = chunk.fetch(f.counters + CNT_TRACE_HEAD, count, !active, anyBusy);
             if (idx != 0xFFFFFFFFu) {
                 pid = identity ? idx : queue[idx];
                 const SceneView& S = scene_of<BATCH>(f, pid);
@@ -313,7 +327,7 @@ __global__ __launch_bounds__(kTraceBlock, WAVES) void k_trace(Frame1 f, uint32_t
                 active = true;
                 ++nrays;
             }
-            if (__ballot(active) == 0ull) break;   // every lane found the queue drained
+            if (chunk.done(__ballot(active) != 0ull)) break;
             bool done = false;
             if (active) done = trav_step<false>(nodes, tris, ray, st);
             const bool hit = done && ray.h.prim != 0xFFFFFFFFu;
@@ -419,10 +433,11 @@ __global__ __launch_bounds__(kTraceBlock, WAVES) void k_shadow(Frame1 f) {
         const Bvh4Node* nodes = nullptr;
         const TriRecord* tris = nullptr;
         uint32_t pid = 0;
-        bool active = false, exhausted = false;
+        bool active = false;
+        WaveChunk chunk;
         while (true) {
             const bool anyBusy = __ballot(active) != 0ull;
-            const uint32_t idx = wave_fetch(f.counters + CNT_SHADOW_HEAD, count, !active && !exhausted, anyBusy, exhausted);
+            const uint32_t idx = chunk.fetch(f.counters + CNT_SHADOW_HEAD, count, !active, anyBusy);
             if (idx != 0xFFFFFFFFu) {
                 pid = f.p.sq[idx];
                 const SceneView& S = scene_of<BATCH>(f, pid);
@@ -433,7 +448,7 @@ __global__ __launch_bounds__(kTraceBlock, WAVES) void k_shadow(Frame1 f) {
                 active = true;
                 ++nrays;
             }
-            if (__ballot(active) == 0ull) break;
+            if (chunk.done(__ballot(active) != 0ull)) break;
             if (active && trav_step<true>(nodes, tris, ray, st)) {
                 if (ray.h.prim == 0xFFFFFFFFu) add_nee(f, pid);
                 active = false;
@@ -978,36 +993,35 @@ __global__ __launch_bounds__(256) void k_accumulate(Frame1 f) {
 // Spill entries per lane of the traversal stack beyond the LDS part.
 size_t trace_spill_words(size_t lanes) { return lanes * (size_t)(kMaxStack4 - kStackLds); }
 
-template <bool BATCH, bool QN>
+template <bool BATCH, bool FETCH, bool QN>
 static void launch_trace_q(rbh_ctx c, const Frame1& f, uint32_t qsel, const LaunchShape& ls) {
     const dim3 g(ls.traceGrid), b(kTraceBlock);
-    if (ls.traceWaves >= 8) hipLaunchKernelGGL((k_trace<BATCH, false, 8, QN>), g, b, 0, c->stream, f, qsel);
-    else if (ls.traceWaves == 7) hipLaunchKernelGGL((k_trace<BATCH, false, 7, QN>), g, b, 0, c->stream, f, qsel);
-    else hipLaunchKernelGGL((k_trace<BATCH, false, 1, QN>), g, b, 0, c->stream, f, qsel);
+    if (ls.traceWaves >= 8) hipLaunchKernelGGL((k_trace<BATCH, FETCH, 8, QN>), g, b, 0, c->stream, f, qsel);
+    else if (ls.traceWaves == 7) hipLaunchKernelGGL((k_trace<BATCH, FETCH, 7, QN>), g, b, 0, c->stream, f, qsel);
+    else hipLaunchKernelGGL((k_trace<BATCH, FETCH, 1, QN>), g, b, 0, c->stream, f, qsel);
 }
 
+// the work-fetch variants read the 128-byte nodes
 template <bool BATCH>
 static void launch_trace(rbh_ctx c, const Frame1& f, uint32_t qsel, const LaunchShape& ls) {
-    if (ls.traceFetch)
-        hipLaunchKernelGGL((k_trace<BATCH, true, 1, false>), dim3(ls.traceGrid), dim3(kTraceBlock), 0, c->stream, f, qsel);
-    else if (ls.quantNodes) launch_trace_q<BATCH, true>(c, f, qsel, ls);
-    else launch_trace_q<BATCH, false>(c, f, qsel, ls);
+    if (ls.traceFetch) launch_trace_q<BATCH, true, false>(c, f, qsel, ls);
+    else if (ls.quantNodes) launch_trace_q<BATCH, false, true>(c, f, qsel, ls);
+    else launch_trace_q<BATCH, false, false>(c, f, qsel, ls);
 }
 
-template <bool BATCH, bool QN>
+template <bool BATCH, bool FETCH, bool QN>
 static void launch_shadow_q(rbh_ctx c, const Frame1& f, const LaunchShape& ls) {
     const dim3 g(ls.traceGrid), b(kTraceBlock);
-    if (ls.traceWaves >= 8) hipLaunchKernelGGL((k_shadow<BATCH, false, 8, QN>), g, b, 0, c->stream, f);
-    else if (ls.traceWaves == 7) hipLaunchKernelGGL((k_shadow<BATCH, false, 7, QN>), g, b, 0, c->stream, f);
-    else hipLaunchKernelGGL((k_shadow<BATCH, false, 1, QN>), g, b, 0, c->stream, f);
+    if (ls.traceWaves >= 8) hipLaunchKernelGGL((k_shadow<BATCH, FETCH, 8, QN>), g, b, 0, c->stream, f);
+    else if (ls.traceWaves == 7) hipLaunchKernelGGL((k_shadow<BATCH, FETCH, 7, QN>), g, b, 0, c->stream, f);
+    else hipLaunchKernelGGL((k_shadow<BATCH, FETCH, 1, QN>), g, b, 0, c->stream, f);
 }
 
 template <bool BATCH>
 static void launch_shadow(rbh_ctx c, const Frame1& f, const LaunchShape& ls) {
-    if (ls.traceFetch)
-        hipLaunchKernelGGL((k_shadow<BATCH, true, 1, false>), dim3(ls.traceGrid), dim3(kTraceBlock), 0, c->stream, f);
-    else if (ls.quantNodes) launch_shadow_q<BATCH, true>(c, f, ls);
-    else launch_shadow_q<BATCH, false>(c, f, ls);
+    if (ls.traceFetch) launch_shadow_q<BATCH, true, false>(c, f, ls);
+    else if (ls.quantNodes) launch_shadow_q<BATCH, false, true>(c, f, ls);
+    else launch_shadow_q<BATCH, false, false>(c, f, ls);
 }
 
 // One wavefront pass: camera rays for every slot, depthMax bounces of
