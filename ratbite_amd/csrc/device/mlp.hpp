@@ -1,7 +1,7 @@
-// Shared device code of the guiding MLP: LDS staging, input encoding and the
-// BF16 MFMA forward pass (v_mfma_f32_16x16x32_bf16; rows = records, columns =
-// features). Used by the training/inference kernels and by the renderer's
-// per-vertex inference.
+// Shared device code of the guiding MLP: LDS staging, input encoding, the
+// block-tiled BF16 MFMA forward pass of the training kernel
+// (v_mfma_f32_16x16x32_bf16; rows = records, columns = features) and the FP8
+// activation scale. Inference runs the wave-resident form in mlp_wave.hpp.
 #pragma once
 #include <hip/hip_runtime.h>
 #include "../guide.hpp"
@@ -110,9 +110,9 @@ __device__ __forceinline__ void forward_rows(Lds& L, const bf16* __restrict__ wf
 
 // ----------------------------------------------------------------- FP8
 // E4M3 inference with per-tensor scales: weights carry one scale per layer,
-// activations one scale per layer and wave (the largest magnitude of the
-// wave's 16 rows maps to kFp8Max). kFp8Max stays within the finite range of
-// both E4M3 encodings (240 and 448).
+// activations one scale per layer and 16-record tile (the largest magnitude
+// of the tile's rows maps to kFp8Max). kFp8Max stays within the finite range
+// of both E4M3 encodings (240 and 448).
 constexpr float kFp8Max = 240.0f;
 
 __device__ __forceinline__ float wave_amax(const bf16* X, int sx, int K, int row0) {
@@ -122,76 +122,6 @@ __device__ __forceinline__ float wave_amax(const bf16* X, int sx, int K, int row
     for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
     return m;
 }
-
-template <int K, int N, bool RELU, bool F32OUT>
-__device__ __forceinline__ void fwd_layer_fp8(const bf16* X, int sx, const uint8_t* __restrict__ Wq, float wscale,
-                                              const float* __restrict__ bias, void* Y, int sy, int row0) {
-    __syncthreads();   // X is complete (written by other lanes); every thread of the block calls this
-    const int lane = __lane_id(), r = lane & 15, g = lane >> 4;
-    const float amax = wave_amax(X, sx, K, row0);
-    const float xs = amax > 0.0f ? amax / kFp8Max : 1.0f;
-    const float inv = 1.0f / xs;
-    const float outScale = xs * wscale;
-#pragma unroll 1
-    for (int ct = 0; ct < N / 16; ++ct) {
-        f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int kb = 0; kb < K / 32; ++kb) {
-            const bf16x8 a16 = *reinterpret_cast<const bf16x8*>(X + (row0 + r) * sx + kb * 32 + 8 * g);
-            int lo = __builtin_amdgcn_cvt_pk_fp8_f32((float)a16[0] * inv, (float)a16[1] * inv, 0, false);
-            lo = __builtin_amdgcn_cvt_pk_fp8_f32((float)a16[2] * inv, (float)a16[3] * inv, lo, true);
-            int hi = __builtin_amdgcn_cvt_pk_fp8_f32((float)a16[4] * inv, (float)a16[5] * inv, 0, false);
-            hi = __builtin_amdgcn_cvt_pk_fp8_f32((float)a16[6] * inv, (float)a16[7] * inv, hi, true);
-            const long a = (long)(uint32_t)lo | ((long)(uint32_t)hi << 32);
-            const long b = *reinterpret_cast<const long*>(Wq + (ct * 16 + r) * K + kb * 32 + 8 * g);
-            c = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a, b, c, 0, 0, 0);
-        }
-        const int col = ct * 16 + r;
-        const float bb = bias[col];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int row = row0 + 4 * g + i;
-            float v = c[i] * outScale + bb;
-            if (RELU) v = fmaxf(v, 0.0f);
-            if (F32OUT) {
-                if (col < kOut) reinterpret_cast<float*>(Y)[row * sy + col] = v;
-            } else {
-                reinterpret_cast<bf16*>(Y)[row * sy + col] = (bf16)v;
-            }
-        }
-    }
-}
-
-// forward pass with FP8 weights (wq: [n_pad][k_pad] bytes per layer at the
-// wf_off offsets) and per-layer weight scales ws[5]
-__device__ __forceinline__ void forward_rows_fp8(const bf16* x0, bf16* h0, bf16* h1, float* out,
-                                                 const uint8_t* __restrict__ wq, const float* __restrict__ ws,
-                                                 const float* __restrict__ bias, int row0) {
-    fwd_layer_fp8<96, 64, true, false>(x0, kSX0, wq + wf_off(0), ws[0], bias + bp_off(0), h0, kSH, row0);
-    fwd_layer_fp8<64, 64, true, false>(h0, kSH, wq + wf_off(1), ws[1], bias + bp_off(1), h1, kSH, row0);
-    fwd_layer_fp8<64, 64, true, false>(h1, kSH, wq + wf_off(2), ws[2], bias + bp_off(2), h0, kSH, row0);
-    fwd_layer_fp8<64, 64, true, false>(h0, kSH, wq + wf_off(3), ws[3], bias + bp_off(3), h1, kSH, row0);
-    fwd_layer_fp8<64, 64, false, true>(h1, kSH, wq + wf_off(4), ws[4], bias + bp_off(4), out, kSO, row0);
-}
-
-// LDS of the forward pass alone (hidden activations ping-pong)
-struct InferLds {
-    bf16 x0[kTile * kSX0];
-    bf16 h[2][kTile * kSH];
-    float out[kTile * kSO];
-};
-
-__device__ __forceinline__ void forward_rows(InferLds& L, const bf16* __restrict__ wf, const float* __restrict__ bias,
-                                             int row0) {
-    forward_rows(L.x0, L.h[0], L.h[1], L.h[0], L.h[1], L.out, wf, bias, row0);
-}
-
-__device__ __forceinline__ void forward_rows_fp8(InferLds& L, const uint8_t* __restrict__ wq,
-                                                 const float* __restrict__ ws, const float* __restrict__ bias,
-                                                 int row0) {
-    forward_rows_fp8(L.x0, L.h[0], L.h[1], L.out, wq, ws, bias, row0);
-}
-
 
 }  // namespace mlp
 }  // namespace rbh

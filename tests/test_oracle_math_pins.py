@@ -302,3 +302,14 @@ def test_refraction_obeys_snell():
     np.testing.assert_allclose(np.linalg.norm(t[ok, :2], axis=1), sin_i[ok] / eta[ok], atol=3e-5)
     # transmitted to the other side, tangential direction reversed with the incident vector convention
     assert np.all(np.sign(t[ok, 2]) == -np.sign(v[ok, 2]))
+
+
+def test_power_heuristic_and_its_overflow_guard():
+    # f^2 / (f^2 + g^2) (src/device/mis.h:14-21); an infinite f^2 (an edge-on
+    # light sample has an infinite solid-angle pdf) gives 1, not inf/inf
+    f = np.array([1.0, 2.0, 0.5, 1e20, np.inf, 3.0, 0.0], np.float32)
+    g = np.array([1.0, 1.0, 2.0, 1.0, 5.0, np.inf, 1.0], np.float32)
+    w = mu(17, _rec(len(f), c0=f, c1=g))[:, 0]
+    ref = np.array([0.5, 0.8, 1.0 / 17.0, 1.0, 1.0, 0.0, 0.0])
+    np.testing.assert_allclose(w, ref, rtol=1e-6)
+    assert np.all(np.isfinite(w))
