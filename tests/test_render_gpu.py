@@ -110,3 +110,31 @@ def test_resolve_matches_oracle(ctx):
     d = np.abs(img.astype(int) - ref.astype(int))
     assert d.max() <= 1
     assert (d == 0).mean() > 0.99
+
+
+def test_full_size_frame_invariants(ctx):
+    # the 1080p workload itself (200 level-5 icospheres, 10k textured emissive
+    # triangles, 2048x1024 env map, depth 16) at 8 spp: size-independent
+    # properties that must hold at full size - one launch equals two
+    # progressive launches and a 3-samples-per-pass budget bit for bit, every
+    # pixel holds exactly its samples, and the image is finite
+    sc = scenes.spheres(spp=8)
+    r = Renderer(sc, ctx=ctx)
+    st = r.render(spp=8, offset=0)
+    a = r.accum()
+    assert np.all(a[..., 3] == 8)
+    assert np.all(np.isfinite(a)) and st["nonFiniteSamples"] == 0
+    r.render(spp=3, offset=0)
+    r.render(spp=5, offset=3)
+    assert np.array_equal(a, r.accum())
+    try:
+        ctx.set_path_slots(3 * sc.width * sc.height)
+        r.render(spp=8, offset=0)
+        assert np.array_equal(a, r.accum())
+    finally:
+        ctx.set_path_slots(0)
+    # rays per sample in the expected range for this scene (most primary rays
+    # see the sky; paths average about 1.4 closest-hit queries)
+    rays_per_sample = st["closestRays"] / (sc.width * sc.height * 8)
+    print("1080p: %.3f closest-hit queries per sample, %d shadow rays" % (rays_per_sample, st["shadowRays"]), flush=True)
+    assert 1.0 <= rays_per_sample <= 3.0
