@@ -179,11 +179,21 @@ int rbh_build_accel(rbh_ctx ctx);
  * on the host from n emissive triangles with their flux, uploads nodes,
  * light pointers, bitmasks and the triangle records. */
 int rbh_build_light_tree(rbh_ctx ctx, const rbh_emissive_tri* tris, const float* flux, uint32_t n);
+enum {
+    RBH_LIGHT_TREE_FP64 = 1,      /* FP64 power mode: flux sums, attribute merges and split costs in double */
+};
+/* rbh_build_light_tree with flags (RBH_LIGHT_TREE_*); flags 0 is the reference's
+ * FP32 builder. In the FP64 mode each node's flux is the float rounding of a
+ * double sum of its lights' fluxes; the topology may differ from FP32 mode. */
+int rbh_build_light_tree_ex(rbh_ctx ctx, const rbh_emissive_tri* tris, const float* flux, uint32_t n,
+                            uint32_t flags);
 /* The same builder without a context or device (host only): fills nodes
  * (capacity 2n), lightPointers (n) and bitmasks (n); returns the node count
  * or a negative RBH_ERR_* code. */
 int rbh_light_tree_build_host(const rbh_emissive_tri* tris, const float* flux, uint32_t n, rbh_packed_node* nodes,
                               uint32_t* lightPointers, uint64_t* bitmasks);
+int rbh_light_tree_build_host_ex(const rbh_emissive_tri* tris, const float* flux, uint32_t n, rbh_packed_node* nodes,
+                                 uint32_t* lightPointers, uint64_t* bitmasks, uint32_t flags);
 /* Read the built tree back: nodes (<= 2n), lightPointers (n), bitmasks (n). */
 int rbh_get_light_tree(rbh_ctx ctx, rbh_packed_node* nodes, uint32_t* nnodes, uint32_t* lightPointers,
                        uint64_t* bitmasks);
@@ -231,6 +241,7 @@ typedef struct rbh_tuning {
     int32_t traceFetch;           /* 1: a trace/shadow lane takes a new ray as soon as its query completes; 0: waves take 64 rays at a time */
     int32_t traceWaves;           /* trace/shadow register budget (wave-synchronous kernels): 1 (unconstrained), 7 or 8 waves per SIMD */
     int32_t quantNodes;           /* 1: the wave-synchronous trace/shadow kernels read the four-wide nodes quantised to 64 bytes */
+    int32_t sortHits;             /* 1: material-sorted shading (conductor hits queued apart from the rest; wave-synchronous trace only) */
     double samplesPerSecond;      /* measured by rbh_autotune */
 } rbh_tuning;
 /* Times trialSpp-sample renders of lp over a grid of slot budgets and grid

@@ -126,11 +126,12 @@ int cdf_build_f32(const float* texels, int w, int h, int channels, float* cdf_x,
     return 0;
 }
 
-// FP64 reference of the same distribution (sequential sums in double), used
-// to state the FP32 build's error. cdf values are returned as double.
+// FP64 reference of the same distribution (sequential sums in double, the
+// row weight sin(pi (y + 1/2) / H) of cdf.cu:112-132 evaluated in double),
+// used to pin the FP64 mode and to state the FP32 build's error. cdf values
+// are returned as double.
 int cdf_build_f64(const float* texels, int w, int h, int channels, double* cdf_x, double* cdf_y, double* integral) {
     if (w % 4 || h % 4) return -1;
-    const int numElY = h / 4;
     for (int y = 0; y < h; ++y) {
         double s = 0.0;
         for (int x = 0; x < w; ++x) {
@@ -143,7 +144,7 @@ int cdf_build_f64(const float* texels, int w, int h, int channels, double* cdf_x
     }
     double s = 0.0;
     for (int y = 0; y < h; ++y) {
-        s += cdf_y[y] * (double)cdf_row_weight(y / 4, y % 4, numElY);
+        s += cdf_y[y] * std::sin(3.14159265358979323846 * ((double)y + 0.5) / (double)h);
         cdf_y[y] = s;
     }
     double n = s != 0.0 ? 1.0 / s : 0.0;

@@ -31,6 +31,7 @@ RBH_RENDER_GUIDE_FP8 = 4
 RBH_RENDER_TIMING = 8
 RBH_RENDER_NO_DROP = 16
 RBH_CDF_FP64 = 1
+RBH_LIGHT_TREE_FP64 = 1
 
 # Record layouts (byte-identical to include/ratbite_hip.h)
 MATERIAL_DTYPE = np.dtype([
@@ -86,7 +87,7 @@ class Stats(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("pathSlots", c_uint64), ("traceBlocksPerCU", ctypes.c_int32), ("shadeBlocksPerCU", ctypes.c_int32),
                 ("useGraph", ctypes.c_int32), ("shadeWaves", ctypes.c_int32), ("traceFetch", ctypes.c_int32), ("traceWaves", ctypes.c_int32),
-                ("quantNodes", ctypes.c_int32),
+                ("quantNodes", ctypes.c_int32), ("sortHits", ctypes.c_int32),
                 ("samplesPerSecond", c_double)]
 
 
@@ -116,7 +117,9 @@ _SIGS = {
     "rbh_upload_geometry": (c_int, [c_void_p, c_void_p, c_uint32, c_void_p, c_uint32, c_void_p, c_uint64, c_void_p, c_void_p]),
     "rbh_build_accel": (c_int, [c_void_p]),
     "rbh_build_light_tree": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32]),
+    "rbh_build_light_tree_ex": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_uint32]),
     "rbh_light_tree_build_host": (c_int, [c_void_p, c_void_p, c_uint32, c_void_p, c_void_p, c_void_p]),
+    "rbh_light_tree_build_host_ex": (c_int, [c_void_p, c_void_p, c_uint32, c_void_p, c_void_p, c_void_p, c_uint32]),
     "rbh_get_light_tree": (c_int, [c_void_p, c_void_p, ctypes.POINTER(c_uint32), c_void_p, c_void_p]),
     "rbh_envmap_build": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_uint32]),
     "rbh_envmap_rebuild": (c_int, [c_void_p, c_uint32, c_int]),

@@ -98,10 +98,11 @@ class Context:
     def build_accel(self):
         self.call("rbh_build_accel")
 
-    def build_light_tree(self, tris, flux):
+    def build_light_tree(self, tris, flux, fp64=False):
         tris = np.ascontiguousarray(tris, L.EMISSIVE_TRI_DTYPE)
         flux = np.ascontiguousarray(flux, np.float32)
-        self.call("rbh_build_light_tree", L.ptr(tris) if len(tris) else None, L.ptr(flux) if len(flux) else None, len(tris))
+        self.call("rbh_build_light_tree_ex", L.ptr(tris) if len(tris) else None, L.ptr(flux) if len(flux) else None,
+                  len(tris), L.RBH_LIGHT_TREE_FP64 if fp64 else 0)
 
     def get_light_tree(self, nlights):
         nodes = np.zeros(max(1, 2 * nlights), L.PACKED_NODE_DTYPE)

@@ -220,15 +220,16 @@ static int envmap_finish(rbh_ctx c, uint32_t flags) {
     float ms = 0.0f;
     RBH_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->stats.cdfMs = ms;
-    double raw = 0.0;
     if (flags & RBH_CDF_FP64) {
+        // the FP64 total divided in double; the renderer rounds it to float once
+        double raw = 0.0;
         RBH_HIP(c, hipMemcpy(&raw, c->env.integralDev.p, sizeof(double), hipMemcpyDeviceToHost));
+        c->env.integral = raw / ((double)c->env.width * (double)c->env.height);
     } else {
         float f = 0.0f;
         RBH_HIP(c, hipMemcpy(&f, c->env.integralDev.p, sizeof(float), hipMemcpyDeviceToHost));
-        raw = f;
+        c->env.integral = f / ((float)c->env.width * (float)c->env.height);
     }
-    c->env.integral = (float)raw / ((float)c->env.width * (float)c->env.height);
     c->env.enabled = true;
     return RBH_OK;
 }

@@ -41,7 +41,7 @@ struct DevBuf {
 struct EnvMap {
     bool enabled = false;
     int width = 0, height = 0, channels = 0;
-    float integral = 0.0f;       // host copy, already divided by w*h
+    double integral = 0.0;       // host copy, already divided by w*h (a float value in FP32 mode)
     DevBuf texels;               // w*h*channels floats
     DevBuf cond, marg;           // uint16 inverted-CDF indices
     DevBuf cdfY;                 // h floats (normalised marginal CDF)
@@ -137,6 +137,7 @@ struct RenderState {
     int traceFetch = 0;          // 1: trace/shadow lanes fetch a new ray as soon as theirs completes
     int traceWaves = 7;          // trace/shadow register budget (1, 7, 8 waves per SIMD)
     int quantNodes = 0;          // traverse the 64-byte quantised four-wide nodes
+    int sortHits = 1;            // shade conductor hits as their own run of the hit queue
     DevBuf sceneTable;           // batched passes: SceneView per scene and pass
     bool useGraph = false;       // replay each pass's launch sequence as a captured HIP graph
     hipGraphExec_t graphExec = nullptr;

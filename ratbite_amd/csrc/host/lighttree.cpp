@@ -11,6 +11,13 @@
 //
 // Compiled without floating-point contraction so its FP32 results match a
 // straightforward scalar evaluation bit for bit.
+//
+// The builder is a template over the type that carries the light attributes
+// (flux sums, flux-weighted means, variances, directions) and the split-cost
+// sums. float is the reference's arithmetic. double is the FP64 power mode:
+// the sums and the SAOH costs run in double, and branch attributes merge the
+// children's unrounded double attributes instead of the packed ones, so each
+// node's flux is the float rounding of a double sum of its lights' fluxes.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -25,7 +32,6 @@ namespace host {
 
 namespace {
 
-constexpr float kPi = 3.14159265358979323846f;
 constexpr uint32_t kMaxPerLeaf = 8, kBins = 32;
 
 struct Aabb {
@@ -43,38 +49,38 @@ inline void grow(Aabb& a, const Aabb& b) {
     }
 }
 inline float center(const Aabb& a, int k) { return a.lo[k] + (a.hi[k] - a.lo[k]) * 0.5f; }
-inline float area(const Aabb& a) {
-    const float x = a.hi[0] - a.lo[0], y = a.hi[1] - a.lo[1], z = a.hi[2] - a.lo[2];
-    return x * y * 2.0f + x * z * 2.0f + y * z * 2.0f;
-}
 inline bool valid(const Aabb& a) { return a.lo[0] <= a.hi[0] && a.lo[1] <= a.hi[1] && a.lo[2] <= a.hi[2]; }
 
 // vMF axis length <-> sharpness (math_util.h:562-587)
-inline float axis_to_sharpness(float l) { return l * (3.0f - l * l) / (1.0f - l * l); }
-inline float sharpness_to_axis(float s) {
-    const float a = s / 3.0f;
-    const float b = a * a * a;
-    const float c = std::sqrt(1.0f + 3.0f * (a * a) * (1.0f + a * a));
-    const float th = std::atan2(c, b) / 3.0f;
-    const float d = std::sin(th) * 1.7320508075688772935274463415059f - std::cos(th);
-    return (s > 0x1.0p25f) ? 1.0f : std::sqrt(1.0f + a * a) * d + a;
+template <typename T>
+inline T axis_to_sharpness(T l) { return l * (T(3) - l * l) / (T(1) - l * l); }
+template <typename T>
+inline T sharpness_to_axis(T s) {
+    const T a = s / T(3);
+    const T b = a * a * a;
+    const T c = std::sqrt(T(1) + T(3) * (a * a) * (T(1) + a * a));
+    const T th = std::atan2(c, b) / T(3);
+    const T d = std::sin(th) * T(1.7320508075688772935274463415059L) - std::cos(th);
+    return (s > T(0x1.0p25)) ? T(1) : std::sqrt(T(1) + a * a) * d + a;
 }
 
+template <typename T>
 struct Attr {
-    float mean[3], var, flux, dir[3], sharp;
+    T mean[3], var, flux, dir[3], sharp;
 };
 
 // light_tree_types.h:65-100
-Attr merge(const Attr& A, const Attr& B) {
-    Attr r{};
-    const float wa = A.flux / (A.flux + B.flux), wb = B.flux / (A.flux + B.flux);
-    const float la = sharpness_to_axis(A.sharp), lb = sharpness_to_axis(B.sharp);
-    float d[3];
+template <typename T>
+Attr<T> merge(const Attr<T>& A, const Attr<T>& B) {
+    Attr<T> r{};
+    const T wa = A.flux / (A.flux + B.flux), wb = B.flux / (A.flux + B.flux);
+    const T la = sharpness_to_axis(A.sharp), lb = sharpness_to_axis(B.sharp);
+    T d[3];
     for (int k = 0; k < 3; ++k) d[k] = wa * la * A.dir[k] + wb * lb * B.dir[k];
-    const float dl = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-    for (int k = 0; k < 3; ++k) r.dir[k] = dl > 0.0001f ? d[k] / dl : 0.0f;
-    const float m[3] = {B.mean[0] - A.mean[0], B.mean[1] - A.mean[1], B.mean[2] - A.mean[2]};
-    const float m2 = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
+    const T dl = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    for (int k = 0; k < 3; ++k) r.dir[k] = dl > T(0.0001f) ? d[k] / dl : T(0);
+    const T m[3] = {B.mean[0] - A.mean[0], B.mean[1] - A.mean[1], B.mean[2] - A.mean[2]};
+    const T m2 = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
     for (int k = 0; k < 3; ++k) r.mean[k] = wa * A.mean[k] + wb * B.mean[k];
     r.var = wa * A.var + wb * B.var + wa * wb * m2;
     r.flux = A.flux + B.flux;
@@ -114,13 +120,15 @@ struct Prim {
     uint32_t id;
 };
 
+template <typename T>
 struct Bin {
     Aabb box = aabb_zero();
     uint32_t n = 0;
-    float flux = 0.0f;
-    float cone[3] = {0.0f, 0.0f, 0.0f};
-    float cosCone = 1.0f;
-    void add(const Aabb& b, uint32_t c, float f, const float* d) {
+    T flux = T(0);
+    T cone[3] = {T(0), T(0), T(0)};
+    T cosCone = T(1);
+    template <typename U>
+    void add(const Aabb& b, uint32_t c, T f, const U* d) {
         grow(box, b);
         n += c;
         flux += f;
@@ -129,116 +137,131 @@ struct Bin {
 };
 
 // light_tree.cpp:167-185
-float cone_union(const float* da, float ca, const float* db, float cb) {
-    if (ca == -1.0f || cb == -1.0f) return -1.0f;
-    const float cd = da[0] * db[0] + da[1] * db[1] + da[2] * db[2];
-    const float sd = std::sqrt(1.0f - cd * cd);
-    const float sb = std::sqrt(1.0f - cb * cb);
-    const float ct = cb * cd - sb * sd;
-    const float st = sb * cd + cb * sd;
-    return st > 0.0f ? std::min(cb, ct) : -1.0f;
+template <typename T, typename U>
+T cone_union(const T* da, T ca, const U* db, T cb) {
+    if (ca == T(-1) || cb == T(-1)) return T(-1);
+    const T cd = da[0] * db[0] + da[1] * db[1] + da[2] * db[2];
+    const T sd = std::sqrt(T(1) - cd * cd);
+    const T sb = std::sqrt(T(1) - cb * cb);
+    const T ct = cb * cd - sb * sd;
+    const T st = sb * cd + cb * sd;
+    return st > T(0) ? std::min(cb, ct) : T(-1);
 }
 // light_tree.cpp:187-199 (surface area x orientation measure x flux)
-float saoh(const Aabb& b, float flux, float cosT) {
-    const float ac = valid(b) ? area(b) : 0.0f;
-    const float tO = cosT != -1.0f ? std::acos(std::max(std::min(cosT, 1.0f), -1.0f)) : kPi;
-    const float tW = std::min(tO + kPi / 2.0f, kPi);
-    const float sO = std::sin(tO), cO = std::cos(tO);
-    const float oc = 2.0f * kPi * (1.0f - cO) + kPi / 2.0f * (2.0f * tW * sO - std::cos(tO - 2.0f * tW) - 2.0f * tO * sO + cO);
+template <typename T>
+T saoh(const Aabb& b, T flux, T cosT) {
+    const T pi = T(3.14159265358979323846L);
+    T ac = T(0);
+    if (valid(b)) {
+        const T x = T(b.hi[0]) - T(b.lo[0]), y = T(b.hi[1]) - T(b.lo[1]), z = T(b.hi[2]) - T(b.lo[2]);
+        ac = x * y * T(2) + x * z * T(2) + y * z * T(2);
+    }
+    const T tO = cosT != T(-1) ? std::acos(std::max(std::min(cosT, T(1)), T(-1))) : pi;
+    const T tW = std::min(tO + pi / T(2), pi);
+    const T sO = std::sin(tO), cO = std::cos(tO);
+    const T oc = T(2) * pi * (T(1) - cO) + pi / T(2) * (T(2) * tW * sO - std::cos(tO - T(2) * tW) - T(2) * tO * sO + cO);
     return flux * ac * oc;
 }
 
+template <typename T>
 struct Builder {
+    static constexpr bool kExact = sizeof(T) > sizeof(float);
     const EmissiveTri* tris;
     std::vector<Prim> prims;
     std::vector<PackedNode> nodes;
+    std::vector<Attr<T>> leafAttrs;  // FP64 mode: unrounded leaf attributes by node index
     std::vector<uint32_t> pointers;
     std::vector<uint64_t> masks;
 
-    Attr leaf_attr(uint32_t id) const {
+    Attr<T> leaf_attr(uint32_t id) const {
         const float* v = tris[id].v;
-        const float e1[3] = {v[3] - v[0], v[4] - v[1], v[5] - v[2]};
-        const float e2[3] = {v[6] - v[0], v[7] - v[1], v[8] - v[2]};
-        float n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
-        const float inv = 1.0f / std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-        Attr a{};
+        const T e1[3] = {T(v[3]) - T(v[0]), T(v[4]) - T(v[1]), T(v[5]) - T(v[2])};
+        const T e2[3] = {T(v[6]) - T(v[0]), T(v[7]) - T(v[1]), T(v[8]) - T(v[2])};
+        T n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+        const T inv = T(1) / std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        Attr<T> a{};
         for (int k = 0; k < 3; ++k) {
-            a.mean[k] = (v[k] + v[3 + k] + v[6 + k]) / 3.0f;
+            a.mean[k] = (T(v[k]) + T(v[3 + k]) + T(v[6 + k])) / T(3);
             a.dir[k] = n[k] * inv;
         }
-        const float d11 = e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2];
-        const float d22 = e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2];
-        const float d12 = e1[0] * e2[0] + e1[1] * e2[1] + e1[2] * e2[2];
-        a.var = (d11 + d22 - d12) / 18.0f;
-        a.sharp = axis_to_sharpness(0.5f);
-        a.flux = 0.0f;
+        const T d11 = e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2];
+        const T d22 = e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2];
+        const T d12 = e1[0] * e2[0] + e1[1] * e2[1] + e1[2] * e2[2];
+        a.var = (d11 + d22 - d12) / T(18);
+        a.sharp = axis_to_sharpness(T(0.5));
+        a.flux = T(0);
         return a;
     }
 
-    static void pack(PackedNode& p, const Attr& a) {
-        for (int k = 0; k < 3; ++k) p.mean[k] = a.mean[k];
-        p.variance = a.var;
-        p.dir = oct_pack(a.dir);
-        p.sharpness = a.sharp;
-        p.flux = a.flux;
+    static void pack(PackedNode& p, const Attr<T>& a) {
+        float dir[3];
+        for (int k = 0; k < 3; ++k) {
+            p.mean[k] = (float)a.mean[k];
+            dir[k] = (float)a.dir[k];
+        }
+        p.variance = (float)a.var;
+        p.dir = oct_pack(dir);
+        p.sharpness = (float)a.sharp;
+        p.flux = (float)a.flux;
     }
 
     // light_tree.cpp:162-394; returns axis/index, or index = UINT32_MAX for none
     void split(uint32_t begin, uint32_t end, const Aabb& bounds, uint32_t& outAxis, uint32_t& outIndex) {
-        float best = std::numeric_limits<float>::max();
+        T best = std::numeric_limits<T>::max();
         outAxis = outIndex = UINT32_MAX;
         const float dims[3] = {bounds.hi[0] - bounds.lo[0], bounds.hi[1] - bounds.lo[1], bounds.hi[2] - bounds.lo[2]};
         const uint32_t largest = dims[0] > dims[1] && dims[0] > dims[2] ? 0u : (dims[1] > dims[2] ? 1u : 2u);
-        Bin bins[kBins];
-        float costs[kBins - 1];
+        Bin<T> bins[kBins];
+        T costs[kBins - 1];
         for (uint32_t ax = 0; ax < 3; ++ax) {
             const float w = bounds.hi[ax] - bounds.lo[ax];
             const float scale = w > std::numeric_limits<float>::min() ? (float)kBins / w : 0.0f;
             auto binOf = [&](const Prim& p) {
                 return std::min((uint32_t)((center(p.box, (int)ax) - bounds.lo[ax]) * scale), kBins - 1);
             };
-            for (auto& b : bins) b = Bin{};
-            for (uint32_t i = begin; i < end; ++i) bins[binOf(prims[i])].add(prims[i].box, 1, prims[i].flux, prims[i].cone);
+            for (auto& b : bins) b = Bin<T>{};
+            for (uint32_t i = begin; i < end; ++i)
+                bins[binOf(prims[i])].add(prims[i].box, 1, T(prims[i].flux), prims[i].cone);
             for (auto& b : bins) {
-                const float l = std::sqrt(b.cone[0] * b.cone[0] + b.cone[1] * b.cone[1] + b.cone[2] * b.cone[2]);
-                if (l < std::numeric_limits<float>::min()) {
-                    b.cosCone = -1.0f;
+                const T l = std::sqrt(b.cone[0] * b.cone[0] + b.cone[1] * b.cone[1] + b.cone[2] * b.cone[2]);
+                if (l < std::numeric_limits<T>::min()) {
+                    b.cosCone = T(-1);
                 } else {
-                    b.cosCone = 1.0f;
+                    b.cosCone = T(1);
                     b.cone[0] /= l; b.cone[1] /= l; b.cone[2] /= l;
                 }
             }
             for (uint32_t i = begin; i < end; ++i) {
-                Bin& b = bins[binOf(prims[i])];
-                b.cosCone = cone_union(b.cone, b.cosCone, prims[i].cone, prims[i].cosCone);
+                Bin<T>& b = bins[binOf(prims[i])];
+                b.cosCone = cone_union(b.cone, b.cosCone, prims[i].cone, T(prims[i].cosCone));
             }
-            auto sweep_cost = [&](const Bin& tot, uint32_t j0, uint32_t j1) {
-                float cosT = -1.0f;
-                const float l = std::sqrt(tot.cone[0] * tot.cone[0] + tot.cone[1] * tot.cone[1] + tot.cone[2] * tot.cone[2]);
-                if (l > std::numeric_limits<float>::min()) {
-                    cosT = 1.0f;
-                    const float cd[3] = {tot.cone[0] / l, tot.cone[1] / l, tot.cone[2] / l};
+            auto sweep_cost = [&](const Bin<T>& tot, uint32_t j0, uint32_t j1) {
+                T cosT = T(-1);
+                const T l = std::sqrt(tot.cone[0] * tot.cone[0] + tot.cone[1] * tot.cone[1] + tot.cone[2] * tot.cone[2]);
+                if (l > std::numeric_limits<T>::min()) {
+                    cosT = T(1);
+                    const T cd[3] = {tot.cone[0] / l, tot.cone[1] / l, tot.cone[2] / l};
                     for (uint32_t j = j0; j <= j1; ++j) cosT = cone_union(cd, cosT, bins[j].cone, bins[j].cosCone);
                 }
                 return saoh(tot.box, tot.flux, cosT);
             };
-            Bin tot;
+            Bin<T> tot;
             for (uint32_t i = 0; i < kBins - 1; ++i) {
                 tot.add(bins[i].box, bins[i].n, bins[i].flux, bins[i].cone);
                 costs[i] = sweep_cost(tot, 0, i);
             }
-            tot = Bin{};
+            tot = Bin<T>{};
             for (uint32_t i = kBins - 1; i > 0; --i) {
                 tot.add(bins[i].box, bins[i].n, bins[i].flux, bins[i].cone);
                 costs[i - 1] += sweep_cost(tot, i, kBins - 1);
             }
-            float axCost = std::numeric_limits<float>::max();
+            T axCost = std::numeric_limits<T>::max();
             uint32_t axIndex = 0, run = begin;
             for (uint32_t i = 0; i < kBins - 1; ++i) {
                 run += bins[i].n;
                 if (costs[i] < axCost) { axCost = costs[i]; axIndex = run; }
             }
-            axCost *= dims[largest] / dims[ax];
+            axCost *= T(dims[largest] / dims[ax]);
             if (axIndex == begin || axIndex == end) continue;
             if (axCost < best) { best = axCost; outAxis = ax; outIndex = axIndex; }
         }
@@ -265,14 +288,18 @@ struct Builder {
             nodes[me].core = ((1u << 31) - 1u) & right;
             return me;
         }
-        Attr a = leaf_attr(prims[begin].id);
-        a.flux = prims[begin].flux;
+        Attr<T> a = leaf_attr(prims[begin].id);
+        a.flux = T(prims[begin].flux);
         for (uint32_t i = begin + 1; i < end; ++i) {
-            Attr o = leaf_attr(prims[i].id);
-            o.flux = prims[i].flux;
+            Attr<T> o = leaf_attr(prims[i].id);
+            o.flux = T(prims[i].flux);
             a = merge(a, o);
         }
         pack(nodes[me], a);
+        if (kExact) {
+            if (leafAttrs.size() <= me) leafAttrs.resize(me + 1);
+            leafAttrs[me] = a;
+        }
         const uint32_t cnt = end - begin;
         nodes[me].core = (1u << 31) | ((cnt & 15u) << 27) | (begin & ((1u << 27) - 1u));
         for (uint32_t i = begin; i < end; ++i) {
@@ -282,38 +309,34 @@ struct Builder {
         return me;
     }
 
-    Attr unpacked(uint32_t i) const {
+    Attr<T> unpacked(uint32_t i) const {
         const PackedNode& p = nodes[i];
-        Attr a{};
+        Attr<T> a{};
         for (int k = 0; k < 3; ++k) a.mean[k] = p.mean[k];
         a.var = p.variance;
         a.flux = p.flux;
         a.sharp = p.sharpness;
-        oct_unpack_unit(p.dir, a.dir);
+        float dir[3];
+        oct_unpack_unit(p.dir, dir);
+        for (int k = 0; k < 3; ++k) a.dir[k] = dir[k];
         return a;
     }
-    // light_tree.cpp:396-418
-    Attr fill(uint32_t i) {
-        if ((nodes[i].core >> 31) != 0u) return unpacked(i);
-        const Attr l = fill(i + 1);
-        const Attr r = fill(nodes[i].core);
-        const Attr a = merge(l, r);
+    // light_tree.cpp:396-418 (float: merges the packed children as the
+    // reference does; double: merges the unrounded children)
+    Attr<T> fill(uint32_t i) {
+        if ((nodes[i].core >> 31) != 0u) return kExact ? leafAttrs[i] : unpacked(i);
+        const Attr<T> l = fill(i + 1);
+        const Attr<T> r = fill(nodes[i].core);
+        const Attr<T> a = merge(l, r);
         pack(nodes[i], a);
         return a;
     }
 };
 
-}  // namespace
-
-// Returns the node count, 0 for no lights, or a negative error.
-int build_light_tree(const EmissiveTri* tris, const float* flux, uint32_t n, std::vector<PackedNode>& nodes,
-                     std::vector<uint32_t>& pointers, std::vector<uint64_t>& masks) {
-    nodes.clear();
-    pointers.clear();
-    masks.clear();
-    if (n == 0) return 0;
-    if (n > (1u << 27) - 1u + kMaxPerLeaf) return -1;
-    Builder b;
+template <typename T>
+int build(const EmissiveTri* tris, const float* flux, uint32_t n, std::vector<PackedNode>& nodes,
+          std::vector<uint32_t>& pointers, std::vector<uint64_t>& masks) {
+    Builder<T> b;
     b.tris = tris;
     b.prims.resize(n);
     for (uint32_t i = 0; i < n; ++i) {
@@ -347,6 +370,21 @@ int build_light_tree(const EmissiveTri* tris, const float* flux, uint32_t n, std
     pointers = std::move(b.pointers);
     masks = std::move(b.masks);
     return (int)nodes.size();
+}
+
+}  // namespace
+
+// Returns the node count, 0 for no lights, or a negative error. fp64 selects
+// the FP64 power mode.
+int build_light_tree(const EmissiveTri* tris, const float* flux, uint32_t n, std::vector<PackedNode>& nodes,
+                     std::vector<uint32_t>& pointers, std::vector<uint64_t>& masks, bool fp64) {
+    nodes.clear();
+    pointers.clear();
+    masks.clear();
+    if (n == 0) return 0;
+    if (n > (1u << 27) - 1u + kMaxPerLeaf) return -1;
+    return fp64 ? build<double>(tris, flux, n, nodes, pointers, masks)
+                : build<float>(tris, flux, n, nodes, pointers, masks);
 }
 
 }  // namespace host

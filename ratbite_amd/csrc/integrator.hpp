@@ -12,6 +12,7 @@ constexpr int kShadeBlock = 128;
 enum : uint32_t {
     CNT_QUEUE0 = 0, CNT_QUEUE1 = 1, CNT_SHADOW = 2, CNT_HIT = 3, CNT_MISS = 4,
     CNT_TRACE_HEAD = 5, CNT_SHADOW_HEAD = 6,   // work-fetch positions of the trace and shadow queues
+    CNT_HIT_BACK = 7,                          // hits queued from the end of the hit queue (conductors)
     CNT_COUNT = 8
 };
 enum : uint32_t { STAT_CLOSEST = 0, STAT_SHADOW = 1, STAT_VERTICES = 2, STAT_NONFINITE = 3, STAT_COUNT = 4 };
@@ -32,7 +33,8 @@ struct PathBuffers {
     float4* srd;    // shadow ray direction
     uint32_t* q0;   // paths to trace (even / odd bounces)
     uint32_t* q1;
-    uint32_t* hq;   // paths whose closest hit found a triangle (to shade)
+    uint32_t* hq;   // paths whose closest hit found a triangle (to shade): CNT_HIT entries from the
+                    // front, CNT_HIT_BACK (conductor hits when sorting by material) from the end
     uint32_t* mq;   // paths whose ray left the scene
     uint32_t* sq;   // paths with a deferred NEE shadow ray
     uint32_t* prec; // guided renders that collect training data: last record of the path
@@ -88,6 +90,7 @@ struct LaunchShape {
     int traceFetch;          // trace/shadow lanes fetch a new ray when theirs completes (1) or wave by wave (0)
     int traceWaves;          // register budget of the wave-synchronous trace/shadow kernels: 1 (free), 7 or 8
     int quantNodes;          // wave-synchronous trace/shadow kernels read the 64-byte quantised nodes
+    int sortHits;            // queue conductor hits apart from the rest (material-sorted shading)
 };
 
 struct Frame1 {
@@ -97,6 +100,7 @@ struct Frame1 {
     uint32_t NP;                  // path slots of the pass (all scenes)
     uint32_t maxP;                // largest pixel count of a scene in the pass
     uint32_t bounce;
+    uint32_t sortHits;            // LaunchShape::sortHits
     // state
     PathBuffers p;
     int* spill;              // traversal-stack spill columns of the trace/shadow grid

@@ -145,6 +145,7 @@ __global__ __launch_bounds__(256) void k_camera(Frame1 f) {
         f.counters[CNT_MISS] = 0u;
         f.counters[CNT_TRACE_HEAD] = 0u;
         f.counters[CNT_SHADOW_HEAD] = 0u;
+        f.counters[CNT_HIT_BACK] = 0u;
     }
 }
 
@@ -158,6 +159,17 @@ __global__ void k_reset_hits(uint32_t* __restrict__ counters) {
     if (threadIdx.x == 0) counters[CNT_HIT] = 0u;
     if (threadIdx.x == 1) counters[CNT_MISS] = 0u;
     if (threadIdx.x == 2) counters[CNT_TRACE_HEAD] = 0u;
+    if (threadIdx.x == 3) counters[CNT_HIT_BACK] = 0u;
+}
+
+// Hit-queue entries in shading order: the CNT_HIT entries pushed from the
+// front, then the CNT_HIT_BACK entries pushed from the end of the NP-entry
+// queue (their total never exceeds the NP rays traced).
+__device__ __forceinline__ uint32_t hit_count(const Frame1& f) {
+    return f.counters[CNT_HIT] + f.counters[CNT_HIT_BACK];
+}
+__device__ __forceinline__ uint32_t hit_entry(const Frame1& f, uint32_t i, uint32_t nFront) {
+    return i < nFront ? f.p.hq[i] : f.p.hq[f.NP - 1u - (i - nFront)];
 }
 
 // Work fetch of the persistent trace and shadow kernels. A wave claims
@@ -342,7 +354,7 @@ __global__ __launch_bounds__(kTraceBlock, WAVES) void k_trace(Frame1 f, uint32_t
         misses.flush(f.counters + CNT_MISS, f.p.mq, 1u);
     } else {
         for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i - threadIdx.x < count; i += gridDim.x * blockDim.x) {
-            bool hit = false;
+            bool hit = false, back = false;
             uint32_t pid = 0;
             if (i < count) {
                 pid = identity ? i : queue[i];
@@ -351,12 +363,20 @@ __global__ __launch_bounds__(kTraceBlock, WAVES) void k_trace(Frame1 f, uint32_t
                 const Hit h = traverse4<false>(scene_nodes<QN>(S), S.tris, S.nnodes4, V3{od.x, od.y, od.z},
                                                V3{dr.x, dr.y, dr.z}, 3.402823466e+38f, st);
                 hit = h.prim != 0xFFFFFFFFu;
-                if (hit) f.p.hit[pid] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.prim));
+                if (hit) {
+                    f.p.hit[pid] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.prim));
+                    // material-sorted shading: conductor hits form their own run of the queue
+                    if (f.sortHits) back = S.materials[S.triMaterial[h.prim]].bxdf == 0u;
+                }
                 ++nrays;
             }
             const bool live = i < count;
-            const uint32_t slot = wave_push(f.counters + CNT_HIT, live && hit);
-            if (live && hit) f.p.hq[slot] = pid;
+            const uint32_t slot = wave_push(f.counters + CNT_HIT, live && hit && !back);
+            if (live && hit && !back) f.p.hq[slot] = pid;
+            if (f.sortHits) {
+                const uint32_t bslot = wave_push(f.counters + CNT_HIT_BACK, back);
+                if (back) f.p.hq[f.NP - 1u - bslot] = pid;
+            }
             const uint32_t mslot = wave_push(f.counters + CNT_MISS, live && !hit);
             if (live && !hit) f.p.mq[mslot] = pid;
         }
@@ -392,9 +412,9 @@ __device__ __forceinline__ void guide_input(const Frame1& f, const SceneView& S,
 // Network input of every queued hit, one thread per hit (the gathers run
 // fully parallel), kept for the training records of the vertex.
 __global__ __launch_bounds__(256) void k_guide_features(Frame1 f) {
-    const uint32_t count = f.counters[CNT_HIT];
+    const uint32_t count = hit_count(f), nFront = f.counters[CNT_HIT];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
-        guide_input(f, f.sc, f.p.hq[i], f.gfeat + (size_t)i * guide::kRec);
+        guide_input(f, f.sc, hit_entry(f, i, nFront), f.gfeat + (size_t)i * guide::kRec);
 }
 
 // Mixture of every queued hit: wave-resident MLP inference over the
@@ -402,7 +422,7 @@ __global__ __launch_bounds__(256) void k_guide_features(Frame1 f) {
 template <bool FP8>
 __global__ __launch_bounds__(256, 2) void k_guide_mlp(Frame1 f) {
     __shared__ mlp::WaveLds L[4];
-    mlp::wave_infer<FP8, true>(L[threadIdx.x >> 6], f.gfeat, f.counters[CNT_HIT], f.gwf, f.gwq, f.gws, f.gbias, f.gmix);
+    mlp::wave_infer<FP8, true>(L[threadIdx.x >> 6], f.gfeat, hit_count(f), f.gwf, f.gwq, f.gws, f.gbias, f.gmix);
 }
 
 template <bool BATCH>
@@ -503,8 +523,7 @@ __device__ __forceinline__ void quat_cols(float4 q, V3& c0, V3& c2) {
 template <bool GUIDED, bool BATCH, int WAVES>
 __global__ __launch_bounds__(kShadeBlock, WAVES) void k_shade(Frame1 f, uint32_t qsel) {
     // shades the paths whose ray hit a triangle (misses were finished by k_trace)
-    const uint32_t count = f.counters[CNT_HIT];
-    const uint32_t* __restrict__ qin = f.p.hq;
+    const uint32_t count = hit_count(f), nFront = f.counters[CNT_HIT];
     uint32_t* __restrict__ qout = qsel ? f.p.q0 : f.p.q1;
     uint32_t* __restrict__ cout = f.counters + (qsel ? CNT_QUEUE0 : CNT_QUEUE1);
     uint32_t nvert = 0;
@@ -516,7 +535,7 @@ __global__ __launch_bounds__(kShadeBlock, WAVES) void k_shade(Frame1 f, uint32_t
         S4 recT = s4(0.0f);           // throughput before the recorded vertex
         uint32_t pid = 0;
         if (i < count) {
-            pid = qin[i];
+            pid = hit_entry(f, i, nFront);
             ++nvert;
             const SceneView& S = scene_of<BATCH>(f, pid);
             const ColorTables ct{S.colors};
@@ -1046,6 +1065,7 @@ static hipError_t launch_pass_t(rbh_ctx c, Frame1 f, uint32_t depthMax, const La
         if (kt && (e = kt->end(c->stream)) != hipSuccess) return e;  \
     } while (0)
     f.bounce = 0;
+    f.sortHits = ls.sortHits && !ls.traceFetch ? 1u : 0u;   // the work-fetch trace kernels push to the front only
     hipLaunchKernelGGL(k_camera<BATCH>, dim3(nb), dim3(256), 0, c->stream, f);
     for (uint32_t b = 0; b < depthMax; ++b) {
         const uint32_t qsel = b & 1u;

@@ -15,7 +15,7 @@ hipError_t resolve_launch(rbh_ctx c, const float* colorspace, float expo, void* 
 size_t trace_spill_words(size_t lanes);
 namespace host {
 int build_light_tree(const EmissiveTri* tris, const float* flux, uint32_t n, std::vector<PackedNode>& nodes,
-                     std::vector<uint32_t>& pointers, std::vector<uint64_t>& masks);
+                     std::vector<uint32_t>& pointers, std::vector<uint64_t>& masks, bool fp64);
 }
 }  // namespace rbh
 
@@ -68,7 +68,12 @@ int rbh_build_accel(rbh_ctx c) {
 }
 
 int rbh_build_light_tree(rbh_ctx c, const rbh_emissive_tri* tris, const float* flux, uint32_t n) {
+    return rbh_build_light_tree_ex(c, tris, flux, n, 0u);
+}
+
+int rbh_build_light_tree_ex(rbh_ctx c, const rbh_emissive_tri* tris, const float* flux, uint32_t n, uint32_t flags) {
     if (!c) return RBH_ERR_INVALID;
+    if (flags & ~(uint32_t)RBH_LIGHT_TREE_FP64) return fail(c, RBH_ERR_INVALID, "unknown light-tree flags 0x%x", flags);
     if (n && (!tris || !flux)) return fail(c, RBH_ERR_INVALID, "light arrays are null");
     for (uint32_t i = 0; i < n; ++i) {
         if (!(flux[i] > 0.0f) || !std::isfinite(flux[i])) return fail(c, RBH_ERR_INVALID, "light %u: flux must be positive and finite", i);
@@ -77,7 +82,7 @@ int rbh_build_light_tree(rbh_ctx c, const rbh_emissive_tri* tris, const float* f
     LightTreeDev& t = c->tree;
     const EmissiveTri* et = reinterpret_cast<const EmissiveTri*>(tris);
     std::vector<PackedNode> nodes;
-    const int r = host::build_light_tree(et, flux, n, nodes, t.hostPointers, t.hostMasks);
+    const int r = host::build_light_tree(et, flux, n, nodes, t.hostPointers, t.hostMasks, (flags & RBH_LIGHT_TREE_FP64) != 0);
     if (r == -1) return fail(c, RBH_ERR_RANGE, "light count %u exceeds the tree's 27-bit leaf offsets", n);
     if (r < 0) return fail(c, RBH_ERR_INVALID, "light tree produced an invalid bitmask");
     t.hostNodes.resize(nodes.size());
@@ -99,11 +104,18 @@ int rbh_build_light_tree(rbh_ctx c, const rbh_emissive_tri* tris, const float* f
 
 int rbh_light_tree_build_host(const rbh_emissive_tri* tris, const float* flux, uint32_t n, rbh_packed_node* nodes,
                               uint32_t* lightPointers, uint64_t* bitmasks) {
+    return rbh_light_tree_build_host_ex(tris, flux, n, nodes, lightPointers, bitmasks, 0u);
+}
+
+int rbh_light_tree_build_host_ex(const rbh_emissive_tri* tris, const float* flux, uint32_t n, rbh_packed_node* nodes,
+                                 uint32_t* lightPointers, uint64_t* bitmasks, uint32_t flags) {
     if (n && (!tris || !flux || !nodes || !lightPointers || !bitmasks)) return RBH_ERR_INVALID;
+    if (flags & ~(uint32_t)RBH_LIGHT_TREE_FP64) return RBH_ERR_INVALID;
     std::vector<PackedNode> nv;
     std::vector<uint32_t> lp;
     std::vector<uint64_t> bm;
-    const int r = host::build_light_tree(reinterpret_cast<const EmissiveTri*>(tris), flux, n, nv, lp, bm);
+    const int r = host::build_light_tree(reinterpret_cast<const EmissiveTri*>(tris), flux, n, nv, lp, bm,
+                                         (flags & RBH_LIGHT_TREE_FP64) != 0);
     if (r == -1) return RBH_ERR_RANGE;
     if (r < 0) return RBH_ERR_INVALID;
     if (r) {
@@ -205,7 +217,7 @@ void fill_view(rbh_ctx c, const rbh_launch_params* lp, SceneView& v) {
         v.envTex = c->env.texels.as<float>();
         v.envCond = c->env.cond.as<uint16_t>();
         v.envMarg = c->env.marg.as<uint16_t>();
-        v.envIntegral = c->env.integral;
+        v.envIntegral = (float)c->env.integral;
     }
     v.accum = c->rs.accum.as<double4>();
 }
@@ -231,6 +243,7 @@ int prepare_pass_state(rbh_ctx c, size_t NP, Frame1& f, LaunchShape& ls) {
     ls.traceFetch = rs.traceFetch;
     ls.traceWaves = rs.traceWaves;
     ls.quantNodes = rs.quantNodes;
+    ls.sortHits = rs.sortHits;
     const size_t lanes = (size_t)ls.traceGrid * kTraceBlock;
     RBH_ALLOC(c, rs.spill, trace_spill_words(lanes) * sizeof(int));
     f.spill = rs.spill.as<int>();
@@ -515,8 +528,11 @@ int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_
     rbh_tuning b{};
     (void)freeB;
     const int keepF = rs.traceFetch, keepW = rs.shadeWaves, keepTW = rs.traceWaves, keepQ = rs.quantNodes;
+    const int keepH = rs.sortHits;
     rs.quantNodes = 0;
     b.quantNodes = 0;
+    rs.sortHits = 0;
+    b.sortHits = 0;
     rs.traceFetch = 0;
     rs.shadeWaves = 1;
     rs.traceWaves = 1;
@@ -535,6 +551,7 @@ int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_
                 rs.shadeWaves = keepW;
                 rs.traceWaves = keepTW;
                 rs.quantNodes = keepQ;
+                rs.sortHits = keepH;
                 return rc;
             }
             const double rate = (double)P * trialSpp / (c->stats.renderMs * 1e-3);
@@ -600,6 +617,25 @@ int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_
         }
     }
     if (bestRate >= 0.0) {
+        // material-sorted shading
+        rs.traceBlocksPerCU = b.traceBlocksPerCU;
+        rs.shadeBlocksPerCU = b.shadeBlocksPerCU;
+        rs.shadeWaves = b.shadeWaves ? b.shadeWaves : 1;
+        rs.traceWaves = b.traceWaves;
+        rs.quantNodes = b.quantNodes;
+        rs.sortHits = 1;
+        const int rc = rbh_render(c, &t);
+        rs.sortHits = 0;
+        rs.quantNodes = 0;
+        if (rc) return rc;
+        const double rate = (double)P * trialSpp / (c->stats.renderMs * 1e-3);
+        if (rate > bestRate) {
+            bestRate = rate;
+            b.samplesPerSecond = rate;
+            b.sortHits = 1;
+        }
+    }
+    if (bestRate >= 0.0) {
         // per-lane work fetch instead of wave-synchronous ray batches
         rs.traceBlocksPerCU = b.traceBlocksPerCU;
         rs.shadeBlocksPerCU = b.shadeBlocksPerCU;
@@ -614,6 +650,8 @@ int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_
             bestRate = rate;
             b.samplesPerSecond = rate;
             b.traceFetch = 1;
+            b.quantNodes = 0;    // the work-fetch kernels read the 128-byte nodes and push hits unsorted
+            b.sortHits = 0;
         }
     }
     if (bestRate >= 0.0) {
@@ -621,6 +659,7 @@ int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_
         rs.traceFetch = b.traceFetch;
         rs.traceWaves = b.traceWaves;
         rs.quantNodes = b.quantNodes;
+        rs.sortHits = b.sortHits;
         rs.slotBudget = b.pathSlots;
         rs.traceBlocksPerCU = b.traceBlocksPerCU;
         rs.shadeBlocksPerCU = b.shadeBlocksPerCU;
@@ -644,6 +683,7 @@ int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_
         rs.shadeWaves = keepW;
         rs.traceWaves = keepTW;
         rs.quantNodes = keepQ;
+        rs.sortHits = keepH;
         return fail(c, RBH_ERR_NOMEM, "no launch shape fits the free device memory");
     }
     rs.slotBudget = b.pathSlots;
@@ -654,6 +694,7 @@ int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_
     rs.traceFetch = b.traceFetch;
     rs.traceWaves = b.traceWaves;
     rs.quantNodes = b.quantNodes;
+    rs.sortHits = b.sortHits;
     if (!b.shadeWaves) b.shadeWaves = 1;
     *best = b;
     return RBH_OK;
@@ -672,6 +713,7 @@ int rbh_set_tuning(rbh_ctx c, const rbh_tuning* t) {
     c->rs.traceFetch = t->traceFetch != 0;
     c->rs.traceWaves = (t->traceWaves == 7 || t->traceWaves == 8) ? t->traceWaves : 1;
     c->rs.quantNodes = t->quantNodes != 0;
+    c->rs.sortHits = t->sortHits != 0;
     return RBH_OK;
 }
 
@@ -686,6 +728,7 @@ int rbh_get_tuning(rbh_ctx c, rbh_tuning* t) {
     t->traceFetch = c->rs.traceFetch;
     t->traceWaves = c->rs.traceWaves;
     t->quantNodes = c->rs.quantNodes;
+    t->sortHits = c->rs.sortHits;
     return RBH_OK;
 }
 

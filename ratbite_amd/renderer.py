@@ -16,7 +16,7 @@ from .context import Context
 
 
 class Renderer:
-    def __init__(self, scene, device=0, ctx=None, fp64_cdf=False):
+    def __init__(self, scene, device=0, ctx=None, fp64_cdf=False, fp64_light_tree=False):
         self.scene = scene
         self.ctx = ctx or Context(device)
         t = scene.tables
@@ -28,7 +28,7 @@ class Renderer:
         c.upload_materials(scene.materials)
         c.upload_geometry(scene.verts, scene.indices, scene.attribs, scene.tri_material, scene.tri_light)
         c.build_accel()
-        c.build_light_tree(scene.lights, scene.flux)
+        c.build_light_tree(scene.lights, scene.flux, fp64=fp64_light_tree)
         if scene.env is not None:
             c.envmap_build(scene.env, fp64=fp64_cdf)
         else:
@@ -63,7 +63,7 @@ class Renderer:
         return {"pathSlots": t.pathSlots, "traceBlocksPerCU": t.traceBlocksPerCU,
                 "shadeBlocksPerCU": t.shadeBlocksPerCU, "useGraph": t.useGraph, "shadeWaves": t.shadeWaves,
                 "traceFetch": t.traceFetch, "traceWaves": t.traceWaves, "quantNodes": t.quantNodes,
-                "samplesPerSecond": t.samplesPerSecond}
+                "sortHits": t.sortHits, "samplesPerSecond": t.samplesPerSecond}
 
     def tuning(self):
         """The launch shape this context renders with."""
@@ -71,13 +71,14 @@ class Renderer:
         self.ctx.call("rbh_get_tuning", ctypes.byref(t))
         return {"pathSlots": t.pathSlots, "traceBlocksPerCU": t.traceBlocksPerCU,
                 "shadeBlocksPerCU": t.shadeBlocksPerCU, "useGraph": t.useGraph, "shadeWaves": t.shadeWaves,
-                "traceFetch": t.traceFetch, "traceWaves": t.traceWaves, "quantNodes": t.quantNodes}
+                "traceFetch": t.traceFetch, "traceWaves": t.traceWaves, "quantNodes": t.quantNodes,
+                "sortHits": t.sortHits}
 
     def set_tuning(self, path_slots=0, trace_blocks=64, shade_blocks=64, use_graph=False, shade_waves=3,
-                   trace_fetch=0, trace_waves=7, quant_nodes=False):
+                   trace_fetch=0, trace_waves=7, quant_nodes=False, sort_hits=True):
         """Sets the launch shape; the defaults are the library's own."""
         t = L.Tuning(path_slots, trace_blocks, shade_blocks, int(use_graph), shade_waves, int(trace_fetch),
-                     trace_waves, int(quant_nodes), 0.0)
+                     trace_waves, int(quant_nodes), int(sort_hits), 0.0)
         self.ctx.call("rbh_set_tuning", ctypes.byref(t))
 
     # ------------------------------------------------------------ guiding

@@ -475,3 +475,110 @@ def spheres(seed=0x5EED0002, n_spheres=200, level=5, n_lights=10000, n_light_tex
     cam["fov"] = math.radians(60.0)
     env = lognormal_envmap(env_size[0], env_size[1], sigma=env_sigma, seed=seed, sun=True) if env_size else None
     return _finish(parts, mats, spectra, tex_list, cam, env, width, height, depth, spp, "spheres", tables)
+
+
+def _value_noise_2d(rng, n, cells):
+    """n x n smooth value noise in [0, 1] from a (cells+1)^2 random lattice."""
+    g = rng.random((cells + 1, cells + 1))
+    t = np.arange(n) / max(1, n - 1) * cells
+    i = np.minimum(np.floor(t).astype(int), cells - 1)
+    f = t - i
+    f = f * f * (3 - 2 * f)
+    fy, fx = f[:, None], f[None, :]
+    a, b = g[i][:, i], g[i + 1][:, i]
+    c, d = g[i][:, i + 1], g[i + 1][:, i + 1]
+    return (a * (1 - fy) + b * fy) * (1 - fx) + (c * (1 - fy) + d * fy) * fx
+
+
+def heightfield(rng, n, x_range, z_range, base, amplitude, octaves=6):
+    """Seeded fBm terrain of n x n vertices (2 (n-1)^2 triangles) with
+    analytic normals from the height gradient."""
+    h = np.zeros((n, n))
+    amp, cells, norm = 1.0, 4, 0.0
+    for _ in range(octaves):
+        h += amp * _value_noise_2d(rng, n, cells)
+        norm += amp
+        amp *= 0.5
+        cells *= 2
+    h = base + amplitude * h / norm
+    xs = np.linspace(x_range[0], x_range[1], n)
+    zs = np.linspace(z_range[0], z_range[1], n)
+    X, Z = np.meshgrid(xs, zs)
+    verts = np.stack([X, h, Z], -1).reshape(-1, 3)
+    dz, dx = np.gradient(h, zs, xs)
+    nrm = np.stack([-dx, np.ones_like(h), -dz], -1).reshape(-1, 3)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    r = np.arange(n - 1, dtype=np.uint32)
+    a = (r[:, None] * n + r[None, :]).ravel()
+    idx = np.empty((2 * len(a), 3), np.uint32)
+    idx[0::2] = np.stack([a, a + n, a + 1], 1)
+    idx[1::2] = np.stack([a + 1, a + n, a + n + 1], 1)
+    uv = np.stack([(X - x_range[0]) / (x_range[1] - x_range[0]), (Z - z_range[0]) / (z_range[1] - z_range[0])],
+                  -1).reshape(-1, 2).astype(np.float32)
+    return verts, idx, nrm, uv
+
+
+def large(seed=0x5EED0005, n_spheres=800, level=4, terrain=1449, n_lights=100000, n_light_tex=16, width=1920,
+          height=1080, spp=256, depth=24, env_size=(8192, 4096), env_sigma=3.0):
+    """The large precision-split scene: 800 displaced level-4 icospheres
+    (5,120 triangles each, 4.10M) over a seeded fBm heightfield terrain of
+    1449^2 vertices (4.19M triangles), 8.29M triangles in total; 60% of the
+    spheres translucent or metal (35% translucent, 25% Cauchy metal), the rest
+    glossy-diffuse (25%) and fuzz (15%); 100,000 textured emissive triangles;
+    an 8192x4096 log-normal env map with sigma 3 and no sun; 256 spp,
+    depth 24."""
+    rng = np.random.default_rng(seed)
+    tables = load_tables()
+    base_v, base_i = icosphere(level)
+    mats, parts, spectra = [], [], []
+    x_range, z_range = (-170.0, 170.0), (-300.0, 15.0)
+    tv, ti, tn, tuv = heightfield(rng, terrain, x_range, z_range, base=-14.0, amplitude=14.0)
+    mats.append(material(base=(0.35, 0.32, 0.28), metal=0.0, rough=0.85))
+    parts.append((tv, ti, tn, tuv, 0))
+    for s in range(n_spheres):
+        R = random_rotation(rng)
+        scale = float(np.exp(rng.uniform(np.log(0.2), np.log(5.0))))
+        center = np.array([rng.uniform(-120, 120), rng.uniform(-2, 35), rng.uniform(-240, -15)])
+        k = rng.normal(size=(4, 3))
+        ph = rng.uniform(0, 2 * np.pi, 4)
+        disp = 1.0 + 0.08 * np.sum(np.sin(base_v @ k.T * 3.0 + ph), axis=1) / 4.0
+        v = (base_v * disp[:, None]) @ R.T * scale + center
+        n = vertex_normals(v, base_i)
+        uv = np.stack([0.5 + np.arctan2(base_v[:, 2], base_v[:, 0]) / (2 * np.pi),
+                       np.arccos(np.clip(base_v[:, 1], -1, 1)) / np.pi], axis=1).astype(np.float32)
+        r = rng.random()
+        rough = float(rng.uniform(0.02, 1.0))
+        if r < 0.35:
+            m = material(base=rng.uniform(0.5, 1.0, 3), metal=0.0, rough=rough, ior=float(rng.uniform(1.3, 1.8)),
+                         transmission=1.0)
+        elif r < 0.60:
+            eta, kk = cauchy_spectra(rng)
+            spectra += [eta, kk]
+            m = material(bxdf=BXDF_CONDUCTOR, rough=rough, eta_k=(len(spectra) - 2, len(spectra) - 1))
+        elif r < 0.85:
+            m = material(base=rng.uniform(0.1, 0.9, 3), metal=0.0, rough=rough)
+        else:
+            m = material(base=rng.uniform(0.1, 0.9, 3), metal=0.0, rough=rough,
+                         sheen=(rng.uniform(0.3, 1.0, 3), float(rng.uniform(0.05, 1.0))))
+        mats.append(m)
+        parts.append((v, base_i, n, uv, len(mats) - 1))
+    tex_list = [value_noise_texture(rng) for _ in range(n_light_tex)]
+    em_first = len(mats)
+    for t in range(n_light_tex):
+        mats.append(material(base=(0.8, 0.8, 0.8), rough=1.0, emissive=rng.uniform(0.5, 2.0, 3), emission_tex=t))
+    c = rng.uniform([-160, 0, -290], [160, 45, -10], size=(n_lights, 3))
+    size = np.exp(rng.uniform(np.log(0.2), np.log(1.0), n_lights))
+    lv = np.zeros((n_lights * 3, 3))
+    for k in range(3):
+        d = rng.normal(size=(n_lights, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        lv[k::3] = c + d * size[:, None]
+    li = np.arange(n_lights * 3, dtype=np.uint32).reshape(-1, 3)
+    ln = vertex_normals(lv, li)
+    luv = np.tile(np.array([[0, 0], [1, 0], [0, 1]], np.float32), (n_lights, 1))
+    lmat = (em_first + rng.integers(0, n_light_tex, n_lights)).astype(np.uint32)
+    parts.append((lv, li, ln, luv, lmat))
+    cam = camera_basis((0.0, 12.0, 10.0), (0.0, 6.0, -80.0), (0.0, 1.0, 0.0))
+    cam["fov"] = math.radians(60.0)
+    env = lognormal_envmap(env_size[0], env_size[1], sigma=env_sigma, seed=seed, sun=False) if env_size else None
+    return _finish(parts, mats, spectra, tex_list, cam, env, width, height, depth, spp, "large", tables)

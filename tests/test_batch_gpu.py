@@ -87,6 +87,42 @@ def test_autotune_keeps_results(ctx):
     r.set_tuning()
 
 
+def test_material_sorted_shading_keeps_results(ctx):
+    # conductor hits queued from the end of the hit queue: the same sums for
+    # plain, guided (the network's mixtures are indexed by queue position)
+    # and batched renders
+    sc = scenes.spheres(n_spheres=24, level=2, n_lights=80, width=48, height=32, spp=6, env_size=(128, 64))
+    assert np.any(sc.materials["bxdf"] == scenes.BXDF_CONDUCTOR)
+    r = Renderer(sc, ctx=ctx)
+    try:
+        r.configure_guide(seed=3)
+        out = {}
+        for sort in (False, True):
+            r.set_tuning(sort_hits=sort)
+            assert r.tuning()["sortHits"] == int(sort)
+            r.render(spp=6, offset=0)
+            plain = r.accum()
+            r.render(spp=6, offset=0, guided=True)
+            out[sort] = (plain, r.accum())
+        assert np.array_equal(out[False][0], out[True][0])
+        assert np.array_equal(out[False][1], out[True][1])
+    finally:
+        r.set_tuning()
+    rs = [Renderer(s) for s in _scenes()]
+    try:
+        alone = []
+        for q in rs:
+            q.render(spp=q.scene.spp, offset=0)
+            alone.append(q.accum())
+        rs[0].set_tuning(sort_hits=True)
+        render_batch(rs)
+        for q, a in zip(rs, alone):
+            assert np.array_equal(a, q.accum()), q.scene.name
+    finally:
+        for q in rs:
+            q.close()
+
+
 def test_graph_replays_match_direct_launches(ctx):
     # several passes per render replay one graph executable (updated in place);
     # eager renders and host synchronisations run between the graph renders

@@ -4,6 +4,8 @@ The oracle follows the kernel's FP32 association order. These tests pin it
 (1) against closed forms, (2) against an independent vectorised numpy
 formulation of the same scan and (3) against the FP64 distribution.
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -99,6 +101,20 @@ def test_fp32_against_fp64_distribution():
     assert np.max(np.abs(cx - dx)) < 2e-5
     assert np.max(np.abs(cy - dy)) < 2e-5
     assert abs(i32 - i64) / i64 < 1e-5
+
+
+@pytest.mark.parametrize("w,h", [(64, 32), (2048, 1024), (512, 4096)])
+def test_fp64_constant_map_closed_forms(w, h):
+    # a map of unit power: every conditional CDF is (x + 1) / W, the marginal
+    # is symmetric about the equator (0.5 after the upper half) and the
+    # integral is W * sum_y sin(pi (y + 1/2) / H) = W / sin(pi / (2 H))
+    tex = np.ones((h, w, 3), np.float32)
+    dx, dy, integ = O.cdf_build_f64(tex)
+    np.testing.assert_allclose(dx, np.broadcast_to((np.arange(w) + 1.0) / w, (h, w)), rtol=0, atol=1e-14)
+    assert abs(dy[h // 2 - 1] - 0.5) < 1e-13 and abs(dy[-1] - 1.0) < 1e-13
+    np.testing.assert_allclose(dy[:h // 2] + dy[::-1][1:h // 2 + 1], 1.0, atol=1e-13)
+    closed = w / math.sin(math.pi / (2 * h))
+    assert abs(integ - closed) / closed < 1e-12
 
 
 def test_zero_rows_and_blocks():

@@ -7,8 +7,12 @@
   holds its light, <= 2n-1 nodes, <= 8 lights per leaf, flux conserved);
 * the oracle's PDF/sampling pair is pinned by invariants: the PDFs of all
   lights sum to 1 - envImportance, the PDF reported by sampling equals the
-  traversal PDF of the sampled light, and sampling frequencies follow the PDF.
+  traversal PDF of the sampled light, and sampling frequencies follow the PDF;
+* the FP64 power mode (RBH_LIGHT_TREE_FP64): every node's flux is the float
+  rounding of the exact sum of its lights' fluxes (math.fsum), and the FP32
+  builder's flux sums stay within a stated error of the exact sums.
 """
+import math
 import ctypes
 
 import numpy as np
@@ -24,15 +28,39 @@ def _lights(n, seed):
     return sc.lights, sc.flux
 
 
-def product_build(lights, flux):
+def product_build(lights, flux, flags=0):
     n = len(lights)
     nodes = np.zeros(max(1, 2 * n), L.PACKED_NODE_DTYPE)
     lp = np.zeros(max(1, n), np.uint32)
     bm = np.zeros(max(1, n), np.uint64)
-    r = L.lib().rbh_light_tree_build_host(L.ptr(np.ascontiguousarray(lights)), L.ptr(np.ascontiguousarray(flux, np.float32)),
-                                          n, L.ptr(nodes), L.ptr(lp), L.ptr(bm))
+    r = L.lib().rbh_light_tree_build_host_ex(L.ptr(np.ascontiguousarray(lights)),
+                                             L.ptr(np.ascontiguousarray(flux, np.float32)), n, L.ptr(nodes), L.ptr(lp),
+                                             L.ptr(bm), flags)
     assert r >= 0, r
     return nodes[:r], lp[:n], bm[:n]
+
+
+def subtree_ranges(nodes):
+    """[lo, hi) of lightPointers under every node (leaves hold contiguous
+    ranges in depth-first order)."""
+    core = nodes["coreData"].astype(np.int64)
+    lo = np.zeros(len(nodes), np.int64)
+    hi = np.zeros(len(nodes), np.int64)
+    for i in range(len(nodes) - 1, -1, -1):
+        c = int(core[i])
+        if c >> 31:
+            lo[i] = c & ((1 << 27) - 1)
+            hi[i] = lo[i] + ((c >> 27) & 15)
+        else:
+            lo[i], hi[i] = lo[i + 1], hi[c]
+    return lo, hi
+
+
+def exact_node_flux(nodes, lp, flux):
+    ids = lp.astype(np.int64) & ((1 << 30) - 1)
+    f = flux.astype(np.float64)[ids]
+    lo, hi = subtree_ranges(nodes)
+    return np.array([math.fsum(f[a:b]) for a, b in zip(lo, hi)])
 
 
 @pytest.mark.parametrize("n,seed", [(1, 1), (7, 2), (9, 3), (100, 4), (2000, 5)])
@@ -148,3 +176,58 @@ def test_fp64_light_pdf_partition_and_fp32_error():
     med, q99, wmean = fp32_error_summary(p32, p64)
     print("fp32 oracle vs fp64: median %.2g, q99 (pdf >= 1e-4) %.2g, pdf-weighted mean %.2g" % (med, q99, wmean))
     assert med < 1e-5 and q99 < 5e-3 and wmean < 2e-4
+
+
+@pytest.mark.parametrize("n,seed", [(2000, 5), (100000, 6)])
+def test_fp64_power_mode_node_flux(n, seed):
+    # the FP64 mode rounds each node's flux once from an exact-enough double
+    # sum; the reference's FP32 builder sums packed (rounded) child fluxes
+    lights, flux = _lights(n, seed)
+    flux = np.ascontiguousarray(flux, np.float32)
+    nodes64, lp64, bm64 = product_build(lights, flux, L.RBH_LIGHT_TREE_FP64)
+    ex64 = exact_node_flux(nodes64, lp64, flux)
+    assert np.array_equal(nodes64["flux"], ex64.astype(np.float32))
+    nodes32, lp32, _ = product_build(lights, flux)
+    ex32 = exact_node_flux(nodes32, lp32, flux)
+    rel32 = np.abs(nodes32["flux"].astype(np.float64) - ex32) / ex32
+    print("FP32 builder node flux vs exact sums (%d lights): max rel %.3g, mean %.3g" % (n, rel32.max(), rel32.mean()))
+    assert rel32.max() < 1e-5
+    # the FP64-mode tree is a valid tree: every light reachable, a partition of unity
+    for light in range(0, n, max(1, n // 500)):
+        leaf, cnt, off = _walk(nodes64, bm64[light])
+        assert light in set(int(x) & ((1 << 30) - 1) for x in lp64[off:off + cnt])
+    assert np.array_equal(np.sort(lp64.astype(np.int64) & ((1 << 30) - 1)), np.arange(n))
+
+
+def test_fp64_power_mode_rejects_unknown_flags():
+    lights, flux = _lights(10, 1)
+    n = len(lights)
+    nodes = np.zeros(2 * n, L.PACKED_NODE_DTYPE)
+    lp, bm = np.zeros(n, np.uint32), np.zeros(n, np.uint64)
+    r = L.lib().rbh_light_tree_build_host_ex(L.ptr(lights), L.ptr(np.ascontiguousarray(flux, np.float32)), n,
+                                             L.ptr(nodes), L.ptr(lp), L.ptr(bm), 2)
+    assert r == L.RBH_ERR_INVALID
+
+
+def test_fp64_light_pdf_error_at_100k_lights():
+    # C5-sized tree (100k emissive triangles, built in the FP64 power mode):
+    # the FP32 pdf (the reference's formulation, equal to the GPU's within
+    # 1e-5) against the FP64 restatement, at 20k random shading points
+    import light_tree64 as T
+    lights, flux = _lights(100000, 7)
+    nodes, lp, bm = product_build(lights, flux, L.RBH_LIGHT_TREE_FP64)
+    rng = np.random.default_rng(8)
+    n = 20000
+    pos = rng.uniform([-60, -8, -120], [60, 25, -10], (n, 3)).astype(np.float32)
+    nrm = rng.normal(size=(n, 3)).astype(np.float32)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    pn = np.ascontiguousarray(np.concatenate([pos, nrm], 1), np.float32)
+    ids = rng.integers(0, len(lights), n).astype(np.uint32)
+    p32 = np.zeros(n, np.float32)
+    O.lib().orc_light_tree_pdf(O.p(nodes), O.p(lp), O.p(bm), 0, O.p(ids), O.p(pn), O.p(p32), n)
+    p64 = T.pdf(nodes, bm, ids, pn)
+    ok = p64 > 0
+    med, q99, wmean = fp32_error_summary(p32[ok], p64[ok])
+    print("100k lights, fp32 vs fp64 pdf: median %.2g, q99 (pdf >= 1e-4) %.2g, pdf-weighted mean %.2g"
+          % (med, q99, wmean))
+    assert med < 1e-5 and wmean < 5e-4

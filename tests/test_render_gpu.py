@@ -21,11 +21,16 @@ def relmse(a, ref, eps=1e-2):
     return float(np.mean((a - ref) ** 2 / (ref ** 2 + eps)))
 
 
-def _compare(sc, ctx, spp, crop=None):
-    r = Renderer(sc, ctx=ctx)
+def _compare(sc, ctx, spp, crop=None, **kw):
+    r = Renderer(sc, ctx=ctx, **kw)
     st = r.render(spp=spp, offset=0)
     acc = r.accum()
-    os_ = O.OracleScene(sc, tree=r.light_tree())
+    env = None
+    if kw.get("fp64_cdf") and sc.env is not None:
+        # the oracle samples the env map with the GPU's FP64-mode tables
+        cond, marg, _, integral = ctx.envmap_read()
+        env = (cond, marg, float(np.float32(integral)))
+    os_ = O.OracleScene(sc, tree=r.light_tree(), env=env)
     oacc, ost = O.render(sc, sc.launch_params(offset=0, count=spp), oscene=os_)
     assert np.all(acc[..., 3] == spp) and np.all(oacc[..., 3] == spp)
     g = acc[..., :3] / spp
@@ -70,6 +75,27 @@ def test_spheres_matches_oracle(ctx):
     frac = close.all(axis=2)[fin].mean()
     e = relmse(g[fin], o[fin])
     print("spheres: pixels within 1e-4: %.4f  relMSE(gpu, oracle) = %.3g" % (frac, e))
+    assert frac > 0.5
+    assert e < 5e-2
+
+
+def test_fp64_precision_modes_match_oracle(ctx):
+    # env-map CDF and light tree built in their FP64 modes: the context's
+    # tree is the host builder's FP64 tree bit for bit, and the image agrees
+    # with the oracle rendering over the same tree and env map tables
+    from ratbite_amd import _lib as L
+    from test_light_tree import product_build
+    sc = scenes.spheres(n_spheres=12, level=3, n_lights=400, width=80, height=48, spp=8, env_size=(256, 128))
+    g, o, st, ost = _compare(sc, ctx, 8, fp64_cdf=True, fp64_light_tree=True)
+    nodes, lp, bm = product_build(sc.lights, sc.flux, L.RBH_LIGHT_TREE_FP64)
+    n2, lp2, bm2 = ctx.get_light_tree(len(sc.lights))
+    assert n2.tobytes() == nodes.tobytes() and np.array_equal(lp2, lp) and np.array_equal(bm2, bm)
+    fin = np.isfinite(g).all(axis=2) & np.isfinite(o).all(axis=2)
+    assert fin.mean() > 0.99
+    close = np.abs(g - o) <= 1e-4 * np.maximum(np.abs(o), 1e-3)
+    frac = close.all(axis=2)[fin].mean()
+    e = relmse(g[fin], o[fin])
+    print("fp64 modes: pixels within 1e-4: %.4f  relMSE(gpu, oracle) = %.3g" % (frac, e))
     assert frac > 0.5
     assert e < 5e-2
 
