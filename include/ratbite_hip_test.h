@@ -35,7 +35,8 @@ int rbh_test_oct(rbh_ctx ctx, const float* v, uint32_t* enc, float* dec, uint32_
  * four-wide nodes quantised to 64 bytes. */
 int rbh_test_trace(rbh_ctx ctx, const float* rays, float* hits, uint32_t n, int mode);
 /* The built LBVH (replaces optixAccelBuild, rendering_interface.cpp:733-1064):
- * sorted 64-bit keys (Morton code << 32 | triangle), n entries; 64-byte nodes
+ * sorted 64-bit keys (Morton code << b | triangle, b the bits that hold n - 1,
+ * the Morton code min(21, (64 - b) / 3) bits per axis), n entries; 64-byte nodes
  * (nnodes entries); 48-byte triangle records in leaf order (n entries). Any
  * output pointer may be null. */
 int rbh_test_lbvh_read(rbh_ctx ctx, uint64_t* keys, void* nodes, void* tris, uint32_t* nnodes);

@@ -73,6 +73,7 @@ struct Accel {
     DevBuf tris;      // TriRecord in leaf order
     DevBuf scratch;   // build scratch
     size_t keysOffset = 0;   // byte offset of the sorted 64-bit keys in scratch (kept after the build)
+    int keyIndexBits = 32;   // low key bits holding the triangle index
 };
 
 // HIP-event timing of individual render-loop kernels (RBH_RENDER_TIMING):

@@ -172,15 +172,6 @@ NODE_DTYPE = np.dtype([(k, "f4") for k in ("lminx", "lmaxx", "lminy", "lmaxy", "
 TRI_DTYPE = np.dtype([("v0", "f4", 3), ("prim", "u4"), ("v1", "f4", 3), ("pad0", "u4"), ("v2", "f4", 3), ("pad1", "u4")])
 
 
-def _spread10(x):
-    x = x.astype(np.uint64) & 0x3FF
-    x = (x | (x << 16)) & 0x030000FF
-    x = (x | (x << 8)) & 0x0300F00F
-    x = (x | (x << 4)) & 0x030C30C3
-    x = (x | (x << 2)) & 0x09249249
-    return x
-
-
 def _read_lbvh(ctx, n):
     import ctypes
     keys = np.zeros(n, np.uint64)
@@ -205,19 +196,27 @@ def test_morton_keys_bit_exact_and_tree_invariants(ctx, which):
     _upload(ctx, v, idx)
     n = len(idx)
     keys, nodes, tris = _read_lbvh(ctx, n)
-    # Morton codes: 10 bits per axis of the triangle-centroid grid, FP32 as the kernel computes them
+    # Morton codes on the triangle-centroid grid, FP32 as the kernel computes
+    # them: the triangle index in the low b bits (b holds n - 1), the code in
+    # min(21, (64 - b) // 3) bits per axis above it, x the most significant
     tv = v[idx]                                            # n x 3 x 3
     lo, hi = tv.min(axis=1), tv.max(axis=1)
     ce = (np.float32(0.5) * (lo + hi)).astype(np.float32)
     cmin, cmax = ce.min(axis=0), ce.max(axis=0)
     ext = (cmax - cmin).astype(np.float32)
     u = np.where(ext > 0, (ce - cmin) / np.where(ext > 0, ext, 1), np.float32(0.5)).astype(np.float32)
-    q = np.clip(u * np.float32(1024.0), 0, 1023).astype(np.uint32)
-    m = (_spread10(q[:, 0]) << 2) | (_spread10(q[:, 1]) << 1) | _spread10(q[:, 2])
-    ref = np.sort((m << 32) | np.arange(n, dtype=np.uint64))
+    b = max(1, int(n - 1).bit_length())
+    ab = min(21, (64 - b) // 3)
+    cells = np.float32(1 << ab)
+    q = np.clip(u * cells, 0, cells - 1).astype(np.uint64)
+    m = np.zeros(n, np.uint64)
+    for bit in range(ab):                                  # interleave bit by bit: ... x1 y1 z1 x0 y0 z0
+        for axis in range(3):
+            m |= ((q[:, axis] >> np.uint64(bit)) & np.uint64(1)) << np.uint64(3 * bit + 2 - axis)
+    ref = np.sort((m << np.uint64(b)) | np.arange(n, dtype=np.uint64))
     assert np.array_equal(keys, ref)
     # leaf order and triangle records
-    assert np.array_equal(tris["prim"], (keys & 0xFFFFFFFF).astype(np.uint32))
+    assert np.array_equal(tris["prim"], (keys & np.uint64((1 << b) - 1)).astype(np.uint32))
     assert np.array_equal(tris["v0"], v[idx[tris["prim"], 0]]) and np.array_equal(tris["v2"], v[idx[tris["prim"], 2]])
     # topology: every internal node but the root and every leaf is a child exactly once
     assert len(nodes) == n - 1
