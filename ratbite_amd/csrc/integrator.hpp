@@ -21,8 +21,6 @@ enum : uint32_t { STAT_CLOSEST = 0, STAT_SHADOW = 1, STAT_VERTICES = 2, STAT_NON
 struct PathBuffers {
     float4* od;     // ray origin, bxdfPDF
     float4* dr;     // ray direction, refractionScale
-    float4* lam;    // hero wavelengths
-    float4* lpdf;   // wavelength pdfs
     float4* L;      // radiance estimate
     float4* T;      // throughput
     float4* nf;     // last shading normal, state flags

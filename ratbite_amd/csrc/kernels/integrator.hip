@@ -96,6 +96,21 @@ __device__ __forceinline__ void slot_pixel(const SceneView& S, uint32_t pix, uin
 }
 
 // ------------------------------------------------------------------ camera
+// The sample's hero wavelengths (spectral.h:227-241 from the pixel's
+// wavelength dimension at the first bounce). They never change along the
+// path, so every kernel recomputes them from the slot instead of keeping
+// 32 bytes per path slot in HBM.
+__device__ __forceinline__ Wl slot_wavelengths(const SceneView& S, uint32_t sk, uint32_t x, uint32_t y) {
+    return sample_wavelengths(qrng_at(S.sample + sk, 0, pixel_hash(x, y)).s1(QD_WAVELENGTH));
+}
+__device__ __forceinline__ Wl path_wavelengths(const SceneView& S, uint32_t pid) {
+    const uint32_t local = pid - S.slot0;
+    const uint32_t sk = local / S.P, pix = local - sk * S.P;
+    uint32_t x, y;
+    slot_pixel(S, pix, x, y);
+    return slot_wavelengths(S, sk, x, y);
+}
+
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_camera(Frame1 f) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -106,7 +121,6 @@ __global__ __launch_bounds__(256) void k_camera(Frame1 f) {
     uint32_t x, y;
     slot_pixel(S, pix, x, y);
     const Qrng q = qrng_at(S.sample + k, 0, pixel_hash(x, y));
-    const Wl wl = sample_wavelengths(q.s1(QD_WAVELENGTH));
     const V2 fo = gaussian_offset(q.s2(QD_FILTER));
     const float cx = (float)x + fo.x, cy = (float)y + fo.y;
     V3 o{0.0f, 0.0f, 0.0f}, d;
@@ -130,8 +144,6 @@ __global__ __launch_bounds__(256) void k_camera(Frame1 f) {
     PathBuffers& p = f.p;
     p.od[i] = make_float4(o.x, o.y, o.z, 1.0f);           // bxdfPdf = 1
     p.dr[i] = make_float4(d.x, d.y, d.z, 1.0f);           // refractionScale = 1
-    p.lam[i] = make_float4(wl.lambda.v[0], wl.lambda.v[1], wl.lambda.v[2], wl.lambda.v[3]);
-    p.lpdf[i] = make_float4(wl.pdf.v[0], wl.pdf.v[1], wl.pdf.v[2], wl.pdf.v[3]);
     p.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     p.T[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
     p.nf[i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
@@ -245,9 +257,7 @@ __device__ __forceinline__ void shade_miss(const Frame1& f, const SceneView& S, 
                                   theta / 3.14159265358979f);
         float lightPdf = (1.0f / (4.0f * 3.14159265358979f)) * ((sky.x + sky.y + sky.z) / 3.0f) * sinf(theta) /
                          S.envIntegral;
-        const float4 la = f.p.lam[pid];
-        Wl wl;
-        wl.lambda = S4{{la.x, la.y, la.z, la.w}};
+        const Wl wl = path_wavelengths(S, pid);
         const S4 Le = ColorTables{S.colors}.rgb(sky, wl);
         lightPdf *= treePdf;
         const float w = prevNEE ? power_heuristic(od.w, lightPdf) : 1.0f;
@@ -405,7 +415,7 @@ __device__ __forceinline__ void guide_input(const Frame1& f, const SceneView& S,
     out[3] = g.x; out[4] = g.y; out[5] = g.z;
     out[6] = wo.x; out[7] = wo.y; out[8] = wo.z;
     out[9] = rough;
-    out[10] = (f.p.lam[pid].x - 360.0f) * (1.0f / 470.0f);
+    out[10] = (path_wavelengths(S, pid).lambda.v[0] - 360.0f) * (1.0f / 470.0f);
     for (int k = 11; k < 16; ++k) out[k] = 0.0f;
 }
 
@@ -548,11 +558,6 @@ __global__ __launch_bounds__(kShadeBlock, WAVES) void k_shade(Frame1 f, uint32_t
             P.refr = dr.w;
             P.bxdfPdf = od.w;
             P.flags = __float_as_uint(nf.w);
-            {
-                const float4 a = f.p.lam[pid], b = f.p.lpdf[pid];
-                P.wl.lambda = S4{{a.x, a.y, a.z, a.w}};
-                P.wl.pdf = S4{{b.x, b.y, b.z, b.w}};
-            }
             float4 Lv = f.p.L[pid], Tv = f.p.T[pid];
             if (GUIDED && f.collect) snapshot_radiance(f, pid, Lv);
             S4 L{{Lv.x, Lv.y, Lv.z, Lv.w}};
@@ -564,6 +569,7 @@ __global__ __launch_bounds__(kShadeBlock, WAVES) void k_shade(Frame1 f, uint32_t
             const uint32_t sk = local / S.P, pix = local - sk * S.P;
             uint32_t px, py;
             slot_pixel(S, pix, px, py);
+            P.wl = slot_wavelengths(S, sk, px, py);
             const Qrng q = qrng_at(S.sample + sk, depth, pixel_hash(px, py));
 
             // ---- interaction (pathtrace.cu:86-149, 906-953)
@@ -984,10 +990,8 @@ __global__ __launch_bounds__(256) void k_accumulate(Frame1 f) {
         if (f.collect) finish_records(f, j, f.p.L[j]);
         const uint32_t flags = __float_as_uint(f.p.nf[j].w);
         if (!(flags & PF_TERMINATED)) {
-            const float4 Lv = f.p.L[j], la = f.p.lam[j], lp = f.p.lpdf[j];
-            Wl wl;
-            wl.lambda = S4{{la.x, la.y, la.z, la.w}};
-            wl.pdf = S4{{lp.x, lp.y, lp.z, lp.w}};
+            const float4 Lv = f.p.L[j];
+            const Wl wl = slot_wavelengths(S, k, px, py);
             S4 L{{Lv.x, Lv.y, Lv.z, Lv.w}};
             for (int c = 0; c < kNL; ++c) L.v[c] = wl.pdf.v[c] != 0.0f ? L.v[c] / wl.pdf.v[c] : 0.0f;
             double xyz[3];

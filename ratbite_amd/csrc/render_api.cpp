@@ -34,7 +34,7 @@ size_t layout_paths(char* base, size_t P, PathBuffers* pb, uint32_t** counters, 
         off = align256(off + bytes);
         return p;
     };
-    float4** f4[] = {&pb->od, &pb->dr, &pb->lam, &pb->lpdf, &pb->L, &pb->T, &pb->nf, &pb->hit, &pb->Lnee, &pb->sro, &pb->srd};
+    float4** f4[] = {&pb->od, &pb->dr, &pb->L, &pb->T, &pb->nf, &pb->hit, &pb->Lnee, &pb->sro, &pb->srd};
     for (float4** q : f4) *q = reinterpret_cast<float4*>(take(P * sizeof(float4)));
     pb->depth = reinterpret_cast<uint32_t*>(take(P * 4));
     pb->q0 = reinterpret_cast<uint32_t*>(take(P * 4));
