@@ -37,7 +37,8 @@ int rbh_test_trace(rbh_ctx ctx, const float* rays, float* hits, uint32_t n, int 
 /* The built LBVH (replaces optixAccelBuild, rendering_interface.cpp:733-1064):
  * sorted 64-bit keys (Morton code << b | triangle, b the bits that hold n - 1,
  * the Morton code min(21, (64 - b) / 3) bits per axis), n entries; 64-byte nodes
- * (nnodes entries); 48-byte triangle records in leaf order (n entries). Any
+ * (nnodes entries); 64-byte triangle records in leaf order (vertices,
+ * triangle, material, light, vertex indices; n entries). Any
  * output pointer may be null. */
 int rbh_test_lbvh_read(rbh_ctx ctx, uint64_t* keys, void* nodes, void* tris, uint32_t* nnodes);
 /* A plain device copy of `bytes` bytes, `repeats` times (no reference

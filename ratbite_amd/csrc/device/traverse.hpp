@@ -32,7 +32,7 @@ __device__ __forceinline__ RayShear make_shear(V3 d) {
 
 struct Hit {
     float t, b1, b2;
-    uint32_t prim;   // 0xFFFFFFFF: miss
+    uint32_t leaf;   // leaf-order triangle record of the hit; 0xFFFFFFFF: miss
 };
 
 __device__ __forceinline__ bool tri_hit(const RayShear& s, V3 o, const TriRecord& tr, float tmax, float& t, float& b1,
@@ -99,7 +99,7 @@ __device__ __forceinline__ Hit traverse(const BvhNode* __restrict__ nodes, const
                                         uint32_t nnodes, V3 o, V3 d, float tmax, const TravStack<LDS_DEPTH>& stack) {
     Hit h;
     h.t = tmax;
-    h.prim = 0xFFFFFFFFu;
+    h.leaf = 0xFFFFFFFFu;
     h.b1 = h.b2 = 0.0f;
     const RayShear sh = make_shear(d);
     const float ix = 1.0f / d.x, iy = 1.0f / d.y, iz = 1.0f / d.z;
@@ -134,13 +134,13 @@ __device__ __forceinline__ Hit traverse(const BvhNode* __restrict__ nodes, const
             if (hl) { node = e.x; continue; }
             if (hr) { node = e.y; continue; }
         } else {
-            const TriRecord tr = tris[~node];
+            const TriRecord& tr = tris[~node];
             float t, b1, b2;
             if (tri_hit(sh, o, tr, h.t, t, b1, b2)) {
                 h.t = t;
                 h.b1 = b1;
                 h.b2 = b2;
-                h.prim = tr.prim;
+                h.leaf = (uint32_t)~node;
                 if (ANY) return h;
             }
         }
@@ -174,7 +174,7 @@ __device__ __forceinline__ void trav_init(TravRay& r, V3 o, V3 d, float tmax, ui
     r.iz = 1.0f / d.z;
     r.sh = make_shear(d);
     r.h.t = tmax;
-    r.h.prim = 0xFFFFFFFFu;
+    r.h.leaf = 0xFFFFFFFFu;
     r.h.b1 = r.h.b2 = 0.0f;
     r.node = nnodes ? 0 : ~0;
     r.sp = 0;
@@ -224,13 +224,13 @@ __device__ __forceinline__ bool trav_step(const NODE* __restrict__ nodes, const 
                                           TravRay& r, const TravStack<LDS_DEPTH>& stack) {
     if (r.node < 0) {
         // a single-triangle scene: the root is the leaf
-        const TriRecord tr = tris[~r.node];
+        const TriRecord& tr = tris[~r.node];
         float t, b1, b2;
         if (tri_hit(r.sh, r.o, tr, r.h.t, t, b1, b2)) {
             r.h.t = t;
             r.h.b1 = b1;
             r.h.b2 = b2;
-            r.h.prim = tr.prim;
+            r.h.leaf = (uint32_t)~r.node;
         }
         return true;
     }
@@ -259,13 +259,13 @@ __device__ __forceinline__ bool trav_step(const NODE* __restrict__ nodes, const 
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (in[k] && cha[k] < 0) {
-            const TriRecord tr = tris[~cha[k]];
+            const TriRecord& tr = tris[~cha[k]];
             float t, b1, b2;
             if (tri_hit(r.sh, r.o, tr, r.h.t, t, b1, b2)) {
                 r.h.t = t;
                 r.h.b1 = b1;
                 r.h.b2 = b2;
-                r.h.prim = tr.prim;
+                r.h.leaf = (uint32_t)~cha[k];
                 if (ANY) return true;
             }
         }

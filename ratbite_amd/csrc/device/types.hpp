@@ -65,13 +65,17 @@ __host__ __device__ inline float qnode_step(uint32_t expByte) {
 }
 __host__ __device__ inline float qnode_decode(float o, uint32_t q, float s) { return o + (float)q * s; }
 
-// Triangle in leaf order: vertices + original triangle index, 48 bytes.
+// Triangle in leaf order, one 64-byte line: the vertices the traversal
+// tests (first 48 bytes), and what shading needs besides them: the original
+// triangle index, its material and light index (0xFFFFFFFF: not emissive)
+// and its vertex indices (for the per-vertex attributes).
 struct alignas(16) TriRecord {
     float v0x, v0y, v0z; uint32_t prim;
-    float v1x, v1y, v1z; uint32_t pad0;
-    float v2x, v2y, v2z; uint32_t pad1;
+    float v1x, v1y, v1z; uint32_t material;
+    float v2x, v2y, v2z; uint32_t light;
+    uint32_t i0, i1, i2, pad;
 };
-static_assert(sizeof(TriRecord) == 48, "TriRecord is 48 bytes");
+static_assert(sizeof(TriRecord) == 64, "TriRecord is 64 bytes");
 
 // LightTree::PackedNode (src/core/light_tree_types.h:119-162)
 struct PackedNode {

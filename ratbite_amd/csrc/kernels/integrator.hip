@@ -352,8 +352,8 @@ __global__ __launch_bounds__(kTraceBlock, WAVES) void k_trace(Frame1 f, uint32_t
             if (chunk.done(__ballot(active) != 0ull)) break;
             bool done = false;
             if (active) done = trav_step<false>(nodes, tris, ray, st);
-            const bool hit = done && ray.h.prim != 0xFFFFFFFFu;
-            if (hit) f.p.hit[pid] = make_float4(ray.h.t, ray.h.b1, ray.h.b2, __uint_as_float(ray.h.prim));
+            const bool hit = done && ray.h.leaf != 0xFFFFFFFFu;
+            if (hit) f.p.hit[pid] = make_float4(ray.h.t, ray.h.b1, ray.h.b2, __uint_as_float(ray.h.leaf));
             hits.add(hit, pid);
             misses.add(done && !hit, pid);
             hits.flush(f.counters + CNT_HIT, f.p.hq, 64u);
@@ -372,11 +372,11 @@ __global__ __launch_bounds__(kTraceBlock, WAVES) void k_trace(Frame1 f, uint32_t
                 const float4 od = f.p.od[pid], dr = f.p.dr[pid];
                 const Hit h = traverse4<false>(scene_nodes<QN>(S), S.tris, S.nnodes4, V3{od.x, od.y, od.z},
                                                V3{dr.x, dr.y, dr.z}, 3.402823466e+38f, st);
-                hit = h.prim != 0xFFFFFFFFu;
+                hit = h.leaf != 0xFFFFFFFFu;
                 if (hit) {
-                    f.p.hit[pid] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.prim));
+                    f.p.hit[pid] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.leaf));
                     // material-sorted shading: conductor hits form their own run of the queue
-                    if (f.sortHits) back = S.materials[S.triMaterial[h.prim]].bxdf == 0u;
+                    if (f.sortHits) back = S.materials[S.tris[h.leaf].material].bxdf == 0u;
                 }
                 ++nrays;
             }
@@ -398,16 +398,15 @@ __global__ __launch_bounds__(kTraceBlock, WAVES) void k_trace(Frame1 f, uint32_t
 // towards the viewer, outgoing direction, roughness factor, hero wavelength.
 __device__ __forceinline__ void guide_input(const Frame1& f, const SceneView& S, uint32_t pid, float* __restrict__ out) {
     const float4 hr = f.p.hit[pid], dr = f.p.dr[pid];
-    const uint32_t prim = __float_as_uint(hr.w);
-    const V3 v0 = ld3(S.verts + 3 * (size_t)S.indices[3 * prim]), v1 = ld3(S.verts + 3 * (size_t)S.indices[3 * prim + 1]),
-             v2 = ld3(S.verts + 3 * (size_t)S.indices[3 * prim + 2]);
+    const TriRecord& tr = S.tris[__float_as_uint(hr.w)];
+    const V3 v0{tr.v0x, tr.v0y, tr.v0z}, v1{tr.v1x, tr.v1y, tr.v1z}, v2{tr.v2x, tr.v2y, tr.v2z};
     const float b1 = hr.y, b2 = hr.z, b0 = 1.0f - b1 - b2;
     const V3 hp{v0.x * b0 + v1.x * b1 + v2.x * b2, v0.y * b0 + v1.y * b1 + v2.y * b2, v0.z * b0 + v1.z * b1 + v2.z * b2};
     V3 g = cross(v1 - v0, v2 - v0);
     g = g * rsqrtf(dot(g, g));
     const V3 wo{-dr.x, -dr.y, -dr.z};
     if (dot(g, wo) < 0.0f) g = -g;
-    const Material& m = S.materials[S.triMaterial[prim]];
+    const Material& m = S.materials[tr.material];
     const float rough = m.bxdf == 0 ? m.mfRoughness : ((m.factors & 4u) ? m.roughness : 1.0f);
     out[0] = clampf((hp.x - f.gmin[0]) * f.ginv[0], 0.0f, 1.0f);
     out[1] = clampf((hp.y - f.gmin[1]) * f.ginv[1], 0.0f, 1.0f);
@@ -480,7 +479,7 @@ __global__ __launch_bounds__(kTraceBlock, WAVES) void k_shadow(Frame1 f) {
             }
             if (chunk.done(__ballot(active) != 0ull)) break;
             if (active && trav_step<true>(nodes, tris, ray, st)) {
-                if (ray.h.prim == 0xFFFFFFFFu) add_nee(f, pid);
+                if (ray.h.leaf == 0xFFFFFFFFu) add_nee(f, pid);
                 active = false;
             }
         }
@@ -492,7 +491,7 @@ __global__ __launch_bounds__(kTraceBlock, WAVES) void k_shadow(Frame1 f) {
                 const float4 so = f.p.sro[pid], sd = f.p.srd[pid];
                 const Hit h = traverse4<true>(scene_nodes<QN>(S), S.tris, S.nnodes4, V3{so.x, so.y, so.z},
                                               V3{sd.x, sd.y, sd.z}, so.w, st);
-                if (h.prim == 0xFFFFFFFFu) add_nee(f, pid);
+                if (h.leaf == 0xFFFFFFFFu) add_nee(f, pid);
                 ++nrays;
             }
         }
@@ -572,23 +571,24 @@ __global__ __launch_bounds__(kShadeBlock, WAVES) void k_shade(Frame1 f, uint32_t
             P.wl = slot_wavelengths(S, sk, px, py);
             const Qrng q = qrng_at(S.sample + sk, depth, pixel_hash(px, py));
 
-            // ---- interaction (pathtrace.cu:86-149, 906-953)
-            const uint32_t prim = __float_as_uint(hr.w);
+            // ---- interaction (pathtrace.cu:86-149, 906-953): the hit's
+            // leaf-order triangle record holds its vertices, vertex
+            // indices, material and light index in one 64-byte line
+            const TriRecord& tr = S.tris[__float_as_uint(hr.w)];
             // k_trace queues only paths that hit a triangle
             uint32_t ltype = LT_NONE, lindex = 0;
             const float b1 = hr.y, b2 = hr.z;
-            const uint32_t vi0 = S.indices[3 * prim], vi1 = S.indices[3 * prim + 1], vi2 = S.indices[3 * prim + 2];
-            const V3 v0 = ld3(S.verts + 3 * (size_t)vi0), v1 = ld3(S.verts + 3 * (size_t)vi1),
-                     v2 = ld3(S.verts + 3 * (size_t)vi2);
+            const uint32_t vi0 = tr.i0, vi1 = tr.i1, vi2 = tr.i2;
+            const V3 v0{tr.v0x, tr.v0y, tr.v0z}, v1{tr.v1x, tr.v1y, tr.v1z}, v2{tr.v2x, tr.v2y, tr.v2z};
             V3 gn;
             {
                 V3 g = cross(v1 - v0, v2 - v0);
                 g = g * rsqrtf(dot(g, g));
                 gn = oct_dec_u32(oct_enc_u32(g));
             }
-            const uint32_t mat = S.triMaterial[prim];
+            const uint32_t mat = tr.material;
             {
-                const uint32_t tl = S.triLight[prim];
+                const uint32_t tl = tr.light;
                 if (tl != 0xFFFFFFFFu) { ltype = LT_TRIANGLE; lindex = tl; }
             }
             P.flags |= PF_TRIANGULAR;

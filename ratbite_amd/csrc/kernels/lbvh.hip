@@ -151,6 +151,8 @@ struct Box { float lo[3], hi[3]; };
 
 // Triangle records in leaf (Morton) order and their boxes.
 __global__ __launch_bounds__(kBuildBlock) void k_leaves(const float* __restrict__ v, const uint32_t* __restrict__ idx,
+                                                        const uint32_t* __restrict__ triMaterial,
+                                                        const uint32_t* __restrict__ triLight,
                                                         const unsigned long long* __restrict__ keys, int n, int idxBits,
                                                         float* __restrict__ lbox, TriRecord* __restrict__ tris) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -159,8 +161,9 @@ __global__ __launch_bounds__(kBuildBlock) void k_leaves(const float* __restrict_
     const uint32_t a = idx[3 * tri], b = idx[3 * tri + 1], c = idx[3 * tri + 2];
     TriRecord r;
     r.v0x = v[3 * a]; r.v0y = v[3 * a + 1]; r.v0z = v[3 * a + 2]; r.prim = tri;
-    r.v1x = v[3 * b]; r.v1y = v[3 * b + 1]; r.v1z = v[3 * b + 2]; r.pad0 = 0;
-    r.v2x = v[3 * c]; r.v2y = v[3 * c + 1]; r.v2z = v[3 * c + 2]; r.pad1 = 0;
+    r.v1x = v[3 * b]; r.v1y = v[3 * b + 1]; r.v1z = v[3 * b + 2]; r.material = triMaterial[tri];
+    r.v2x = v[3 * c]; r.v2y = v[3 * c + 1]; r.v2z = v[3 * c + 2]; r.light = triLight[tri];
+    r.i0 = a; r.i1 = b; r.i2 = c; r.pad = 0u;
     tris[j] = r;
     float* lb = lbox + 6 * (size_t)j;
     lb[0] = fminf(fminf(r.v0x, r.v1x), r.v2x); lb[3] = fmaxf(fmaxf(r.v0x, r.v1x), r.v2x);
@@ -443,8 +446,8 @@ int lbvh_build(rbh_ctx c) {
     RBH_HIP(c, hipcub::DeviceRadixSort::SortKeys(s + oSort, sortBytes, keys, keys2, n, 0, 64, c->stream));
     const int nbi = (int)((nInt + kBuildBlock - 1) / kBuildBlock);
     if (n > 1) hipLaunchKernelGGL(k_karras, dim3(nbi), dim3(kBuildBlock), 0, c->stream, keys2, n, child, parent);
-    hipLaunchKernelGGL(k_leaves, dim3(nb), dim3(kBuildBlock), 0, c->stream, v, idx, keys2, n, idxBits, lbox,
-                       acc.tris.as<TriRecord>());
+    hipLaunchKernelGGL(k_leaves, dim3(nb), dim3(kBuildBlock), 0, c->stream, v, idx, g.triMaterial.as<uint32_t>(),
+                       g.triLight.as<uint32_t>(), keys2, n, idxBits, lbox, acc.tris.as<TriRecord>());
     if (n > 1) {
         int cus = 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0) cus = 256;

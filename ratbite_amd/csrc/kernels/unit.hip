@@ -88,8 +88,8 @@ __global__ __launch_bounds__(128) void k_trace_test(const BvhNode* nodes, const 
                 : tree == 2 ? traverse4<ANY>(nodes4q, tris, nnodes4, ro, rd, r[7], st)
                             : traverse4<ANY>(nodes4, tris, nnodes4, ro, rd, r[7], st);
     float* o = hits + 4 * (size_t)i;
-    o[0] = h.prim == 0xFFFFFFFFu ? -1.0f : h.t;
-    o[1] = __uint_as_float(h.prim);
+    o[0] = h.leaf == 0xFFFFFFFFu ? -1.0f : h.t;
+    o[1] = __uint_as_float(h.leaf == 0xFFFFFFFFu ? 0xFFFFFFFFu : tris[h.leaf].prim);
     o[2] = h.b1;
     o[3] = h.b2;
 }

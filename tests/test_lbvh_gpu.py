@@ -169,7 +169,8 @@ def test_spheres_scene_depth_and_hits(ctx):
 NODE_DTYPE = np.dtype([(k, "f4") for k in ("lminx", "lmaxx", "lminy", "lmaxy", "rminx", "rmaxx", "rminy", "rmaxy",
                                            "lminz", "lmaxz", "rminz", "rmaxz")] +
                       [("left", "i4"), ("right", "i4"), ("lcnt", "u4"), ("rcnt", "u4")])
-TRI_DTYPE = np.dtype([("v0", "f4", 3), ("prim", "u4"), ("v1", "f4", 3), ("pad0", "u4"), ("v2", "f4", 3), ("pad1", "u4")])
+TRI_DTYPE = np.dtype([("v0", "f4", 3), ("prim", "u4"), ("v1", "f4", 3), ("material", "u4"), ("v2", "f4", 3),
+                      ("light", "u4"), ("idx", "u4", 3), ("pad", "u4")])
 
 
 def _read_lbvh(ctx, n):
@@ -218,6 +219,8 @@ def test_morton_keys_bit_exact_and_tree_invariants(ctx, which):
     # leaf order and triangle records
     assert np.array_equal(tris["prim"], (keys & np.uint64((1 << b) - 1)).astype(np.uint32))
     assert np.array_equal(tris["v0"], v[idx[tris["prim"], 0]]) and np.array_equal(tris["v2"], v[idx[tris["prim"], 2]])
+    assert np.array_equal(tris["idx"], idx[tris["prim"]])
+    assert np.all(tris["material"] == 0) and np.all(tris["light"] == 0xFFFFFFFF)
     # topology: every internal node but the root and every leaf is a child exactly once
     assert len(nodes) == n - 1
     ch = np.concatenate([nodes["left"], nodes["right"]])
