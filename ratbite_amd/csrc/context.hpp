@@ -52,6 +52,7 @@ struct EnvMap {
 struct LightTreeDev {
     uint32_t nnodes = 0, nlights = 0;
     DevBuf nodes, lightPointers, bitmasks, tris;
+    DevBuf nodesX;               // expanded nodes the kernels traverse (decoded direction, SG integral)
     std::vector<rbh_packed_node> hostNodes;
     std::vector<uint32_t> hostPointers;
     std::vector<uint64_t> hostMasks;

@@ -90,9 +90,20 @@ __device__ __forceinline__ float sg_clamped_cos_integral_over_pi(float cosine, f
 struct Node {
     V3 mean, dir;
     float variance, flux, sharpness;
+    float integral;          // sg_integral(sharpness)
     bool leaf;
     uint32_t right, count, offset;
 };
+
+// A packed node as the traversal reads it: the decoded direction and the
+// node's spherical-Gaussian integral computed once when the tree is built
+// (by the same device functions, so importances are bit-identical), 48 B.
+struct TreeNodeX {
+    float4 meanVar;          // mean, variance
+    float4 dirSharp;         // decoded average direction, sharpness
+    float4 fluxIntCore;      // flux, sg_integral(sharpness), core bits, 0
+};
+static_assert(sizeof(TreeNodeX) == 48, "TreeNodeX is 48 bytes");
 
 // light_tree_types.h:165-189; the direction is decoded from [0,1]-scaled
 // octahedral components without the [-1,1] remap, as the reference does.
@@ -107,6 +118,30 @@ __device__ __forceinline__ Node load_node(const PackedNode* __restrict__ nodes, 
     n.sharpness = b.y;
     n.flux = b.z;
     const uint32_t core = __float_as_uint(b.w);
+    n.leaf = (core >> 31) != 0u;
+    n.right = core;
+    n.count = (core >> 27) & 15u;
+    n.offset = core & ((1u << 27) - 1u);
+    n.integral = sg_integral(n.sharpness);
+    return n;
+}
+__device__ __forceinline__ TreeNodeX expand_node(const Node& n) {
+    TreeNodeX x;
+    x.meanVar = make_float4(n.mean.x, n.mean.y, n.mean.z, n.variance);
+    x.dirSharp = make_float4(n.dir.x, n.dir.y, n.dir.z, n.sharpness);
+    x.fluxIntCore = make_float4(n.flux, n.integral, __uint_as_float(n.right), 0.0f);
+    return x;
+}
+__device__ __forceinline__ Node load_node(const TreeNodeX* __restrict__ nodes, uint32_t i) {
+    const TreeNodeX x = nodes[i];
+    Node n;
+    n.mean = {x.meanVar.x, x.meanVar.y, x.meanVar.z};
+    n.variance = x.meanVar.w;
+    n.dir = {x.dirSharp.x, x.dirSharp.y, x.dirSharp.z};
+    n.sharpness = x.dirSharp.w;
+    n.flux = x.fluxIntCore.x;
+    n.integral = x.fluxIntCore.y;
+    const uint32_t core = __float_as_uint(x.fluxIntCore.z);
     n.leaf = (core >> 31) != 0u;
     n.right = core;
     n.count = (core >> 27) & 15u;
@@ -131,11 +166,11 @@ __device__ __forceinline__ float node_importance(const Node& n, V3 pos, V3 nrmv)
     const V3 axis = ax / fmaxf(sh, 1.17549435e-38f);
     const float amp = expf(logA);
     const float c = clampf(dot(axis, nrmv), -1.0f, 1.0f);
-    return (n.flux / (var * sg_integral(n.sharpness))) * amp * sg_clamped_cos_integral_over_pi(c, sh);
+    return (n.flux / (var * n.integral)) * amp * sg_clamped_cos_integral_over_pi(c, sh);
 }
 
 struct LightTree {
-    const PackedNode* __restrict__ nodes;   // nullptr: no tree
+    const TreeNodeX* __restrict__ nodes;    // nullptr: no tree
     const uint32_t* __restrict__ pointers;
     const uint64_t* __restrict__ masks;
     bool env;

@@ -5,6 +5,9 @@
 #include "device/vec.hpp"
 
 namespace rbh {
+namespace dev {
+struct TreeNodeX;
+}
 
 constexpr int kTraceBlock = 128;   // lanes per block of the trace and shadow kernels
 constexpr int kShadeBlock = 128;
@@ -68,7 +71,7 @@ struct SceneView {
     const float* colors;
     const float* luts;
     // lights
-    const PackedNode* treeNodes;
+    const dev::TreeNodeX* treeNodes;   // expanded light-tree nodes (48 B)
     const uint32_t* treePointers;
     const uint64_t* treeMasks;
     const EmissiveTri* lights;

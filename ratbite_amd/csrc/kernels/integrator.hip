@@ -1012,7 +1012,22 @@ __global__ __launch_bounds__(256) void k_accumulate(Frame1 f) {
 
 }  // namespace
 
+// Expanded light-tree nodes: the packed nodes decoded once.
+__global__ __launch_bounds__(256) void k_expand_tree(const PackedNode* __restrict__ in, TreeNodeX* __restrict__ out,
+                                                     uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = expand_node(load_node(in, i));
+}
+
 // ------------------------------------------------------------------ host
+hipError_t light_tree_expand(rbh_ctx c) {
+    const uint32_t n = c->tree.nnodes;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_expand_tree, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->tree.nodes.as<PackedNode>(),
+                       c->tree.nodesX.as<TreeNodeX>(), n);
+    return hipGetLastError();
+}
+
 // Spill entries per lane of the traversal stack beyond the LDS part.
 size_t trace_spill_words(size_t lanes) { return lanes * (size_t)(kMaxStack4 - kStackLds); }
 

@@ -342,7 +342,7 @@ int rbh_test_lbvh_read(rbh_ctx c, uint64_t* keys, void* nodes, void* tris, uint3
 }
 
 static LightTree tree_of(rbh_ctx c) {
-    return LightTree{c->tree.nlights ? c->tree.nodes.as<PackedNode>() : nullptr, c->tree.lightPointers.as<uint32_t>(),
+    return LightTree{c->tree.nlights ? c->tree.nodesX.as<TreeNodeX>() : nullptr, c->tree.lightPointers.as<uint32_t>(),
                      c->tree.bitmasks.as<uint64_t>(), c->env.enabled};
 }
 

@@ -13,6 +13,7 @@ int lbvh_build(rbh_ctx c);
 hipError_t launch_pass(rbh_ctx c, const Frame1& f, uint32_t depthMax, const LaunchShape& ls, KernelTimer* kt);
 hipError_t resolve_launch(rbh_ctx c, const float* colorspace, float expo, void* out);
 size_t trace_spill_words(size_t lanes);
+hipError_t light_tree_expand(rbh_ctx c);
 namespace host {
 int build_light_tree(const EmissiveTri* tris, const float* flux, uint32_t n, std::vector<PackedNode>& nodes,
                      std::vector<uint32_t>& pointers, std::vector<uint64_t>& masks, bool fp64);
@@ -98,6 +99,8 @@ int rbh_build_light_tree_ex(rbh_ctx c, const rbh_emissive_tri* tris, const float
     RBH_HIP(c, hipMemcpyAsync(t.lightPointers.p, t.hostPointers.data(), (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
     RBH_HIP(c, hipMemcpyAsync(t.bitmasks.p, t.hostMasks.data(), (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
     RBH_HIP(c, hipMemcpyAsync(t.tris.p, tris, (size_t)n * sizeof(EmissiveTri), hipMemcpyHostToDevice, c->stream));
+    RBH_ALLOC(c, t.nodesX, nodes.size() * 48);
+    RBH_HIP(c, light_tree_expand(c));
     RBH_HIP(c, hipStreamSynchronize(c->stream));
     return RBH_OK;
 }
@@ -205,7 +208,7 @@ void fill_view(rbh_ctx c, const rbh_launch_params* lp, SceneView& v) {
     v.spectra = c->spectra.as<float>();
     v.colors = c->colorTables.as<float>();
     v.luts = c->luts.as<float>();
-    v.treeNodes = treeOn ? c->tree.nodes.as<PackedNode>() : nullptr;
+    v.treeNodes = treeOn ? c->tree.nodesX.as<dev::TreeNodeX>() : nullptr;
     v.treePointers = treeOn ? c->tree.lightPointers.as<uint32_t>() : nullptr;
     v.treeMasks = treeOn ? c->tree.bitmasks.as<uint64_t>() : nullptr;
     v.lights = treeOn ? c->tree.tris.as<EmissiveTri>() : nullptr;
