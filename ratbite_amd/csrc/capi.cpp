@@ -9,6 +9,7 @@
 
 namespace rbh {
 hipError_t envmap_cdf_launch(rbh_ctx c, bool fp64);
+hipError_t srgb_lut_fill(rbh_ctx c);
 size_t envmap_scratch_bytes(int h);
 }  // namespace rbh
 
@@ -127,6 +128,8 @@ int rbh_upload_textures(rbh_ctx c, const rbh_texture_desc* descs, uint32_t n, co
     RBH_ALLOC(c, c->texels, blobBytes ? blobBytes : 16);
     if (n) RBH_HIP(c, hipMemcpyAsync(c->textures.p, descs, (size_t)n * sizeof(TextureDesc), hipMemcpyHostToDevice, c->stream));
     if (blobBytes) RBH_HIP(c, hipMemcpyAsync(c->texels.p, blob, blobBytes, hipMemcpyHostToDevice, c->stream));
+    RBH_ALLOC(c, c->srgbLut, 256 * sizeof(float));
+    RBH_HIP(c, rbh::srgb_lut_fill(c));
     RBH_HIP(c, hipStreamSynchronize(c->stream));
     c->ntextures = n;
     return RBH_OK;

@@ -162,6 +162,7 @@ struct rbh_context {
     rbh::DevBuf luts;         // conductor | dielOut | dielIn | reflOut | reflIn | sheen
     bool haveLuts = false;
     rbh::DevBuf textures, texels; uint32_t ntextures = 0;
+    rbh::DevBuf srgbLut;         // sRGB decode of the 256 8-bit values, filled with the textures
     rbh::DevBuf materials; uint32_t nmaterials = 0;
     rbh::Geometry geom;
     rbh::Accel accel;

@@ -5,6 +5,7 @@
 #pragma once
 #include "vec.hpp"
 #include "types.hpp"
+#include "spectral.hpp"
 
 namespace rbh {
 namespace dev {
@@ -69,17 +70,25 @@ __device__ __forceinline__ float4 lut2x4(const float* __restrict__ t, float u, f
     return r;
 }
 
-__device__ __forceinline__ float4 texel(const TextureDesc& d, const uint8_t* __restrict__ blob, int x, int y) {
+// 8-bit sRGB channels are decoded through a 256-entry table that the
+// context fills with srgb_decode(k / 255) (the same function the float
+// formats use), so the results are those of decoding each texel.
+__device__ __forceinline__ float4 texel(const TextureDesc& d, const uint8_t* __restrict__ blob, int x, int y,
+                                        const float* __restrict__ srgbLut) {
     const size_t i = (size_t)y * d.width + x;
     const uint8_t* base = blob + d.offset;
     float4 o;
     switch (d.format) {
         case 0: {
             const uchar4 c = reinterpret_cast<const uchar4*>(base)[i];
+            if (d.srgb) return make_float4(srgbLut[c.x], srgbLut[c.y], srgbLut[c.z], c.w / 255.0f);
             o = make_float4(c.x / 255.0f, c.y / 255.0f, c.z / 255.0f, c.w / 255.0f);
             break;
         }
-        case 1: o = make_float4(base[i] / 255.0f, 0.0f, 0.0f, 1.0f); break;
+        case 1:
+            if (d.srgb) return make_float4(srgbLut[base[i]], srgbLut[0], srgbLut[0], 1.0f);
+            o = make_float4(base[i] / 255.0f, 0.0f, 0.0f, 1.0f);
+            break;
         case 2: o = reinterpret_cast<const float4*>(base)[i]; break;
         case 3: {
             const float* f = reinterpret_cast<const float*>(base) + 3 * i;
@@ -89,26 +98,27 @@ __device__ __forceinline__ float4 texel(const TextureDesc& d, const uint8_t* __r
         default: o = make_float4(reinterpret_cast<const float*>(base)[i], 0.0f, 0.0f, 1.0f); break;
     }
     if (d.srgb) {
-        o.x = o.x < 0.04045f ? o.x / 12.92f : powf((o.x + 0.055f) / 1.055f, 2.4f);
-        o.y = o.y < 0.04045f ? o.y / 12.92f : powf((o.y + 0.055f) / 1.055f, 2.4f);
-        o.z = o.z < 0.04045f ? o.z / 12.92f : powf((o.z + 0.055f) / 1.055f, 2.4f);
+        o.x = srgb_decode(o.x);
+        o.y = srgb_decode(o.y);
+        o.z = srgb_decode(o.z);
     }
     return o;
 }
 
 __device__ __forceinline__ float4 tex2d(const TextureDesc* __restrict__ descs, const uint8_t* __restrict__ blob, int ti,
-                                        float u, float v) {
+                                        float u, float v, const float* __restrict__ srgbLut) {
     const TextureDesc d = descs[ti];
     const int W = (int)d.width, H = (int)d.height;
     if (d.filter == 0)
-        return texel(d, blob, wrap_index((int)floorf(u * W), W, d.addr[0]), wrap_index((int)floorf(v * H), H, d.addr[1]));
+        return texel(d, blob, wrap_index((int)floorf(u * W), W, d.addr[0]), wrap_index((int)floorf(v * H), H, d.addr[1]),
+                     srgbLut);
     const float x = u * W - 0.5f, y = v * H - 0.5f;
     const float fx0 = floorf(x), fy0 = floorf(y);
     const float ax = x - fx0, ay = y - fy0;
     const int x0 = wrap_index((int)fx0, W, d.addr[0]), x1 = wrap_index((int)fx0 + 1, W, d.addr[0]);
     const int y0 = wrap_index((int)fy0, H, d.addr[1]), y1 = wrap_index((int)fy0 + 1, H, d.addr[1]);
-    const float4 c00 = texel(d, blob, x0, y0), c10 = texel(d, blob, x1, y0);
-    const float4 c01 = texel(d, blob, x0, y1), c11 = texel(d, blob, x1, y1);
+    const float4 c00 = texel(d, blob, x0, y0, srgbLut), c10 = texel(d, blob, x1, y0, srgbLut);
+    const float4 c01 = texel(d, blob, x0, y1, srgbLut), c11 = texel(d, blob, x1, y1, srgbLut);
     float4 r;
     r.x = (c00.x * (1.0f - ax) + c10.x * ax) * (1.0f - ay) + (c01.x * (1.0f - ax) + c11.x * ax) * ay;
     r.y = (c00.y * (1.0f - ax) + c10.y * ax) * (1.0f - ay) + (c01.y * (1.0f - ax) + c11.y * ax) * ay;

@@ -67,6 +67,7 @@ struct SceneView {
     const Material* materials;
     const TextureDesc* texDescs;
     const uint8_t* texels;
+    const float* srgbLut;    // 256 sRGB-decoded 8-bit values
     const float* spectra;
     const float* colors;
     const float* luts;

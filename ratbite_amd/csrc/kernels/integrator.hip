@@ -508,7 +508,7 @@ __device__ __forceinline__ S4 tri_emission(const SceneView& S, const EmissiveTri
     if (m.textures & 64u) {
         const float u = ba * tri.uv[0] + bb * tri.uv[2] + bc * tri.uv[4];
         const float v = ba * tri.uv[1] + bb * tri.uv[3] + bc * tri.uv[5];
-        const float4 t = tex2d(S.texDescs, S.texels, m.tex[6], u, v);
+        const float4 t = tex2d(S.texDescs, S.texels, m.tex[6], u, v, S.srgbLut);
         e = V3{e.x * t.x, e.y * t.y, e.z * t.z};
     }
     return ct.rgb(e, wl);
@@ -694,7 +694,7 @@ __global__ __launch_bounds__(kShadeBlock, WAVES) void k_shade(Frame1 f, uint32_t
                 V3 n{0.0f, 0.0f, 1.0f};
                 if (m.textures & 2u) {
                     const V2 uv = uvOf(1);
-                    const float4 t = tex2d(S.texDescs, S.texels, m.tex[1], uv.x, uv.y);
+                    const float4 t = tex2d(S.texDescs, S.texels, m.tex[1], uv.x, uv.y, S.srgbLut);
                     n = nrm(V3{t.x * 2.0f - 1.0f, t.y * 2.0f - 1.0f, t.z * 2.0f - 1.0f});
                 }
                 n = tbn.to_world(n);
@@ -725,7 +725,7 @@ __global__ __launch_bounds__(kShadeBlock, WAVES) void k_shade(Frame1 f, uint32_t
                     float alphaV = 0.0f;
                     if (bcTex) {
                         const V2 uv = uvOf(1);  // the reference reads the normal map's set here (pathtrace.cu:1453)
-                        const float4 t = tex2d(S.texDescs, S.texels, m.tex[0], uv.x, uv.y);
+                        const float4 t = tex2d(S.texDescs, S.texels, m.tex[0], uv.x, uv.y, S.srgbLut);
                         s.base = V3{t.x, t.y, t.z};
                         alphaV = t.w;
                         if (bcFac) {
@@ -744,7 +744,7 @@ __global__ __launch_bounds__(kShadeBlock, WAVES) void k_shade(Frame1 f, uint32_t
                     const bool metF = (m.factors & 2u) != 0, rF = (m.factors & 4u) != 0;
                     if (m.textures & 4u) {
                         const V2 uv = uvOf(2);
-                        const float4 t = tex2d(S.texDescs, S.texels, m.tex[2], uv.x, uv.y);
+                        const float4 t = tex2d(S.texDescs, S.texels, m.tex[2], uv.x, uv.y, S.srgbLut);
                         s.metalness = t.z * (metF ? m.metalness : 1.0f);
                         s.roughness = t.y * (rF ? m.roughness : 1.0f);
                     } else {
@@ -754,7 +754,7 @@ __global__ __launch_bounds__(kShadeBlock, WAVES) void k_shade(Frame1 f, uint32_t
                     const bool trF = (m.factors & 8u) != 0;
                     if (m.textures & 8u) {
                         const V2 uv = uvOf(3);
-                        const float4 t = tex2d(S.texDescs, S.texels, m.tex[3], uv.x, uv.y);
+                        const float4 t = tex2d(S.texDescs, S.texels, m.tex[3], uv.x, uv.y, S.srgbLut);
                         s.transmission = t.x * (trF ? m.transmission : 1.0f);
                     } else if (trF) {
                         s.transmission = m.transmission;
@@ -765,7 +765,7 @@ __global__ __launch_bounds__(kShadeBlock, WAVES) void k_shade(Frame1 f, uint32_t
                         const bool shcF = (m.factors & 64u) != 0;
                         if (m.textures & 16u) {
                             const V2 uv = uvOf(4);
-                            const float4 t = tex2d(S.texDescs, S.texels, m.tex[4], uv.x, uv.y);
+                            const float4 t = tex2d(S.texDescs, S.texels, m.tex[4], uv.x, uv.y, S.srgbLut);
                             s.sheenColor = V3{t.x * (shcF ? m.sheenColor[0] : 1.0f), t.y * (shcF ? m.sheenColor[1] : 1.0f),
                                               t.z * (shcF ? m.sheenColor[2] : 1.0f)};
                         } else if (shcF) {
@@ -774,7 +774,7 @@ __global__ __launch_bounds__(kShadeBlock, WAVES) void k_shade(Frame1 f, uint32_t
                         const bool shrF = (m.factors & 128u) != 0;
                         if (m.textures & 32u) {
                             const V2 uv = uvOf(5);
-                            const float4 t = tex2d(S.texDescs, S.texels, m.tex[5], uv.x, uv.y);
+                            const float4 t = tex2d(S.texDescs, S.texels, m.tex[5], uv.x, uv.y, S.srgbLut);
                             const float r = t.w * (shrF ? m.sheenRoughness : 1.0f);
                             s.sheenAlpha = r * r;
                         } else if (shrF) {
@@ -1019,7 +1019,18 @@ __global__ __launch_bounds__(256) void k_expand_tree(const PackedNode* __restric
     if (i < n) out[i] = expand_node(load_node(in, i));
 }
 
+// sRGB decode of the 256 8-bit channel values
+__global__ void k_srgb_lut(float* __restrict__ out) {
+    const uint32_t k = threadIdx.x;
+    if (k < 256u) out[k] = srgb_decode((float)k / 255.0f);
+}
+
 // ------------------------------------------------------------------ host
+hipError_t srgb_lut_fill(rbh_ctx c) {
+    hipLaunchKernelGGL(k_srgb_lut, dim3(1), dim3(256), 0, c->stream, c->srgbLut.as<float>());
+    return hipGetLastError();
+}
+
 hipError_t light_tree_expand(rbh_ctx c) {
     const uint32_t n = c->tree.nnodes;
     if (n == 0) return hipSuccess;

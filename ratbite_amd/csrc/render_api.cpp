@@ -205,6 +205,7 @@ void fill_view(rbh_ctx c, const rbh_launch_params* lp, SceneView& v) {
     v.materials = c->materials.as<Material>();
     v.texDescs = c->textures.as<TextureDesc>();
     v.texels = c->texels.as<uint8_t>();
+    v.srgbLut = c->srgbLut.as<float>();
     v.spectra = c->spectra.as<float>();
     v.colors = c->colorTables.as<float>();
     v.luts = c->luts.as<float>();
