@@ -277,6 +277,7 @@ int orc_math_unit(int fn, const float* in, float* out, uint32_t n) {
                 break;
             }
             case 17: o[0] = power_heuristic(a[0], a[1]); break;
+            case 18: o[0] = pow5(a[0]); o[1] = schlick(a[1], a[0]); break;
             case 16: {
                 const Wavelengths wl = sample_visible(a[0]);
                 o[0] = wl.lambda.v[0]; o[1] = wl.pdf.v[0]; o[2] = wl.lambda.v[1]; o[3] = wl.pdf.v[1];

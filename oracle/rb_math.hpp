@@ -596,9 +596,15 @@ inline float fresnel_complex1(float ct, float er, float ei) {
     float rpe = (cr * cr + ci * ci) / (dr * dr + di * di);
     return (rpa + rpe) / 2.0f;
 }
+// (1 - mu)^5 of surface.h:245-252 (the reference's pow(x, 5)), evaluated as
+// a double product rounded once, the same bits on the CPU and the GPU
+inline float pow5(float x) {
+    const double a = (double)x * (double)x;
+    return (float)(a * a * (double)x);
+}
 // src/device/surface.h:245-252
-inline float schlick(float f0, float mu) { return f0 + (1.0f - f0) * std::pow(1.0f - mu, 5.0f); }
-inline V3 schlick3(V3 f0, float mu) { float p = std::pow(1.0f - mu, 5.0f); return f0 + (v3s(1.0f) - f0) * p; }
+inline float schlick(float f0, float mu) { return f0 + (1.0f - f0) * pow5(1.0f - mu); }
+inline V3 schlick3(V3 f0, float mu) { float p = pow5(1.0f - mu); return f0 + (v3s(1.0f) - f0) * p; }
 // src/device/surface.h:254-264 (specularColor = 1)
 inline V3 f82(V3 f0, float mu) {
     const float muD = 1.0f / 7.0f;

@@ -82,11 +82,17 @@ __device__ __forceinline__ float fresnel_conductor(float ct, float er, float ei)
     const float rpe = (cr * cr + ei2 * ei2) / (dr * dr + ei2 * ei2);
     return (rpa + rpe) / 2.0f;
 }
-__device__ __forceinline__ float schlick(float f0, float mu) { return f0 + (1.0f - f0) * powf(1.0f - mu, 5.0f); }
+// (1 - mu)^5 of surface.h:245-252 (the reference's pow(x, 5)) as a double
+// product rounded once: a few FP64 multiplies instead of a general powf
+__device__ __forceinline__ float pow5(float x) {
+    const double a = (double)x * (double)x;
+    return (float)(a * a * (double)x);
+}
+__device__ __forceinline__ float schlick(float f0, float mu) { return f0 + (1.0f - f0) * pow5(1.0f - mu); }
 // F82 with specularColor = 1 is Schlick (surface.h:254-264), times the
 // Turquin multiple-scattering factor (1 + F_avg (1 - E) / E)
 __device__ __forceinline__ V3 conductor_color(V3 f0, float mu, float E) {
-    const float p = powf(1.0f - mu, 5.0f);
+    const float p = pow5(1.0f - mu);
     const V3 fav = 0.877105f * f0 + (V3{0.0648148f, 0.0648148f, 0.0648148f} + 0.0752755f * f0) - V3{0.0171958f, 0.0171958f, 0.0171958f};
     const float k = (1.0f - E) / E;
     return V3{f0.x + (1.0f - f0.x) * p, f0.y + (1.0f - f0.y) * p, f0.z + (1.0f - f0.z) * p} *

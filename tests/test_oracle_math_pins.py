@@ -313,3 +313,20 @@ def test_power_heuristic_and_its_overflow_guard():
     ref = np.array([0.5, 0.8, 1.0 / 17.0, 1.0, 1.0, 0.0, 0.0])
     np.testing.assert_allclose(w, ref, rtol=1e-6)
     assert np.all(np.isfinite(w))
+
+
+def test_pow5_is_the_correctly_rounded_fifth_power():
+    # the (1 - mu)^5 of the Schlick/F82 Fresnel terms: a double product rounded
+    # once, equal to the exactly rounded x^5 (rational arithmetic) for every
+    # tested x, and Schlick's f0 + (1 - f0) (1 - mu)^5 built on it
+    from fractions import Fraction
+    rng = np.random.default_rng(5)
+    x = np.concatenate([rng.random(4000), np.linspace(0, 1, 1001), [1e-8, 0.5, 1.0 - 2 ** -24]]).astype(np.float32)
+    f0 = rng.random(len(x)).astype(np.float32)
+    out = mu(18, _rec(len(x), c0=x, c1=f0))
+    exact = np.array([float(Fraction(float(v)) ** 5) for v in x]).astype(np.float32)
+    assert np.array_equal(out[:, 0], exact)
+    mu_ = (np.float32(1) - x).astype(np.float32)       # schlick(f0, mu) with mu = x uses (1 - x)^5
+    p5 = np.array([float(Fraction(float(v)) ** 5) for v in mu_]).astype(np.float32)
+    ref = (f0 + (np.float32(1) - f0) * p5).astype(np.float32)
+    assert np.array_equal(out[:, 1], ref)
