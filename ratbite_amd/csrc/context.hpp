@@ -135,7 +135,7 @@ struct RenderState {
     // path slot); rbh_autotune / rbh_set_tuning replace it per context
     int traceBlocksPerCU = 64;
     int shadeBlocksPerCU = 64;
-    int shadeWaves = 3;          // shade-kernel variant (1, 3, 4 waves per SIMD)
+    int shadeWaves = 4;          // shade-kernel variant (1, 3, 4 waves per SIMD)
     int traceFetch = 0;          // 1: trace/shadow lanes fetch a new ray as soon as theirs completes
     int traceWaves = 7;          // trace/shadow register budget (1, 7, 8 waves per SIMD)
     int quantNodes = 0;          // traverse the 64-byte quantised four-wide nodes
