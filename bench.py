@@ -95,7 +95,17 @@ def cpu_baseline(sc, args, tree):
     lp = sc.launch_params(offset=0, count=args.cpu_spp, depth=args.depth)
     _, st = O.render(sc, lp, threads=threads, crop=crop, oscene=O.OracleScene(sc, tree=tree))
     v = cw * ch * args.cpu_spp / st["renderSeconds"]
-    return {"value": v, "unit": "samples/s", "cores": threads, "kind": "port",
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fi:
+            for line in fi:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": v, "unit": "samples/s", "cores": threads, "kind": "port", "nproc": os.cpu_count(),
+            "cpu_model": model,
             "sample": "centre %dx%d crop of the same frame at %d spp, depth %d (%.1f s of path tracing; "
                       "CPU BVH build of %.1f s excluded)" % (cw, ch, args.cpu_spp, args.depth, st["renderSeconds"],
                                                              st["bvhSeconds"])}
