@@ -104,11 +104,39 @@ def cpu_baseline(sc, args, tree):
                     break
     except OSError:
         pass
+    c1 = cpu_c1(threads)
     return {"value": v, "unit": "samples/s", "cores": threads, "kind": "port", "nproc": os.cpu_count(),
-            "cpu_model": model,
+            "cpu_model": model, "c1": c1,
             "sample": "centre %dx%d crop of the same frame at %d spp, depth %d (%.1f s of path tracing; "
                       "CPU BVH build of %.1f s excluded)" % (cw, ch, args.cpu_spp, args.depth, st["renderSeconds"],
                                                              st["bvhSeconds"])}
+
+
+def cpu_c1(threads):
+    """The CPU-runnable configuration in full (256x256 Cornell box with 8 UV
+    spheres and a 4000 K blackbody emitter, 16 spp, depth 8): the GPU render
+    (best of 3) beside the oracle on the same host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_lib as O
+    from ratbite_amd import scenes
+    from ratbite_amd.renderer import Renderer
+    sc = scenes.cornell()
+    r = Renderer(sc)
+    best = None
+    for _ in range(4):
+        r.ctx.synchronize()
+        t = time.perf_counter()
+        r.render(spp=sc.spp, offset=0)
+        r.ctx.synchronize()
+        el = time.perf_counter() - t
+        best = el if best is None else min(best, el)
+    lp = sc.launch_params(offset=0, count=sc.spp)
+    _, st = O.render(sc, lp, threads=threads, oscene=O.OracleScene(sc, tree=r.light_tree()))
+    r.close()
+    n = sc.width * sc.height * sc.spp
+    return {"workload": "cornell 256x256, 8 UV spheres, blackbody 4000 K emitter, 16 spp, depth 8", "samples": n,
+            "gpu_ms": best * 1e3, "gpu_samples_per_s": n / best, "cpu_seconds": st["renderSeconds"],
+            "cpu_samples_per_s": n / st["renderSeconds"]}
 
 
 def batch_workload(args):
