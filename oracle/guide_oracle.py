@@ -197,7 +197,8 @@ def loss_and_dout(out, rec):
     s = (wt / W * a * q / P)[:, None]                  # share of the lobes in P
     dlogit = -s * (g - w)
     # d(-log q)/d kappa = -g (1/kappa - 2/(e^{2k}-1) + dot - 1); kappa = exp(lk)
-    dlnorm = 1.0 / kappa - 2.0 / np.expm1(2.0 * kappa)
+    with np.errstate(over="ignore"):               # expm1 -> inf for large kappa: 2 / inf = 0, the limit
+        dlnorm = 1.0 / kappa - 2.0 / np.expm1(2.0 * kappa)
     dk = -s * g * (dlnorm + dot - 1.0)
     lk_raw = o[..., 3]
     inside = (lk_raw > LOG_KAPPA_MIN) & (lk_raw < LOG_KAPPA_MAX)
