@@ -133,8 +133,8 @@ struct RenderState {
     // launch shape: the fastest of a sweep of full 1080p 64-spp frames of the
     // 4.1M-triangle spheres scene (persistent grids capped at one lane per
     // path slot); rbh_autotune / rbh_set_tuning replace it per context
-    int traceBlocksPerCU = 64;
-    int shadeBlocksPerCU = 64;
+    int traceBlocksPerCU = 256;
+    int shadeBlocksPerCU = 256;
     int shadeWaves = 4;          // shade-kernel variant (1, 3, 4 waves per SIMD)
     int traceFetch = 0;          // 1: trace/shadow lanes fetch a new ray as soon as theirs completes
     int traceWaves = 7;          // trace/shadow register budget (1, 7, 8 waves per SIMD)

@@ -527,7 +527,7 @@ int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_
     t.flags &= ~(uint32_t)(RBH_RENDER_COLLECT | RBH_RENDER_TIMING);
     // the slot budget stays automatic (all samples of a launch in one pass
     // when memory allows); the grids of the persistent kernels are searched
-    const int gridList[] = {8, 16, 32, 64};
+    const int gridList[] = {8, 16, 32, 64, 128, 256};
     double bestRate = -1.0;
     rbh_tuning b{};
     (void)freeB;

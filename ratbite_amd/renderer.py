@@ -74,7 +74,7 @@ class Renderer:
                 "traceFetch": t.traceFetch, "traceWaves": t.traceWaves, "quantNodes": t.quantNodes,
                 "sortHits": t.sortHits}
 
-    def set_tuning(self, path_slots=0, trace_blocks=64, shade_blocks=64, use_graph=False, shade_waves=4,
+    def set_tuning(self, path_slots=0, trace_blocks=256, shade_blocks=256, use_graph=False, shade_waves=4,
                    trace_fetch=0, trace_waves=7, quant_nodes=False, sort_hits=True):
         """Sets the launch shape; the defaults are the library's own."""
         t = L.Tuning(path_slots, trace_blocks, shade_blocks, int(use_graph), shade_waves, int(trace_fetch),
