@@ -170,6 +170,17 @@ int rbh_upload_materials(rbh_ctx ctx, const rbh_material* mats, uint32_t n);
 int rbh_upload_geometry(rbh_ctx ctx, const float* verts, uint32_t nverts, const uint32_t* indices, uint32_t ntris,
                         const void* attribs, uint64_t attribBytes, const uint32_t* triMaterial,
                         const uint32_t* triLight);
+/* The same with 16-bit index buffers as well (the reference's per-geometry
+ * index type, read at pathtrace.cu:1295-1299 and set up at
+ * rendering_interface.cpp:201-301): indexData holds 3*ntris indices of
+ * indexFormat. */
+enum {
+    RBH_INDEX_U32 = 0,
+    RBH_INDEX_U16 = 1,
+};
+int rbh_upload_geometry_ex(rbh_ctx ctx, const float* verts, uint32_t nverts, const void* indexData, uint32_t indexFormat,
+                           uint32_t ntris, const void* attribs, uint64_t attribBytes, const uint32_t* triMaterial,
+                           const uint32_t* triLight);
 /* optixAccelBuild GAS/IAS (rendering_interface.cpp:733-1064): builds the LBVH */
 int rbh_build_accel(rbh_ctx ctx);
 
@@ -194,6 +205,15 @@ int rbh_light_tree_build_host(const rbh_emissive_tri* tris, const float* flux, u
                               uint32_t* lightPointers, uint64_t* bitmasks);
 int rbh_light_tree_build_host_ex(const rbh_emissive_tri* tris, const float* flux, uint32_t n, rbh_packed_node* nodes,
                                  uint32_t* lightPointers, uint64_t* bitmasks, uint32_t flags);
+/* fillLightTreeDataBuffers (rendering_interface.cpp:1470-1510): uploads a
+ * tree the caller built (nnodes packed nodes in pre-order: the left child of
+ * an inner node follows it, its right child index is in coreData; leaves of
+ * 1..15 lights), n light pointers (triangle lights), n bitmasks (the path of
+ * each light from the root) and the n emissive-triangle records. The tree is
+ * checked (every walk ends; every light's bitmask reaches the leaf holding
+ * it) before the kernels use it. */
+int rbh_set_light_tree(rbh_ctx ctx, const rbh_packed_node* nodes, uint32_t nnodes, const uint32_t* lightPointers,
+                       const uint64_t* bitmasks, const rbh_emissive_tri* tris, uint32_t n);
 /* Read the built tree back: nodes (<= 2n), lightPointers (n), bitmasks (n). */
 int rbh_get_light_tree(rbh_ctx ctx, rbh_packed_node* nodes, uint32_t* nnodes, uint32_t* lightPointers,
                        uint64_t* bitmasks);
@@ -262,13 +282,25 @@ int rbh_stats_get(rbh_ctx ctx, rbh_stats* out);
 /* ------------------------------------------------------------- path guiding
  * No reference counterpart: a learned proposal that extends the BSDF
  * sampling of src/kernels/pathtrace.cu:548-905 through one-sample MIS.
- * An MLP 80 -> 64 x4 (ReLU) -> 40 maps a shading point to 8 von Mises-Fisher
- * lobes; BF16 weights with FP32 accumulation and FP32 master weights, Adam.
- * Training records are 16 floats: position normalised to [0,1]^3 by the
+ * An MLP 80 -> 64 x4 (ReLU) -> 41 maps a shading point to 8 von Mises-Fisher
+ * lobes (per lobe: mean xyz, log sharpness, mixture logit; outputs 0..39) and
+ * a selection logit c (output 40): a renderer draws a direction from the lobes
+ * with probability a = 0.1 + 0.8 sigmoid(c), otherwise from the BSDF (or with
+ * a = 0.5 when the guide is configured with fixedSelection). BF16 weights
+ * with FP32 accumulation and FP32 master weights, Adam.
+ * Records are RBH_GUIDE_RECORD floats: position normalised to [0,1]^3 by the
  * configured scene bounds [0:3], shading normal [3:6], outgoing direction
- * [6:9], roughness [9], wavelength normalised to [0,1] over 360..830 nm [10],
- * sampled incident direction [11:14], weight = radiance / sampling pdf [14].
- * The loss is sum(weight * -log q(direction)) / sum(weight). */
+ * [6:9], roughness [9], wavelength normalised to [0,1] over 360..830 nm [10]
+ * (inference reads these 11), then for training the sampled incident
+ * direction [11:14], weight w = f |cos| L_i / pdf of the sampled contribution
+ * [14] and the BSDF sampler's density of that direction p_bsdf [15].
+ * The loss is sum(w * -log(a q(d) + (1 - a) p_bsdf(d))) / sum(w), q the lobe
+ * mixture's density. */
+enum {
+    RBH_GUIDE_RECORD = 16,        /* floats per record */
+    RBH_GUIDE_OUTPUTS = 41,       /* floats per rbh_guide_infer output row */
+    RBH_GUIDE_PARAMS = 20329,     /* FP32 master parameters: 20032 weights + 297 biases */
+};
 typedef struct rbh_guide_config {
     float lr, beta1, beta2, eps;      /* Adam (1e-3, 0.9, 0.999, 1e-8) */
     float sceneMin[3], sceneMax[3];   /* position normalisation */
@@ -276,21 +308,24 @@ typedef struct rbh_guide_config {
     uint32_t recordCapacity;          /* records kept per RBH_RENDER_COLLECT render */
 } rbh_guide_config;
 int rbh_guide_configure(rbh_ctx ctx, const rbh_guide_config* cfg);
-/* FP32 master parameters, per layer W[out][in] then b[out] (20264 floats);
- * setting them resets the Adam moments */
+/* FP32 master parameters, per layer W[out][in] then b[out]
+ * (n = RBH_GUIDE_PARAMS floats); setting them resets the Adam moments */
 int rbh_guide_set_params(rbh_ctx ctx, const float* params, uint32_t n);
 int rbh_guide_get_params(rbh_ctx ctx, float* params, uint32_t n);
-/* gradient of the loss over n records (no update) */
+/* gradient of the loss over n records (no update; grad holds
+ * RBH_GUIDE_PARAMS floats) */
 int rbh_guide_gradient(rbh_ctx ctx, const float* records, uint32_t n, float* grad, float* loss);
 /* steps Adam steps over consecutive batches of the n records */
 int rbh_guide_train(rbh_ctx ctx, const float* records, uint32_t n, uint32_t batch, uint32_t steps, float* lastLoss);
-/* raw network outputs (n x 40: per lobe mean xyz, log sharpness, logit) */
+/* raw network outputs, n x RBH_GUIDE_OUTPUTS floats: per lobe mean xyz, log
+ * sharpness, logit, then the selection logit */
 int rbh_guide_infer(rbh_ctx ctx, const float* records, uint32_t n, float* out);
 /* the same with FP8 E4M3 matrix products: per-layer weight scales and
  * per-layer, per-wave activation scales */
 int rbh_guide_infer_fp8(rbh_ctx ctx, const float* records, uint32_t n, float* out);
 /* Training records gathered by the last RBH_RENDER_COLLECT render (count in
- * *n; up to max copied when out is not null). */
+ * *n; up to max records of RBH_GUIDE_RECORD floats copied when out is not
+ * null). */
 int rbh_guide_records(rbh_ctx ctx, float* out, uint32_t max, uint32_t* n);
 /* steps Adam steps over consecutive batches of the gathered records */
 int rbh_guide_train_collected(rbh_ctx ctx, uint32_t batch, uint32_t steps, float* lastLoss);

@@ -100,6 +100,9 @@ class GuideConfig(ctypes.Structure):
 
 GUIDE_PARAMS = 20329
 GUIDE_REC = 16
+GUIDE_OUTPUTS = 41
+RBH_INDEX_U32 = 0
+RBH_INDEX_U16 = 1
 
 
 # name -> (restype, argtypes)
@@ -116,6 +119,9 @@ _SIGS = {
     "rbh_upload_materials": (c_int, [c_void_p, c_void_p, c_uint32]),
     "rbh_upload_geometry": (c_int, [c_void_p, c_void_p, c_uint32, c_void_p, c_uint32, c_void_p, c_uint64, c_void_p, c_void_p]),
     "rbh_build_accel": (c_int, [c_void_p]),
+    "rbh_upload_geometry_ex": (c_int, [c_void_p, c_void_p, c_uint32, c_void_p, c_uint32, c_uint32, c_void_p, c_uint64,
+                                       c_void_p, c_void_p]),
+    "rbh_set_light_tree": (c_int, [c_void_p, c_void_p, c_uint32, c_void_p, c_void_p, c_void_p, c_uint32]),
     "rbh_build_light_tree": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32]),
     "rbh_build_light_tree_ex": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_uint32]),
     "rbh_light_tree_build_host": (c_int, [c_void_p, c_void_p, c_uint32, c_void_p, c_void_p, c_void_p]),

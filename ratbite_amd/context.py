@@ -87,12 +87,14 @@ class Context:
         self.call("rbh_upload_materials", L.ptr(mats), len(mats))
 
     def upload_geometry(self, verts, indices, attribs, tri_material, tri_light):
+        """indices: uint32, or uint16 (uploaded as a 16-bit index buffer)."""
         verts = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
-        indices = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
+        fmt = L.RBH_INDEX_U16 if np.asarray(indices).dtype == np.uint16 else L.RBH_INDEX_U32
+        indices = np.ascontiguousarray(indices, np.uint16 if fmt == L.RBH_INDEX_U16 else np.uint32).reshape(-1, 3)
         attribs = np.ascontiguousarray(attribs, np.uint8).reshape(-1)
         tri_material = np.ascontiguousarray(tri_material, np.uint32)
         tri_light = np.ascontiguousarray(tri_light, np.uint32)
-        self.call("rbh_upload_geometry", L.ptr(verts), len(verts), L.ptr(indices), len(indices),
+        self.call("rbh_upload_geometry_ex", L.ptr(verts), len(verts), L.ptr(indices), fmt, len(indices),
                   L.ptr(attribs) if attribs.size else None, attribs.size, L.ptr(tri_material), L.ptr(tri_light))
 
     def build_accel(self):
@@ -103,6 +105,16 @@ class Context:
         flux = np.ascontiguousarray(flux, np.float32)
         self.call("rbh_build_light_tree_ex", L.ptr(tris) if len(tris) else None, L.ptr(flux) if len(flux) else None,
                   len(tris), L.RBH_LIGHT_TREE_FP64 if fp64 else 0)
+
+    def set_light_tree(self, nodes, pointers, masks, tris):
+        """Uploads a light tree built by the caller (checked before use)."""
+        nodes = np.ascontiguousarray(nodes, L.PACKED_NODE_DTYPE)
+        pointers = np.ascontiguousarray(pointers, np.uint32)
+        masks = np.ascontiguousarray(masks, np.uint64)
+        tris = np.ascontiguousarray(tris, L.EMISSIVE_TRI_DTYPE)
+        n = len(tris)
+        self.call("rbh_set_light_tree", L.ptr(nodes) if len(nodes) else None, len(nodes),
+                  L.ptr(pointers) if n else None, L.ptr(masks) if n else None, L.ptr(tris) if n else None, n)
 
     def get_light_tree(self, nlights):
         nodes = np.zeros(max(1, 2 * nlights), L.PACKED_NODE_DTYPE)

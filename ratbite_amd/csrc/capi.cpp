@@ -156,12 +156,24 @@ int rbh_upload_materials(rbh_ctx c, const rbh_material* m, uint32_t n) {
     return RBH_OK;
 }
 
-int rbh_upload_geometry(rbh_ctx c, const float* verts, uint32_t nverts, const uint32_t* indices, uint32_t ntris,
-                        const void* attribs, uint64_t attribBytes, const uint32_t* triMaterial, const uint32_t* triLight) {
+int rbh_upload_geometry_ex(rbh_ctx c, const float* verts, uint32_t nverts, const void* indexData, uint32_t indexFormat,
+                           uint32_t ntris, const void* attribs, uint64_t attribBytes, const uint32_t* triMaterial,
+                           const uint32_t* triLight) {
     CHECK_CTX(c);
-    if (!verts || !indices || !triMaterial || !triLight || nverts == 0 || ntris == 0)
+    if (indexFormat != RBH_INDEX_U32 && indexFormat != RBH_INDEX_U16)
+        return fail(c, RBH_ERR_INVALID, "unknown index format %u", indexFormat);
+    if (!verts || !indexData || !triMaterial || !triLight || nverts == 0 || ntris == 0)
         return fail(c, RBH_ERR_INVALID, "geometry arrays are null or empty");
     if (ntris > (1u << 30)) return fail(c, RBH_ERR_RANGE, "at most 2^30 triangles");
+    // 16-bit index buffers (pathtrace.cu:1295-1299) are widened once here; the
+    // leaf-order triangle records the kernels read hold 32-bit indices
+    std::vector<uint32_t> widened;
+    const uint32_t* indices = static_cast<const uint32_t*>(indexData);
+    if (indexFormat == RBH_INDEX_U16) {
+        const uint16_t* i16 = static_cast<const uint16_t*>(indexData);
+        widened.assign(i16, i16 + 3ull * ntris);
+        indices = widened.data();
+    }
     for (uint64_t i = 0; i < 3ull * ntris; ++i)
         if (indices[i] >= nverts) return fail(c, RBH_ERR_INVALID, "index %llu out of range", (unsigned long long)i);
     int64_t maxLight = -1;
@@ -208,6 +220,12 @@ int rbh_upload_geometry(rbh_ctx c, const float* verts, uint32_t nverts, const ui
     return RBH_OK;
 }
 
+int rbh_upload_geometry(rbh_ctx c, const float* verts, uint32_t nverts, const uint32_t* indices, uint32_t ntris,
+                        const void* attribs, uint64_t attribBytes, const uint32_t* triMaterial, const uint32_t* triLight) {
+    return rbh_upload_geometry_ex(c, verts, nverts, indices, RBH_INDEX_U32, ntris, attribs, attribBytes, triMaterial,
+                                  triLight);
+}
+
 // ------------------------------------------------------------ env map
 static int envmap_run(rbh_ctx c, uint32_t flags) {
     RBH_HIP(c, hipEventRecord(c->ev0, c->stream));
@@ -244,8 +262,6 @@ int rbh_envmap_build(rbh_ctx c, const float* texels, int w, int h, int ch, uint3
     if (w < 4 || h < 4 || (w % 4) || (h % 4)) return fail(c, RBH_ERR_INVALID, "env-map dimensions must be multiples of 4 (got %dx%d)", w, h);
     if (w > 32768) return fail(c, RBH_ERR_RANGE, "env-map width above 32768");
     if (h > 32768) return fail(c, RBH_ERR_RANGE, "env-map height above 32768");
-    if ((flags & RBH_CDF_FP64) && (size_t)std::max(w, h) * 12 + 8 * 2048 + 64 > 160 * 1024)
-        return fail(c, RBH_ERR_RANGE, "FP64 CDF mode supports max(w, h) up to 12288");
     EnvMap& e = c->env;
     e.enabled = false;
     RBH_ALLOC(c, e.texels, (size_t)w * h * ch * sizeof(float));

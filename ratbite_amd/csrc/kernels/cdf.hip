@@ -69,147 +69,256 @@ __device__ inline double row_weight_f64(int row, int h) {
     return sin(3.14159265358979323846 * ((double)row + 0.5) / (double)h);
 }
 
-// Block-wide inclusive Hillis-Steele scan over .w lanes with the same
-// step/association as cdf.cu:264-282. Double-buffered LDS: one barrier per
-// step is enough because step k+2 reuses a buffer only after every lane has
-// passed the barrier of step k+1.
-template <int B, typename T>
-__device__ inline void block_scan(V4<T>& v, T* __restrict__ sbuf, T& total, T* __restrict__ stotal) {
-    const int t = threadIdx.x;
-    v.y += v.x; v.z += v.y; v.w += v.z;
-    int k = 0;
+// The reference's block-wide inclusive Hillis-Steele scan (cdf.cu:264-282:
+// per-thread component prefix, then steps 1, 2, 4, ..., B/2 that add the
+// .w value of lane t - i from the previous step to all four components of
+// lane t >= i) run by ONE wave with no LDS and no barrier: virtual lane
+// t = 64 k + lane of the B = 64 V lanes lives in register k of hardware lane
+// `lane`. A step i < 64 reads lane (lane - i) mod 64 of register k (t >= i
+// within the same register) or of register k - 1 (the source wrapped into
+// the previous 64), one cross-lane read per register; a step i = 64 m reads
+// register k - m of the same lane. Every virtual lane adds the same operands
+// in the same order as the reference, so the sums are bit-identical.
+template <int V, typename T>
+__device__ __forceinline__ void wave_scan(T (&x)[V], T (&y)[V], T (&z)[V], T (&w)[V], T& total) {
+    const int lane = __lane_id();
 #pragma unroll
-    for (int i = 1; i < B; i <<= 1, ++k) {
-        T* buf = sbuf + (k & 1) * B;
-        buf[t] = v.w;
-        __syncthreads();
-        if (t >= i) {
-            T o = buf[t - i];
-            v.x += o; v.y += o; v.z += o; v.w += o;
+    for (int k = 0; k < V; ++k) {
+        y[k] += x[k];
+        z[k] += y[k];
+        w[k] += z[k];
+    }
+#pragma unroll
+    for (int i = 1; i < 64; i <<= 1) {
+        T s[V];
+        const int src = (lane - i) & 63;
+#pragma unroll
+        for (int k = 0; k < V; ++k) s[k] = __shfl(w[k], src);
+        const bool own = lane >= i;
+#pragma unroll
+        for (int k = 0; k < V; ++k) {
+            if (own || k > 0) {
+                const T o = own ? s[k] : s[k > 0 ? k - 1 : 0];
+                x[k] += o; y[k] += o; z[k] += o; w[k] += o;
+            }
         }
     }
-    v.x += total; v.y += total; v.z += total; v.w += total;
-    if (t == B - 1) *stotal = v.w;
-    __syncthreads();
-    total = *stotal;
+#pragma unroll
+    for (int m = 1; m < V; m <<= 1) {
+        // descending k: register k - m still holds the previous step's value
+#pragma unroll
+        for (int k = V - 1; k >= m; --k) {
+            const T o = w[k - m];
+            x[k] += o; y[k] += o; z[k] += o; w[k] += o;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        x[k] += total; y[k] += total; z[k] += total; w[k] += total;
+    }
+    total = __shfl(w[V - 1], 63);
 }
 
-// Binary search of cdf.cu:517-534 over an LDS-resident CDF.
-__device__ inline uint16_t invert_one(const float* __restrict__ cdf, int n, int j) {
-    int low = 0, high = n - 1;
-    float p = (float)j / (float)(n - 1);
-    int mid = (high - low) / 2;
-    do {
-        if (p > cdf[mid]) low = mid; else high = mid;
-        mid = low + (high - low) / 2;
-    } while (low + 1 != high);
-    return (uint16_t)high;
+// Loads chunk c of a row's power prefixes: virtual lane t covers texels
+// 4t .. 4t+3 (zero past the row).
+template <int V, typename T>
+__device__ __forceinline__ void load_chunk(const float* __restrict__ rowp, int numEl, int c, int channels, T (&x)[V],
+                                           T (&y)[V], T (&z)[V], T (&w)[V]) {
+    const int lane = __lane_id();
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        const int idx = c * 64 * V + 64 * k + lane;
+        x[k] = y[k] = z[k] = w[k] = (T)0;
+        if (idx < numEl) {
+            const V4<float> f = load4(rowp, idx, channels);
+            x[k] = (T)f.x; y[k] = (T)f.y; z[k] = (T)f.z; w[k] = (T)f.w;
+        }
+    }
 }
 
-template <int B, typename T>
-__global__ __launch_bounds__(B) void k_envmap_cdf(const float* __restrict__ texels, int w, int h, int channels,
-                                                   uint16_t* __restrict__ cond, uint16_t* __restrict__ marg,
-                                                   float* __restrict__ cdfY, T* __restrict__ rowTotals,
-                                                   T* __restrict__ integral, unsigned int* __restrict__ counter) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* sbuf = reinterpret_cast<T*>(smem_raw);                    // 2*B scan buffers
-    T* stotal = sbuf + 2 * B;                                    // broadcast slot
+// The row weights of the marginal pass (cdf.cu:112-132) for chunk c.
+template <int V, typename T>
+__device__ __forceinline__ void load_marginal(const T* __restrict__ rowTotals, int numElY, int h, int c, T (&x)[V],
+                                              T (&y)[V], T (&z)[V], T (&w)[V]) {
+    const int lane = __lane_id();
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        const int idx = c * 64 * V + 64 * k + lane;
+        x[k] = y[k] = z[k] = w[k] = (T)0;
+        if (idx < numElY) {
+            const T* rt = rowTotals + (size_t)idx * 4;
+            if constexpr (sizeof(T) == 4) {
+                x[k] = rt[0] * row_weight_f32(idx, 0, numElY);
+                y[k] = rt[1] * row_weight_f32(idx, 1, numElY);
+                z[k] = rt[2] * row_weight_f32(idx, 2, numElY);
+                w[k] = rt[3] * row_weight_f32(idx, 3, numElY);
+            } else {
+                x[k] = rt[0] * row_weight_f64(4 * idx + 0, h);
+                y[k] = rt[1] * row_weight_f64(4 * idx + 1, h);
+                z[k] = rt[2] * row_weight_f64(4 * idx + 2, h);
+                w[k] = rt[3] * row_weight_f64(4 * idx + 3, h);
+            }
+        }
+    }
+}
+
+template <int V, typename T>
+__device__ __forceinline__ void store_chunk(T* __restrict__ dst, int numEl, int c, const T (&x)[V], const T (&y)[V],
+                                            const T (&z)[V], const T (&w)[V]) {
+    const int lane = __lane_id();
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        const int idx = c * 64 * V + 64 * k + lane;
+        if (idx < numEl) {
+            dst[4 * idx + 0] = x[k]; dst[4 * idx + 1] = y[k]; dst[4 * idx + 2] = z[k]; dst[4 * idx + 3] = w[k];
+        }
+    }
+}
+
+// The binary search of cdf.cu:517-534 (invert_one) for the n outputs of an
+// LDS-resident CDF, eight searches per lane interleaved so that their LDS
+// reads overlap; outputs j, j+1 are stored as one 32-bit word.
+__device__ __forceinline__ void invert_row(const float* __restrict__ cdf, int n, uint16_t* __restrict__ out) {
+    constexpr int K = 8;   // 4 pairs
+    const int lane = __lane_id();
+    const int pairs = n >> 1;
+    uint32_t* out32 = reinterpret_cast<uint32_t*>(out);
+    const float inv = (float)(n - 1);
+    for (int base = 0; base < pairs; base += 64 * (K / 2)) {
+        int low[K], high[K], mid[K];
+        float p[K];
+        bool act[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int pr = base + 64 * (k >> 1) + lane;
+            const int j = 2 * pr + (k & 1);
+            act[k] = pr < pairs;
+            p[k] = (float)j / inv;
+            low[k] = 0;
+            high[k] = n - 1;
+            mid[k] = (high[k] - low[k]) / 2;
+        }
+        bool any = true;
+        while (any) {
+            float c[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) c[k] = cdf[mid[k]];
+            any = false;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const bool gt = p[k] > c[k];
+                const int nl = gt ? mid[k] : low[k], nh = gt ? high[k] : mid[k];
+                low[k] = act[k] ? nl : low[k];
+                high[k] = act[k] ? nh : high[k];
+                mid[k] = low[k] + (high[k] - low[k]) / 2;
+                act[k] = act[k] && low[k] + 1 != high[k];
+                any = any || act[k];
+            }
+            any = __ballot(any) != 0ull;
+        }
+#pragma unroll
+        for (int k = 0; k < K; k += 2) {
+            const int pr = base + 64 * (k >> 1) + lane;
+            if (pr < pairs) out32[pr] = (uint32_t)high[k] | ((uint32_t)high[k + 1] << 16);
+        }
+    }
+}
+
+__device__ __forceinline__ void wave_sync_lds() {
+    // this wave's LDS writes are visible to all its lanes
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// One wave per row: the row's power prefixes in registers (chunks of the
+// reference's B = 64 V lanes, running total between chunks), the normalised
+// CDF in the wave's LDS row, the row's inverted indices from LDS. The last
+// workgroup to finish (device-scope counter) builds the marginal with its
+// first wave the same way, publishes the integral and re-arms the counter.
+// FP32 keeps the un-normalised prefixes in LDS and normalises them in place;
+// FP64 scans the row twice (the second time from L2) and rounds each
+// normalised value to float once.
+template <int V, typename T, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_envmap_cdf(const float* __restrict__ texels, int w, int h, int channels,
+                                                            uint16_t* __restrict__ cond, uint16_t* __restrict__ marg,
+                                                            float* __restrict__ cdfY, T* __restrict__ rowTotals,
+                                                            T* __restrict__ integral, unsigned int* __restrict__ counter) {
+    extern __shared__ __align__(16) float lds[];
+    __shared__ bool last;
+    constexpr int B = 64 * V;
+    constexpr bool kF32 = sizeof(T) == 4;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int maxwh = w > h ? w : h;
-    T* raw = stotal + 2;                                         // max(w, h) un-normalised prefixes
-    float* cdf = sizeof(T) == 4 ? reinterpret_cast<float*>(raw) : reinterpret_cast<float*>(raw + maxwh);
-    __shared__ bool done;
-
-    const int t = threadIdx.x;
-    const int row = blockIdx.x;
+    float* cdf = lds + (size_t)wave * maxwh;
+    const int row = blockIdx.x * WAVES + wave;     // h is a multiple of WAVES
     const int numEl = w >> 2;
     const int nChunks = (numEl + B - 1) / B;
     const float* rowp = texels + (size_t)row * w * channels;
 
-    // ---- row CDF into LDS
     T total = (T)0;
     for (int c = 0; c < nChunks; ++c) {
-        const int idx = t + B * c;
-        V4<T> v{(T)0, (T)0, (T)0, (T)0};
-        if (idx < numEl) {
-            V4<float> f = load4(rowp, idx, channels);
-            v = V4<T>{(T)f.x, (T)f.y, (T)f.z, (T)f.w};
-        }
-        block_scan<B, T>(v, sbuf, total, stotal);
-        if (idx < numEl) {
-            raw[4 * idx + 0] = v.x; raw[4 * idx + 1] = v.y; raw[4 * idx + 2] = v.z; raw[4 * idx + 3] = v.w;
-        }
+        T x[V], y[V], z[V], q[V];
+        load_chunk<V, T>(rowp, numEl, c, channels, x, y, z, q);
+        wave_scan<V, T>(x, y, z, q, total);
+        if constexpr (kF32) store_chunk<V, T>(reinterpret_cast<T*>(cdf), numEl, c, x, y, z, q);
     }
-    {
-        // each lane normalises the entries it wrote (cdf.cu:380-392)
-        const T norm = total != (T)0 ? (T)1 / total : (T)0;
+    const T norm = total != (T)0 ? (T)1 / total : (T)0;
+    if constexpr (kF32) {
+        // cdf.cu:380-392, in place (each lane rescales what it wrote)
+        for (int c = 0; c < nChunks; ++c)
+#pragma unroll
+            for (int k = 0; k < V; ++k) {
+                const int idx = c * B + 64 * k + lane;
+                if (idx < numEl)
+                    for (int e = 0; e < 4; ++e) cdf[4 * idx + e] = cdf[4 * idx + e] * norm;
+            }
+    } else {
+        T t2 = (T)0;
         for (int c = 0; c < nChunks; ++c) {
-            const int idx = t + B * c;
-            if (idx < numEl)
-                for (int k = 0; k < 4; ++k) cdf[4 * idx + k] = (float)(raw[4 * idx + k] * norm);
-        }
-    }
-    if (t == 0) rowTotals[row] = total;
-    __syncthreads();
-
-    // ---- invert the row from LDS: two indices per thread, one 32-bit store
-    uint32_t* cond32 = reinterpret_cast<uint32_t*>(cond + (size_t)row * w);
-    for (int j2 = t; j2 < (w >> 1); j2 += B) {
-        uint32_t a = invert_one(cdf, w, 2 * j2);
-        uint32_t b = invert_one(cdf, w, 2 * j2 + 1);
-        cond32[j2] = a | (b << 16);
-    }
-
-    // ---- last workgroup builds the marginal
-    __threadfence();
-    if (t == 0) done = atomicAdd(counter, 1u) == (unsigned)(h - 1);
-    __syncthreads();
-    if (!done) return;
-    __threadfence();
-
-    const int numElY = h >> 2;
-    const int nBlocksY = (numElY + B - 1) / B;
-    T mtotal = (T)0;
-    for (int c = 0; c < nBlocksY; ++c) {
-        const int idx = t + B * c;
-        V4<T> v{(T)0, (T)0, (T)0, (T)0};
-        if (idx < numElY) {
-            const T* rt = rowTotals + (size_t)idx * 4;
-            if constexpr (sizeof(T) == 4) {
-                v = V4<T>{rt[0] * row_weight_f32(idx, 0, numElY), rt[1] * row_weight_f32(idx, 1, numElY),
-                          rt[2] * row_weight_f32(idx, 2, numElY), rt[3] * row_weight_f32(idx, 3, numElY)};
-            } else {
-                v = V4<T>{rt[0] * row_weight_f64(4 * idx + 0, h), rt[1] * row_weight_f64(4 * idx + 1, h),
-                          rt[2] * row_weight_f64(4 * idx + 2, h), rt[3] * row_weight_f64(4 * idx + 3, h)};
+            T x[V], y[V], z[V], q[V];
+            load_chunk<V, T>(rowp, numEl, c, channels, x, y, z, q);
+            wave_scan<V, T>(x, y, z, q, t2);
+#pragma unroll
+            for (int k = 0; k < V; ++k) {
+                const int idx = c * B + 64 * k + lane;
+                if (idx < numEl) {
+                    cdf[4 * idx + 0] = (float)(x[k] * norm); cdf[4 * idx + 1] = (float)(y[k] * norm);
+                    cdf[4 * idx + 2] = (float)(z[k] * norm); cdf[4 * idx + 3] = (float)(q[k] * norm);
+                }
             }
         }
-        block_scan<B, T>(v, sbuf, mtotal, stotal);
-        if (idx < numElY) {
-            T* rt = rowTotals + (size_t)idx * 4;  // reuse as the un-normalised marginal prefix
-            rt[0] = v.x; rt[1] = v.y; rt[2] = v.z; rt[3] = v.w;
-        }
     }
+    if (lane == 0) rowTotals[row] = total;
+    wave_sync_lds();
+    invert_row(cdf, w, cond + (size_t)row * w);
+
+    // ---- the last workgroup builds the marginal
+    __threadfence();
     __syncthreads();
-    if constexpr (sizeof(T) == 4) {
-        const float norm = mtotal != 0.0f ? 1.0f / mtotal : 0.0f;
-        for (int c = 0; c < nBlocksY; ++c) {
-            const int idx = t + B * c;
-            if (idx < numElY)
-                for (int k = 0; k < 4; ++k) cdf[4 * idx + k] = rowTotals[4 * idx + k] * norm;
-        }
-    } else {
-        const double norm = mtotal != 0.0 ? 1.0 / mtotal : 0.0;
-        for (int c = 0; c < nBlocksY; ++c) {
-            const int idx = t + B * c;
-            if (idx < numElY)
-                for (int k = 0; k < 4; ++k) cdf[4 * idx + k] = (float)(rowTotals[4 * idx + k] * norm);
-        }
+    if (threadIdx.x == 0) last = atomicAdd(counter, 1u) == gridDim.x - 1u;
+    __syncthreads();
+    if (!last || wave != 0) return;
+    __threadfence();
+    const int numElY = h >> 2;
+    const int nChunksY = (numElY + B - 1) / B;
+    T mtotal = (T)0;
+    for (int c = 0; c < nChunksY; ++c) {
+        T x[V], y[V], z[V], q[V];
+        load_marginal<V, T>(rowTotals, numElY, h, c, x, y, z, q);
+        wave_scan<V, T>(x, y, z, q, mtotal);
+        store_chunk<V, T>(rowTotals, numElY, c, x, y, z, q);   // the un-normalised marginal prefix, in place
     }
-    __syncthreads();
-    for (int i = t; i < h; i += B) cdfY[i] = cdf[i];
-    for (int j = t; j < h; j += B) marg[j] = invert_one(cdf, h, j);
-    if (t == 0) {
+    const T mnorm = mtotal != (T)0 ? (T)1 / mtotal : (T)0;
+    for (int i = lane; i < h; i += 64) {
+        const float v = (float)(rowTotals[i] * mnorm);
+        cdf[i] = v;
+        cdfY[i] = v;
+    }
+    wave_sync_lds();
+    invert_row(cdf, h, marg);
+    if (lane == 0) {
         *integral = mtotal;
-        *counter = 0u;  // re-arm for the next build
+        *counter = 0u;   // re-arm for the next build
     }
 }
 
@@ -218,37 +327,59 @@ int block_size_for(int w) {
     return numElements <= 256 ? 256 : (numElements <= 512 ? 512 : 1024);
 }
 
-template <int B, typename T>
-hipError_t launch_cdf(rbh_ctx c, int w, int h, int ch, void* rowTotals, void* integral) {
-    const size_t maxwh = (size_t)(w > h ? w : h);
-    size_t lds = (size_t)(2 * B + 2) * sizeof(T) + maxwh * sizeof(T) + (sizeof(T) == 4 ? 0 : maxwh * sizeof(float));
-    auto fn = k_envmap_cdf<B, T>;
+// Rows per workgroup: as many waves as fit 64 KB of LDS rows (at least one).
+int waves_for(int w, int h) {
+    const size_t row = (size_t)(w > h ? w : h) * sizeof(float);
+    return row * 4 <= 65536 ? 4 : (row * 2 <= 65536 ? 2 : 1);
+}
+
+template <int V, typename T, int WAVES>
+hipError_t launch_cdf_w(hipStream_t s, const float* texels, int w, int h, int ch, uint16_t* cond, uint16_t* marg,
+                        float* cdfY, void* rowTotals, void* integral, unsigned int* counter) {
+    const size_t lds = (size_t)WAVES * (size_t)(w > h ? w : h) * sizeof(float);
+    auto fn = k_envmap_cdf<V, T, WAVES>;
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(fn, dim3(h), dim3(B), lds, c->stream, c->env.texels.as<float>(), w, h, ch,
-                       c->env.cond.as<uint16_t>(), c->env.marg.as<uint16_t>(), c->env.cdfY.as<float>(),
-                       static_cast<T*>(rowTotals), static_cast<T*>(integral), c->env.counter.as<unsigned int>());
+    hipLaunchKernelGGL(fn, dim3(h / WAVES), dim3(64 * WAVES), lds, s, texels, w, h, ch, cond, marg, cdfY,
+                       static_cast<T*>(rowTotals), static_cast<T*>(integral), counter);
     return hipGetLastError();
+}
+
+template <int V, typename T>
+hipError_t launch_cdf_v(hipStream_t s, const float* texels, int w, int h, int ch, uint16_t* cond, uint16_t* marg,
+                        float* cdfY, void* rowTotals, void* integral, unsigned int* counter) {
+    const int waves = waves_for(w, h);
+    if (waves == 4) return launch_cdf_w<V, T, 4>(s, texels, w, h, ch, cond, marg, cdfY, rowTotals, integral, counter);
+    if (waves == 2) return launch_cdf_w<V, T, 2>(s, texels, w, h, ch, cond, marg, cdfY, rowTotals, integral, counter);
+    return launch_cdf_w<V, T, 1>(s, texels, w, h, ch, cond, marg, cdfY, rowTotals, integral, counter);
+}
+
+// The fused build over device buffers: texels (w*h*ch floats), outputs cond
+// (w*h), marg (h), cdfY (h); scratch: rowTotals (h values of T, T = double
+// in FP64 mode), integral (one T) and the last-workgroup counter (zero
+// before the first launch; every launch leaves it at zero).
+hipError_t envmap_cdf_run(hipStream_t s, bool fp64, const float* texels, int w, int h, int ch, uint16_t* cond,
+                          uint16_t* marg, float* cdfY, void* rowTotals, void* integral, unsigned int* counter) {
+    const int B = block_size_for(w);
+    if (!fp64) {
+        if (B == 256) return launch_cdf_v<4, float>(s, texels, w, h, ch, cond, marg, cdfY, rowTotals, integral, counter);
+        if (B == 512) return launch_cdf_v<8, float>(s, texels, w, h, ch, cond, marg, cdfY, rowTotals, integral, counter);
+        return launch_cdf_v<16, float>(s, texels, w, h, ch, cond, marg, cdfY, rowTotals, integral, counter);
+    }
+    if (B == 256) return launch_cdf_v<4, double>(s, texels, w, h, ch, cond, marg, cdfY, rowTotals, integral, counter);
+    if (B == 512) return launch_cdf_v<8, double>(s, texels, w, h, ch, cond, marg, cdfY, rowTotals, integral, counter);
+    return launch_cdf_v<16, double>(s, texels, w, h, ch, cond, marg, cdfY, rowTotals, integral, counter);
 }
 
 }  // namespace
 
 // Launches the fused build on the resident env map; returns a HIP error.
 hipError_t envmap_cdf_launch(rbh_ctx c, bool fp64) {
-    const int w = c->env.width, h = c->env.height, ch = c->env.channels;
-    const int B = block_size_for(w);
     // rowTotals (h values of T) + integral (one T) live in integralDev
     char* base = c->env.integralDev.as<char>();
-    void* integral = base;
-    void* rowTotals = base + 16;
-    if (!fp64) {
-        if (B == 256) return launch_cdf<256, float>(c, w, h, ch, rowTotals, integral);
-        if (B == 512) return launch_cdf<512, float>(c, w, h, ch, rowTotals, integral);
-        return launch_cdf<1024, float>(c, w, h, ch, rowTotals, integral);
-    }
-    if (B == 256) return launch_cdf<256, double>(c, w, h, ch, rowTotals, integral);
-    if (B == 512) return launch_cdf<512, double>(c, w, h, ch, rowTotals, integral);
-    return launch_cdf<1024, double>(c, w, h, ch, rowTotals, integral);
+    return envmap_cdf_run(c->stream, fp64, c->env.texels.as<float>(), c->env.width, c->env.height, c->env.channels,
+                          c->env.cond.as<uint16_t>(), c->env.marg.as<uint16_t>(), c->env.cdfY.as<float>(), base + 16,
+                          base, c->env.counter.as<unsigned int>());
 }
 
 size_t envmap_scratch_bytes(int h) { return 16 + (size_t)h * sizeof(double); }

@@ -22,6 +22,11 @@
 #include "../guide.hpp"
 #include "../device/mlp_wave.hpp"
 
+// the sizes the public header documents
+static_assert(RBH_GUIDE_RECORD == rbh::guide::kRec, "record size");
+static_assert(RBH_GUIDE_OUTPUTS == rbh::guide::kOut, "output row size");
+static_assert(RBH_GUIDE_PARAMS == rbh::guide::kParams, "parameter count");
+
 namespace rbh {
 
 using namespace guide;

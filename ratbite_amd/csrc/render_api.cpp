@@ -72,37 +72,114 @@ int rbh_build_light_tree(rbh_ctx c, const rbh_emissive_tri* tris, const float* f
     return rbh_build_light_tree_ex(c, tris, flux, n, 0u);
 }
 
+}  // extern "C"
+
+namespace {
+
+// Checks a packed light tree before the kernels walk it: nodes in pre-order
+// (the left child of inner node i is i + 1, its right child lies after it,
+// so every walk ends), leaves holding 1..15 triangle lights inside the
+// pointer array, every light's bitmask path from the root ending in the leaf
+// that holds it (light_tree_types.h:119-206, light_tree.cpp:71-73).
+int check_tree(rbh_ctx c, const PackedNode* nodes, uint32_t nnodes, const uint32_t* ptrs, const uint64_t* masks,
+               uint32_t n) {
+    if (nnodes == 0 || nnodes > 2 * (uint64_t)n) return fail(c, RBH_ERR_INVALID, "%u nodes for %u lights", nnodes, n);
+    for (uint32_t i = 0; i < nnodes; ++i) {
+        const uint32_t core = nodes[i].core;
+        if (core >> 31) {
+            const uint32_t cnt = (core >> 27) & 15u, off = core & ((1u << 27) - 1u);
+            if (cnt == 0 || (uint64_t)off + cnt > n)
+                return fail(c, RBH_ERR_INVALID, "leaf %u holds lights [%u, %u) of %u", i, off, off + cnt, n);
+        } else if (i + 1 >= nnodes || core <= i + 1 || core >= nnodes) {
+            return fail(c, RBH_ERR_INVALID, "inner node %u: children %u and %u are not in pre-order below %u nodes", i,
+                        i + 1, core, nnodes);
+        }
+    }
+    std::vector<uint8_t> seen(n, 0);
+    for (uint32_t k = 0; k < n; ++k) {
+        if ((ptrs[k] >> 30) != 0u || (ptrs[k] & ((1u << 30) - 1u)) >= n)
+            return fail(c, RBH_ERR_INVALID, "light pointer %u (0x%08x) is not a triangle light below %u", k, ptrs[k], n);
+        if (seen[ptrs[k]]++) return fail(c, RBH_ERR_INVALID, "light %u appears twice in the pointer array", ptrs[k]);
+    }
+    for (uint32_t l = 0; l < n; ++l) {
+        uint32_t cur = 0;
+        for (int d = 0; d < 64 && !(nodes[cur].core >> 31); ++d) cur = ((masks[l] >> d) & 1ull) ? nodes[cur].core : cur + 1;
+        const uint32_t core = nodes[cur].core;
+        if (!(core >> 31)) return fail(c, RBH_ERR_INVALID, "light %u: its bitmask does not reach a leaf", l);
+        const uint32_t cnt = (core >> 27) & 15u, off = core & ((1u << 27) - 1u);
+        bool found = false;
+        for (uint32_t k = off; k < off + cnt; ++k) found = found || ptrs[k] == l;
+        if (!found) return fail(c, RBH_ERR_INVALID, "light %u: its bitmask ends in leaf %u, which does not hold it", l, cur);
+    }
+    return RBH_OK;
+}
+
+// Uploads a checked tree with its triangle records and expands the nodes.
+int upload_tree(rbh_ctx c, const PackedNode* nodes, uint32_t nnodes, const uint32_t* ptrs, const uint64_t* masks,
+                const rbh_emissive_tri* tris, uint32_t n) {
+    LightTreeDev& t = c->tree;
+    t.hostNodes.resize(nnodes);
+    if (nnodes) std::memcpy(t.hostNodes.data(), nodes, (size_t)nnodes * sizeof(PackedNode));
+    t.hostPointers.assign(ptrs, ptrs + n);
+    t.hostMasks.assign(masks, masks + n);
+    t.nnodes = nnodes;
+    t.nlights = n;
+    if (n == 0) return RBH_OK;
+    RBH_ALLOC(c, t.nodes, (size_t)nnodes * sizeof(PackedNode));
+    RBH_ALLOC(c, t.lightPointers, (size_t)n * 4);
+    RBH_ALLOC(c, t.bitmasks, (size_t)n * 8);
+    RBH_ALLOC(c, t.tris, (size_t)n * sizeof(EmissiveTri));
+    RBH_HIP(c, hipMemcpyAsync(t.nodes.p, nodes, (size_t)nnodes * sizeof(PackedNode), hipMemcpyHostToDevice, c->stream));
+    RBH_HIP(c, hipMemcpyAsync(t.lightPointers.p, ptrs, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+    RBH_HIP(c, hipMemcpyAsync(t.bitmasks.p, masks, (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+    RBH_HIP(c, hipMemcpyAsync(t.tris.p, tris, (size_t)n * sizeof(EmissiveTri), hipMemcpyHostToDevice, c->stream));
+    RBH_ALLOC(c, t.nodesX, (size_t)nnodes * 48);
+    RBH_HIP(c, light_tree_expand(c));
+    RBH_HIP(c, hipStreamSynchronize(c->stream));
+    return RBH_OK;
+}
+
+int check_light_records(rbh_ctx c, const rbh_emissive_tri* tris, uint32_t n) {
+    for (uint32_t i = 0; i < n; ++i)
+        if (tris[i].materialIndex >= c->nmaterials)
+            return fail(c, RBH_ERR_INVALID, "light %u: material %u of %u", i, tris[i].materialIndex, c->nmaterials);
+    return RBH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int rbh_build_light_tree_ex(rbh_ctx c, const rbh_emissive_tri* tris, const float* flux, uint32_t n, uint32_t flags) {
     if (!c) return RBH_ERR_INVALID;
     if (flags & ~(uint32_t)RBH_LIGHT_TREE_FP64) return fail(c, RBH_ERR_INVALID, "unknown light-tree flags 0x%x", flags);
     if (n && (!tris || !flux)) return fail(c, RBH_ERR_INVALID, "light arrays are null");
-    for (uint32_t i = 0; i < n; ++i) {
+    for (uint32_t i = 0; i < n; ++i)
         if (!(flux[i] > 0.0f) || !std::isfinite(flux[i])) return fail(c, RBH_ERR_INVALID, "light %u: flux must be positive and finite", i);
-        if (tris[i].materialIndex >= c->nmaterials) return fail(c, RBH_ERR_INVALID, "light %u: material %u of %u", i, tris[i].materialIndex, c->nmaterials);
-    }
-    LightTreeDev& t = c->tree;
+    int r = check_light_records(c, tris, n);
+    if (r) return r;
     const EmissiveTri* et = reinterpret_cast<const EmissiveTri*>(tris);
     std::vector<PackedNode> nodes;
-    const int r = host::build_light_tree(et, flux, n, nodes, t.hostPointers, t.hostMasks, (flags & RBH_LIGHT_TREE_FP64) != 0);
+    std::vector<uint32_t> ptrs;
+    std::vector<uint64_t> masks;
+    r = host::build_light_tree(et, flux, n, nodes, ptrs, masks, (flags & RBH_LIGHT_TREE_FP64) != 0);
     if (r == -1) return fail(c, RBH_ERR_RANGE, "light count %u exceeds the tree's 27-bit leaf offsets", n);
     if (r < 0) return fail(c, RBH_ERR_INVALID, "light tree produced an invalid bitmask");
-    t.hostNodes.resize(nodes.size());
-    std::memcpy(t.hostNodes.data(), nodes.data(), nodes.size() * sizeof(PackedNode));
-    t.nnodes = (uint32_t)nodes.size();
-    t.nlights = n;
-    if (n == 0) return RBH_OK;
-    RBH_ALLOC(c, t.nodes, nodes.size() * sizeof(PackedNode));
-    RBH_ALLOC(c, t.lightPointers, (size_t)n * 4);
-    RBH_ALLOC(c, t.bitmasks, (size_t)n * 8);
-    RBH_ALLOC(c, t.tris, (size_t)n * sizeof(EmissiveTri));
-    RBH_HIP(c, hipMemcpyAsync(t.nodes.p, nodes.data(), nodes.size() * sizeof(PackedNode), hipMemcpyHostToDevice, c->stream));
-    RBH_HIP(c, hipMemcpyAsync(t.lightPointers.p, t.hostPointers.data(), (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
-    RBH_HIP(c, hipMemcpyAsync(t.bitmasks.p, t.hostMasks.data(), (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
-    RBH_HIP(c, hipMemcpyAsync(t.tris.p, tris, (size_t)n * sizeof(EmissiveTri), hipMemcpyHostToDevice, c->stream));
-    RBH_ALLOC(c, t.nodesX, nodes.size() * 48);
-    RBH_HIP(c, light_tree_expand(c));
-    RBH_HIP(c, hipStreamSynchronize(c->stream));
-    return RBH_OK;
+    return upload_tree(c, nodes.data(), (uint32_t)nodes.size(), ptrs.data(), masks.data(), tris, n);
+}
+
+int rbh_set_light_tree(rbh_ctx c, const rbh_packed_node* nodes, uint32_t nnodes, const uint32_t* lightPointers,
+                       const uint64_t* bitmasks, const rbh_emissive_tri* tris, uint32_t n) {
+    if (!c) return RBH_ERR_INVALID;
+    if (n == 0) return upload_tree(c, nullptr, 0, nullptr, nullptr, nullptr, 0);
+    if (!nodes || !lightPointers || !bitmasks || !tris) return fail(c, RBH_ERR_INVALID, "light-tree arrays are null");
+    if (n > (1u << 27)) return fail(c, RBH_ERR_RANGE, "light count %u exceeds the tree's 27-bit leaf offsets", n);
+    const PackedNode* pn = reinterpret_cast<const PackedNode*>(nodes);
+    int r = check_light_records(c, tris, n);
+    if (r) return r;
+    r = check_tree(c, pn, nnodes, lightPointers, bitmasks, n);
+    if (r) return r;
+    return upload_tree(c, pn, nnodes, lightPointers, bitmasks, tris, n);
 }
 
 int rbh_light_tree_build_host(const rbh_emissive_tri* tris, const float* flux, uint32_t n, rbh_packed_node* nodes,

@@ -43,3 +43,32 @@ def test_record_layouts():
 def test_host_only_entry_rejects_bad_arguments():
     assert L.lib().rbh_light_tree_build_host(None, None, 3, None, None, None) == L.RBH_ERR_INVALID
     assert L.lib().rbh_light_tree_build_host(None, None, 0, None, None, None) == 0
+
+
+ABI_C = os.path.join(ROOT, "tests", "abi_c")
+
+
+def _abi_binary():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", ABI_C], check=True)
+    return os.path.join(ABI_C, "_build", "abi_check")
+
+
+def test_c_caller_host_entry_points():
+    """A C program compiled against include/ratbite_hip.h alone: record and
+    guide sizes as documented, host-only light-tree builds write no more than
+    the documented capacities."""
+    import subprocess
+    r = subprocess.run([_abi_binary(), "host"], capture_output=True, text=True)
+    assert r.returncode == 0 and "abi_check: ok" in r.stdout, r.stdout + r.stderr
+
+
+@pytest.mark.gpu
+def test_c_caller_every_entry_point():
+    """The same C program calls every entry point of the header on the GPU
+    with buffers of exactly the documented sizes plus a guard band, and checks
+    the bands after every call that writes into caller memory."""
+    import subprocess
+    print("abi_check gpu", flush=True)
+    r = subprocess.run([_abi_binary(), "gpu"], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0 and "abi_check: ok" in r.stdout, r.stdout + r.stderr

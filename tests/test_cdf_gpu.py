@@ -13,8 +13,10 @@ from ratbite_amd.envmaps import lognormal_envmap
 
 pytestmark = pytest.mark.gpu
 
+# one to four rows per workgroup (LDS rows of max(w, h) floats), one to eight
+# chunks of the reference's block per row, non-power-of-two sizes
 SIZES = [(64, 32, 3), (256, 128, 4), (1024, 512, 3), (2048, 1024, 3), (4096, 2048, 3), (1000, 500, 3),
-         (8192, 16, 4), (4, 4, 3)]
+         (8192, 16, 4), (4, 4, 3), (16384, 8, 3), (32768, 4, 3), (12, 4096, 3)]
 
 
 @pytest.mark.parametrize("w,h,ch", SIZES)
@@ -60,7 +62,7 @@ def test_cdf_rebuild_is_repeatable(ctx):
         assert np.array_equal(np.asarray(a), np.asarray(b))
 
 
-@pytest.mark.parametrize("w,h", [(2048, 1024), (512, 256), (8192, 4096)])
+@pytest.mark.parametrize("w,h", [(2048, 1024), (512, 256), (8192, 4096), (16384, 64)])
 def test_cdf_fp64_mode(ctx, w, h):
     tex = lognormal_envmap(w, h, sigma=3.0, seed=w, sun=True)
     ctx.envmap_build(tex, fp64=True)

@@ -47,8 +47,10 @@ def test_cornell_matches_oracle(ctx):
     frac = close.all(axis=2).mean()
     e = relmse(g, o)
     print("cornell: pixels within 1e-4: %.4f  relMSE(gpu, oracle) = %.3g" % (frac, e))
-    assert frac > 0.5
-    assert e < 2e-2
+    # measured: 0.9995 of the pixels, relMSE ~1e-9 (only paths that diverge
+    # within an ulp of a threshold differ)
+    assert frac >= 0.99
+    assert e < 1e-5
     # the same number of closest-hit queries up to diverged paths
     assert abs(st["closestRays"] - ost["closestRays"]) / ost["closestRays"] < 0.02
 
@@ -62,8 +64,9 @@ def test_odd_film_sizes_match_oracle(ctx, size):
     close = np.abs(g - o) <= 1e-4 * np.maximum(np.abs(o), 1e-3)
     frac = close.all(axis=2).mean()
     print("cornell %dx%d: pixels within 1e-4: %.4f" % (size[0], size[1], frac), flush=True)
-    assert frac > 0.5
-    assert relmse(g, o) < 5e-2
+    # at most one pixel of these tiny films may hold a diverged path
+    assert frac >= 0.99 or (1.0 - frac) * size[0] * size[1] <= 1.0 + 1e-9
+    assert relmse(g, o) < 1e-4
 
 
 def test_spheres_matches_oracle(ctx):
@@ -75,8 +78,8 @@ def test_spheres_matches_oracle(ctx):
     frac = close.all(axis=2)[fin].mean()
     e = relmse(g[fin], o[fin])
     print("spheres: pixels within 1e-4: %.4f  relMSE(gpu, oracle) = %.3g" % (frac, e))
-    assert frac > 0.5
-    assert e < 5e-2
+    assert frac >= 0.99
+    assert e < 1e-5
 
 
 def test_fp64_precision_modes_match_oracle(ctx):
@@ -96,8 +99,8 @@ def test_fp64_precision_modes_match_oracle(ctx):
     frac = close.all(axis=2)[fin].mean()
     e = relmse(g[fin], o[fin])
     print("fp64 modes: pixels within 1e-4: %.4f  relMSE(gpu, oracle) = %.3g" % (frac, e))
-    assert frac > 0.5
-    assert e < 5e-2
+    assert frac >= 0.99
+    assert e < 1e-5
 
 
 def test_progressive_equals_single_launch(ctx):
@@ -164,3 +167,36 @@ def test_full_size_frame_invariants(ctx):
     rays_per_sample = st["closestRays"] / (sc.width * sc.height * 8)
     print("1080p: %.3f closest-hit queries per sample, %d shadow rays" % (rays_per_sample, st["shadowRays"]), flush=True)
     assert 1.0 <= rays_per_sample <= 3.0
+
+
+def test_u16_indices_and_caller_built_tree(ctx):
+    """A 16-bit index buffer renders the same image as the 32-bit one, and a
+    light tree uploaded by the caller (rbh_set_light_tree, here the tree the
+    context built, read back) renders the same image as the built one; trees
+    whose walks would not end or whose bitmasks miss their leaves are refused."""
+    from ratbite_amd import RbhError
+    from ratbite_amd import _lib as L
+    sc = scenes.spheres(n_spheres=4, level=2, n_lights=50, width=24, height=16, spp=4, env_size=(64, 32))
+    assert sc.verts.shape[0] < 65536
+    r = Renderer(sc, ctx=ctx)
+    r.render(spp=4, offset=0)
+    a32 = r.accum()
+    nodes, lp, bm = r.light_tree()
+    ctx.upload_geometry(sc.verts, np.asarray(sc.indices).astype(np.uint16), sc.attribs, sc.tri_material, sc.tri_light)
+    ctx.build_accel()
+    r.render(spp=4, offset=0)
+    assert np.array_equal(r.accum(), a32)
+    ctx.set_light_tree(nodes, lp, bm, sc.lights)
+    r.render(spp=4, offset=0)
+    assert np.array_equal(r.accum(), a32)
+    assert len(nodes) > 1
+    bad = nodes.copy()
+    bad["coreData"][0] = 0              # the root's right child is itself
+    with pytest.raises(RbhError) as e:
+        ctx.set_light_tree(bad, lp, bm, sc.lights)
+    assert e.value.code == L.RBH_ERR_INVALID
+    bad_masks = bm ^ np.uint64(1)       # every light turns the other way at the root
+    with pytest.raises(RbhError) as e:
+        ctx.set_light_tree(nodes, lp, bad_masks, sc.lights)
+    assert e.value.code == L.RBH_ERR_INVALID
+    ctx.set_light_tree(nodes, lp, bm, sc.lights)
