@@ -424,6 +424,11 @@ void render_pixel(const Ctx& C, uint32_t px, uint32_t py, double res[4], Stats& 
                 if (M->bxdf == 0) bxdf_conductor(sc, *M, dl, q, local, path);
                 else if (M->bxdf == 2) bxdf_complex(sc, dl, q, local, surf, path);
                 // bxdf 1 (PureDielectric) has an empty body in the reference
+                // RBH_RENDER_NO_DROP (flag 16, no reference counterpart): an
+                // invalid BxDF sample ends the path and keeps its radiance
+                // instead of discarding the whole sample
+                if ((P->flags & 16u) && (path.flags & PF_PATH_TERMINATED))
+                    path.flags = (path.flags & ~PF_PATH_TERMINATED) | PF_FINISHED;
             }
 
             // ---- Russian roulette (pathtrace.cu:1736-1744)

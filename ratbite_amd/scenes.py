@@ -410,6 +410,40 @@ def cornell(seed=0x5EED0001, width=256, height=256, spp=16, depth=8, light_up=Fa
     return _finish(parts, mats, spectra, [], cam, None, width, height, depth, spp, "cornell", tables)
 
 
+def furnace(inner="metal", rough=0.3, width=64, height=64, spp=64, depth=8, level=3, radius=1.5):
+    """An emissive enclosure around a sphere: the inside of a radius-10
+    icosphere whose triangles all emit white (1, 1, 1) and reflect almost
+    nothing (ComplexSurface metal with a black base colour: Schlick with
+    f0 = 0, seen near normal incidence from the centre), and a level-`level`
+    icosphere of `radius` at the centre with a white, non-absorbing material:
+    "metal" (metalness 1, base white, multiple-scattering compensation) or
+    "dielectric" (metalness 0, base white, ior 1.5). Every direction seen from
+    the inner sphere carries the wall radiance Le, so its pixels converge to
+    Le times its directional albedo (one within the LUTs' accuracy), like the
+    pixels that see the wall directly. The camera looks at the centre from
+    distance 5."""
+    tables = load_tables()
+    ev, ei = icosphere(3)
+    ev = ev * 10.0
+    ei = ei[:, [0, 2, 1]].copy()                 # normals towards the centre
+    en = -ev / 10.0
+    euv = np.zeros((len(ev), 2), np.float32)
+    iv, ii = icosphere(level)
+    iv = iv * radius
+    inn = vertex_normals(iv, ii)
+    iuv = np.zeros((len(iv), 2), np.float32)
+    mats = [material(base=(0.0, 0.0, 0.0), metal=1.0, rough=0.5, emissive=(1.0, 1.0, 1.0))]
+    if inner == "metal":
+        mats.append(material(base=(1.0, 1.0, 1.0), metal=1.0, rough=rough))
+    else:
+        mats.append(material(base=(1.0, 1.0, 1.0), metal=0.0, rough=rough, ior=1.5))
+    parts = [(ev, ei, en, euv, 0), (iv, ii, inn, iuv, 1)]
+    cam = camera_basis((0.0, 0.0, 5.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0))
+    cam["fov"] = math.radians(60.0)
+    return _finish(parts, mats, [], [], cam, None, width, height, depth, spp, "furnace-" + inner, tables,
+                   light_uv=np.zeros((3, 2), np.float32))
+
+
 def cauchy_spectra(rng):
     """Seeded Cauchy fits n = A + B / l^2, k = C + D / l^2 (l in um) at 1 nm."""
     lam = np.arange(360, 831, dtype=np.float64) * 1e-3
