@@ -268,7 +268,9 @@ def guided_workload(args):
         return float(e.mean()), float(np.sort(e)[:-k].mean())
 
     if args.scene == "bounce":
-        sc = scenes.cornell(width=args.width, height=args.height, spp=16, bounce=True)
+        raise SystemExit("bench.py --workload guided --scene bounce is switched off: its plain no-drop reference "
+                         "renders from sample 2^24 ended in an illegal memory access on the GPU whose cause has not "
+                         "been found (a sanitized CPU build of the same kernel bodies replays such ranges cleanly)")
     else:
         sc = scenes.spheres(width=args.width, height=args.height, spp=16, depth=args.depth)
     r = Renderer(sc)

@@ -26,6 +26,15 @@ namespace rbh {
 namespace {
 
 template <typename T> struct V4 { T x, y, z, w; };
+
+// LDS floats per wave: the row (or marginal) CDF of max(w, h) floats, then
+// as many 16-bit marks for the linear inversion (max(w, h) is a multiple of
+// 4). Rows wider than 16384 leave the marks out (they would not fit 160 KB)
+// and are inverted by the binary search.
+__host__ __device__ inline bool use_marks(int maxwh) { return maxwh <= 16384; }
+__host__ __device__ inline size_t wave_lds_floats(int maxwh) {
+    return (size_t)maxwh + (use_marks(maxwh) ? (size_t)maxwh / 2 : 0);
+}
 typedef float f4v __attribute__((ext_vector_type(4)));
 
 __device__ inline float texel_power(float r, float g, float b) {
@@ -136,28 +145,18 @@ __device__ __forceinline__ void load_chunk(const float* __restrict__ rowp, int n
     }
 }
 
-// The row weights of the marginal pass (cdf.cu:112-132) for chunk c.
+// Chunk c of the weighted row totals (the marginal pass's input).
 template <int V, typename T>
-__device__ __forceinline__ void load_marginal(const T* __restrict__ rowTotals, int numElY, int h, int c, T (&x)[V],
-                                              T (&y)[V], T (&z)[V], T (&w)[V]) {
+__device__ __forceinline__ void load_marginal(const T* __restrict__ rowTerms, int numElY, int c, T (&x)[V], T (&y)[V],
+                                              T (&z)[V], T (&w)[V]) {
     const int lane = __lane_id();
 #pragma unroll
     for (int k = 0; k < V; ++k) {
         const int idx = c * 64 * V + 64 * k + lane;
         x[k] = y[k] = z[k] = w[k] = (T)0;
         if (idx < numElY) {
-            const T* rt = rowTotals + (size_t)idx * 4;
-            if constexpr (sizeof(T) == 4) {
-                x[k] = rt[0] * row_weight_f32(idx, 0, numElY);
-                y[k] = rt[1] * row_weight_f32(idx, 1, numElY);
-                z[k] = rt[2] * row_weight_f32(idx, 2, numElY);
-                w[k] = rt[3] * row_weight_f32(idx, 3, numElY);
-            } else {
-                x[k] = rt[0] * row_weight_f64(4 * idx + 0, h);
-                y[k] = rt[1] * row_weight_f64(4 * idx + 1, h);
-                z[k] = rt[2] * row_weight_f64(4 * idx + 2, h);
-                w[k] = rt[3] * row_weight_f64(4 * idx + 3, h);
-            }
+            const T* rt = rowTerms + (size_t)idx * 4;
+            x[k] = rt[0]; y[k] = rt[1]; z[k] = rt[2]; w[k] = rt[3];
         }
     }
 }
@@ -175,10 +174,24 @@ __device__ __forceinline__ void store_chunk(T* __restrict__ dst, int numEl, int 
     }
 }
 
+template <int V, typename T>
+__device__ __forceinline__ void store_chunk_lds(float* __restrict__ cdf, int numEl, int c, const T (&x)[V],
+                                                const T (&y)[V], const T (&z)[V], const T (&w)[V]) {
+    const int lane = __lane_id();
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        const int idx = c * 64 * V + 64 * k + lane;
+        if (idx < numEl) {
+            cdf[4 * idx + 0] = (float)x[k]; cdf[4 * idx + 1] = (float)y[k];
+            cdf[4 * idx + 2] = (float)z[k]; cdf[4 * idx + 3] = (float)w[k];
+        }
+    }
+}
+
 // The binary search of cdf.cu:517-534 (invert_one) for the n outputs of an
 // LDS-resident CDF, eight searches per lane interleaved so that their LDS
 // reads overlap; outputs j, j+1 are stored as one 32-bit word.
-__device__ __forceinline__ void invert_row(const float* __restrict__ cdf, int n, uint16_t* __restrict__ out) {
+__device__ __forceinline__ void invert_row_search(const float* __restrict__ cdf, int n, uint16_t* __restrict__ out) {
     constexpr int K = 8;   // 4 pairs
     const int lane = __lane_id();
     const int pairs = n >> 1;
@@ -230,6 +243,87 @@ __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_wave_barrier();
 }
 
+// The first output j in [0, n-1] whose query q_j = RN(j / (n-1)) (a
+// correctly rounded float quotient, as invert_one computes it) exceeds c >= 0,
+// or n, without dividing: RN(x) > c exactly when x lies above the midpoint
+// m of c and the next float, or on it when ties-to-even rounds m up (c's
+// significand odd). m (n-1) is exact in double (25 + 15 significant bits).
+__device__ __forceinline__ int first_above(float c, int n) {
+    const uint32_t b = __float_as_uint(c);
+    if (b >= 0x7f7fffffu) return n;                      // the largest float, inf: no query exceeds them
+    const double t = 0.5 * ((double)c + (double)__uint_as_float(b + 1u)) * (double)(n - 1);
+    const double f = floor(t);
+    const double j = (f == t && (b & 1u)) ? f : f + 1.0;
+    return j >= (double)n ? n : (int)j;
+}
+
+// The outputs of invert_one for all j = 0..n-1 in O(n). The search only
+// probes entries 1..n-2, and while those are non-decreasing it returns
+// 1 + #{m in [1, n-2] : cdf[m] < p_j} (at least 1, at most n-1). Entry m is
+// below the queries from first_above(cdf[m]) on, so output j is the largest
+// m + 1 whose first_above(cdf[m]) <= j: each m marks its first query (only
+// the last of several entries sharing one first query writes it), and a
+// running maximum over the marks gives every output. A row whose probed
+// entries decrease somewhere (float rounding of the reference's prefix sums
+// can make a later entry an ulp smaller) or hold a NaN takes the exact
+// binary search instead, as do all rows when `marks` (n 16-bit words of
+// this wave) is null.
+__device__ __forceinline__ void invert_row(const float* __restrict__ cdf, uint16_t* __restrict__ marks, int n,
+                                           uint16_t* __restrict__ out) {
+    const int lane = __lane_id();
+    if (marks == nullptr) {
+        invert_row_search(cdf, n, out);
+        return;
+    }
+    uint2* z64 = reinterpret_cast<uint2*>(marks);
+    for (int q = lane; 4 * q < n; q += 64) z64[q] = make_uint2(0u, 0u);
+    wave_sync_lds();
+    // one pass over the probed entries: the order check and the marks (a
+    // row that fails the check is redone by the binary search)
+    bool bad = false;
+    for (int m = 1 + lane; m <= n - 2; m += 64) {
+        const float c = cdf[m];
+        const bool last = m + 1 > n - 2;
+        const float cn = last ? 0.0f : cdf[m + 1];
+        bad = bad || !(c >= 0.0f) || (!last && !(cn >= c));
+        const int jm = first_above(c, n);
+        const int jn = last ? n : first_above(cn, n);
+        if (jm < jn && jm < n) marks[jm] = (uint16_t)(m + 1);
+    }
+    if (__ballot(bad) != 0ull) {
+        invert_row_search(cdf, n, out);
+        return;
+    }
+    wave_sync_lds();
+    // running maximum from 1, four consecutive outputs per lane (n is a
+    // multiple of 4): in-lane maximum, then a wave-wide exclusive maximum
+    uint32_t carry = 1u;
+    const uint2* m64 = reinterpret_cast<const uint2*>(marks);
+    uint2* o64 = reinterpret_cast<uint2*>(out);
+    for (int base = 0; base < n; base += 256) {
+        const int q = (base >> 2) + lane;          // group of four outputs
+        const bool live = 4 * q < n;
+        uint32_t v0 = 0u, v1 = 0u, v2 = 0u, v3 = 0u;
+        if (live) {
+            const uint2 g = m64[q];
+            v0 = g.x & 0xFFFFu; v1 = g.x >> 16; v2 = g.y & 0xFFFFu; v3 = g.y >> 16;
+        }
+        v1 = max(v1, v0); v2 = max(v2, v1); v3 = max(v3, v2);
+        uint32_t sc = v3;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(sc, o);
+            if (lane >= o) sc = max(sc, t);
+        }
+        uint32_t ex = __shfl_up(sc, 1);
+        if (lane == 0) ex = 0u;
+        ex = max(ex, carry);
+        if (live)
+            o64[q] = make_uint2(max(ex, v0) | (max(ex, v1) << 16), max(ex, v2) | (max(ex, v3) << 16));
+        carry = max(carry, __shfl(sc, 63));
+    }
+}
+
 // One wave per row: the row's power prefixes in registers (chunks of the
 // reference's B = 64 V lanes, running total between chunks), the normalised
 // CDF in the wave's LDS row, the row's inverted indices from LDS. The last
@@ -249,7 +343,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_envmap_cdf(const float* __restri
     constexpr bool kF32 = sizeof(T) == 4;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int maxwh = w > h ? w : h;
-    float* cdf = lds + (size_t)wave * maxwh;
+    float* cdf = lds + (size_t)wave * wave_lds_floats(maxwh);
+    uint16_t* marks = use_marks(maxwh) ? reinterpret_cast<uint16_t*>(cdf + maxwh) : nullptr;
     const int row = blockIdx.x * WAVES + wave;     // h is a multiple of WAVES
     const int numEl = w >> 2;
     const int nChunks = (numEl + B - 1) / B;
@@ -260,7 +355,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_envmap_cdf(const float* __restri
         T x[V], y[V], z[V], q[V];
         load_chunk<V, T>(rowp, numEl, c, channels, x, y, z, q);
         wave_scan<V, T>(x, y, z, q, total);
-        if constexpr (kF32) store_chunk<V, T>(reinterpret_cast<T*>(cdf), numEl, c, x, y, z, q);
+        if constexpr (kF32) store_chunk_lds<V, T>(cdf, numEl, c, x, y, z, q);
     }
     const T norm = total != (T)0 ? (T)1 / total : (T)0;
     if constexpr (kF32) {
@@ -288,9 +383,14 @@ __global__ __launch_bounds__(64 * WAVES) void k_envmap_cdf(const float* __restri
             }
         }
     }
-    if (lane == 0) rowTotals[row] = total;
+    // the row's marginal term: its total times the row weight (cdf.cu:112-132),
+    // formed here so that the last workgroup only scans
+    if (lane == 0) {
+        if constexpr (kF32) rowTotals[row] = total * row_weight_f32(row >> 2, row & 3, h >> 2);
+        else rowTotals[row] = total * row_weight_f64(row, h);
+    }
     wave_sync_lds();
-    invert_row(cdf, w, cond + (size_t)row * w);
+    invert_row(cdf, marks, w, cond + (size_t)row * w);
 
     // ---- the last workgroup builds the marginal
     __threadfence();
@@ -304,7 +404,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_envmap_cdf(const float* __restri
     T mtotal = (T)0;
     for (int c = 0; c < nChunksY; ++c) {
         T x[V], y[V], z[V], q[V];
-        load_marginal<V, T>(rowTotals, numElY, h, c, x, y, z, q);
+        load_marginal<V, T>(rowTotals, numElY, c, x, y, z, q);
         wave_scan<V, T>(x, y, z, q, mtotal);
         store_chunk<V, T>(rowTotals, numElY, c, x, y, z, q);   // the un-normalised marginal prefix, in place
     }
@@ -315,7 +415,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_envmap_cdf(const float* __restri
         cdfY[i] = v;
     }
     wave_sync_lds();
-    invert_row(cdf, h, marg);
+    invert_row(cdf, marks, h, marg);
     if (lane == 0) {
         *integral = mtotal;
         *counter = 0u;   // re-arm for the next build
@@ -327,16 +427,17 @@ int block_size_for(int w) {
     return numElements <= 256 ? 256 : (numElements <= 512 ? 512 : 1024);
 }
 
-// Rows per workgroup: as many waves as fit 64 KB of LDS rows (at least one).
+// Rows per workgroup: as many waves (up to four) as fit 80 KB of LDS,
+// so that two workgroups share a CU; at least one.
 int waves_for(int w, int h) {
-    const size_t row = (size_t)(w > h ? w : h) * sizeof(float);
-    return row * 4 <= 65536 ? 4 : (row * 2 <= 65536 ? 2 : 1);
+    const size_t row = wave_lds_floats(w > h ? w : h) * sizeof(float);
+    return row * 4 <= 81920 ? 4 : (row * 2 <= 81920 ? 2 : 1);
 }
 
 template <int V, typename T, int WAVES>
 hipError_t launch_cdf_w(hipStream_t s, const float* texels, int w, int h, int ch, uint16_t* cond, uint16_t* marg,
                         float* cdfY, void* rowTotals, void* integral, unsigned int* counter) {
-    const size_t lds = (size_t)WAVES * (size_t)(w > h ? w : h) * sizeof(float);
+    const size_t lds = (size_t)WAVES * wave_lds_floats(w > h ? w : h) * sizeof(float);
     auto fn = k_envmap_cdf<V, T, WAVES>;
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
