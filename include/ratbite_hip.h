@@ -227,6 +227,21 @@ int rbh_envmap_build(rbh_ctx ctx, const float* texels, int w, int h, int channel
 /* Re-run the CDF build on the resident env map (device-resident input). */
 int rbh_envmap_rebuild(rbh_ctx ctx, uint32_t flags, int iterations);
 int rbh_envmap_read(rbh_ctx ctx, uint16_t* condIdx, uint16_t* margIdx, float* cdfY, double* integral);
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). The build and inversion of rbh_envmap_build, from and into
+ * caller-owned DEVICE buffers: texels (w*h*channels floats), condIdx (w*h),
+ * margIdx (h), cdfY (h) and integralOut (one double: the integral divided by
+ * w*h, as rbh_envmap_read returns it). The build's scratch (h row terms,
+ * a counter) is a fixed array of the library on each device, so
+ * h <= RBH_ENVMAP_NOALLOC_MAX_ROWS; the call allocates nothing and only
+ * launches kernels on the context's stream (no host synchronisation: the
+ * outputs are ready when the stream reaches them). Calls that overlap on one
+ * GPU (several contexts or threads) race on that array. The context's own
+ * env map is not touched. */
+enum { RBH_ENVMAP_NOALLOC_MAX_ROWS = 32768 };
+int rbh_envmap_build_noalloc(rbh_ctx ctx, const float* texels, int w, int h, int channels, uint32_t flags,
+                             uint16_t* condIdx, uint16_t* margIdx, float* cdfY, double* integralOut);
 int rbh_envmap_disable(rbh_ctx ctx);
 
 /* ------------------------------------------------------------------ render

@@ -31,6 +31,7 @@ RBH_RENDER_GUIDE_FP8 = 4
 RBH_RENDER_TIMING = 8
 RBH_RENDER_NO_DROP = 16
 RBH_CDF_FP64 = 1
+RBH_ENVMAP_NOALLOC_MAX_ROWS = 32768
 RBH_LIGHT_TREE_FP64 = 1
 
 # Record layouts (byte-identical to include/ratbite_hip.h)
@@ -131,6 +132,8 @@ _SIGS = {
     "rbh_envmap_rebuild": (c_int, [c_void_p, c_uint32, c_int]),
     "rbh_envmap_read": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_double)]),
     "rbh_envmap_disable": (c_int, [c_void_p]),
+    "rbh_envmap_build_noalloc": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_uint32, c_void_p, c_void_p, c_void_p,
+                                         c_void_p]),
     "rbh_render": (c_int, [c_void_p, ctypes.POINTER(LaunchParams)]),
     "rbh_set_path_slots": (c_int, [c_void_p, c_uint64]),
     "rbh_render_batch": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_void_p]),

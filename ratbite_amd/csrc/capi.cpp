@@ -11,6 +11,9 @@ namespace rbh {
 hipError_t envmap_cdf_launch(rbh_ctx c, bool fp64);
 hipError_t srgb_lut_fill(rbh_ctx c);
 size_t envmap_scratch_bytes(int h);
+hipError_t envmap_noalloc_init(int device);
+hipError_t envmap_cdf_noalloc(int device, hipStream_t s, bool fp64, const float* texels, int w, int h, int ch,
+                              uint16_t* cond, uint16_t* marg, float* cdfY, double* normIntegral);
 }  // namespace rbh
 
 using namespace rbh;
@@ -34,8 +37,11 @@ int rbh_create(int device, rbh_ctx* out) {
     rbh_context* c = new (std::nothrow) rbh_context();
     if (!c) return RBH_ERR_NOMEM;
     c->device = device;
+    // envmap_noalloc_init: this device's address of the no-allocation CDF
+    // build's static scratch array, looked up once per device
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        envmap_noalloc_init(device) != hipSuccess) {
         delete c;
         return RBH_ERR_HIP;
     }
@@ -289,6 +295,23 @@ int rbh_envmap_rebuild(rbh_ctx c, uint32_t flags, int iterations) {
     int r = envmap_finish(c, flags);
     c->stats.cdfMs /= iterations;
     return r;
+}
+
+int rbh_envmap_build_noalloc(rbh_ctx c, const float* texels, int w, int h, int ch, uint32_t flags, uint16_t* cond,
+                             uint16_t* marg, float* cdfY, double* integralOut) {
+    CHECK_CTX(c);
+    if (!texels || !cond || !marg || !cdfY || !integralOut) return fail(c, RBH_ERR_INVALID, "a device buffer is null");
+    if (flags & ~(uint32_t)RBH_CDF_FP64) return fail(c, RBH_ERR_INVALID, "unknown CDF flags 0x%x", flags);
+    if (ch != 3 && ch != 4) return fail(c, RBH_ERR_INVALID, "channels must be 3 or 4");
+    if (w < 4 || h < 4 || (w % 4) || (h % 4)) return fail(c, RBH_ERR_INVALID, "env-map dimensions must be multiples of 4 (got %dx%d)", w, h);
+    if (w > 32768) return fail(c, RBH_ERR_RANGE, "env-map width above 32768");
+    if (h > RBH_ENVMAP_NOALLOC_MAX_ROWS)
+        return fail(c, RBH_ERR_RANGE, "env-map height %d above the %d rows of the static scratch", h,
+                    (int)RBH_ENVMAP_NOALLOC_MAX_ROWS);
+    const hipError_t e = envmap_cdf_noalloc(c->device, c->stream, (flags & RBH_CDF_FP64) != 0, texels, w, h, ch, cond,
+                                            marg, cdfY, integralOut);
+    if (e != hipSuccess) return fail(c, RBH_ERR_HIP, "no-allocation CDF build: %s", hipGetErrorString(e));
+    return RBH_OK;
 }
 
 int rbh_envmap_read(rbh_ctx c, uint16_t* cond, uint16_t* marg, float* cdfY, double* integral) {

@@ -20,6 +20,7 @@
 // normalisation in double and rounds each CDF value once.
 #include "../context.hpp"
 #include <cmath>
+#include <mutex>
 
 namespace rbh {
 
@@ -336,7 +337,8 @@ template <int V, typename T, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void k_envmap_cdf(const float* __restrict__ texels, int w, int h, int channels,
                                                             uint16_t* __restrict__ cond, uint16_t* __restrict__ marg,
                                                             float* __restrict__ cdfY, T* __restrict__ rowTotals,
-                                                            T* __restrict__ integral, unsigned int* __restrict__ counter) {
+                                                            T* __restrict__ integral, unsigned int* __restrict__ counter,
+                                                            double* __restrict__ normIntegral) {
     extern __shared__ __align__(16) float lds[];
     __shared__ bool last;
     constexpr int B = 64 * V;
@@ -418,6 +420,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_envmap_cdf(const float* __restri
     invert_row(cdf, marks, h, marg);
     if (lane == 0) {
         *integral = mtotal;
+        // the integral divided by w h as the host does it after the build
+        // (FP32: in float, FP64: in double), for callers that read no host value
+        if (normIntegral) {
+            if constexpr (kF32) *normIntegral = (double)(mtotal / ((float)w * (float)h));
+            else *normIntegral = mtotal / ((double)w * (double)h);
+        }
         *counter = 0u;   // re-arm for the next build
     }
 }
@@ -436,23 +444,25 @@ int waves_for(int w, int h) {
 
 template <int V, typename T, int WAVES>
 hipError_t launch_cdf_w(hipStream_t s, const float* texels, int w, int h, int ch, uint16_t* cond, uint16_t* marg,
-                        float* cdfY, void* rowTotals, void* integral, unsigned int* counter) {
+                        float* cdfY, void* rowTotals, void* integral, unsigned int* counter, double* normIntegral) {
     const size_t lds = (size_t)WAVES * wave_lds_floats(w > h ? w : h) * sizeof(float);
     auto fn = k_envmap_cdf<V, T, WAVES>;
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(fn, dim3(h / WAVES), dim3(64 * WAVES), lds, s, texels, w, h, ch, cond, marg, cdfY,
-                       static_cast<T*>(rowTotals), static_cast<T*>(integral), counter);
+                       static_cast<T*>(rowTotals), static_cast<T*>(integral), counter, normIntegral);
     return hipGetLastError();
 }
 
 template <int V, typename T>
 hipError_t launch_cdf_v(hipStream_t s, const float* texels, int w, int h, int ch, uint16_t* cond, uint16_t* marg,
-                        float* cdfY, void* rowTotals, void* integral, unsigned int* counter) {
+                        float* cdfY, void* rowTotals, void* integral, unsigned int* counter, double* normIntegral) {
     const int waves = waves_for(w, h);
-    if (waves == 4) return launch_cdf_w<V, T, 4>(s, texels, w, h, ch, cond, marg, cdfY, rowTotals, integral, counter);
-    if (waves == 2) return launch_cdf_w<V, T, 2>(s, texels, w, h, ch, cond, marg, cdfY, rowTotals, integral, counter);
-    return launch_cdf_w<V, T, 1>(s, texels, w, h, ch, cond, marg, cdfY, rowTotals, integral, counter);
+    if (waves == 4)
+        return launch_cdf_w<V, T, 4>(s, texels, w, h, ch, cond, marg, cdfY, rowTotals, integral, counter, normIntegral);
+    if (waves == 2)
+        return launch_cdf_w<V, T, 2>(s, texels, w, h, ch, cond, marg, cdfY, rowTotals, integral, counter, normIntegral);
+    return launch_cdf_w<V, T, 1>(s, texels, w, h, ch, cond, marg, cdfY, rowTotals, integral, counter, normIntegral);
 }
 
 // The fused build over device buffers: texels (w*h*ch floats), outputs cond
@@ -460,19 +470,70 @@ hipError_t launch_cdf_v(hipStream_t s, const float* texels, int w, int h, int ch
 // in FP64 mode), integral (one T) and the last-workgroup counter (zero
 // before the first launch; every launch leaves it at zero).
 hipError_t envmap_cdf_run(hipStream_t s, bool fp64, const float* texels, int w, int h, int ch, uint16_t* cond,
-                          uint16_t* marg, float* cdfY, void* rowTotals, void* integral, unsigned int* counter) {
+                          uint16_t* marg, float* cdfY, void* rowTotals, void* integral, unsigned int* counter,
+                          double* normIntegral = nullptr) {
     const int B = block_size_for(w);
     if (!fp64) {
-        if (B == 256) return launch_cdf_v<4, float>(s, texels, w, h, ch, cond, marg, cdfY, rowTotals, integral, counter);
-        if (B == 512) return launch_cdf_v<8, float>(s, texels, w, h, ch, cond, marg, cdfY, rowTotals, integral, counter);
-        return launch_cdf_v<16, float>(s, texels, w, h, ch, cond, marg, cdfY, rowTotals, integral, counter);
+        if (B == 256)
+            return launch_cdf_v<4, float>(s, texels, w, h, ch, cond, marg, cdfY, rowTotals, integral, counter, normIntegral);
+        if (B == 512)
+            return launch_cdf_v<8, float>(s, texels, w, h, ch, cond, marg, cdfY, rowTotals, integral, counter, normIntegral);
+        return launch_cdf_v<16, float>(s, texels, w, h, ch, cond, marg, cdfY, rowTotals, integral, counter, normIntegral);
     }
-    if (B == 256) return launch_cdf_v<4, double>(s, texels, w, h, ch, cond, marg, cdfY, rowTotals, integral, counter);
-    if (B == 512) return launch_cdf_v<8, double>(s, texels, w, h, ch, cond, marg, cdfY, rowTotals, integral, counter);
-    return launch_cdf_v<16, double>(s, texels, w, h, ch, cond, marg, cdfY, rowTotals, integral, counter);
+    if (B == 256)
+        return launch_cdf_v<4, double>(s, texels, w, h, ch, cond, marg, cdfY, rowTotals, integral, counter, normIntegral);
+    if (B == 512)
+        return launch_cdf_v<8, double>(s, texels, w, h, ch, cond, marg, cdfY, rowTotals, integral, counter, normIntegral);
+    return launch_cdf_v<16, double>(s, texels, w, h, ch, cond, marg, cdfY, rowTotals, integral, counter, normIntegral);
+}
+
+// ---- the build without allocations (rbh_envmap_build_noalloc)
+//
+// Its scratch (the counter, the raw integral, the h weighted row totals)
+// lives in this fixed device array: every GPU the library runs on holds its
+// own copy until the process ends. Host code never passes the array's name
+// to a kernel (in host code it is only a stand-in for the symbol): it uses
+// the address hipGetSymbolAddress returned for the device, kept per device.
+constexpr size_t kNoAllocCounter = 0, kNoAllocIntegral = 256, kNoAllocRows = 512;
+constexpr size_t kNoAllocBytes = kNoAllocRows + (size_t)RBH_ENVMAP_NOALLOC_MAX_ROWS * sizeof(double);
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+static void* g_scratch[kMaxDevices];
+static std::once_flag g_scratchOnce[kMaxDevices];
+static hipError_t g_scratchErr[kMaxDevices];
+
+__global__ void k_zero_counter(unsigned int* __restrict__ c) {
+    if (threadIdx.x == 0) *c = 0u;
 }
 
 }  // namespace
+
+// Looks up this device's copy of the scratch array (once per device; the
+// device must be current). Called when a context is created.
+hipError_t envmap_noalloc_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::call_once(g_scratchOnce[device], [device]() {
+        void* p = nullptr;
+        g_scratchErr[device] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        g_scratch[device] = g_scratchErr[device] == hipSuccess ? p : nullptr;
+    });
+    return g_scratchErr[device];
+}
+
+// The fused build over caller device buffers with the static scratch of
+// `device` (looked up by envmap_noalloc_init): no allocation, kernel
+// launches on `s` only (the counter is cleared by a kernel, not a memset, so
+// the call can be captured into a graph).
+hipError_t envmap_cdf_noalloc(int device, hipStream_t s, bool fp64, const float* texels, int w, int h, int ch,
+                              uint16_t* cond, uint16_t* marg, float* cdfY, double* normIntegral) {
+    if (device < 0 || device >= kMaxDevices || !g_scratch[device]) return hipErrorNotInitialized;
+    char* base = static_cast<char*>(g_scratch[device]);
+    unsigned int* counter = reinterpret_cast<unsigned int*>(base + kNoAllocCounter);
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, s, counter);
+    return envmap_cdf_run(s, fp64, texels, w, h, ch, cond, marg, cdfY, base + kNoAllocRows, base + kNoAllocIntegral,
+                          counter, normIntegral);
+}
 
 // Launches the fused build on the resident env map; returns a HIP error.
 hipError_t envmap_cdf_launch(rbh_ctx c, bool fp64) {

@@ -206,6 +206,9 @@ static void gpu_part(void) {
     double* done = (double*)guarded(2 * sizeof(double));
     OK(c, rbh_render_batch(c, both, lps, 2, done));
     check_guard(done, 2 * sizeof(double), "rbh_render_batch completionMs");
+    /* the no-allocation build takes device buffers only; null ones are refused */
+    EXPECT(rbh_envmap_build_noalloc(c, NULL, ew, eh, 3, 0, NULL, NULL, NULL, NULL) == RBH_ERR_INVALID,
+           "rbh_envmap_build_noalloc accepted null buffers");
     OK(c, rbh_envmap_disable(c));
     /* guiding */
     rbh_guide_config gc;
