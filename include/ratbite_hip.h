@@ -321,6 +321,9 @@ typedef struct rbh_guide_config {
     float sceneMin[3], sceneMax[3];   /* position normalisation */
     uint32_t seed;                    /* He-uniform initialisation */
     uint32_t recordCapacity;          /* records kept per RBH_RENDER_COLLECT render */
+    uint32_t fixedSelection;          /* 1: one-sample MIS at a fixed selection probability a = 0.5; the
+                                         selection output's weights and bias are held at zero (set_params
+                                         zeroes them too), so only the 40 lobe outputs are trained */
 } rbh_guide_config;
 int rbh_guide_configure(rbh_ctx ctx, const rbh_guide_config* cfg);
 /* FP32 master parameters, per layer W[out][in] then b[out]

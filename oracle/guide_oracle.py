@@ -12,6 +12,9 @@ CPU oracle its HIP kernels are checked against. It defines:
     (normalised), log sharpness (clamped) and a mixture logit (softmax), and
     a selection logit c: the renderer draws from the lobes with probability
     a = SEL_MIN + (SEL_MAX - SEL_MIN) sigmoid(c), otherwise from the BSDF;
+    in the fixed-selection mode the selection output is held at zero (its
+    weights and bias are zero and get no gradient), so a = 0.5 and the
+    network is the 40-output mixture network;
   * the loss of one training record, weight * -log(a q(d) + (1 - a) p_bsdf(d)),
     whose batch sum divided by the weight sum is minimised: a KL divergence
     between the density of the sampled contribution (BSDF x cosine x incident
@@ -174,9 +177,19 @@ def frame(n):
     return t, bt
 
 
-def loss_and_dout(out, rec):
+def fix_selection(ps):
+    """Parameters of the fixed-selection mode: the last layer's selection row
+    (weights and bias of output N_MIX) set to zero."""
+    ps = [(W.copy(), b.copy()) for W, b in ps]
+    ps[-1][0][N_MIX, :] = 0.0
+    ps[-1][1][N_MIX] = 0.0
+    return ps
+
+
+def loss_and_dout(out, rec, fixed_selection=False):
     """Batch loss sum(w * -log(a q + (1 - a) p_bsdf)) / sum(w) and its
-    gradient w.r.t. the raw outputs."""
+    gradient w.r.t. the raw outputs. With fixed_selection, a = 0.5 and the
+    selection output gets no gradient."""
     rec = np.asarray(rec, np.float64)
     d = rec[:, 11:14]
     wt = rec[:, 14]
@@ -190,7 +203,7 @@ def loss_and_dout(out, rec):
     lse = (m + np.log(np.exp(lc - m).sum(axis=1, keepdims=True)))[:, 0]
     q = np.exp(lse)
     sig = 1.0 / (1.0 + np.exp(-np.asarray(out, np.float64)[:, N_MIX]))
-    a = SEL_MIN + (SEL_MAX - SEL_MIN) * sig
+    a = np.full_like(sig, 0.5) if fixed_selection else SEL_MIN + (SEL_MAX - SEL_MIN) * sig
     P = np.maximum(a * q + (1.0 - a) * pb, 1e-30)
     loss = float(np.sum(wt * -np.log(P)) / W)
     g = np.exp(lc - lse[:, None])                      # responsibilities within the lobes
@@ -214,7 +227,8 @@ def loss_and_dout(out, rec):
     dl[..., 4] = dlogit
     dout[:, :N_MIX] = dl.reshape(-1, N_MIX)
     # selection: d(-log P)/da = -(q - p_bsdf) / P, da/dc = (SEL_MAX - SEL_MIN) sig (1 - sig)
-    dout[:, N_MIX] = -(wt / W) * (q - pb) / P * (SEL_MAX - SEL_MIN) * sig * (1.0 - sig)
+    if not fixed_selection:
+        dout[:, N_MIX] = -(wt / W) * (q - pb) / P * (SEL_MAX - SEL_MIN) * sig * (1.0 - sig)
     return loss, dout
 
 
@@ -231,10 +245,12 @@ def backward(ps, acts, dout, emulate_bf16=False):
     return grads
 
 
-def loss_grad(ps, rec, emulate_bf16=False):
+def loss_grad(ps, rec, emulate_bf16=False, fixed_selection=False):
+    if fixed_selection:
+        ps = fix_selection(ps)
     x = encode(rec)
     out, acts = forward(ps, x, emulate_bf16)
-    loss, dout = loss_and_dout(out, rec)
+    loss, dout = loss_and_dout(out, rec, fixed_selection)
     return loss, backward(ps, acts, dout, emulate_bf16)
 
 

@@ -96,6 +96,7 @@ class GuideConfig(ctypes.Structure):
     _fields_ = [
         ("lr", c_float), ("beta1", c_float), ("beta2", c_float), ("eps", c_float),
         ("sceneMin", c_float * 3), ("sceneMax", c_float * 3), ("seed", c_uint32), ("recordCapacity", c_uint32),
+        ("fixedSelection", c_uint32),
     ]
 
 

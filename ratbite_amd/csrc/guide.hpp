@@ -48,6 +48,7 @@ struct GuideState {
     float lr = 1e-3f, beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
     float sceneMin[3] = {0, 0, 0}, sceneInvExtent[3] = {1, 1, 1};
     uint32_t step = 0;                 // Adam steps taken
+    bool fixedSel = false;             // selection fixed at 0.5 (the selection output held at zero)
     void* master = nullptr;            // float[kParams]
     void* adamM = nullptr;             // float[kParams]
     void* adamV = nullptr;             // float[kParams]

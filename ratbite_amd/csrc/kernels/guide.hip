@@ -60,7 +60,8 @@ __device__ __forceinline__ LobeView lobe_at(const float* o, int j) {
 // d(loss)/d(outputs) * weight / weightSum as bf16 into dst[0..63] (zeros past
 // the 41 outputs). The lobes are re-derived from the outputs in each pass to
 // keep few values live.
-__device__ float record_grad(const float* __restrict__ o, const float* __restrict__ rec, float invWsum, bf16* dst) {
+__device__ float record_grad(const float* __restrict__ o, const float* __restrict__ rec, float invWsum, bf16* dst,
+                             bool fixedSel) {
     const float dx = rec[11], dy = rec[12], dz = rec[13], wt = rec[14], pb = rec[15];
     float lmax = -3.4e38f;
     for (int j = 0; j < kLobes; ++j) lmax = fmaxf(lmax, o[5 * j + 4]);
@@ -105,7 +106,8 @@ __device__ float record_grad(const float* __restrict__ o, const float* __restric
         dst[5 * j + 4] = (bf16)(-s * (gj - w));
     }
     // selection: d(-log P)/da = -(q - p_bsdf) / P, da/dc = (kSelMax - kSelMin) sig (1 - sig)
-    dst[kMix] = (bf16)(-s0 * (q - pb) / P * (kSelMax - kSelMin) * sig * (1.0f - sig));
+    // (with a fixed selection the selection output stays zero, a = 0.5)
+    dst[kMix] = fixedSel ? (bf16)0.0f : (bf16)(-s0 * (q - pb) / P * (kSelMax - kSelMin) * sig * (1.0f - sig));
     for (int c2 = kOut; c2 < 64; ++c2) dst[c2] = (bf16)0.0f;
     return wt * -logf(P);
 }
@@ -164,7 +166,8 @@ __device__ __forceinline__ f32x4 grad_tile_mfma(const bf16* dY, const bf16* X, i
 __global__ __launch_bounds__(256) void k_guide_train(const float* __restrict__ recs, uint32_t n,
                                                      const float* __restrict__ scalars, const bf16* __restrict__ wf,
                                                      const bf16* __restrict__ wt, const float* __restrict__ bias,
-                                                     float* __restrict__ partials, float* __restrict__ lossOut) {
+                                                     float* __restrict__ partials, float* __restrict__ lossOut,
+                                                     uint32_t fixedSel) {
     __shared__ Lds L;
     const int wave = threadIdx.x >> 6, lane = __lane_id(), r = lane & 15, g = lane >> 4;
     const int row0 = wave * kRows;
@@ -186,7 +189,7 @@ __global__ __launch_bounds__(256) void k_guide_train(const float* __restrict__ r
             const uint32_t s = base + row0 + r;
             bf16* dst = L.dy[0] + (row0 + r) * kSH;
             if (s < n) {
-                lossAcc += record_grad(L.out + (row0 + r) * kSO, recs + (size_t)s * kRec, invW, dst);
+                lossAcc += record_grad(L.out + (row0 + r) * kSO, recs + (size_t)s * kRec, invW, dst, fixedSel != 0u);
             } else {
                 for (int c = 0; c < 64; ++c) dst[c] = (bf16)0.0f;
             }
@@ -406,8 +409,23 @@ void init_params(uint32_t seed, std::vector<float>& p) {
     }
 }
 
+// The selection output's weights and bias (output kMix of the last layer):
+// zero and never trained when the selection is fixed at 0.5, which makes the
+// network the 40-output mixture network.
+void clear_selection_row(std::vector<float>& p) {
+    const int l = kLayers - 1;
+    for (int i = 0; i < fan_in(l); ++i) p[w_off(l) + kMix * fan_in(l) + i] = 0.0f;
+    p[b_off(l) + kMix] = 0.0f;
+}
+
 int upload_master(rbh_ctx c, const float* params) {
     GuideState& g = gs(c);
+    std::vector<float> fixedParams;
+    if (g.fixedSel) {
+        fixedParams.assign(params, params + kParams);
+        clear_selection_row(fixedParams);
+        params = fixedParams.data();
+    }
     RBH_HIP(c, hipMemcpyAsync(g.master, params, kParams * 4, hipMemcpyHostToDevice, c->stream));
     RBH_HIP(c, hipMemsetAsync(g.adamM, 0, kParams * 4, c->stream));
     RBH_HIP(c, hipMemsetAsync(g.adamV, 0, kParams * 4, c->stream));
@@ -436,7 +454,7 @@ int run_train(rbh_ctx c, const float* drecs, uint32_t n, int mode, float* gradOu
     hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(64), 0, c->stream, sc, 4);
     hipLaunchKernelGGL(k_weight_sum, dim3(256), dim3(256), 0, c->stream, drecs, n, sc);
     hipLaunchKernelGGL(k_guide_train, dim3(blocks), dim3(256), 0, c->stream, drecs, n, sc, dptr<bf16>(g.wf),
-                       dptr<bf16>(g.wt), dptr<float>(g.bias), dptr<float>(g.partials), sc + 1);
+                       dptr<bf16>(g.wt), dptr<float>(g.bias), dptr<float>(g.partials), sc + 1, g.fixedSel ? 1u : 0u);
     if (mode == 0) ++g.step;
     const float bc1 = 1.0f - std::pow(g.beta1, (float)std::max<uint32_t>(1, g.step));
     const float bc2 = 1.0f - std::pow(g.beta2, (float)std::max<uint32_t>(1, g.step));
@@ -466,6 +484,7 @@ int rbh_guide_configure(rbh_ctx c, const rbh_guide_config* cfg) {
     if (!(cfg->lr > 0.0f) || !(cfg->beta1 >= 0.0f && cfg->beta1 < 1.0f) || !(cfg->beta2 >= 0.0f && cfg->beta2 < 1.0f) ||
         !(cfg->eps > 0.0f))
         return fail(c, RBH_ERR_INVALID, "Adam parameters out of range");
+    if (cfg->fixedSelection > 1u) return fail(c, RBH_ERR_INVALID, "fixedSelection must be 0 or 1");
     GuideState& g = gs(c);
     RBH_HIP(c, hipSetDevice(c->device));
     RBH_HIP(c, alloc(&g.master, kParams * 4));
@@ -492,6 +511,7 @@ int rbh_guide_configure(rbh_ctx c, const rbh_guide_config* cfg) {
     g.beta1 = cfg->beta1;
     g.beta2 = cfg->beta2;
     g.eps = cfg->eps;
+    g.fixedSel = cfg->fixedSelection != 0u;
     for (int a = 0; a < 3; ++a) {
         const float e = cfg->sceneMax[a] - cfg->sceneMin[a];
         if (!(e > 0.0f)) return fail(c, RBH_ERR_INVALID, "scene bounds must have positive extent");

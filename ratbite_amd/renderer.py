@@ -82,9 +82,11 @@ class Renderer:
         self.ctx.call("rbh_set_tuning", ctypes.byref(t))
 
     # ------------------------------------------------------------ guiding
-    def configure_guide(self, seed=1, lr=1e-3, capacity=1 << 22):
+    def configure_guide(self, seed=1, lr=1e-3, capacity=1 << 22, fixed_selection=False):
         """Sets up the guiding network (random initialisation) with the
-        scene's bounding box as the position normalisation."""
+        scene's bounding box as the position normalisation. With
+        fixed_selection the guided render draws from the lobes with
+        probability 0.5 and only the 40 lobe outputs are trained."""
         v = np.asarray(self.scene.verts, np.float64)
         lo, hi = v.min(axis=0), v.max(axis=0)
         pad = 1e-3 * max(1e-6, float((hi - lo).max()))
@@ -94,6 +96,7 @@ class Renderer:
         cfg.sceneMax[:] = [float(x) for x in hi + pad]
         cfg.seed = seed
         cfg.recordCapacity = capacity
+        cfg.fixedSelection = int(bool(fixed_selection))
         self.ctx.call("rbh_guide_configure", ctypes.byref(cfg))
 
     def train_guide(self, batch=1 << 18, steps=1):

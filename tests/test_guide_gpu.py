@@ -13,13 +13,14 @@ from ratbite_amd import _lib as L
 pytestmark = pytest.mark.gpu
 
 
-def _configure(ctx, seed=1, lr=1e-3):
+def _configure(ctx, seed=1, lr=1e-3, fixed=False):
     cfg = L.GuideConfig()
     cfg.lr, cfg.beta1, cfg.beta2, cfg.eps = lr, 0.9, 0.999, 1e-8
     cfg.sceneMin[:] = [0, 0, 0]
     cfg.sceneMax[:] = [1, 1, 1]
     cfg.seed = seed
     cfg.recordCapacity = 0
+    cfg.fixedSelection = int(fixed)
     ctx.check(L.lib().rbh_guide_configure(ctx.h, ctypes.byref(cfg)))
 
 
@@ -111,6 +112,45 @@ def test_gradient_matches_oracle(ctx):
             # the gradient this far from the FP64 one
             assert rel64 < 0.3
             off += size
+
+
+def test_fixed_selection_mode_matches_oracle(ctx):
+    """Fixed selection a = 0.5: set_params zeroes the selection row, the
+    selection output is exactly zero, the gradient matches the oracle's
+    fixed-mode gradient (zero on the selection row) and Adam leaves the row
+    at zero."""
+    _configure(ctx, fixed=True)
+    p = _params(5, scale_out=2.0)
+    _set(ctx, p)
+    back = np.zeros_like(p)
+    ctx.check(L.lib().rbh_guide_get_params(ctx.h, L.ptr(back), len(back)))
+    ps = G.unflatten(p.astype(np.float64))
+    want = G.flatten(G.fix_selection(ps)).astype(np.float32)
+    assert np.array_equal(back, want)
+    rng = np.random.default_rng(6)
+    n = 4096
+    rec = _records(n, rng)
+    out = np.zeros((n, G.N_OUT), np.float32)
+    ctx.check(L.lib().rbh_guide_infer(ctx.h, L.ptr(rec), n, L.ptr(out)))
+    assert np.all(out[:, G.N_MIX] == 0.0)
+    grad = np.zeros(L.GUIDE_PARAMS, np.float32)
+    loss = ctypes.c_float()
+    ctx.check(L.lib().rbh_guide_gradient(ctx.h, L.ptr(rec), n, L.ptr(grad), ctypes.byref(loss)))
+    l_ref, g_ref = G.loss_grad(ps, rec.astype(np.float64), emulate_bf16=True, fixed_selection=True)
+    gr = G.flatten(g_ref)
+    print("fixed selection: loss gpu %.6f oracle(bf16) %.6f" % (loss.value, l_ref), flush=True)
+    assert abs(loss.value - l_ref) / abs(l_ref) < 2e-3
+    fi, fo = G.LAYERS[-1]
+    w0 = len(gr) - fi * fo - fo
+    sel_rows = np.r_[w0 + G.N_MIX * fi: w0 + (G.N_MIX + 1) * fi, len(gr) - 1]
+    assert np.all(grad[sel_rows] == 0.0)
+    rel = np.linalg.norm(grad - gr) / np.linalg.norm(gr)
+    cos = grad @ gr / (np.linalg.norm(grad) * np.linalg.norm(gr))
+    print("fixed selection gradient: rel L2 %.2e cosine %.6f" % (rel, cos), flush=True)
+    assert rel < 2e-2 and cos > 0.9998
+    ctx.check(L.lib().rbh_guide_train(ctx.h, L.ptr(rec), n, 1024, 5, ctypes.byref(loss)))
+    ctx.check(L.lib().rbh_guide_get_params(ctx.h, L.ptr(back), len(back)))
+    assert np.all(back[sel_rows] == 0.0) and not np.array_equal(back, want)
 
 
 def test_adam_training_reaches_target(ctx):

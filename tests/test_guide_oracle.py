@@ -61,6 +61,39 @@ def test_gradient_matches_finite_differences():
         assert abs(fd - gflat[i]) <= 1e-6 + 1e-5 * abs(fd), (i, fd, gflat[i])
 
 
+def test_fixed_selection_mode():
+    """With the selection fixed at 0.5 the loss is the batch mean of
+    w * -log(q / 2 + p_bsdf / 2) with q from the mixture density, the
+    gradient still matches finite differences, and the selection row (which
+    the loss no longer depends on) gets exactly zero gradient."""
+    rng = np.random.default_rng(11)
+    ps = G.init_params(seed=12)
+    ps[-1] = (ps[-1][0] * 3.0, rng.normal(size=41) * 0.5)
+    rec = _records(64, rng)
+    loss, grads = G.loss_grad(ps, rec, fixed_selection=True)
+    fixed = G.fix_selection(ps)
+    out, _ = G.forward(fixed, G.encode(rec))
+    assert np.all(out[:, G.N_MIX] == 0.0) and np.allclose(G.selection(out), 0.5)
+    q = G.pdf(out, rec[:, 11:14])
+    direct = np.sum(rec[:, 14] * -np.log(0.5 * q + 0.5 * rec[:, 15])) / rec[:, 14].sum()
+    assert abs(loss - direct) <= 1e-12 * abs(direct)
+    # the learned selection's loss differs: the mode changes the objective
+    assert abs(G.loss_grad(ps, rec)[0] - loss) > 1e-3
+    Wg, bg = grads[-1]
+    assert np.all(Wg[G.N_MIX] == 0.0) and bg[G.N_MIX] == 0.0
+    flat = G.flatten(fixed)
+    gflat = G.flatten(grads)
+    idx = np.concatenate([rng.choice(len(flat) - 41, 60, replace=False), np.arange(len(flat) - 41, len(flat) - 1)])
+    h = 1e-6
+    for i in idx:
+        p1 = flat.copy(); p1[i] += h
+        p2 = flat.copy(); p2[i] -= h
+        l1, _ = G.loss_grad(G.unflatten(p1), rec, fixed_selection=True)
+        l2, _ = G.loss_grad(G.unflatten(p2), rec, fixed_selection=True)
+        fd = (l1 - l2) / (2 * h)
+        assert abs(fd - gflat[i]) <= 1e-6 + 1e-5 * abs(fd), (i, fd, gflat[i])
+
+
 def _sphere_grid(nc, nphi):
     c = -1 + (np.arange(nc) + 0.5) * 2 / nc
     ph = (np.arange(nphi) + 0.5) * 2 * np.pi / nphi
