@@ -9,9 +9,12 @@ tree, a 2048x1024 log-normal env map with a sun, the four-way material mix,
 LBVH, light tree and env-map tables are built once before the timed region.
 
 Printed: one JSON line with samples/s (whole frames), Mrays/s, the roofline
-of the dominant kernel measured with HIP events on the render stream, and the
-CPU baseline: the scalar C++ restatement (oracle/) on the centre 256x256 crop
-of the same frame, timed on this machine's host cores.
+of the dominant kernel measured with HIP events on the render stream, the
+same frames replayed as HIP graphs, the guided study on the same scene
+(relMSE of guided vs plain at equal wall time, one-sample MIS at selection
+0.5, against a plain FP64-accumulated reference), and the CPU baseline: the
+scalar C++ restatement (oracle/, built -march=native on this host) on the
+centre 256x256 crop of the same frame, timed on this process's host cores.
 """
 import argparse
 import json
@@ -44,7 +47,8 @@ def parse():
     ap.add_argument("--depth", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-spp", type=int, default=256, help="spp of the CPU baseline crop")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, host cores)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="0: the host cores this process may use (OMP_NUM_THREADS, else the affinity mask)")
     ap.add_argument("--tune", action="store_true",
                     help="autotune the launch shape first (short 8-spp renders) instead of the library default, "
                          "which was picked by a sweep of full frames of this workload")
@@ -64,7 +68,14 @@ def parse():
     ap.add_argument("--train-iters", type=int, default=8, help="guided workload: collect+train rounds")
     ap.add_argument("--train-steps", type=int, default=8, help="guided workload: Adam steps per round")
     ap.add_argument("--train-batch", type=int, default=1 << 18, help="guided workload: records per Adam step")
-    ap.add_argument("--train-lr", type=float, default=2e-3, help="guided workload: Adam learning rate")
+    ap.add_argument("--train-lr", type=float, default=1e-3, help="guided study: Adam learning rate")
+    ap.add_argument("--fixed-selection", type=int, default=1,
+                    help="guided study: 1 = one-sample MIS at selection 0.5, 0 = learned selection")
+    ap.add_argument("--guided-budget-ms", type=float, default=500.0,
+                    help="default line: wall time per method of the guided study (0: no study)")
+    ap.add_argument("--guided-ref-spp", type=int, default=4096, help="default line: spp of the study's reference")
+    ap.add_argument("--graph-steps", type=int, default=2,
+                    help="default line: frames timed again with the passes replayed as HIP graphs (0: none)")
     return ap.parse_args()
 
 
@@ -84,12 +95,28 @@ def traffic_per_launch(kernel):
     return None
 
 
+def host_threads():
+    """The host cores this process may use: the CPU share the machine gives
+    it (OMP_NUM_THREADS where the environment sets one, else the affinity
+    mask)."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
 def cpu_baseline(sc, args, tree):
     """The oracle (scalar C++ restatement, OpenMP over 16x16 tiles) on the centre
     256x256 crop of the frame; returns samples/s of path tracing alone."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import tempfile
     import oracle_lib as O
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    threads = args.cpu_threads or host_threads()
+    build_s = O.use_native_build(os.path.join(tempfile.gettempdir(), "rbh_oracle_native_%d" % os.getpid()),
+                                 jobs=threads)
     cw, ch = min(256, sc.width), min(256, sc.height)
     crop = ((sc.width - cw) // 2, (sc.height - ch) // 2, cw, ch)
     lp = sc.launch_params(offset=0, count=args.cpu_spp, depth=args.depth)
@@ -106,7 +133,7 @@ def cpu_baseline(sc, args, tree):
         pass
     c1 = cpu_c1(threads)
     return {"value": v, "unit": "samples/s", "cores": threads, "kind": "port", "nproc": os.cpu_count(),
-            "cpu_model": model, "c1": c1,
+            "cpu_model": model, "c1": c1, "build": "g++ -O3 -march=native -ffp-contract=off -fopenmp (%.0f s)" % build_s,
             "sample": "centre %dx%d crop of the same frame at %d spp, depth %d (%.1f s of path tracing; "
                       "CPU BVH build of %.1f s excluded)" % (cw, ch, args.cpu_spp, args.depth, st["renderSeconds"],
                                                              st["bvhSeconds"])}
@@ -122,20 +149,31 @@ def cpu_c1(threads):
     from ratbite_amd.renderer import Renderer
     sc = scenes.cornell()
     r = Renderer(sc)
-    best = None
-    for _ in range(4):
-        r.ctx.synchronize()
-        t = time.perf_counter()
-        r.render(spp=sc.spp, offset=0)
-        r.ctx.synchronize()
-        el = time.perf_counter() - t
-        best = el if best is None else min(best, el)
+
+    def best_of():
+        best = None
+        for _ in range(4):
+            r.ctx.synchronize()
+            t = time.perf_counter()
+            r.render(spp=sc.spp, offset=0)
+            r.ctx.synchronize()
+            el = time.perf_counter() - t
+            best = el if best is None else min(best, el)
+        return best
+
+    best = best_of()
+    t = r.tuning()
+    r.set_tuning(path_slots=t["pathSlots"], trace_blocks=t["traceBlocksPerCU"], shade_blocks=t["shadeBlocksPerCU"],
+                 use_graph=True, shade_waves=t["shadeWaves"], trace_fetch=t["traceFetch"],
+                 trace_waves=t["traceWaves"], quant_nodes=t["quantNodes"], sort_hits=t["sortHits"])
+    best_graph = best_of()
     lp = sc.launch_params(offset=0, count=sc.spp)
     _, st = O.render(sc, lp, threads=threads, oscene=O.OracleScene(sc, tree=r.light_tree()))
     r.close()
     n = sc.width * sc.height * sc.spp
     return {"workload": "cornell 256x256, 8 UV spheres, blackbody 4000 K emitter, 16 spp, depth 8", "samples": n,
-            "gpu_ms": best * 1e3, "gpu_samples_per_s": n / best, "cpu_seconds": st["renderSeconds"],
+            "gpu_ms": best * 1e3, "gpu_samples_per_s": n / best, "gpu_graph_ms": best_graph * 1e3,
+            "gpu_graph_samples_per_s": n / best_graph, "cpu_seconds": st["renderSeconds"],
             "cpu_samples_per_s": n / st["renderSeconds"]}
 
 
@@ -150,21 +188,8 @@ def batch_workload(args):
     from ratbite_amd import scenes
     from ratbite_amd.envmaps import lognormal_envmap
     from ratbite_amd.renderer import Renderer, render_batch
-    rng = np.random.default_rng(0x5EED0003)
-    sides = [64, 128, 256, 512, 1024]
-    rs = []
     t0 = time.time()
-    for i in range(args.scenes):
-        tris = float(np.exp(rng.uniform(np.log(1e3), np.log(5e5))))
-        level = 3 if tris > 2e5 else 2
-        per = 20 * 4 ** level
-        n_sph = max(1, int(round(tris / per)))
-        side = int(rng.choice(sides))
-        n_lights = int(np.exp(rng.uniform(0.0, np.log(5000.0))))
-        env = None if i % 2 else (256, 128)
-        sc = scenes.spheres(seed=0x5EED0003 + i, n_spheres=n_sph, level=level, n_lights=n_lights, width=side,
-                            height=side, spp=32, depth=12, env_size=env)
-        rs.append(Renderer(sc))
+    rs = [Renderer(sc) for sc in scenes.batch(args.scenes)]
     print("batch: %d scenes built in %.1f s" % (len(rs), time.time() - t0), file=sys.stderr, flush=True)
     tuning = rs[0].autotune(trial_spp=2)
     print("autotuned launch shape %s" % tuning, file=sys.stderr, flush=True)
@@ -245,42 +270,30 @@ def mlp_rates(r, chunk):
     return out
 
 
-def guided_workload(args):
-    """Equal-time study of the learned guide: the plain tracer renders for the
-    budget; the guided tracer spends the same wall time on collecting
-    training records, training and rendering. The collecting renders are
-    guided renders of the image's own sample range, so their samples count;
-    only the Adam steps are overhead. relMSE (raw, and without the
-    0.1% largest per-pixel errors) against a plain render of --ref-spp
-    samples is taken at eight checkpoints of the budget with the clock
-    paused, giving the final ratio and the time at which the guided tracer
-    reaches the plain tracer's final error. Both sides use the no-drop sample
-    semantics, so they converge to the same image. --scene spheres is the
-    1080p workload's scene (at --width x --height), --scene bounce the
-    Cornell box lit by the patch a small emitter throws on the back wall."""
+def guided_study(r, budget_ms, ref_spp, chunk, train_iters, train_steps, train_batch, lr, fixed_selection):
+    """Equal-time study of the learned guide on the renderer's scene: the
+    plain tracer renders for the budget; the guided tracer spends the same
+    wall time on collecting training records, training and rendering. The
+    collecting renders are guided renders of the image's own sample range,
+    so their samples count; only the Adam steps are overhead. relMSE (raw,
+    and without the 0.1% largest per-pixel errors) against a plain render of
+    ref_spp samples from sample 2^24 (FP64-accumulated) is taken at eight
+    checkpoints of the budget with the clock paused, giving the final ratio
+    and the time at which the guided tracer reaches the plain tracer's final
+    error. Both sides use the no-drop sample semantics, so they converge to
+    the same image. fixed_selection: one-sample MIS at selection 0.5 with the
+    40-output network; otherwise the learned selection in [0.1, 0.9]."""
     import numpy as np
-    from ratbite_amd import scenes
-    from ratbite_amd.renderer import Renderer
 
     def errs(a, ref, eps=1e-2):
         e = ((a - ref) ** 2 / (ref ** 2 + eps)).mean(axis=2).ravel()
         k = max(1, int(np.ceil(1e-3 * e.size)))
         return float(e.mean()), float(np.sort(e)[:-k].mean())
 
-    if args.scene == "bounce":
-        raise SystemExit("bench.py --workload guided --scene bounce is switched off: its plain no-drop reference "
-                         "renders from sample 2^24 ended in an illegal memory access on the GPU whose cause has not "
-                         "been found (a sanitized CPU build of the same kernel bodies replays such ranges cleanly)")
-    else:
-        sc = scenes.spheres(width=args.width, height=args.height, spp=16, depth=args.depth)
-    r = Renderer(sc)
-    chunk = args.chunk_spp
-
     def image():
         a = r.accum()
         return a[..., :3] / a[..., 3:4]
 
-    ref_spp = args.ref_spp or 8192
     t0 = time.perf_counter()
     done = 0
     while done < ref_spp:
@@ -288,22 +301,23 @@ def guided_workload(args):
         r.render(spp=step, offset=1 << 24 | done, no_drop=True, fresh=(done == 0))
         done += step
     ref = image()
-    print("reference: %d spp in %.1f s" % (ref_spp, time.perf_counter() - t0), file=sys.stderr, flush=True)
+    ref_s = time.perf_counter() - t0
+    print("guided study: reference %d spp in %.1f s" % (ref_spp, ref_s), file=sys.stderr, flush=True)
 
     def run(method):
-        checkpoints = [args.budget_ms * k / 8.0 for k in range(1, 9)]
+        checkpoints = [budget_ms * k / 8.0 for k in range(1, 9)]
         spent, spp, trained, curve = 0.0, 0, 0, []
         train_ms = 0.0
         r.ctx.synchronize()
         while checkpoints:
             t = time.perf_counter()
-            if method == "guided" and trained < args.train_iters:
+            if method == "guided" and trained < train_iters:
                 # collecting renders are guided renders too (unbiased for any
                 # state of the network), so their samples join the image
                 r.render(spp=chunk, offset=spp, guided=True, collect=True)
                 spp += chunk
                 t1 = time.perf_counter()
-                r.train_guide(batch=args.train_batch, steps=args.train_steps)
+                r.train_guide(batch=train_batch, steps=train_steps)
                 trained += 1
                 train_ms += (time.perf_counter() - t1) * 1e3
             else:
@@ -316,7 +330,7 @@ def guided_workload(args):
         return curve, train_ms
 
     plain, _ = run("plain")
-    r.configure_guide(seed=1, lr=args.train_lr, capacity=1 << 22)
+    r.configure_guide(seed=1, lr=lr, capacity=1 << 22, fixed_selection=fixed_selection)
     guided, train_ms = run("guided")
 
     def reach(curve, target, idx):
@@ -327,22 +341,45 @@ def guided_workload(args):
 
     fp, fg = plain[-1], guided[-1]
     t_raw, t_trim = reach(guided, fp[2], 2), reach(guided, fp[3], 3)
-    mlp = mlp_rates(r, chunk)
-    out = {
-        "metric": "relMSE at equal time, guided vs plain", "value": fg[2] / fp[2], "unit": "ratio",
-        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": args.budget_ms, "higher_is_better": False,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32+bf16", "data": "synthetic (seeded scene generator)",
-        "config": {"workload": "%s %dx%d, %d ms per method, reference %d spp plain, guide trained %d x (%d spp "
-                               "collect + %d Adam steps of %d records)"
-                               % (args.scene, sc.width, sc.height, args.budget_ms, ref_spp, args.train_iters,
-                                  chunk, args.train_steps, args.train_batch)},
+    return {
+        "relmse_ratio": fg[2] / fp[2], "relmse_ratio_trimmed": fg[3] / fp[3],
+        "time_to_plain_relmse": None if t_raw is None else t_raw / budget_ms,
+        "time_to_plain_relmse_trimmed": None if t_trim is None else t_trim / budget_ms,
         "relmse_plain": fp[2], "relmse_guided": fg[2], "relmse_trimmed_plain": fp[3], "relmse_trimmed_guided": fg[3],
-        "ratio_trimmed": fg[3] / fp[3], "spp_plain": fp[1], "spp_guided": fg[1], "guided_training_ms": train_ms,
-        "time_to_plain_relmse": None if t_raw is None else t_raw / args.budget_ms,
-        "time_to_plain_relmse_trimmed": None if t_trim is None else t_trim / args.budget_ms,
+        "spp_plain": fp[1], "spp_guided": fg[1], "guided_training_ms": train_ms, "budget_ms": budget_ms,
+        "reference_spp": ref_spp, "reference_s": ref_s,
+        "selection": "fixed 0.5 (40 trained outputs)" if fixed_selection else "learned in [0.1, 0.9] (41 outputs)",
+        "training": "%d x (%d spp guided collect + %d Adam steps of %d records), lr %g" % (
+            train_iters, chunk, train_steps, train_batch, lr),
         "curve_plain": [[round(c[0], 1), c[1], c[2], c[3]] for c in plain],
         "curve_guided": [[round(c[0], 1), c[1], c[2], c[3]] for c in guided],
-        "mlp": mlp,
+    }
+
+
+def guided_workload(args):
+    """The guided study as its own bench line (metric: relMSE ratio at equal
+    time). --scene spheres is the 1080p workload's scene (at --width x
+    --height)."""
+    from ratbite_amd import scenes
+    from ratbite_amd.renderer import Renderer
+
+    if args.scene == "bounce":
+        raise SystemExit("bench.py --workload guided --scene bounce is switched off: its plain no-drop reference "
+                         "renders from sample 2^24 ended in an illegal memory access on the GPU whose cause has not "
+                         "been found (a sanitized CPU build of the same kernel bodies replays such ranges cleanly)")
+    sc = scenes.spheres(width=args.width, height=args.height, spp=16, depth=args.depth)
+    r = Renderer(sc)
+    g = guided_study(r, args.budget_ms, args.ref_spp or 8192, args.chunk_spp, args.train_iters, args.train_steps,
+                     args.train_batch, args.train_lr, args.fixed_selection)
+    mlp = mlp_rates(r, args.chunk_spp)
+    out = {
+        "metric": "relMSE at equal time, guided vs plain", "value": g["relmse_ratio"], "unit": "ratio",
+        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": args.budget_ms, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+bf16", "data": "synthetic (seeded scene generator)",
+        "config": {"workload": "%s %dx%d, %d ms per method, reference %d spp plain, guide %s, %s"
+                               % (args.scene, sc.width, sc.height, args.budget_ms, g["reference_spp"],
+                                  g["selection"], g["training"])},
+        "guided": g, "mlp": mlp,
     }
     print(json.dumps(out), flush=True)
     r.close()
@@ -467,6 +504,30 @@ def large_workload(args):
     r.close()
 
 
+def graph_rate(r, ctx, sc, args, eager_rate):
+    """The same frames rendered with each pass captured as a HIP graph and
+    replayed (no per-kernel timing), timed on the host around whole frames."""
+    t = r.tuning()
+    r.set_tuning(path_slots=t["pathSlots"], trace_blocks=t["traceBlocksPerCU"], shade_blocks=t["shadeBlocksPerCU"],
+                 use_graph=True, shade_waves=t["shadeWaves"], trace_fetch=t["traceFetch"],
+                 trace_waves=t["traceWaves"], quant_nodes=t["quantNodes"], sort_hits=t["sortHits"])
+    params = sc.launch_params(offset=0, count=args.spp)
+    ctx.render(params)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.graph_steps):
+        ctx.render(params)
+    ctx.synchronize()
+    el = time.perf_counter() - t1
+    r.set_tuning(path_slots=t["pathSlots"], trace_blocks=t["traceBlocksPerCU"], shade_blocks=t["shadeBlocksPerCU"],
+                 use_graph=bool(t["useGraph"]), shade_waves=t["shadeWaves"], trace_fetch=t["traceFetch"],
+                 trace_waves=t["traceWaves"], quant_nodes=t["quantNodes"], sort_hits=t["sortHits"])
+    rate = sc.width * sc.height * args.spp * args.graph_steps / el
+    print("graph replay: %.1f ms per frame" % (el * 1e3 / args.graph_steps), file=sys.stderr, flush=True)
+    return {"samples_per_s": rate, "ms_per_step": el * 1e3 / args.graph_steps, "steps": args.graph_steps,
+            "over_eager": rate / eager_rate}
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -537,18 +598,23 @@ def main():
     launches = tot["traceLaunches"]
     achieved = kbytes[dom] / (kms[dom] * 1e-3) / 1e9
     traffic = traffic_per_launch(KERNEL_NAMES[dom])
-    # every traced ray is a path vertex (misses are shaded by the miss kernel)
+    # every traced ray is a path vertex (misses are shaded by the miss kernel);
+    # the FP64 accumulation is read and written once per pixel per pass, and a
+    # pass traces `depth` bounces
+    passes = tot["traceLaunches"] // max(1, args.depth)
     path_bytes = (BYTES_CLOSEST + BYTES_SHADOW + BYTES_VERTEX) * tot["closestRays"] + \
-        BYTES_PIXEL_LAUNCH * pixels * args.spp * args.steps
+        BYTES_PIXEL_LAUNCH * pixels * passes
+    graph = graph_rate(r, ctx, sc, args, samples / elapsed) if args.graph_steps > 0 else None
 
     out = {
         "metric": "1080p samples/s", "value": samples / elapsed, "unit": "samples/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene generator)",
-        "config": {"workload": "spheres 1920x1080: 200 level-5 icospheres (%d triangles), %d textured emissive "
-                               "triangles, env 2048x1024 log-normal + sun, 64 spp, depth 16" % (len(sc.indices),
-                                                                                                 len(sc.lights)),
+        "config": {"workload": "spheres %dx%d: 200 level-5 icospheres (%d triangles), %d emissive triangles "
+                               "sharing 16 seeded 256x256 RGBA8 value-noise emission textures, env 2048x1024 "
+                               "log-normal + sun, %d spp, depth %d" % (sc.width, sc.height, len(sc.indices),
+                                                                       len(sc.lights), args.spp, args.depth),
                    "width": sc.width, "height": sc.height, "spp": args.spp, "depth": args.depth},
         "mrays_per_s": rays / elapsed / 1e6,
         "rays": {"closest": tot["closestRays"], "shadow": tot["shadowRays"], "vertices": tot["pathVertices"]},
@@ -560,10 +626,16 @@ def main():
                      "traffic_gbs": None if traffic is None else traffic / (kms[dom] / max(1, launches) * 1e-3) / 1e9,
                      "copy_gbs_measured": copy_gbs},
         "path_roofline": {"achieved": path_bytes / elapsed / 1e9, "unit": "GB/s",
-                          "frac": path_bytes / elapsed / 1e9 / HBM_PEAK_GBS},
+                          "frac": path_bytes / elapsed / 1e9 / HBM_PEAK_GBS, "passes": passes},
+        "graph": graph,
         "cpu_baseline": None,
         "tuning": tuning,
     }
+    if args.guided_budget_ms > 0:
+        out["guided"] = guided_study(r, args.guided_budget_ms, args.guided_ref_spp, args.chunk_spp,
+                                     args.train_iters, args.train_steps, args.train_batch, args.train_lr,
+                                     bool(args.fixed_selection))
+        out["guided"]["workload"] = "the same scene and frame size, plain vs guided, no-drop sample semantics"
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, args, r.light_tree())
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]

@@ -19,6 +19,22 @@ def build():
     subprocess.run(["make", "-s", "-C", ORACLE_DIR, "-j8"], check=True)
 
 
+def use_native_build(build_dir, jobs=16):
+    """Builds the oracle with -march=native for the host this process runs
+    on into build_dir and loads that copy (for the CPU-baseline timing; the
+    tests use the portable build). Call before anything else loads the
+    oracle; returns the seconds the build took."""
+    global ORACLE_SO
+    import time
+    if _lib is not None:
+        raise RuntimeError("the oracle is already loaded")
+    t = time.perf_counter()
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR, "-j%d" % jobs, "MARCH=native", "BUILD=" + build_dir],
+                   check=True)
+    ORACLE_SO = os.path.join(build_dir, "liboracle.so")
+    return time.perf_counter() - t
+
+
 def lib():
     global _lib
     if _lib is None:

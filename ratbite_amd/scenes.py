@@ -444,6 +444,259 @@ def furnace(inner="metal", rough=0.3, width=64, height=64, spp=64, depth=8, leve
                    light_uv=np.zeros((3, 2), np.float32))
 
 
+def quat_from_frame(t, b, n):
+    """Unit quaternions (x, y, z, w) of the rotations whose matrix columns are
+    the orthonormal frames (t, b, n), rows of (k, 3) arrays (Shepperd's
+    method, the largest of the four candidates as pivot)."""
+    t, b, n = (np.asarray(a, np.float64) for a in (t, b, n))
+    m00, m10, m20 = t[:, 0], t[:, 1], t[:, 2]
+    m01, m11, m21 = b[:, 0], b[:, 1], b[:, 2]
+    m02, m12, m22 = n[:, 0], n[:, 1], n[:, 2]
+    tr = m00 + m11 + m22
+    cand = np.stack([tr, m00, m11, m22], 1)
+    k = cand.argmax(axis=1)
+    q = np.zeros((len(t), 4))
+    for case in range(4):
+        sel = k == case
+        if not sel.any():
+            continue
+        a00, a11, a22 = m00[sel], m11[sel], m22[sel]
+        a01, a10, a02, a20, a12, a21 = m01[sel], m10[sel], m02[sel], m20[sel], m12[sel], m21[sel]
+        if case == 0:
+            r = np.sqrt(1.0 + a00 + a11 + a22) * 2.0
+            q[sel] = np.stack([(a21 - a12) / r, (a02 - a20) / r, (a10 - a01) / r, 0.25 * r], 1)
+        elif case == 1:
+            r = np.sqrt(1.0 + a00 - a11 - a22) * 2.0
+            q[sel] = np.stack([0.25 * r, (a01 + a10) / r, (a02 + a20) / r, (a21 - a12) / r], 1)
+        elif case == 2:
+            r = np.sqrt(1.0 + a11 - a00 - a22) * 2.0
+            q[sel] = np.stack([(a01 + a10) / r, 0.25 * r, (a12 + a21) / r, (a02 - a20) / r], 1)
+        else:
+            r = np.sqrt(1.0 + a22 - a00 - a11) * 2.0
+            q[sel] = np.stack([(a02 + a20) / r, (a12 + a21) / r, 0.25 * r, (a10 - a01) / r], 1)
+    # w >= 0, so that the sign of w is free to carry the bitangent flip
+    q *= np.where(q[:, 3:4] < 0, -1.0, 1.0)
+    q[:, 3] = np.maximum(q[:, 3], 1e-6)
+    return q / np.linalg.norm(q, axis=1, keepdims=True)
+
+
+def sphere_frames(normals):
+    """Tangent (along increasing u of uv_sphere), bitangent n x t, normal."""
+    n = np.asarray(normals, np.float64)
+    t = np.cross(n, np.array([0.0, 1.0, 0.0]))
+    pole = np.linalg.norm(t, axis=1) < 1e-6
+    t[pole] = (1.0, 0.0, 0.0)
+    t = -t / np.linalg.norm(t, axis=1, keepdims=True)
+    return t, np.cross(n, t), n
+
+
+def _noise_tex(rng, size, cells, channels):
+    return np.stack([_value_noise_2d(rng, size, cells) for _ in range(channels)], -1)
+
+
+def features(seed=0x5EED0009, width=96, height=96, spp=4, depth=8):
+    """The Cornell box with objects that exercise every surface-attribute and
+    texture branch of the shading code (pathtrace.cu:1279-1567), which the
+    benchmark scenes leave unused:
+      - a sphere whose tangent frames are per-vertex quaternions (AT_FRAME),
+        with a normal map read through TEX_COORD_2 and a base-colour texture;
+      - the same with the bitangent flipped (negative quaternion w) and an
+        RGB32F normal map on TEX_COORD_1 with mirror addressing;
+      - a metal-roughness texture (TEX_COORD_2, nearest filtering, clamp);
+      - a transmission texture (R8, ior 1.4);
+      - sheen colour and sheen roughness textures;
+      - an RGBA8 base-colour texture with checkered alpha and an alpha
+        cutoff, and a quad with alpha blending (RGBA32F texture);
+      - a PureDielectric (bxdf 1) sphere, a wall without vertex attributes
+        (the geometric normal) and a PureConductor sphere.
+    Every vertex holds 32 bytes: a frame quaternion or an octahedral normal
+    at 0, TEX_COORD_1 at 16 and TEX_COORD_2 at 24."""
+    rng = np.random.default_rng(seed)
+    tables = load_tables()
+    spectra = [tables["blackbody4000"]]
+    mats, parts, frame_parts = [], [], set()
+    walls = [
+        ((-1, 0, -1), (-1, 0, 1), (1, 0, 1), (1, 0, -1)),
+        ((-1, 2, -1), (1, 2, -1), (1, 2, 1), (-1, 2, 1)),
+        ((-1, 0, -1), (-1, 2, -1), (1, 2, -1), (1, 0, -1)),
+        ((-1, 0, -1), (-1, 0, 1), (-1, 2, 1), (-1, 2, -1)),
+        ((1, 0, -1), (1, 2, -1), (1, 2, 1), (1, 0, 1)),
+    ]
+    for w in walls:
+        v, i, n, uv = quad(*w)
+        if np.dot(n[0], np.array([0.0, 1.0, 0.0]) - v.mean(axis=0)) < 0:
+            v, i, n, uv = quad(*w[::-1])
+        mats.append(material(base=rng.uniform(0.2, 0.9, 3), rough=float(rng.uniform(0.3, 1.0))))
+        parts.append((v, i, n, uv, len(mats) - 1))
+    no_attrib_mat = 2          # the back wall shades with its geometric normal
+    v, i, n, uv = quad((-0.3, 1.98, -0.3), (0.3, 1.98, -0.3), (0.3, 1.98, 0.3), (-0.3, 1.98, 0.3))
+    mats.append(material(base=(0.8, 0.8, 0.8), rough=1.0, spectral_emission=(0, 12.0)))
+    mats[-1]["factors"] |= FC_EMISSION
+    parts.append((v, i, n, uv, len(mats) - 1))
+
+    # textures: (array, format, srgb, addr, filter)
+    tex = []
+
+    def add_tex(a, fmt, srgb, addr=(0, 0, 0), filt=1):
+        tex.append((a, fmt, srgb, addr, filt))
+        return len(tex) - 1
+
+    def normal_map(size, amp):
+        h = _value_noise_2d(rng, size, 4)
+        gy, gx = np.gradient(h * amp * size / 4.0)
+        nm = np.stack([-gx, -gy, np.ones_like(h)], -1)
+        return nm / np.linalg.norm(nm, axis=-1, keepdims=True)
+
+    nm8 = normal_map(64, 0.6)
+    t_nm8 = add_tex(np.concatenate([np.round((nm8 * 0.5 + 0.5) * 255), np.full((64, 64, 1), 255)], -1)
+                    .astype(np.uint8), 0, 0)
+    t_nm32 = add_tex(np.ascontiguousarray(normal_map(32, 0.8), np.float32), 3, 0, addr=(2, 2, 2))
+    base = np.round(_noise_tex(rng, 64, 4, 3) * 255)
+    checker = ((np.arange(64)[:, None] // 8 + np.arange(64)[None, :] // 8) % 2).astype(np.float64)
+    alpha = np.where(checker > 0, 255, 40)[..., None]
+    t_base = add_tex(np.concatenate([base, alpha], -1).astype(np.uint8), 0, 1)
+    mr = np.zeros((32, 32, 4))
+    mr[..., 1] = np.round(_value_noise_2d(rng, 32, 4) * 255)
+    mr[..., 2] = np.where(_value_noise_2d(rng, 32, 2) > 0.5, 255, 0)
+    mr[..., 3] = 255
+    t_mr = add_tex(mr.astype(np.uint8), 0, 0, addr=(1, 1, 1), filt=0)
+    t_tr = add_tex(np.round(_value_noise_2d(rng, 32, 4) * 255).astype(np.uint8), 1, 0)
+    t_shc = add_tex(np.concatenate([np.round(_noise_tex(rng, 32, 4, 3) * 255), np.full((32, 32, 1), 255)], -1)
+                    .astype(np.uint8), 0, 1)
+    shr = np.full((32, 32, 4), 255.0)
+    shr[..., 3] = np.round((0.2 + 0.8 * _value_noise_2d(rng, 32, 4)) * 255)
+    t_shr = add_tex(shr.astype(np.uint8), 0, 0)
+    blend = np.ones((16, 16, 4), np.float32)
+    blend[..., :3] = rng.uniform(0.2, 0.9, 3)
+    blend[..., 3] = np.linspace(0.1, 0.9, 16)[None, :]
+    t_blend = add_tex(blend, 2, 0, addr=(1, 1, 1))
+
+    def sphere(c, r, m, frame=False):
+        v, i, n, uv = uv_sphere(c, r)
+        mats.append(m)
+        parts.append((v, i, n, uv, len(mats) - 1))
+        if frame:
+            frame_parts.add(len(parts) - 1)
+        return mats[-1]
+
+    m = material(base=(1, 1, 1), rough=0.35)
+    m["textures"] = TX_NORMAL | TX_BASE_COLOR
+    m["tex"][1], m["tex"][0] = t_nm8, t_base
+    m["texCoordSets"] = 1 << 1                    # the normal map (and so the base colour) read TEX_COORD_2
+    sphere((-0.55, 0.3, -0.4), 0.28, m, frame=True)
+    m = material(base=(0.9, 0.6, 0.3), metal=0.0, rough=0.25)
+    m["textures"] = TX_NORMAL
+    m["tex"][1] = t_nm32
+    sphere((0.55, 0.3, -0.4), 0.28, m, frame=True)   # flipped below
+    m = material(base=(0.8, 0.8, 0.85), metal=1.0, rough=0.8)
+    m["textures"] = TX_MET_ROUGH
+    m["tex"][2] = t_mr
+    m["texCoordSets"] = 1 << 2
+    sphere((0.0, 0.25, 0.1), 0.25, m)
+    m = material(base=(0.9, 0.95, 1.0), rough=0.15, ior=1.4, transmission=1.0)
+    m["textures"] = TX_TRANSMISSION
+    m["tex"][3] = t_tr
+    sphere((-0.5, 1.0, 0.2), 0.22, m)
+    m = material(base=(0.3, 0.2, 0.4), rough=0.6, sheen=((1.0, 1.0, 1.0), 1.0))
+    m["textures"] = TX_SHEEN_COLOR | TX_SHEEN_ROUGH
+    m["tex"][4], m["tex"][5] = t_shc, t_shr
+    sphere((0.5, 1.0, 0.2), 0.22, m)
+    sphere((0.0, 1.2, -0.5), 0.2, material(bxdf=BXDF_DIELECTRIC, base=(1, 1, 1), rough=0.1, ior=1.5))
+    eta, kk = cauchy_spectra(rng)
+    spectra += [eta, kk]
+    sphere((0.0, 0.7, 0.45), 0.15, material(bxdf=BXDF_CONDUCTOR, rough=0.3, eta_k=(1, 2)))
+    # alpha-cutoff quad (base-colour alpha 40/255 or 1 in a checker) and an
+    # alpha-blended quad in front of the back half of the box
+    m = material(base=(1, 1, 1), rough=0.7)
+    m["textures"] = TX_BASE_COLOR
+    m["tex"][0] = t_base
+    m["factors"] |= FC_CUTOFF
+    m["alphaCutoff"] = 0.5
+    mats.append(m)
+    v, i, n, uv = quad((-0.9, 0.05, -0.75), (-0.1, 0.05, -0.75), (-0.1, 0.85, -0.75), (-0.9, 0.85, -0.75))
+    parts.append((v, i, n, uv * 2.0, len(mats) - 1))
+    m = material(base=(1, 1, 1), rough=0.5)
+    m["textures"] = TX_BASE_COLOR
+    m["tex"][0] = t_blend
+    m["factors"] = (int(m["factors"]) & ~FC_BASE_COLOR & 0xFFFFFFFF) | FC_BLEND
+    mats.append(m)
+    v, i, n, uv = quad((0.1, 0.05, -0.7), (0.9, 0.05, -0.7), (0.9, 0.85, -0.7), (0.1, 0.85, -0.7))
+    parts.append((v, i, n, uv, len(mats) - 1))
+
+    cam = camera_basis((0.0, 1.0, 3.4), (0.0, 1.0, 0.0), (0.0, 1.0, 0.0))
+    cam["fov"] = math.radians(45.0)
+    sc = _finish(parts, mats, spectra, [], cam, None, width, height, depth, spp, "features", tables)
+
+    # 32-byte vertex attributes
+    stride = 32
+    nv = sum(len(p[0]) for p in parts)
+    attr = np.zeros((nv, 8), np.float32)
+    off = 0
+    for pi, pt in enumerate(parts):
+        k = len(pt[0])
+        uv1 = np.asarray(pt[3], np.float32)
+        attr[off:off + k, 4:6] = uv1
+        attr[off:off + k, 6:8] = uv1 * np.float32(3.0) + np.float32(0.25)      # TEX_COORD_2
+        if pi in frame_parts:
+            t, b, nn = sphere_frames(pt[2])
+            attr[off:off + k, 0:4] = quat_from_frame(t, b, nn)
+        else:
+            attr[off:off + k, 0:2] = oct_encode(pt[2])
+        off += k
+    # the second frame sphere stores its bitangent flipped (w < 0)
+    fparts = sorted(frame_parts)
+    off = sum(len(parts[j][0]) for j in range(fparts[1]))
+    k = len(parts[fparts[1]][0])
+    attr[off:off + k, 0:4] *= -1.0
+    sc.attribs = attr.view(np.uint8).reshape(-1).copy()
+    mt = sc.materials
+    mt["attributeStride"] = stride
+    mt["normalOffset"] = 0
+    mt["frameOffset"] = 0
+    mt["texCoord1Offset"] = 16
+    mt["texCoord2Offset"] = 24
+    mt["attributes"] = AT_NORMAL | AT_TEX1 | AT_TEX2
+    for j in fparts:
+        mt["attributes"][parts[j][4]] = AT_FRAME | AT_TEX1 | AT_TEX2
+    mt["attributes"][no_attrib_mat] = 0
+    # textures
+    descs = np.zeros(len(tex), TEXTURE_DTYPE)
+    blob, offs = [], 0
+    for ti, (a, fmt, srgb, addr, filt) in enumerate(tex):
+        raw = np.ascontiguousarray(a).view(np.uint8).reshape(-1)
+        descs[ti]["width"], descs[ti]["height"], descs[ti]["depth"] = a.shape[1], a.shape[0], 1
+        descs[ti]["format"], descs[ti]["srgb"], descs[ti]["addr"] = fmt, srgb, addr
+        descs[ti]["filter"], descs[ti]["offset"] = filt, offs
+        pad = (-len(raw)) % 256
+        blob += [raw, np.zeros(pad, np.uint8)]
+        offs += len(raw) + pad
+    sc.textures = descs
+    sc.texels = np.concatenate(blob)
+    return sc
+
+
+def batch(n=64, seed=0x5EED0003):
+    """The batched uneven workload: n seeded spheres scenes with triangle
+    counts log-uniform in [1k, 500k] (level-2 or level-3 icospheres), films
+    drawn from 64^2 .. 1024^2, emissive-triangle counts log-uniform in
+    [1, 5000], the four-way material mix, 32 spp, depth 12; every other
+    scene has a 256x128 log-normal env map."""
+    rng = np.random.default_rng(seed)
+    sides = [64, 128, 256, 512, 1024]
+    out = []
+    for i in range(n):
+        tris = float(np.exp(rng.uniform(np.log(1e3), np.log(5e5))))
+        level = 3 if tris > 2e5 else 2
+        per = 20 * 4 ** level
+        n_sph = max(1, int(round(tris / per)))
+        side = int(rng.choice(sides))
+        n_lights = int(np.exp(rng.uniform(0.0, np.log(5000.0))))
+        env = None if i % 2 else (256, 128)
+        out.append(spheres(seed=seed + i, n_spheres=n_sph, level=level, n_lights=n_lights, width=side,
+                           height=side, spp=32, depth=12, env_size=env))
+    return out
+
+
 def cauchy_spectra(rng):
     """Seeded Cauchy fits n = A + B / l^2, k = C + D / l^2 (l in um) at 1 nm."""
     lam = np.arange(360, 831, dtype=np.float64) * 1e-3

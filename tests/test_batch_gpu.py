@@ -55,6 +55,43 @@ def test_batch_with_small_pass_budget():
             r.close()
 
 
+def test_batch_of_64_full_size_scenes():
+    """The batched uneven workload at its full size (64 scenes, 1k..500k
+    triangles, films up to 1024^2, 32 spp, depth 12) in one render_batch:
+    every pixel of every scene receives exactly 32 finite samples, the
+    completion times are ordered like the passes that finished each scene,
+    and a subset of the scenes accumulates bit-identically to single renders
+    in fresh contexts."""
+    scs = scenes.batch(64)
+    rs = [Renderer(sc) for sc in scs]
+    try:
+        done = np.asarray(render_batch(rs))
+        print("64 scenes: %d triangles, %d pixels; completion ms p50 %.1f p99 %.1f" % (
+            sum(len(sc.indices) for sc in scs), sum(sc.width * sc.height for sc in scs),
+            np.percentile(done, 50), np.percentile(done, 99)), flush=True)
+        assert np.all(done > 0) and np.all(np.isfinite(done))
+        accs = [r.accum() for r in rs]
+        for sc, a in zip(scs, accs):
+            assert a.shape == (sc.height, sc.width, 4)
+            assert np.all(a[..., 3] == 32), sc.name
+            assert np.all(np.isfinite(a[..., :3])), sc.name
+            assert np.all(a[..., 1] >= 0.0)
+        # larger films finish no earlier than smaller ones of no fewer samples
+        px = np.array([sc.width * sc.height for sc in scs])
+        assert done[px.argmax()] >= done[px.argmin()]
+        subset = [int(px.argmin()), int(px.argmax()), 7, 40]
+    finally:
+        for r in rs:
+            r.close()
+    for k in subset:
+        r = Renderer(scs[k])
+        try:
+            r.render(spp=32, offset=0)
+            assert np.array_equal(r.accum(), accs[k]), k
+        finally:
+            r.close()
+
+
 def test_autotune_keeps_results(ctx):
     sc = scenes.cornell(width=48, height=48, spp=8)
     r = Renderer(sc, ctx=ctx)

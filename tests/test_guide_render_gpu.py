@@ -103,3 +103,34 @@ def test_fp8_guided_render_is_unbiased(ctx):
     mp, mg = plain.mean(axis=(0, 1)), g8.mean(axis=(0, 1))
     print("image mean plain", mp, "guided fp8", mg, flush=True)
     assert np.all(np.abs(mg - mp) / mp < 0.03)
+
+
+def test_guided_1080p_spheres_is_unbiased():
+    """The learned guide on the 1920x1080 spheres workload as configured for
+    guiding (one-sample MIS at selection 0.5, 40 trained outputs, Adam at lr
+    1e-3 on batches of 2^18 records from the first 32 spp): its image agrees
+    with the plain no-drop image of the same number of samples within Monte
+    Carlo noise, over the whole frame and over 120x120 blocks."""
+    sc = scenes.spheres(spp=16)
+    r = Renderer(sc)
+    try:
+        r.configure_guide(seed=1, lr=1e-3, capacity=1 << 22, fixed_selection=True)
+        losses = []
+        for it in range(8):
+            r.render(spp=4, offset=4 * it, guided=True, collect=True)
+            losses.append(r.train_guide(batch=1 << 18, steps=8))
+        print("losses", np.round(losses, 3), flush=True)
+        assert np.all(np.isfinite(losses))
+        plain = _img(r, 64, offset=1 << 16, no_drop=True)
+        guided = _img(r, 64, offset=1 << 16, guided=True)
+        assert np.all(np.isfinite(guided))
+        mp, mg = plain.mean(axis=(0, 1)), guided.mean(axis=(0, 1))
+        print("image mean plain", mp, "guided", mg, flush=True)
+        assert np.all(np.abs(mg - mp) / mp < 0.02)
+        bp = plain.reshape(9, 120, 16, 120, 3).mean(axis=(1, 3))
+        bg = guided.reshape(9, 120, 16, 120, 3).mean(axis=(1, 3))
+        rel = np.abs(bg - bp) / np.maximum(bp, 1e-3)
+        print("block rel diff median %.3g p90 %.3g" % (np.median(rel), np.percentile(rel, 90)), flush=True)
+        assert np.median(rel) < 0.03
+    finally:
+        r.close()

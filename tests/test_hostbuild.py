@@ -55,6 +55,21 @@ def test_product_kernels_on_cpu_match_oracle_spheres():
     assert abs(st["closestRays"] - ost["closestRays"]) <= 0.001 * ost["closestRays"]
 
 
+@pytest.mark.parametrize("flags", [0, L.RBH_RENDER_NO_DROP])
+def test_product_kernels_on_cpu_match_oracle_features(flags):
+    """Every surface-attribute and texture branch (quaternion frames, normal
+    maps, the material textures, TEX_COORD_2, alpha cutoff and blend,
+    PureDielectric, no vertex attributes): scenes.features."""
+    sc = scenes.features(width=64, height=64, spp=4)
+    lp = sc.launch_params(offset=0, count=4, flags=flags)
+    a, st = H.render(sc, lp)
+    o, ost = O.render(sc, lp)
+    frac, fin = _close_frac(a, o)
+    print("features: %.4f of pixels within 1e-4 of the oracle" % frac)
+    assert fin == 1.0 and frac >= 0.999
+    assert st["closestRays"] == ost["closestRays"] and st["shadowRays"] == ost["shadowRays"]
+
+
 def test_product_kernels_furnace():
     from test_oracle_furnace import furnace_ratios
 
@@ -76,6 +91,7 @@ cases = [
     scenes.cornell(width=32, height=32, spp=2),
     scenes.cornell(width=24, height=20, spp=2, bounce=True),
     scenes.spheres(n_spheres=8, level=2, n_lights=200, width=40, height=24, spp=2, env_size=(64, 32)),
+    scenes.features(width=32, height=32, spp=2),
 ]
 for sc in cases:
     for flags in (0, L.RBH_RENDER_NO_DROP):

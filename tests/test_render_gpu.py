@@ -55,6 +55,44 @@ def test_cornell_matches_oracle(ctx):
     assert abs(st["closestRays"] - ost["closestRays"]) / ost["closestRays"] < 0.02
 
 
+def test_cpu_runnable_config_in_full_matches_oracle(ctx):
+    """The CPU-runnable configuration at its full size: 256x256 Cornell box
+    with 8 UV spheres and the 4000 K blackbody emitter, 16 spp, depth 8."""
+    sc = scenes.cornell()
+    assert (sc.width, sc.height, sc.spp, sc.depth) == (256, 256, 16, 8)
+    g, o, st, ost = _compare(sc, ctx, 16)
+    close = np.abs(g - o) <= 1e-4 * np.maximum(np.abs(o), 1e-3)
+    frac = close.all(axis=2).mean()
+    e = relmse(g, o)
+    print("cornell 256x256x16: pixels within 1e-4: %.4f  relMSE(gpu, oracle) = %.3g" % (frac, e), flush=True)
+    assert frac >= 0.99
+    assert e < 1e-5
+
+
+@pytest.mark.parametrize("flags", [0, 16])
+def test_feature_branches_match_oracle(ctx, flags):
+    """Quaternion tangent frames (one with the bitangent flipped), normal maps
+    (RGBA8 and RGB32F, mirror addressing), base-colour / metal-roughness /
+    transmission / sheen-colour / sheen-roughness textures, TEX_COORD_2,
+    nearest filtering and clamp addressing, alpha cutoff and alpha blending,
+    a PureDielectric and a PureConductor sphere, a wall without vertex
+    attributes (pathtrace.cu:1279-1567)."""
+    sc = scenes.features(width=96, height=96, spp=8)
+    r = Renderer(sc, ctx=ctx)
+    r.render(spp=8, offset=0, flags=flags)
+    acc = r.accum()
+    oacc, ost = O.render(sc, sc.launch_params(offset=0, count=8, flags=flags),
+                         oscene=O.OracleScene(sc, tree=r.light_tree()))
+    g, o = acc[..., :3] / 8, oacc[..., :3] / 8
+    assert np.all(np.isfinite(g)) and np.all(acc[..., 3] == 8)
+    close = np.abs(g - o) <= 1e-4 * np.maximum(np.abs(o), 1e-3)
+    frac = close.all(axis=2).mean()
+    e = relmse(g, o)
+    print("features flags %d: pixels within 1e-4: %.4f  relMSE(gpu, oracle) = %.3g" % (flags, frac, e), flush=True)
+    assert frac >= 0.99
+    assert e < 1e-5
+
+
 @pytest.mark.parametrize("size", [(37, 21), (5, 3), (8, 13)])
 def test_odd_film_sizes_match_oracle(ctx, size):
     # consecutive path slots cover 8x8 pixel tiles, with partial tiles at the
