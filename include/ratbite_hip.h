@@ -214,6 +214,22 @@ int rbh_light_tree_build_host_ex(const rbh_emissive_tri* tris, const float* flux
  * it) before the kernels use it. */
 int rbh_set_light_tree(rbh_ctx ctx, const rbh_packed_node* nodes, uint32_t nnodes, const uint32_t* lightPointers,
                        const uint64_t* bitmasks, const rbh_emissive_tri* tris, uint32_t n);
+/* Emitted flux of n emissive triangles of one material, as the glTF loader
+ * computes it when it cuts emitters out of a mesh (scene.cpp:485-632,
+ * sw_rast.h): mean(emissiveFactor) x the mean (r+g+b)a/3 of the RGBA8
+ * emission texels whose centres the triangle's UV footprint covers (the
+ * texels under its corners when it covers none) x area x pi. verts: n x 9
+ * floats (model-space corners), uvs: n x 6 (emission texture coordinates),
+ * rgba8: texW x texH texels or NULL for an untextured emitter, addrX/addrY:
+ * 0 wrap, 1 clamp, 2 mirror (the loader's mirror rule). Host only. */
+int rbh_emissive_flux(const float* verts, const float* uvs, uint32_t n, const float* emissiveFactor,
+                      const uint8_t* rgba8, uint32_t texW, uint32_t texH, uint32_t addrX, uint32_t addrY,
+                      float* flux);
+/* The texel set of that rasterisation for one UV-space triangle (corners in
+ * texel units): up to capacity (x, y) pairs, *count = the full count
+ * (RBH_ERR_RANGE when it exceeds capacity). Host only. */
+int rbh_rasterise_uv_triangle(const float* a, const float* b, const float* c, int* xy, uint32_t capacity,
+                              uint32_t* count);
 /* Read the built tree back: nodes (<= 2n), lightPointers (n), bitmasks (n). */
 int rbh_get_light_tree(rbh_ctx ctx, rbh_packed_node* nodes, uint32_t* nnodes, uint32_t* lightPointers,
                        uint64_t* bitmasks);

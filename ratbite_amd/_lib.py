@@ -128,6 +128,9 @@ _SIGS = {
     "rbh_build_light_tree_ex": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_uint32]),
     "rbh_light_tree_build_host": (c_int, [c_void_p, c_void_p, c_uint32, c_void_p, c_void_p, c_void_p]),
     "rbh_light_tree_build_host_ex": (c_int, [c_void_p, c_void_p, c_uint32, c_void_p, c_void_p, c_void_p, c_uint32]),
+    "rbh_emissive_flux": (c_int, [c_void_p, c_void_p, c_uint32, c_void_p, c_void_p, c_uint32, c_uint32, c_uint32,
+                                  c_uint32, c_void_p]),
+    "rbh_rasterise_uv_triangle": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_void_p]),
     "rbh_get_light_tree": (c_int, [c_void_p, c_void_p, ctypes.POINTER(c_uint32), c_void_p, c_void_p]),
     "rbh_envmap_build": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_uint32]),
     "rbh_envmap_rebuild": (c_int, [c_void_p, c_uint32, c_int]),

@@ -11,7 +11,7 @@ from published definitions, not copied:
     spirit of Mallett and Yuksel, "Spectral Primary Decomposition for
     Rendering with sRGB Reflectance", EGSR 2019: the smoothest basis whose
     reflectance under a 6504 K Planckian illuminant reproduces the linear
-    sRGB primaries (solved as an equality-constrained least-squares problem).
+    sRGB primaries (solved as a bounded linear least-squares problem).
   * sRGB XYZ->RGB matrix built from the primaries and the D65 white point
     exactly as src/core/color.h:60-99 builds it.
   * Microfacet albedo LUTs (32 x 32 and 32^3, texel centres at (i+0.5)/32,
@@ -73,46 +73,65 @@ def xyz_to_srgb_matrix(white_xy=(0.31271, 0.32902)):
     return np.linalg.inv(rgb @ np.diag(c))  # XYZ -> linear sRGB
 
 
-def rgb_basis(cmf):
+def white_illuminant(cmf):
+    """A smooth illuminant at 1 nm whose XYZ under `cmf` maps to linear sRGB
+    (1, 1, 1): the 6504 K Planckian times 1 + c1 x + c2 x^2, x = (lambda -
+    555 nm) / 100 nm, with c1, c2 and the scale solved from the three
+    conditions."""
+    P = planck(LAMBDA, 6504.0)
+    x = (LAMBDA - 555.0) / 100.0
+    M = xyz_to_srgb_matrix()
+    cols = np.stack([M @ (cmf * P * x ** p).sum(axis=1) for p in range(3)], axis=1)   # 3 x 3
+    # k (1, 1, 1) = cols @ (1, c1, c2)  ->  cols[:, 1:] @ (c1, c2) - k 1 = -cols[:, 0]
+    sys_ = np.concatenate([cols[:, 1:], -np.ones((3, 1))], axis=1)
+    c1, c2, k = np.linalg.solve(sys_, -cols[:, 0])
+    return P * (1.0 + c1 * x + c2 * x * x) / k
+
+
+def rgb_basis(cmf, weight=1e4):
     """Smoothest non-negative partition of unity b_r + b_g + b_b = 1 whose
-    reflectance under a 6504 K Planckian maps to the sRGB primaries."""
-    from scipy.optimize import minimize
-    lam5 = np.arange(380, 781, 5, dtype=np.float64)
+    reflectances render as the sRGB primaries under a D65-white illuminant
+    (white_illuminant: the 6504 K Planckian tilted by a quadratic in
+    wavelength until it is exactly the sRGB white under these colour-matching
+    functions; the Planckian alone is 4% off white, below the daylight locus).
+
+    Solved on a 5 nm grid over the whole 360..830 nm range the renderer
+    samples (so that the 1 nm resampling keeps the conditions to ~1e-3) as
+    one bounded linear least-squares
+    problem (scipy's bounded-variable least squares): the rows are the first
+    differences of each spectrum (smoothness) and, scaled by `weight`, the
+    3 x 3 primary conditions and the partition of unity at every sample, with
+    every value bounded to [0, 1]."""
+    from scipy.optimize import lsq_linear
+    lam5 = np.arange(360, 831, 5, dtype=np.float64)
     n = len(lam5)
     cmf5 = np.stack([np.interp(lam5, LAMBDA, c) for c in cmf])
-    illum = planck(lam5, 6504.0)
-    illum = illum / np.sum(illum * cmf5[1])
+    illum = np.interp(lam5, LAMBDA, white_illuminant(cmf))
     M = xyz_to_srgb_matrix()
-    A = M @ (cmf5 * illum)                  # 3 x n: basis spectrum -> linear sRGB
-    # unknowns: b (3 x n) flattened; constraints: A b_i = e_i, sum_i b_i = 1
-    def obj(v):
-        b = v.reshape(3, n)
-        d = np.diff(b, axis=1)
-        return np.sum(d * d)
-    def jac(v):
-        b = v.reshape(3, n)
-        d = np.diff(b, axis=1)
-        g = np.zeros_like(b)
-        g[:, 1:] += 2 * d
-        g[:, :-1] -= 2 * d
-        return g.ravel()
-    cons = []
+    A = M @ (cmf5 * illum) * (len(LAMBDA) - 1) / (len(lam5) - 1)   # 3 x n: spectrum -> linear sRGB
+    rows, rhs = [], []
+    D = np.diff(np.eye(n), axis=0)          # (n-1) x n first differences
     for i in range(3):
+        blk = np.zeros((n - 1, 3 * n))
+        blk[:, i * n:(i + 1) * n] = D
+        rows.append(blk)
+        rhs.append(np.zeros(n - 1))
+    for i in range(3):                      # basis i -> primary e_i
         for j in range(3):
-            row = np.zeros((3, n))
-            row[i] = A[j]
-            cons.append({"type": "eq", "fun": (lambda v, r=row.ravel(), t=float(i == j): r @ v - t),
-                         "jac": (lambda v, r=row.ravel(): r)})
-    S = np.zeros((n, 3 * n))
-    for k in range(n):
-        S[k, k] = S[k, n + k] = S[k, 2 * n + k] = 1.0
-    cons.append({"type": "eq", "fun": lambda v: S @ v - 1.0, "jac": lambda v: S})
-    x0 = np.full(3 * n, 1.0 / 3.0)
-    res = minimize(obj, x0, jac=jac, constraints=cons, bounds=[(0.0, 1.0)] * (3 * n), method="SLSQP",
-                   options={"maxiter": 500, "ftol": 1e-12})
+            r = np.zeros((1, 3 * n))
+            r[0, i * n:(i + 1) * n] = A[j]
+            rows.append(weight * r)
+            rhs.append(np.array([weight * float(i == j)]))
+    S = np.hstack([np.eye(n)] * 3)          # partition of unity
+    rows.append(weight * S)
+    rhs.append(weight * np.ones(n))
+    res = lsq_linear(np.vstack(rows), np.concatenate(rhs), bounds=(0.0, 1.0), method="bvls", tol=1e-14,
+                     max_iter=10000)
+    if not res.success:
+        raise RuntimeError("RGB basis solve failed: %s" % res.message)
     b5 = res.x.reshape(3, n)
-    # piecewise-linear resampling at 1 nm with constant ends, as
-    # src/core/spectral.cpp:1282-1286 and 1344-1380 do
+    # piecewise-linear resampling at 1 nm (src/core/spectral.cpp:1282-1286
+    # and 1344-1380 resample the reference's basis the same way)
     return np.stack([np.interp(LAMBDA, lam5, b5[i]) for i in range(3)])
 
 
