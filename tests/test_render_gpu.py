@@ -93,6 +93,23 @@ def test_feature_branches_match_oracle(ctx, flags):
     assert e < 1e-5
 
 
+def test_gltf_scene_with_hdr_env_matches_oracle(ctx, tmp_path):
+    """A glTF 2.0 file (node hierarchy, frames, normal maps, textures,
+    textured emitters) and a Radiance .hdr env map through the loaders."""
+    import gltf_fixture as F
+    from ratbite_amd import gltf, hdr
+    from ratbite_amd.envmaps import lognormal_envmap
+    F.write_gltf(str(tmp_path / "scene.gltf"))
+    hdr.write_hdr(str(tmp_path / "sky.hdr"), lognormal_envmap(128, 64, sigma=1.0, seed=3, sun=False))
+    sc = gltf.scene(str(tmp_path / "scene.gltf"), env=str(tmp_path / "sky.hdr"), width=80, height=64, spp=8, depth=6)
+    g, o, st, ost = _compare(sc, ctx, 8)
+    close = np.abs(g - o) <= 1e-4 * np.maximum(np.abs(o), 1e-3)
+    frac = close.all(axis=2).mean()
+    print("glTF + HDR: pixels within 1e-4: %.4f  relMSE %.3g" % (frac, relmse(g, o)), flush=True)
+    assert np.all(np.isfinite(g)) and frac >= 0.99
+    assert relmse(g, o) < 1e-5
+
+
 @pytest.mark.parametrize("size", [(37, 21), (5, 3), (8, 13)])
 def test_odd_film_sizes_match_oracle(ctx, size):
     # consecutive path slots cover 8x8 pixel tiles, with partial tiles at the
