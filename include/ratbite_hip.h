@@ -183,6 +183,26 @@ int rbh_upload_geometry_ex(rbh_ctx ctx, const float* verts, uint32_t nverts, con
                            const uint32_t* triLight);
 /* optixAccelBuild GAS/IAS (rendering_interface.cpp:733-1064): builds the LBVH */
 int rbh_build_accel(rbh_ctx ctx);
+/* ------------------------------------------------------ partial updates
+ * The interactive loop's partial rebuilds (rendering_interface.cpp:1534-1845:
+ * CHANGE_MODEL_TRANSFORM / CHANGE_CAMERA_POS rebuild the instance structure,
+ * refill the light records and shift the light-tree spatial data;
+ * CHANGE_MODEL_MATERIAL rewrites material records). With one flattened mesh
+ * the instance rebuild becomes a refit of the LBVH. */
+/* Rewrites count vertex positions from `first` (float3 each). Rendering is
+ * refused until rbh_refit_accel or rbh_build_accel has run. */
+int rbh_update_vertices(rbh_ctx ctx, uint32_t first, uint32_t count, const float* verts);
+/* Recomputes every box, triangle record and four-wide node for the current
+ * vertices over the topology of the last rbh_build_accel (same triangles);
+ * the traversal stays exact, its cost grows with the drift. */
+int rbh_refit_accel(rbh_ctx ctx);
+/* Camera-relative coordinates after a camera move by delta (world units):
+ * subtracts delta from every vertex, emitter record corner and light-tree
+ * spatial mean, then refits (CHANGE_CAMERA_POS). */
+int rbh_translate_scene(rbh_ctx ctx, const float* delta);
+/* Rewrites count material records from `first`, with rbh_upload_materials'
+ * checks (CHANGE_MODEL_MATERIAL). */
+int rbh_update_materials(rbh_ctx ctx, uint32_t first, uint32_t count, const rbh_material* mats);
 
 /* ------------------------------------------------------------ light tree */
 /* RenderingInterface::buildLightTree + LightTree::Builder::build

@@ -131,6 +131,10 @@ _SIGS = {
     "rbh_emissive_flux": (c_int, [c_void_p, c_void_p, c_uint32, c_void_p, c_void_p, c_uint32, c_uint32, c_uint32,
                                   c_uint32, c_void_p]),
     "rbh_rasterise_uv_triangle": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_void_p]),
+    "rbh_update_vertices": (c_int, [c_void_p, c_uint32, c_uint32, c_void_p]),
+    "rbh_refit_accel": (c_int, [c_void_p]),
+    "rbh_translate_scene": (c_int, [c_void_p, c_void_p]),
+    "rbh_update_materials": (c_int, [c_void_p, c_uint32, c_uint32, c_void_p]),
     "rbh_get_light_tree": (c_int, [c_void_p, c_void_p, ctypes.POINTER(c_uint32), c_void_p, c_void_p]),
     "rbh_envmap_build": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_uint32]),
     "rbh_envmap_rebuild": (c_int, [c_void_p, c_uint32, c_int]),

@@ -100,6 +100,21 @@ class Context:
     def build_accel(self):
         self.call("rbh_build_accel")
 
+    def update_vertices(self, first, verts):
+        verts = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
+        self.call("rbh_update_vertices", int(first), len(verts), L.ptr(verts))
+
+    def refit_accel(self):
+        self.call("rbh_refit_accel")
+
+    def translate_scene(self, delta):
+        d = np.ascontiguousarray(delta, np.float32).reshape(3)
+        self.call("rbh_translate_scene", L.ptr(d))
+
+    def update_materials(self, first, mats):
+        mats = np.ascontiguousarray(mats, L.MATERIAL_DTYPE)
+        self.call("rbh_update_materials", int(first), len(mats), L.ptr(mats))
+
     def build_light_tree(self, tris, flux, fp64=False):
         tris = np.ascontiguousarray(tris, L.EMISSIVE_TRI_DTYPE)
         flux = np.ascontiguousarray(flux, np.float32)

@@ -75,6 +75,8 @@ struct Accel {
     DevBuf scratch;   // build scratch
     size_t keysOffset = 0;   // byte offset of the sorted 64-bit keys in scratch (kept after the build)
     int keyIndexBits = 32;   // low key bits holding the triangle index
+    uint32_t ntris = 0;      // triangles the structure was built over
+    bool stale = false;      // vertices changed since the last build or refit
 };
 
 // HIP-event timing of individual render-loop kernels (RBH_RENDER_TIMING):

@@ -386,100 +386,91 @@ inline size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
 }  // namespace
 
-int lbvh_build(rbh_ctx c) {
+// p[i * stride + a] -= delta[a] for npoints points (stride 3: packed float3;
+// stride 0: the 9 corner floats of 68-byte emissive records, 3 points each)
+__global__ __launch_bounds__(kBuildBlock) void k_translate(float* __restrict__ p, size_t npoints, uint32_t stride,
+                                                           float dx, float dy, float dz) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npoints) return;
+    float* q = stride ? p + i * stride : p + (i / 3) * 17 + (i % 3) * 3;
+    q[0] -= dx;
+    q[1] -= dy;
+    q[2] -= dz;
+}
+
+// Scratch layout of a build over n triangles (kept after the build: the
+// sorted keys, the binary topology and the depth order are what a refit
+// reuses).
+struct BuildLayout {
+    size_t oBounds, oKeys, oKeys2, oChild, oParent, oIbox, oLbox, oDepth, oOrder, oLevels, oListA, oListB, oCounts,
+        oSort, sortBytes, total;
+};
+
+hipError_t build_layout(rbh_ctx c, int n, BuildLayout& L) {
+    const size_t nInt = n > 1 ? (size_t)n - 1 : 1;
+    size_t off = 0;
+    L.oBounds = off; off = align_up(off + 64);
+    L.oKeys = off; off = align_up(off + (size_t)n * 8);
+    L.oKeys2 = off; off = align_up(off + (size_t)n * 8);
+    L.oChild = off; off = align_up(off + nInt * 8);
+    L.oParent = off; off = align_up(off + ((size_t)n + nInt) * 4);
+    L.oIbox = off; off = align_up(off + nInt * 24);
+    L.oLbox = off; off = align_up(off + (size_t)n * 24);
+    L.oDepth = off; off = align_up(off + nInt * 4);
+    L.oOrder = off; off = align_up(off + nInt * 4);
+    L.oLevels = off; off = align_up(off + 3 * (kLevels + 1) * 4);
+    L.oListA = off; off = align_up(off + nInt * 8);
+    L.oListB = off; off = align_up(off + nInt * 8);
+    L.oCounts = off; off = align_up(off + 16);
+    size_t sortBytes = 0, levelSortBytes = 0;
+    hipError_t e = hipcub::DeviceRadixSort::SortKeys(nullptr, sortBytes, (unsigned long long*)nullptr,
+                                                     (unsigned long long*)nullptr, n, 0, 64, c->stream);
+    if (e != hipSuccess) return e;
+    e = hipcub::DeviceRadixSort::SortPairs(nullptr, levelSortBytes, (unsigned int*)nullptr, (unsigned int*)nullptr,
+                                           (int*)nullptr, (int*)nullptr, (int)nInt, 0, 7, c->stream);
+    if (e != hipSuccess) return e;
+    L.sortBytes = std::max(sortBytes, levelSortBytes);
+    L.oSort = off; off = align_up(off + L.sortBytes);
+    L.total = off;
+    return hipSuccess;
+}
+
+int cu_count_of(int device) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0) cus = 256;
+    return cus;
+}
+
+// The build's last stages, which a refit repeats over the kept topology:
+// leaf boxes and leaf-order triangle records from the current vertices,
+// internal boxes level by level, the two-child nodes and the four-wide
+// collapse.
+int build_boxes_and_nodes(rbh_ctx c, const BuildLayout& L) {
     Geometry& g = c->geom;
     Accel& acc = c->accel;
     const int n = (int)g.ntris;
-    if (n <= 0) return fail(c, RBH_ERR_STATE, "no geometry uploaded");
     const size_t nInt = n > 1 ? (size_t)n - 1 : 1;
-    // scratch layout
-    size_t off = 0;
-    const size_t oBounds = off; off = align_up(off + 64);
-    const size_t oKeys = off; off = align_up(off + (size_t)n * 8);
-    const size_t oKeys2 = off; off = align_up(off + (size_t)n * 8);
-    const size_t oChild = off; off = align_up(off + nInt * 8);
-    const size_t oParent = off; off = align_up(off + ((size_t)n + nInt) * 4);
-    const size_t oIbox = off; off = align_up(off + nInt * 24);
-    const size_t oLbox = off; off = align_up(off + (size_t)n * 24);
-    const size_t oDepth = off; off = align_up(off + nInt * 4);
-    const size_t oOrder = off; off = align_up(off + nInt * 4);
-    const size_t oLevels = off; off = align_up(off + 3 * (kLevels + 1) * 4);
-    const size_t oListA = off; off = align_up(off + nInt * 8);
-    const size_t oListB = off; off = align_up(off + nInt * 8);
-    const size_t oCounts = off; off = align_up(off + 16);
-    size_t sortBytes = 0, levelSortBytes = 0;
-    RBH_HIP(c, hipcub::DeviceRadixSort::SortKeys(nullptr, sortBytes, (unsigned long long*)nullptr,
-                                                 (unsigned long long*)nullptr, n, 0, 64, c->stream));
-    RBH_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, levelSortBytes, (unsigned int*)nullptr, (unsigned int*)nullptr,
-                                                  (int*)nullptr, (int*)nullptr, (int)nInt, 0, 7, c->stream));
-    sortBytes = std::max(sortBytes, levelSortBytes);
-    const size_t oSort = off; off = align_up(off + sortBytes);
-    RBH_ALLOC(c, acc.scratch, off);
-    RBH_ALLOC(c, acc.nodes, nInt * sizeof(BvhNode));
-    RBH_ALLOC(c, acc.tris, (size_t)n * sizeof(TriRecord));
     char* s = acc.scratch.as<char>();
-    acc.keysOffset = oKeys2;
-    uint32_t* bounds = reinterpret_cast<uint32_t*>(s + oBounds);
-    auto* keys = reinterpret_cast<unsigned long long*>(s + oKeys);
-    auto* keys2 = reinterpret_cast<unsigned long long*>(s + oKeys2);
-    int2* child = reinterpret_cast<int2*>(s + oChild);
-    int* parent = reinterpret_cast<int*>(s + oParent);
-    float* ibox = reinterpret_cast<float*>(s + oIbox);
-    float* lbox = reinterpret_cast<float*>(s + oLbox);
-    int* ndepth = reinterpret_cast<int*>(s + oDepth);
-    int* order = reinterpret_cast<int*>(s + oOrder);
-    int* lstart = reinterpret_cast<int*>(s + oLevels);
-
-    const float* v = g.verts.as<float>();
-    const uint32_t* idx = g.indices.as<uint32_t>();
+    auto* keys2 = reinterpret_cast<unsigned long long*>(s + L.oKeys2);
+    int2* child = reinterpret_cast<int2*>(s + L.oChild);
+    float* ibox = reinterpret_cast<float*>(s + L.oIbox);
+    float* lbox = reinterpret_cast<float*>(s + L.oLbox);
+    int* order = reinterpret_cast<int*>(s + L.oOrder);
+    int* lstart = reinterpret_cast<int*>(s + L.oLevels);
     const int nb = (n + kBuildBlock - 1) / kBuildBlock;
-    RBH_HIP(c, hipEventRecord(c->ev0, c->stream));
-    hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, c->stream, bounds);
-    hipLaunchKernelGGL(k_centroid_bounds, dim3(std::min(nb, 2048)), dim3(kBuildBlock), 0, c->stream, v, idx, (uint32_t)n, bounds);
-    // key bits: the triangle index below, the Morton code above
-    int idxBits = 1;
-    while (idxBits < 32 && ((uint64_t)1 << idxBits) < (uint64_t)n) ++idxBits;
-    const int axisBits = std::min(21, (64 - idxBits) / 3);
-    acc.keyIndexBits = idxBits;
-    hipLaunchKernelGGL(k_morton, dim3(nb), dim3(kBuildBlock), 0, c->stream, v, idx, (uint32_t)n, bounds, axisBits,
-                       idxBits, keys);
-    RBH_HIP(c, hipcub::DeviceRadixSort::SortKeys(s + oSort, sortBytes, keys, keys2, n, 0, 64, c->stream));
     const int nbi = (int)((nInt + kBuildBlock - 1) / kBuildBlock);
-    if (n > 1) hipLaunchKernelGGL(k_karras, dim3(nbi), dim3(kBuildBlock), 0, c->stream, keys2, n, child, parent);
-    hipLaunchKernelGGL(k_leaves, dim3(nb), dim3(kBuildBlock), 0, c->stream, v, idx, g.triMaterial.as<uint32_t>(),
-                       g.triLight.as<uint32_t>(), keys2, n, idxBits, lbox, acc.tris.as<TriRecord>());
+    const int cus = cu_count_of(c->device);
+    hipLaunchKernelGGL(k_leaves, dim3(nb), dim3(kBuildBlock), 0, c->stream, g.verts.as<float>(), g.indices.as<uint32_t>(),
+                       g.triMaterial.as<uint32_t>(), g.triLight.as<uint32_t>(), keys2, n, acc.keyIndexBits, lbox,
+                       acc.tris.as<TriRecord>());
     if (n > 1) {
-        int cus = 0;
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0) cus = 256;
-        // internal nodes in depth order (list buffers are free until the collapse)
-        int2* ja = reinterpret_cast<int2*>(s + oListA);
-        int2* jb = reinterpret_cast<int2*>(s + oListB);
-        hipLaunchKernelGGL(k_jump_init, dim3(nbi), dim3(kBuildBlock), 0, c->stream, (int)nInt, parent, ja);
-        for (int r = 0; r < kJumpRounds; ++r) {
-            hipLaunchKernelGGL(k_jump, dim3(nbi), dim3(kBuildBlock), 0, c->stream, (int)nInt, ja, jb);
-            std::swap(ja, jb);
-        }
-        unsigned int* dkey = reinterpret_cast<unsigned int*>(ndepth);
-        int* ids = reinterpret_cast<int*>(jb);                       // free after the last round
-        unsigned int* dsorted = reinterpret_cast<unsigned int*>(ids + nInt);
-        hipLaunchKernelGGL(k_depth_keys, dim3(nbi), dim3(kBuildBlock), 0, c->stream, (int)nInt, ja, dkey, ids);
-        RBH_HIP(c, hipcub::DeviceRadixSort::SortPairs(s + oSort, sortBytes, dkey, dsorted, ids, order, (int)nInt, 0, 7,
-                                                      c->stream));
-        hipLaunchKernelGGL(k_level_bounds, dim3(1), dim3(128), 0, c->stream, dsorted, (int)nInt, lstart);
         const int lg = std::min(nbi, cus * 8);
         for (int d = kLevels - 1; d >= 0; --d)
             hipLaunchKernelGGL(k_refit_level, dim3(lg), dim3(kBuildBlock), 0, c->stream, d, lstart, order, child, ibox, lbox);
-    }
-    hipLaunchKernelGGL(k_depth, dim3(nb), dim3(kBuildBlock), 0, c->stream, n, parent, bounds + 6);
-    if (n > 1) hipLaunchKernelGGL(k_emit, dim3(nbi), dim3(kBuildBlock), 0, c->stream, n, child, ibox, lbox, acc.nodes.as<BvhNode>());
-    RBH_ALLOC(c, acc.nodes4, nInt * sizeof(Bvh4Node));
-    RBH_ALLOC(c, acc.nodes4q, nInt * sizeof(Bvh4QNode));
-    if (n > 1) {
-        int2* la = reinterpret_cast<int2*>(s + oListA);
-        int2* lb = reinterpret_cast<int2*>(s + oListB);
-        unsigned int* counts = reinterpret_cast<unsigned int*>(s + oCounts);
-        int cus = 0;
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0) cus = 256;
+        hipLaunchKernelGGL(k_emit, dim3(nbi), dim3(kBuildBlock), 0, c->stream, n, child, ibox, lbox, acc.nodes.as<BvhNode>());
+        int2* la = reinterpret_cast<int2*>(s + L.oListA);
+        int2* lb = reinterpret_cast<int2*>(s + L.oListB);
+        unsigned int* counts = reinterpret_cast<unsigned int*>(s + L.oCounts);
         hipLaunchKernelGGL(k_collapse_init, dim3(1), dim3(64), 0, c->stream, la, counts);
         // each level consumes one list and fills the other; every level
         // descends at least one binary level, so 64 levels drain the tree
@@ -494,24 +485,133 @@ int lbvh_build(rbh_ctx c) {
         }
     }
     RBH_HIP(c, hipGetLastError());
+    return RBH_OK;
+}
+
+// Synchronises, times the stages since ev0 and checks the collapse.
+int finish_build(rbh_ctx c, const BuildLayout& L, float* ms) {
+    Accel& acc = c->accel;
+    const int n = (int)c->geom.ntris;
+    const size_t nInt = n > 1 ? (size_t)n - 1 : 1;
+    char* s = acc.scratch.as<char>();
     RBH_HIP(c, hipEventRecord(c->ev1, c->stream));
     RBH_HIP(c, hipStreamSynchronize(c->stream));
-    float ms = 0.0f;
-    RBH_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    uint32_t depth = 0;
-    RBH_HIP(c, hipMemcpy(&depth, bounds + 6, 4, hipMemcpyDeviceToHost));
+    RBH_HIP(c, hipEventElapsedTime(ms, c->ev0, c->ev1));
     // counts: pending entries of both level lists, four-wide nodes allocated
     uint32_t cnt4[3] = {0u, 0u, 0u};
-    if (n > 1) RBH_HIP(c, hipMemcpy(cnt4, s + oCounts, 12, hipMemcpyDeviceToHost));
+    if (n > 1) RBH_HIP(c, hipMemcpy(cnt4, s + L.oCounts, 12, hipMemcpyDeviceToHost));
     if (n > 1 && (cnt4[0] != 0u || cnt4[2] > nInt))
         return fail(c, RBH_ERR_RANGE, "four-wide collapse left %u nodes unwritten (%u allocated)", cnt4[0], cnt4[2]);
     acc.nnodes4 = cnt4[2];
+    return RBH_OK;
+}
+
+int lbvh_build(rbh_ctx c) {
+    Geometry& g = c->geom;
+    Accel& acc = c->accel;
+    const int n = (int)g.ntris;
+    if (n <= 0) return fail(c, RBH_ERR_STATE, "no geometry uploaded");
+    const size_t nInt = n > 1 ? (size_t)n - 1 : 1;
+    BuildLayout L;
+    RBH_HIP(c, build_layout(c, n, L));
+    acc.built = false;
+    RBH_ALLOC(c, acc.scratch, L.total);
+    RBH_ALLOC(c, acc.nodes, nInt * sizeof(BvhNode));
+    RBH_ALLOC(c, acc.tris, (size_t)n * sizeof(TriRecord));
+    RBH_ALLOC(c, acc.nodes4, nInt * sizeof(Bvh4Node));
+    RBH_ALLOC(c, acc.nodes4q, nInt * sizeof(Bvh4QNode));
+    char* s = acc.scratch.as<char>();
+    acc.keysOffset = L.oKeys2;
+    uint32_t* bounds = reinterpret_cast<uint32_t*>(s + L.oBounds);
+    auto* keys = reinterpret_cast<unsigned long long*>(s + L.oKeys);
+    auto* keys2 = reinterpret_cast<unsigned long long*>(s + L.oKeys2);
+    int2* child = reinterpret_cast<int2*>(s + L.oChild);
+    int* parent = reinterpret_cast<int*>(s + L.oParent);
+    int* ndepth = reinterpret_cast<int*>(s + L.oDepth);
+    int* order = reinterpret_cast<int*>(s + L.oOrder);
+    int* lstart = reinterpret_cast<int*>(s + L.oLevels);
+
+    const float* v = g.verts.as<float>();
+    const uint32_t* idx = g.indices.as<uint32_t>();
+    const int nb = (n + kBuildBlock - 1) / kBuildBlock;
+    RBH_HIP(c, hipEventRecord(c->ev0, c->stream));
+    hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, c->stream, bounds);
+    hipLaunchKernelGGL(k_centroid_bounds, dim3(std::min(nb, 2048)), dim3(kBuildBlock), 0, c->stream, v, idx, (uint32_t)n, bounds);
+    // key bits: the triangle index below, the Morton code above
+    int idxBits = 1;
+    while (idxBits < 32 && ((uint64_t)1 << idxBits) < (uint64_t)n) ++idxBits;
+    const int axisBits = std::min(21, (64 - idxBits) / 3);
+    acc.keyIndexBits = idxBits;
+    hipLaunchKernelGGL(k_morton, dim3(nb), dim3(kBuildBlock), 0, c->stream, v, idx, (uint32_t)n, bounds, axisBits,
+                       idxBits, keys);
+    RBH_HIP(c, hipcub::DeviceRadixSort::SortKeys(s + L.oSort, L.sortBytes, keys, keys2, n, 0, 64, c->stream));
+    const int nbi = (int)((nInt + kBuildBlock - 1) / kBuildBlock);
+    if (n > 1) {
+        hipLaunchKernelGGL(k_karras, dim3(nbi), dim3(kBuildBlock), 0, c->stream, keys2, n, child, parent);
+        // internal nodes in depth order (list buffers are free until the collapse)
+        int2* ja = reinterpret_cast<int2*>(s + L.oListA);
+        int2* jb = reinterpret_cast<int2*>(s + L.oListB);
+        hipLaunchKernelGGL(k_jump_init, dim3(nbi), dim3(kBuildBlock), 0, c->stream, (int)nInt, parent, ja);
+        for (int r = 0; r < kJumpRounds; ++r) {
+            hipLaunchKernelGGL(k_jump, dim3(nbi), dim3(kBuildBlock), 0, c->stream, (int)nInt, ja, jb);
+            std::swap(ja, jb);
+        }
+        unsigned int* dkey = reinterpret_cast<unsigned int*>(ndepth);
+        int* ids = reinterpret_cast<int*>(jb);                       // free after the last round
+        unsigned int* dsorted = reinterpret_cast<unsigned int*>(ids + nInt);
+        hipLaunchKernelGGL(k_depth_keys, dim3(nbi), dim3(kBuildBlock), 0, c->stream, (int)nInt, ja, dkey, ids);
+        RBH_HIP(c, hipcub::DeviceRadixSort::SortPairs(s + L.oSort, L.sortBytes, dkey, dsorted, ids, order, (int)nInt, 0,
+                                                      7, c->stream));
+        hipLaunchKernelGGL(k_level_bounds, dim3(1), dim3(128), 0, c->stream, dsorted, (int)nInt, lstart);
+    }
+    hipLaunchKernelGGL(k_depth, dim3(nb), dim3(kBuildBlock), 0, c->stream, n, parent, bounds + 6);
+    const int rc = build_boxes_and_nodes(c, L);
+    if (rc) return rc;
+    float ms = 0.0f;
+    const int fr = finish_build(c, L, &ms);
+    if (fr) return fr;
+    uint32_t depth = 0;
+    RBH_HIP(c, hipMemcpy(&depth, bounds + 6, 4, hipMemcpyDeviceToHost));
     c->stats.bvhMs = ms;
     c->stats.bvhDepth = depth;
     acc.nnodes = n > 1 ? (uint32_t)(n - 1) : 0u;
     acc.depth = depth;
+    acc.ntris = (uint32_t)n;
     if (depth > 64) return fail(c, RBH_ERR_RANGE, "LBVH depth %u exceeds the 64-entry traversal stack", depth);
     acc.built = true;
+    acc.stale = false;
+    return RBH_OK;
+}
+
+hipError_t translate_points(rbh_ctx c, float* p, size_t npoints, uint32_t strideFloats, const float* delta) {
+    if (!npoints) return hipSuccess;
+    const size_t blocks = (npoints + kBuildBlock - 1) / kBuildBlock;
+    hipLaunchKernelGGL(k_translate, dim3((unsigned)blocks), dim3(kBuildBlock), 0, c->stream, p, npoints, strideFloats,
+                       delta[0], delta[1], delta[2]);
+    return hipGetLastError();
+}
+
+// Refit: new vertex positions under the kept topology (the triangles' leaf
+// order, the binary hierarchy and its depth order stay; every box, the
+// triangle records and the four-wide nodes are recomputed). The traversal
+// stays exact for any positions; its cost grows as the positions drift from
+// the ones the hierarchy was built for.
+int lbvh_refit(rbh_ctx c) {
+    Accel& acc = c->accel;
+    const int n = (int)c->geom.ntris;
+    if (!acc.built || acc.ntris != (uint32_t)n || n <= 0)
+        return fail(c, RBH_ERR_STATE, "refit needs an acceleration structure built over the same triangles");
+    BuildLayout L;
+    RBH_HIP(c, build_layout(c, n, L));
+    if (acc.scratch.bytes < L.total) return fail(c, RBH_ERR_STATE, "build scratch was released");
+    RBH_HIP(c, hipEventRecord(c->ev0, c->stream));
+    const int rc = build_boxes_and_nodes(c, L);
+    if (rc) return rc;
+    float ms = 0.0f;
+    const int fr = finish_build(c, L, &ms);
+    if (fr) return fr;
+    c->stats.bvhMs = ms;
+    acc.stale = false;
     return RBH_OK;
 }
 

@@ -1,6 +1,7 @@
 // C ABI entry points of the acceleration structure, light tree and render
 // loop (include/ratbite_hip.h).
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -10,6 +11,8 @@
 
 namespace rbh {
 int lbvh_build(rbh_ctx c);
+int lbvh_refit(rbh_ctx c);
+hipError_t translate_points(rbh_ctx c, float* p, size_t npoints, uint32_t strideFloats, const float* delta);
 hipError_t launch_pass(rbh_ctx c, const Frame1& f, uint32_t depthMax, const LaunchShape& ls, KernelTimer* kt);
 hipError_t resolve_launch(rbh_ctx c, const float* colorspace, float expo, void* out);
 size_t trace_spill_words(size_t lanes);
@@ -70,6 +73,81 @@ int rbh_build_accel(rbh_ctx c) {
 
 int rbh_build_light_tree(rbh_ctx c, const rbh_emissive_tri* tris, const float* flux, uint32_t n) {
     return rbh_build_light_tree_ex(c, tris, flux, n, 0u);
+}
+
+int rbh_refit_accel(rbh_ctx c) {
+    if (!c) return RBH_ERR_INVALID;
+    return lbvh_refit(c);
+}
+
+int rbh_update_vertices(rbh_ctx c, uint32_t first, uint32_t count, const float* verts) {
+    if (!c || (count && !verts)) return RBH_ERR_INVALID;
+    Geometry& g = c->geom;
+    if ((uint64_t)first + count > g.nverts)
+        return fail(c, RBH_ERR_RANGE, "vertices %u..%u of %u", first, first + count, g.nverts);
+    if (!count) return RBH_OK;
+    RBH_HIP(c, hipMemcpyAsync(g.verts.as<char>() + (size_t)first * 12, verts, (size_t)count * 12,
+                              hipMemcpyHostToDevice, c->stream));
+    RBH_HIP(c, hipStreamSynchronize(c->stream));
+    c->accel.stale = true;
+    return RBH_OK;
+}
+
+int rbh_translate_scene(rbh_ctx c, const float* delta) {
+    if (!c || !delta) return RBH_ERR_INVALID;
+    if (!c->accel.built) return fail(c, RBH_ERR_STATE, "rbh_build_accel first");
+    for (int a = 0; a < 3; ++a)
+        if (!std::isfinite(delta[a])) return fail(c, RBH_ERR_INVALID, "translation must be finite");
+    Geometry& g = c->geom;
+    RBH_HIP(c, translate_points(c, g.verts.as<float>(), g.nverts, 3, delta));
+    LightTreeDev& t = c->tree;
+    if (t.nlights) {
+        // emitter corners (9 floats at the start of each 68-byte record)
+        RBH_HIP(c, translate_points(c, t.tris.as<float>(), (size_t)t.nlights * 3, 0, delta));
+        for (auto& nd : t.hostNodes)
+            for (int a = 0; a < 3; ++a) nd.spatialMean[a] -= delta[a];
+        RBH_HIP(c, hipMemcpyAsync(t.nodes.p, t.hostNodes.data(), t.hostNodes.size() * sizeof(rbh_packed_node),
+                                  hipMemcpyHostToDevice, c->stream));
+        RBH_HIP(c, light_tree_expand(c));
+    }
+    c->accel.stale = true;
+    return lbvh_refit(c);
+}
+
+int rbh_update_materials(rbh_ctx c, uint32_t first, uint32_t count, const rbh_material* m) {
+    if (!c || (count && !m)) return RBH_ERR_INVALID;
+    if ((uint64_t)first + count > c->nmaterials)
+        return fail(c, RBH_ERR_RANGE, "materials %u..%u of %u", first, first + count, c->nmaterials);
+    for (uint32_t i = 0; i < count; ++i) {
+        const rbh_material& x = m[i];
+        if (x.bxdf > 2) return fail(c, RBH_ERR_INVALID, "material %u: bxdf index %u", first + i, x.bxdf);
+        for (int k = 0; k < 7; ++k)
+            if (((x.textures >> k) & 1u) && (x.tex[k] < 0 || (uint32_t)x.tex[k] >= c->ntextures))
+                return fail(c, RBH_ERR_INVALID, "material %u: texture slot %d references %d of %u", first + i, k,
+                            x.tex[k], c->ntextures);
+        if (x.bxdf == 0 && (x.iorSpectrum >= c->nspectra || x.acSpectrum >= c->nspectra))
+            return fail(c, RBH_ERR_INVALID, "material %u: conductor spectra out of range", first + i);
+        if ((x.flags & 4u) && x.emissionSpectrum >= c->nspectra)
+            return fail(c, RBH_ERR_INVALID, "material %u: emission spectrum out of range", first + i);
+    }
+    if (!count) return RBH_OK;
+    // the vertex-attribute layout must stay inside the attribute buffer
+    const uint64_t attribBytes = c->geom.attribs.bytes;
+    for (uint32_t i = 0; i < count && c->geom.nverts; ++i) {
+        const rbh_material& x = m[i];
+        struct { uint32_t bit, off, size; } at[] = {{1u, x.normalOffset, 8u}, {2u, x.frameOffset, 16u},
+                                                    {8u, x.texCoord1Offset, 8u}, {16u, x.texCoord2Offset, 8u}};
+        for (auto& a : at)
+            if ((x.attributes & a.bit) &&
+                ((uint64_t)(c->geom.nverts - 1) * x.attributeStride + a.off + a.size > attribBytes ||
+                 (a.off % 4) || (x.attributeStride % 4)))
+                return fail(c, RBH_ERR_INVALID, "material %u: attribute (bit %u) outside the attribute buffer",
+                            first + i, a.bit);
+    }
+    RBH_HIP(c, hipMemcpyAsync(c->materials.as<char>() + (size_t)first * sizeof(Material), m,
+                              (size_t)count * sizeof(Material), hipMemcpyHostToDevice, c->stream));
+    RBH_HIP(c, hipStreamSynchronize(c->stream));
+    return RBH_OK;
 }
 
 }  // extern "C"
@@ -223,6 +301,8 @@ namespace {
 // Checks that a context holds a renderable scene for these launch parameters.
 int check_scene(rbh_ctx c, const rbh_launch_params* lp) {
     if (!c->accel.built) return fail(c, RBH_ERR_STATE, "rbh_build_accel has not been run on the current geometry");
+    if (c->accel.stale)
+        return fail(c, RBH_ERR_STATE, "vertices changed since the last build: rbh_refit_accel or rbh_build_accel first");
     if (!c->haveColor || !c->haveLuts) return fail(c, RBH_ERR_STATE, "colour tables and LUTs must be uploaded first");
     if (lp->width == 0 || lp->height == 0) return fail(c, RBH_ERR_INVALID, "empty film");
     if ((uint64_t)lp->width * lp->height > (1ull << 28)) return fail(c, RBH_ERR_RANGE, "film above 2^28 pixels");
