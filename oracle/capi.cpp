@@ -311,7 +311,10 @@ int orc_resolve(const double* accum, uint32_t w, uint32_t h, const float* m3, fl
         c = V3{std::max(c.x, 0.0f), std::max(c.y, 0.0f), std::max(c.z, 0.0f)} * expo;
         c = mul(cs, c);
         c = mul(agx, c);
-        c = {clampf(std::log2(c.x), minEv, maxEv), clampf(std::log2(c.y), minEv, maxEv), clampf(std::log2(c.z), minEv, maxEv)};
+        // a channel that the matrices leave negative has no logarithm: the
+        // clamp maps its NaN to minEv (fmax / fmin semantics), as the product does
+        c = {std::fmin(std::fmax(std::log2(c.x), minEv), maxEv), std::fmin(std::fmax(std::log2(c.y), minEv), maxEv),
+             std::fmin(std::fmax(std::log2(c.z), minEv), maxEv)};
         c = (c - v3s(minEv)) / (maxEv - minEv);
         auto poly = [](float x) {
             float x2 = x * x, x4 = x2 * x2;

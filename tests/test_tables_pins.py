@@ -196,3 +196,17 @@ def test_agx_monotone_in_exposure():
     rgb = np.array([[0.3, 0.2, 0.1]])
     vals = [_resolve_rgb(rgb, exposure=e)[0] for e in (0.25, 0.5, 1.0, 2.0, 4.0, 8.0)]
     assert all(np.all(b >= a) for a, b in zip(vals, vals[1:]))
+
+
+def test_agx_channel_without_a_logarithm_is_black():
+    """A saturated XYZ colour leaves some linear-sRGB channels negative; they
+    have no logarithm, and the clamp sends them to the bottom of the AgX range
+    (the product's fmax / fmin clamp) instead of propagating NaN."""
+    m = np.array([[3.2406, -1.5372, -0.4986], [-0.9689, 1.8758, 0.0415], [0.0557, -0.2040, 1.0570]], np.float32)
+    acc = np.zeros((1, 2, 4))
+    acc[0, :, 3] = 1.0
+    acc[0, 0, :3] = (0.0, 1.0, 0.0)
+    acc[0, 1, :3] = (0.0, 0.0, 1.0)
+    out = O.resolve(acc, m, 1.0)[0, :, :3].astype(int)
+    assert out[0, 0] == 0 and out[0, 2] == 0 and out[0, 1] > 200
+    assert out[1, 0] == 0 and out[1, 2] > 200
