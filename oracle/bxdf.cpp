@@ -141,6 +141,162 @@ void bxdf_conductor(const orc_scene* sc, const orc_material& m, const DirectLigh
     p.rD = normalize(local.from_local(wi));
 }
 
+// PureDielectric with spectral dispersion: the body pathtrace.cu:238-463
+// keeps commented out, with two readings of its text: a secondary wavelength
+// is terminated when its configuration is invalid (the text's condition is
+// inverted), and each wavelength's transmission is divided by its own
+// relative eta squared (the text divides by the last one computed).
+void bxdf_dielectric(const orc_scene* sc, const orc_material& m, const DirectLight& dl, const QState& q,
+                     const Local& local, PathIO& p) {
+    V3 wo = local.to_local(-p.rD);
+    V3 li = local.to_local(dl.dir);
+    V3 rnd = sobol3(q, D_BXDF0);
+    float alpha = m.mfRoughness * m.mfRoughness;
+    Alpha a{alpha, alpha};
+    if (p.flags & PF_REGULARIZED) a.regularize();
+    const float cosO = wo.z;
+    CtxW co = ctx_w(wo);
+    Spec eta = dense_sample(sc->spectra + (size_t)m.iorSpectrum * NDENSE, p.wl);
+    if (alpha < 0.001f || eta.v[0] == 1.0f) {
+        Spec R;
+        for (int i = 0; i < NL; ++i) R.v[i] = fresnel_real(cosO, eta.v[i]);
+        float pr = R.v[0];
+        V3 wi;
+        if (rnd.z < pr) {
+            wi = {-wo.x, -wo.y, wo.z};
+            p.bxdfPDF = pr;
+            float pdfSum = p.bxdfPDF;
+            for (int i = 1; i < NL; ++i)
+                if (p.wl.pdf.v[i] != 0.0f) pdfSum += R.v[i];
+            p.T *= R * p.wl.active * (1.0f / pdfSum);
+        } else {
+            wl_terminate_all_secondary(p.wl);
+            float T = clampf(1.0f - R.v[0], 0.0f, 1.0f);
+            bool valid = false;
+            float etaRel = 1.0f;
+            wi = refract_rel(wo, {0.0f, 0.0f, 1.0f}, eta.v[0], valid, etaRel);
+            if (!valid) { p.flags |= PF_PATH_TERMINATED; return; }
+            p.refrScale = etaRel;
+            p.bxdfPDF = std::max(0.0f, 1.0f - pr);
+            p.T = p.T * (T / (etaRel * etaRel) / p.bxdfPDF);
+            p.flags |= PF_RAY_REFRACTED;
+        }
+        p.flags |= PF_CURRENT_HIT_SPECULAR;
+        p.rD = local.from_local(wi);
+        return;
+    }
+    if (!dl.occluded) {
+        V3 wi = li;
+        CtxW ci = ctx_w(wi);
+        float cosI = wi.z;
+        float cosF = std::fabs(cosI);
+        float t = cosO * cosI;
+        bool refl = t > 0.0f;
+        float G = ggx_G(ci, co, a);
+        float lpdf = 0.0f;
+        Spec f{};
+        for (int i = 0; i < NL; ++i) {
+            if (p.wl.pdf.v[i] == 0.0f) continue;
+            float etaRel = 1.0f;
+            if (!refl) etaRel = cosO > 0.0f ? eta.v[i] : 1.0f / eta.v[i];
+            V3 wm = normalize(wi * etaRel + wo);
+            wm = wm.z > 0.0f ? wm : -wm;
+            float dmo = dot(wm, wo), dmi = dot(wm, wi);
+            if (t != 0.0f && !(dmo * cosO < 0.0f || dmi * cosI < 0.0f)) {
+                CtxM cm = ctx_m(wm);
+                float R = fresnel_real(dmo, eta.v[i]);
+                float T = clampf(1.0f - R, 0.0f, 1.0f);
+                float pR = R / (R + T);
+                float pT = std::max(0.0f, 1.0f - pR);
+                float awm = std::fabs(dmo);
+                if (refl) {
+                    f.v[i] = ggx_D(cm, a) * R * G / (4.0f * cosO * cosI);
+                    if (i == 0) lpdf = vndf_pdf_cap(wo, awm, co, cm, a) / (4.0f * awm) * pR;
+                } else {
+                    float tt = dmi + dmo / etaRel;
+                    float den = tt * tt;
+                    float dwmdwi = std::fabs(dmi) / den;
+                    f.v[i] = ggx_D(cm, a) * T * G * std::fabs(dmo * dmi / (cosO * cosI * den)) / (etaRel * etaRel);
+                    if (i == 0) lpdf = vndf_pdf_cap(wo, awm, co, cm, a) * dwmdwi * pT;
+                }
+            }
+        }
+        p.L += dl.spectrum * f * p.T * (cosF * power_heuristic(dl.pdf, lpdf) / dl.pdf);
+    }
+    V3 wm = vndf_sample_cap(wo, a, {rnd.x, rnd.y});
+    CtxM cm = ctx_m(wm);
+    float dmo = dot(wo, wm);
+    float awm = std::fabs(dmo);
+    float heroR = fresnel_real(dmo, eta.v[0]);
+    float heroT = clampf(1.0f - heroR, 0.0f, 1.0f);
+    float pR = heroR;
+    Spec f{};
+    float cosF = 0.0f, condCount = 0.0f, condSum = 0.0f;
+    V3 wi;
+    if (rnd.z < pR) {
+        wi = reflect(wo, wm);
+        if (wo.z * wi.z <= 0.0f) { p.flags |= PF_PATH_TERMINATED; return; }
+        CtxW ci = ctx_w(wi);
+        float cosI = wi.z;
+        Spec R;
+        R.v[0] = heroR;
+        for (int i = 1; i < NL; ++i) R.v[i] = fresnel_real(dmo, eta.v[i]);
+        f = ggx_D(cm, a) * R * ggx_G(ci, co, a) / (4.0f * cosO * cosI);
+        float pdfTerm = vndf_pdf_cap(wo, awm, co, cm, a) / (4.0f * awm);
+        p.bxdfPDF = pdfTerm * heroR;
+        condCount = p.wl.active;
+        condSum = p.bxdfPDF;
+        for (int i = 1; i < NL; ++i)
+            if (p.wl.pdf.v[i] != 0.0f) condSum += pdfTerm * R.v[i];
+        cosF = std::fabs(cosI);
+    } else {
+        bool valid = false;
+        float etaRel = 1.0f;
+        wi = refract_rel(wo, wm, eta.v[0], valid, etaRel);
+        if (wo.z * wi.z >= 0.0f || !valid) { p.flags |= PF_PATH_TERMINATED; return; }
+        CtxW ci = ctx_w(wi);
+        float cosI = wi.z;
+        float dmi = dot(wm, wi);
+        float tt = dmi + dmo / etaRel;
+        float den = tt * tt;
+        float dwmdwi = std::fabs(dmi) / den;
+        float G = ggx_G(ci, co, a);
+        f.v[0] = ggx_D(cm, a) * heroT * G * std::fabs(dmo * dmi / (cosO * cosI * den)) / (etaRel * etaRel);
+        float pT = std::max(0.0f, 1.0f - pR);
+        p.bxdfPDF = vndf_pdf_cap(wo, awm, co, cm, a) * dwmdwi * pT;
+        p.refrScale = etaRel;
+        condSum = p.bxdfPDF;
+        cosF = std::fabs(cosI);
+        for (int i = 1; i < NL; ++i) {
+            if (p.wl.pdf.v[i] == 0.0f) continue;
+            float er = cosO > 0.0f ? eta.v[i] : 1.0f / eta.v[i];
+            V3 wmi = normalize(wi * er + wo);
+            wmi = wmi.z > 0.0f ? wmi : -wmi;
+            CtxM cmi = ctx_m(wmi);
+            float dmoi = dot(wmi, wo), dmii = dot(wmi, wi);
+            float awmi = std::fabs(dmoi);
+            float secR = fresnel_real(dmoi, eta.v[i]);
+            float secT = clampf(1.0f - secR, 0.0f, 1.0f);
+            float ti = dmii + dmoi / er;
+            float deni = ti * ti;
+            float dwi = std::fabs(dmii) / deni;
+            float fi = ggx_D(cmi, a) * secT * G * std::fabs(dmoi * dmii / (cosO * cosI * deni)) / (er * er);
+            float pdf = vndf_pdf_cap(wo, awmi, co, cmi, a) * dwi * secT;
+            if (pdf <= 0.0f || dmoi * cosO < 0.0f || dmii * cosI < 0.0f) {
+                wl_terminate_secondary(p.wl, i);
+            } else {
+                f.v[i] = fi;
+                condSum += pdf;
+            }
+        }
+        condCount = p.wl.active;
+        p.flags |= PF_RAY_REFRACTED;
+    }
+    if (p.bxdfPDF == 0.0f) { p.flags |= PF_PATH_TERMINATED; return; }
+    p.T *= condCount * (1.0f / condSum) * f * cosF;
+    p.rD = normalize(local.from_local(wi));
+}
+
 // src/kernels/pathtrace.cu:548-905
 void bxdf_complex(const orc_scene* sc, const DirectLight& dl, const QState& q, const Local& local,
                   const Surface& s, PathIO& p) {

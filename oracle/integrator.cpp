@@ -423,7 +423,7 @@ void render_pixel(const Ctx& C, uint32_t px, uint32_t py, double res[4], Stats& 
                 }
                 if (M->bxdf == 0) bxdf_conductor(sc, *M, dl, q, local, path);
                 else if (M->bxdf == 2) bxdf_complex(sc, dl, q, local, surf, path);
-                // bxdf 1 (PureDielectric) has an empty body in the reference
+                else bxdf_dielectric(sc, *M, dl, q, local, path);
                 // RBH_RENDER_NO_DROP (flag 16, no reference counterpart): an
                 // invalid BxDF sample ends the path and keeps its radiance
                 // instead of discarding the whole sample

@@ -185,6 +185,31 @@ inline Wavelengths sample_visible(float u) {
     w.active = (float)NL;
     return w;
 }
+// SampledWavelengths::terminateSecondary / terminateAllSecondary
+// (src/device/spectral.h:250-276): the remaining pdfs are rescaled by
+// (N / active) * (active' / N) at each termination.
+inline void wl_terminate_secondary(Wavelengths& w, int i) {
+    if (w.pdf.v[i] == 0.0f) return;
+    w.pdf.v[i] = 0.0f;
+    float m = (float)NL / w.active;
+    w.active -= 1.0f;
+    m *= w.active / (float)NL;
+    for (int k = 0; k < NL; ++k)
+        if (w.pdf.v[k] != 0.0f) w.pdf.v[k] *= m;
+}
+inline bool wl_all_secondary_terminated(const Wavelengths& w) {
+    for (int k = 1; k < NL; ++k)
+        if (w.pdf.v[k] != 0.0f) return false;
+    return true;
+}
+inline void wl_terminate_all_secondary(Wavelengths& w) {
+    if (wl_all_secondary_terminated(w)) return;
+    float m = (float)NL / w.active;
+    w.active = 1.0f;
+    m *= w.active / (float)NL;
+    w.pdf.v[0] *= m;
+    for (int k = 1; k < NL; ++k) w.pdf.v[k] = 0.0f;
+}
 // src/device/spectral.h:304-327 (lroundf index into the 1 nm table)
 inline float dense_sample(const float* tab, float l) { return tab[std::lround(l) - LMIN]; }
 inline Spec dense_sample(const float* tab, const Wavelengths& w) {
@@ -512,6 +537,29 @@ inline V3 offset_point(V3 p, V3 n) {
 }
 // src/device/util.h:177-180
 inline V3 reflect(V3 v, V3 n) { return -v + 2.0f * dot(v, n) * n; }
+// src/device/util.h:71-96
+inline V3 refract_rel(V3 v, V3 n, float eta, bool& valid, float& etaRel) {
+    float ci = dot(v, n);
+    if (ci < 0.0f) { eta = 1.0f / eta; ci = -ci; n = -n; }
+    float s2i = std::max(0.0f, 1.0f - ci * ci);
+    float s2t = s2i / (eta * eta);
+    if (s2t >= 1.0f) { valid = false; return {0.0f, 0.0f, 0.0f}; }
+    valid = true;
+    etaRel = eta;
+    return -v / eta + (ci / eta - std::sqrt(1.0f - s2t)) * n;
+}
+// microfacet::FReal, src/device/surface.h:197-216
+inline float fresnel_real(float c, float eta) {
+    c = clampf(c, -1.0f, 1.0f);
+    if (c < 0.0f) { eta = 1.0f / eta; c = -c; }
+    float s2 = 1.0f - c * c;
+    float s2t = s2 / (eta * eta);
+    if (s2t >= 1.0f) return 1.0f;
+    float ct = std::sqrt(std::max(0.0f, 1.0f - s2t));
+    float rParl = (eta * c - ct) / (eta * c + ct);
+    float rPerp = (c - eta * ct) / (c + eta * ct);
+    return (rParl * rParl + rPerp * rPerp) / 2.0f;
+}
 // src/device/util.h:181-198
 inline V3 refract(V3 v, V3 n, float eta) {
     float ci = dot(v, n);

@@ -55,5 +55,7 @@ void bxdf_conductor(const orc_scene* sc, const orc_material& m, const DirectLigh
                     const Local& local, PathIO& p);
 void bxdf_complex(const orc_scene* sc, const DirectLight& dl, const QState& q, const Local& local,
                   const Surface& s, PathIO& p);
+void bxdf_dielectric(const orc_scene* sc, const orc_material& m, const DirectLight& dl, const QState& q,
+                     const Local& local, PathIO& p);
 
 }  // namespace orc

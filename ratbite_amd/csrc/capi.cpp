@@ -152,6 +152,8 @@ int rbh_upload_materials(rbh_ctx c, const rbh_material* m, uint32_t n) {
                     return fail(c, RBH_ERR_INVALID, "material %u: texture slot %d references %d of %u", i, k, m[i].tex[k], c->ntextures);
         if (m[i].bxdf == 0 && (m[i].iorSpectrum >= c->nspectra || m[i].acSpectrum >= c->nspectra))
             return fail(c, RBH_ERR_INVALID, "material %u: conductor spectra out of range", i);
+        if (m[i].bxdf == 1 && m[i].iorSpectrum >= c->nspectra)
+            return fail(c, RBH_ERR_INVALID, "material %u: dielectric ior spectrum out of range", i);
         if ((m[i].flags & 4u) && m[i].emissionSpectrum >= c->nspectra)
             return fail(c, RBH_ERR_INVALID, "material %u: emission spectrum out of range", i);
     }

@@ -169,6 +169,24 @@ __device__ __forceinline__ void ltc_eval(V3 wo, V3 wi, float A, float B, float& 
     pdf = r;
 }
 
+// Dielectric Fresnel reflectance (microfacet::FReal, src/device/surface.h:
+// 197-216): eta is inverted for a cosine below zero; 1 under total internal
+// reflection.
+__device__ __forceinline__ float fresnel_real(float c, float eta) {
+    c = clampf(c, -1.0f, 1.0f);
+    if (c < 0.0f) {
+        eta = 1.0f / eta;
+        c = -c;
+    }
+    const float s2 = 1.0f - c * c;
+    const float s2t = s2 / (eta * eta);
+    if (s2t >= 1.0f) return 1.0f;
+    const float ct = sqrtf(fmaxf(0.0f, 1.0f - s2t));
+    const float rParl = (eta * c - ct) / (eta * c + ct);
+    const float rPerp = (c - eta * ct) / (c + eta * ct);
+    return (rParl * rParl + rPerp * rPerp) / 2.0f;
+}
+
 // ------------------------------------------------------------------ inputs
 struct Surface {
     bool sheen;
@@ -192,6 +210,35 @@ struct PathIO {
     float refr;
 };
 struct Luts { const float* __restrict__ t; };
+
+// Secondary wavelengths a dispersive refraction has terminated
+// (SampledWavelengths::terminateSecondary / terminateAllSecondary,
+// src/device/spectral.h:250-276): bit i-1 of PF_WL_TERMINATED marks
+// wavelength i (1..3). The hero wavelength is never terminated. The
+// accumulation gives a terminated wavelength pdf 0 and scales the active ones
+// by (active count) / (initially active count), the product of the
+// reference's per-termination factors.
+constexpr uint32_t PF_WL_SHIFT = 9, PF_WL_TERMINATED = 7u << PF_WL_SHIFT;
+__device__ __forceinline__ bool wl_active(const Wl& wl, uint32_t flags, int i) {
+    return wl.pdf.v[i] != 0.0f && (i == 0 || !((flags >> (PF_WL_SHIFT + i - 1)) & 1u));
+}
+__device__ __forceinline__ float wl_active_count(const Wl& wl, uint32_t flags) {
+    float n = 0.0f;
+    for (int i = 0; i < kNL; ++i) n += wl_active(wl, flags, i) ? 1.0f : 0.0f;
+    return n;
+}
+__device__ __forceinline__ void wl_terminate(uint32_t& flags, int i) { flags |= 1u << (PF_WL_SHIFT + i - 1); }
+__device__ __forceinline__ void wl_terminate_all(uint32_t& flags) { flags |= PF_WL_TERMINATED; }
+// The pdfs the accumulation divides by.
+__device__ __forceinline__ S4 wl_effective_pdf(const Wl& wl, uint32_t flags) {
+    S4 pdf = wl.pdf;
+    if (!(flags & PF_WL_TERMINATED)) return pdf;
+    float initial = 0.0f;
+    for (int i = 0; i < kNL; ++i) initial += wl.pdf.v[i] != 0.0f ? 1.0f : 0.0f;
+    const float keep = wl_active_count(wl, flags) / initial;
+    for (int i = 0; i < kNL; ++i) pdf.v[i] = wl_active(wl, flags, i) ? pdf.v[i] * keep : 0.0f;
+    return pdf;
+}
 
 // Conductor callable (pathtrace.cu:464-547). Adds the unoccluded NEE term
 // to Lnee and samples the next direction.
@@ -241,6 +288,173 @@ __device__ inline void conductor_bxdf(const float* __restrict__ eta_tab, const f
     const S4 f = ggx_D(cm, a) * F * smith_G(ci, co, a) / fabsf(4.0f * wo.z * wi.z);
     p.bxdfPdf = vndf_bounded_pdf(wo, awm, cm, a) / (4.0f * awm);
     p.T *= f * fabsf(wi.z) / p.bxdfPdf;
+    p.rd = nrm(fr.to_world(wi));
+}
+
+// PureDielectric callable with spectral dispersion: the body the reference
+// keeps commented out (pathtrace.cu:238-463), a rough or smooth dielectric
+// whose index of refraction is a dense spectrum sampled at the path's
+// wavelengths. Reflection keeps every wavelength and weights by the hero
+// MIS over the active wavelengths' Fresnel terms (smooth) or pdfs (rough);
+// a smooth refraction follows the hero wavelength alone and terminates the
+// secondaries; a rough refraction keeps each secondary whose own half vector
+// for the sampled direction is a valid configuration and terminates the
+// others. Two places where the commented-out text cannot be what it means
+// are read as intended: a secondary is terminated when its configuration is
+// INVALID (the text terminates the valid ones), and each wavelength's
+// transmission is divided by its own relative eta squared (the text divides
+// all of them by the last one computed); the refraction scale is the hero's.
+__device__ inline void dielectric_bxdf(const float* __restrict__ eta_tab, float rough, const Light& dl, const Qrng& q,
+                                       const Frame& fr, PathIO& p, S4& Lnee) {
+    const V3 wo = fr.to_local(-p.rd);
+    const V3 li = fr.to_local(dl.dir);
+    const V3 rnd = q.s3(QD_BXDF0);
+    const float alpha = rough * rough;
+    Rough a{alpha, alpha};
+    if (p.flags & PF_REGULARIZED) {
+        if (a.ax < 0.3f) a.ax = clampf(2.0f * a.ax, 0.1f, 0.3f);
+        if (a.ay < 0.3f) a.ay = clampf(2.0f * a.ay, 0.1f, 0.3f);
+    }
+    const float cosO = wo.z;
+    const DirCtx co = dir_ctx(wo);
+    const S4 eta = dense4(eta_tab, p.wl);
+    if (alpha < 0.001f || eta.v[0] == 1.0f) {
+        S4 R;
+        for (int i = 0; i < kNL; ++i) R.v[i] = fresnel_real(cosO, eta.v[i]);
+        const float pr = R.v[0];
+        V3 wi;
+        if (rnd.z < pr) {
+            wi = V3{-wo.x, -wo.y, wo.z};
+            p.bxdfPdf = pr;
+            float pdfSum = p.bxdfPdf;
+            for (int i = 1; i < kNL; ++i)
+                if (wl_active(p.wl, p.flags, i)) pdfSum += R.v[i];
+            p.T *= R * wl_active_count(p.wl, p.flags) * (1.0f / pdfSum);
+        } else {
+            wl_terminate_all(p.flags);
+            const float T = clampf(1.0f - R.v[0], 0.0f, 1.0f);
+            bool valid = false;
+            float etaRel = 1.0f;
+            wi = refract_rel(wo, V3{0.0f, 0.0f, 1.0f}, eta.v[0], valid, etaRel);
+            if (!valid) { p.flags |= PF_TERMINATED; return; }
+            p.refr = etaRel;
+            p.bxdfPdf = fmaxf(0.0f, 1.0f - pr);
+            p.T *= s4(T / (etaRel * etaRel) / p.bxdfPdf);
+            p.flags |= PF_RAY_REFRACTED;
+        }
+        p.flags |= PF_CUR_SPECULAR;
+        p.rd = fr.to_world(wi);
+        return;
+    }
+    if (!dl.occluded) {
+        const V3 wi = li;
+        const DirCtx ci = dir_ctx(wi);
+        const float cosI = wi.z;
+        const float cosF = fabsf(cosI);
+        const float t = cosO * cosI;
+        const bool reflect = t > 0.0f;
+        const float G = smith_G(ci, co, a);
+        float lpdf = 0.0f;
+        S4 f = s4(0.0f);
+        for (int i = 0; i < kNL; ++i) {
+            if (!wl_active(p.wl, p.flags, i)) continue;
+            float etaRel = 1.0f;
+            if (!reflect) etaRel = cosO > 0.0f ? eta.v[i] : 1.0f / eta.v[i];
+            V3 wm = nrm(wi * etaRel + wo);
+            wm = wm.z > 0.0f ? wm : -wm;
+            const float dmo = dot(wm, wo), dmi = dot(wm, wi);
+            if (t != 0.0f && !(dmo * cosO < 0.0f || dmi * cosI < 0.0f)) {
+                const MCtx cm = m_ctx(wm);
+                const float R = fresnel_real(dmo, eta.v[i]);
+                const float T = clampf(1.0f - R, 0.0f, 1.0f);
+                const float pR = R / (R + T);
+                const float pT = fmaxf(0.0f, 1.0f - pR);
+                const float awm = fabsf(dmo);
+                if (reflect) {
+                    f.v[i] = ggx_D(cm, a) * R * G / (4.0f * cosO * cosI);
+                    if (i == 0) lpdf = vndf_cap_pdf(wo, awm, co, cm, a) / (4.0f * awm) * pR;
+                } else {
+                    const float tt = dmi + dmo / etaRel;
+                    const float den = tt * tt;
+                    const float dwmdwi = fabsf(dmi) / den;
+                    f.v[i] = ggx_D(cm, a) * T * G * fabsf(dmo * dmi / (cosO * cosI * den)) / (etaRel * etaRel);
+                    if (i == 0) lpdf = vndf_cap_pdf(wo, awm, co, cm, a) * dwmdwi * pT;
+                }
+            }
+        }
+        Lnee += dl.spectrum * f * p.T * (cosF * power_heuristic(dl.pdf, lpdf) / dl.pdf);
+    }
+    V3 wm = vndf_cap(wo, a, V2{rnd.x, rnd.y});
+    MCtx cm = m_ctx(wm);
+    float dmo = dot(wo, wm);
+    float awm = fabsf(dmo);
+    const float heroR = fresnel_real(dmo, eta.v[0]);
+    const float heroT = clampf(1.0f - heroR, 0.0f, 1.0f);
+    const float pR = heroR;
+    S4 f = s4(0.0f);
+    float cosF = 0.0f, condCount = 0.0f, condSum = 0.0f;
+    V3 wi;
+    if (rnd.z < pR) {
+        wi = reflect_about(wo, wm);
+        if (wo.z * wi.z <= 0.0f) { p.flags |= PF_TERMINATED; return; }
+        const DirCtx ci = dir_ctx(wi);
+        const float cosI = wi.z;
+        S4 R;
+        R.v[0] = heroR;
+        for (int i = 1; i < kNL; ++i) R.v[i] = fresnel_real(dmo, eta.v[i]);
+        f = ggx_D(cm, a) * R * smith_G(ci, co, a) / (4.0f * cosO * cosI);
+        const float pdfTerm = vndf_cap_pdf(wo, awm, co, cm, a) / (4.0f * awm);
+        p.bxdfPdf = pdfTerm * heroR;
+        condCount = wl_active_count(p.wl, p.flags);
+        condSum = p.bxdfPdf;
+        for (int i = 1; i < kNL; ++i)
+            if (wl_active(p.wl, p.flags, i)) condSum += pdfTerm * R.v[i];
+        cosF = fabsf(cosI);
+    } else {
+        bool valid = false;
+        float etaRel = 1.0f;
+        wi = refract_rel(wo, wm, eta.v[0], valid, etaRel);
+        if (wo.z * wi.z >= 0.0f || !valid) { p.flags |= PF_TERMINATED; return; }
+        const DirCtx ci = dir_ctx(wi);
+        const float cosI = wi.z;
+        float dmi = dot(wm, wi);
+        float tt = dmi + dmo / etaRel;
+        float den = tt * tt;
+        float dwmdwi = fabsf(dmi) / den;
+        const float G = smith_G(ci, co, a);
+        f.v[0] = ggx_D(cm, a) * heroT * G * fabsf(dmo * dmi / (cosO * cosI * den)) / (etaRel * etaRel);
+        const float pT = fmaxf(0.0f, 1.0f - pR);
+        p.bxdfPdf = vndf_cap_pdf(wo, awm, co, cm, a) * dwmdwi * pT;
+        p.refr = etaRel;
+        condSum = p.bxdfPdf;
+        cosF = fabsf(cosI);
+        for (int i = 1; i < kNL; ++i) {
+            if (!wl_active(p.wl, p.flags, i)) continue;
+            const float er = cosO > 0.0f ? eta.v[i] : 1.0f / eta.v[i];
+            V3 wmi = nrm(wi * er + wo);
+            wmi = wmi.z > 0.0f ? wmi : -wmi;
+            const MCtx cmi = m_ctx(wmi);
+            const float dmoi = dot(wmi, wo), dmii = dot(wmi, wi);
+            const float awmi = fabsf(dmoi);
+            const float secR = fresnel_real(dmoi, eta.v[i]);
+            const float secT = clampf(1.0f - secR, 0.0f, 1.0f);
+            const float ti = dmii + dmoi / er;
+            const float deni = ti * ti;
+            const float dwi = fabsf(dmii) / deni;
+            const float fi = ggx_D(cmi, a) * secT * G * fabsf(dmoi * dmii / (cosO * cosI * deni)) / (er * er);
+            const float pdf = vndf_cap_pdf(wo, awmi, co, cmi, a) * dwi * secT;
+            if (pdf <= 0.0f || dmoi * cosO < 0.0f || dmii * cosI < 0.0f) {
+                wl_terminate(p.flags, i);
+            } else {
+                f.v[i] = fi;
+                condSum += pdf;
+            }
+        }
+        condCount = wl_active_count(p.wl, p.flags);
+        p.flags |= PF_RAY_REFRACTED;
+    }
+    if (p.bxdfPdf == 0.0f) { p.flags |= PF_TERMINATED; return; }
+    p.T *= condCount * (1.0f / condSum) * f * cosF;
     p.rd = nrm(fr.to_world(wi));
 }
 

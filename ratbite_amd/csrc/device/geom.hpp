@@ -57,6 +57,26 @@ __device__ __forceinline__ V3 refract_through(V3 v, V3 n, float eta) {
     return -v / eta + (ci / eta - sqrtf(1.0f - s2t)) * n;
 }
 
+// utility::refract with its validity flag and relative eta
+// (src/device/util.h:71-96): eta is inverted when v comes from below n.
+__device__ __forceinline__ V3 refract_rel(V3 v, V3 n, float eta, bool& valid, float& etaRel) {
+    float ci = dot(v, n);
+    if (ci < 0.0f) {
+        eta = 1.0f / eta;
+        ci = -ci;
+        n = -n;
+    }
+    const float s2i = fmaxf(0.0f, 1.0f - ci * ci);
+    const float s2t = s2i / (eta * eta);
+    if (s2t >= 1.0f) {
+        valid = false;
+        return {0.0f, 0.0f, 0.0f};
+    }
+    valid = true;
+    etaRel = eta;
+    return -v / eta + (ci / eta - sqrtf(1.0f - s2t)) * n;
+}
+
 __device__ __forceinline__ V2 disk_polar(V2 u) {
     const float r = sqrtf(u.x);
     float s, c;

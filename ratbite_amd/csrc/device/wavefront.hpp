@@ -551,8 +551,9 @@ __device__ __forceinline__ void shade_hit(const Frame1& f, const SceneView& S, u
                                m.mfRoughness, dl, q, local, P, Lnee);
             } else if (m.bxdf == 2) {
                 complex_bxdf(ct, lut, dl, q, local, s, P, Lnee);
+            } else {
+                dielectric_bxdf(S.spectra + (size_t)m.iorSpectrum * kDense, m.mfRoughness, dl, q, local, P, Lnee);
             }
-            // bxdf 1: PureDielectric has an empty body in the reference
             if (f.noDrop && (P.flags & PF_TERMINATED)) P.flags = (P.flags & ~PF_TERMINATED) | PF_FINISHED;
             if (!dl.occluded) {
                 shadow = true;
@@ -648,8 +649,9 @@ __device__ __forceinline__ void accumulate_slot(const Frame1& f, const SceneView
         if (!(flags & PF_TERMINATED)) {
             const float4 Lv = f.p.L[j];
             const Wl wl = slot_wavelengths(S, k, px, py);
+            const S4 pdf = wl_effective_pdf(wl, flags);
             S4 L{{Lv.x, Lv.y, Lv.z, Lv.w}};
-            for (int c = 0; c < kNL; ++c) L.v[c] = wl.pdf.v[c] != 0.0f ? L.v[c] / wl.pdf.v[c] : 0.0f;
+            for (int c = 0; c < kNL; ++c) L.v[c] = pdf.v[c] != 0.0f ? L.v[c] / pdf.v[c] : 0.0f;
             double xyz[3];
             ct.xyz(wl, L, xyz);
             if (isfinite(xyz[0]) && isfinite(xyz[1]) && isfinite(xyz[2])) {

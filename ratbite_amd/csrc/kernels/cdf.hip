@@ -2,22 +2,27 @@
 //
 // Replaces buildCDFLatLongRGBUpTo{4,8,16,32}k + invertCDFToIndices
 // (src/kernels/cdf.cu:287-535, launched at rendering_interface.cpp:1364-1419)
-// with ONE fused kernel: each workgroup owns a row, streams its texels from
-// HBM once, builds the row CDF in LDS, inverts it from LDS into uint16
+// with ONE fused kernel. Each wave owns a row (up to four rows per
+// workgroup, as many as fit 80 KB of LDS so that two workgroups share a CU):
+// it streams the row's texels from HBM once, scans them in registers, keeps
+// the normalised row CDF in its own LDS row, inverts it from LDS into uint16
 // indices and writes only those (14 B per RGB32F texel of HBM traffic instead
 // of the reference's texel read + FP32 CDF write + CDF re-read + index
 // write). The last workgroup to finish (device-scope counter) builds the
-// marginal CDF with the sin(theta) Jacobian, inverts it and publishes the
-// integral, then re-arms the counter so the launch can be replayed without a
-// memset.
+// marginal CDF with the sin(theta) Jacobian the same way with its first wave,
+// inverts it, publishes the integral and re-arms the counter, so the launch
+// can be replayed without a memset.
 //
 // FP32 mode keeps the reference's association order exactly: per-thread
-// float4 component scan, then a block-wide Hillis-Steele scan over the .w
+// float4 component scan, then the block-wide Hillis-Steele scan over the .w
 // lanes with the reference's block size B (256/512/1024, chosen from W/4 as
 // rendering_interface.cpp:1380-1382), then the running total of previous
-// chunks. The CDF floats, and therefore the inverted indices, are
-// reproducible bit for bit. FP64 mode carries the prefix sums and the
-// normalisation in double and rounds each CDF value once.
+// chunks. Here one wave runs all B virtual lanes with cross-lane reads and no
+// barrier (wave_scan). The CDF floats, and therefore the inverted indices,
+// are reproducible bit for bit. The inversion is linear in the row length
+// for rows whose probed entries are non-decreasing (the usual case) and falls
+// back to the reference's binary search otherwise. FP64 mode carries the
+// prefix sums and the normalisation in double and rounds each CDF value once.
 #include "../context.hpp"
 #include <cmath>
 #include <mutex>

@@ -127,6 +127,8 @@ int rbh_update_materials(rbh_ctx c, uint32_t first, uint32_t count, const rbh_ma
                             x.tex[k], c->ntextures);
         if (x.bxdf == 0 && (x.iorSpectrum >= c->nspectra || x.acSpectrum >= c->nspectra))
             return fail(c, RBH_ERR_INVALID, "material %u: conductor spectra out of range", first + i);
+        if (x.bxdf == 1 && x.iorSpectrum >= c->nspectra)
+            return fail(c, RBH_ERR_INVALID, "material %u: dielectric ior spectrum out of range", first + i);
         if ((x.flags & 4u) && x.emissionSpectrum >= c->nspectra)
             return fail(c, RBH_ERR_INVALID, "material %u: emission spectrum out of range", first + i);
     }

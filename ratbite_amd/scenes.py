@@ -417,7 +417,8 @@ def furnace(inner="metal", rough=0.3, width=64, height=64, spp=64, depth=8, leve
     f0 = 0, seen near normal incidence from the centre), and a level-`level`
     icosphere of `radius` at the centre with a white, non-absorbing material:
     "metal" (metalness 1, base white, multiple-scattering compensation) or
-    "dielectric" (metalness 0, base white, ior 1.5). Every direction seen from
+    "dielectric" (metalness 0, base white, ior 1.5) or "glass" (a PureDielectric
+    with a dispersive index of refraction). Every direction seen from
     the inner sphere carries the wall radiance Le, so its pixels converge to
     Le times its directional albedo (one within the LUTs' accuracy), like the
     pixels that see the wall directly. The camera looks at the centre from
@@ -433,14 +434,20 @@ def furnace(inner="metal", rough=0.3, width=64, height=64, spp=64, depth=8, leve
     inn = vertex_normals(iv, ii)
     iuv = np.zeros((len(iv), 2), np.float32)
     mats = [material(base=(0.0, 0.0, 0.0), metal=1.0, rough=0.5, emissive=(1.0, 1.0, 1.0))]
+    spectra = []
     if inner == "metal":
         mats.append(material(base=(1.0, 1.0, 1.0), metal=1.0, rough=rough))
+    elif inner == "glass":
+        # PureDielectric with a dispersive index of refraction (Cauchy b = 0.03 um^2)
+        mats.append(material(bxdf=BXDF_DIELECTRIC, rough=rough))
+        mats[-1]["iorSpectrum"] = 0
+        spectra = [dispersive_ior(1.45, 0.03)]
     else:
         mats.append(material(base=(1.0, 1.0, 1.0), metal=0.0, rough=rough, ior=1.5))
     parts = [(ev, ei, en, euv, 0), (iv, ii, inn, iuv, 1)]
     cam = camera_basis((0.0, 0.0, 5.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0))
     cam["fov"] = math.radians(60.0)
-    return _finish(parts, mats, [], [], cam, None, width, height, depth, spp, "furnace-" + inner, tables,
+    return _finish(parts, mats, spectra, [], cam, None, width, height, depth, spp, "furnace-" + inner, tables,
                    light_uv=np.zeros((3, 2), np.float32))
 
 
@@ -601,9 +608,16 @@ def features(seed=0x5EED0009, width=96, height=96, spp=4, depth=8):
     m["textures"] = TX_SHEEN_COLOR | TX_SHEEN_ROUGH
     m["tex"][4], m["tex"][5] = t_shc, t_shr
     sphere((0.5, 1.0, 0.2), 0.22, m)
-    sphere((0.0, 1.2, -0.5), 0.2, material(bxdf=BXDF_DIELECTRIC, base=(1, 1, 1), rough=0.1, ior=1.5))
     eta, kk = cauchy_spectra(rng)
-    spectra += [eta, kk]
+    spectra += [eta, kk, dispersive_ior(1.45, 0.03)]
+    # PureDielectric spheres with a dispersive index of refraction: rough
+    # (alpha 0.01) and smooth
+    m = material(bxdf=BXDF_DIELECTRIC, base=(1, 1, 1), rough=0.1, ior=1.5)
+    m["iorSpectrum"] = 3
+    sphere((0.0, 1.2, -0.5), 0.2, m)
+    m = material(bxdf=BXDF_DIELECTRIC, base=(1, 1, 1), rough=0.0, ior=1.5)
+    m["iorSpectrum"] = 3
+    sphere((0.45, 1.55, -0.3), 0.15, m)
     sphere((0.0, 0.7, 0.45), 0.15, material(bxdf=BXDF_CONDUCTOR, rough=0.3, eta_k=(1, 2)))
     # alpha-cutoff quad (base-colour alpha 40/255 or 1 in a checker) and an
     # alpha-blended quad in front of the back half of the box
@@ -695,6 +709,13 @@ def batch(n=64, seed=0x5EED0003):
         out.append(spheres(seed=seed + i, n_spheres=n_sph, level=level, n_lights=n_lights, width=side,
                            height=side, spp=32, depth=12, env_size=env))
     return out
+
+
+def dispersive_ior(a, b):
+    """A dense 1 nm index-of-refraction spectrum n = a + b / lambda^2
+    (lambda in um): Cauchy's dispersion law for a glass."""
+    lam = np.arange(360, 831, dtype=np.float64) * 1e-3
+    return (a + b / lam ** 2).astype(np.float32)
 
 
 def cauchy_spectra(rng):

@@ -79,6 +79,11 @@ def test_product_kernels_furnace():
     wall, ratio = furnace_ratios(host, "metal", 0.3, 128, L.RBH_RENDER_NO_DROP)
     print("product furnace metal: wall %.5f, sphere / wall %.4f" % (wall, ratio))
     assert abs(wall - 1.0) < 2e-3 and abs(ratio - 1.0) < 0.01
+    # the dispersive smooth glass sphere (secondary wavelengths terminated at
+    # refraction) conserves energy too
+    wall, ratio = furnace_ratios(host, "glass", 0.0, 64, L.RBH_RENDER_NO_DROP)
+    print("product furnace dispersive glass: wall %.5f, sphere / wall %.4f" % (wall, ratio))
+    assert abs(wall - 1.0) < 2e-3 and abs(ratio - 1.0) < 0.015
 
 
 SAN_SCRIPT = r'''

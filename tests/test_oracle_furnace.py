@@ -191,3 +191,18 @@ def test_oracle_relmse_falls_as_inverse_spp():
     slope = np.polyfit(np.log(spps), np.log(errs), 1)[0]
     print("relMSE", dict(zip(spps, errs)), "slope %.3f" % slope)
     assert -1.3 < slope < -0.7
+
+
+@pytest.mark.parametrize("rough,lo,hi", [(0.0, 0.985, 1.015), (0.3, 0.85, 1.0)])
+def test_furnace_dispersive_glass(rough, lo, hi):
+    """A PureDielectric sphere whose index of refraction varies with
+    wavelength (Cauchy b = 0.03 um^2). Smooth, it conserves energy exactly:
+    the sphere converges to the wall radiance, which also pins the hero-
+    wavelength weighting of refraction (the secondary wavelengths are
+    terminated and the hero carries the estimate; a wrong pdf rescale would
+    be off by up to 4x). Rough, the single-scattering microfacet interfaces
+    lose energy but never create it."""
+    wall, ratio = furnace_ratios(oracle_render, "glass", rough, 64, L.RBH_RENDER_NO_DROP)
+    print("furnace dispersive glass rough %.1f: wall %.5f, sphere / wall %.4f" % (rough, wall, ratio))
+    assert abs(wall - 1.0) < 2e-3
+    assert lo < ratio < hi
