@@ -88,6 +88,26 @@ typedef struct rbh_emissive_tri {
     uint32_t primitiveDataIndex;
 } rbh_emissive_tri;
 
+/* DiskLightData / SphereLightData (src/core/light_tree_types.h:24-39,
+ * src/core/scene.h:319-492) as one 48-byte record. The reference declares
+ * these lights but leaves their programs commented out
+ * (src/kernels/pathtrace.cu:150-228). Here a disk emits from the side its
+ * normal (the frame's third column) faces and occludes from both; a sphere
+ * emits outwards and gives nothing to points inside it; a path that hits
+ * either ends there (pure emitter). Emission = the material's emission
+ * (emissionSpectrum x emissive[0] with material flag 4, else the RGB
+ * emissive colour; no texture) x powerScale. */
+typedef struct rbh_shape_light {
+    float position[3];
+    float powerScale;
+    float frame[4];           /* unit quaternion x, y, z, w (the vertex-frame order) */
+    float radius;
+    uint32_t materialIndex;
+    uint32_t type;            /* RBH_SHAPE_DISK or RBH_SHAPE_SPHERE */
+    uint32_t pad;
+} rbh_shape_light;
+enum { RBH_SHAPE_DISK = 0, RBH_SHAPE_SPHERE = 1 };
+
 /* LightTree::PackedNode (src/core/light_tree_types.h:119-162), 32 bytes. */
 typedef struct rbh_packed_node {
     float spatialMean[3];
@@ -207,8 +227,9 @@ int rbh_update_materials(rbh_ctx ctx, uint32_t first, uint32_t count, const rbh_
 /* ------------------------------------------------------------ light tree */
 /* RenderingInterface::buildLightTree + LightTree::Builder::build
  * (rendering_interface.cpp:1065-1186, light_tree.cpp:47-79): builds the tree
- * on the host from n emissive triangles with their flux, uploads nodes,
- * light pointers, bitmasks and the triangle records. */
+ * on the host from n emissive triangles with their flux and the context's
+ * shape lights (rbh_upload_shape_lights), uploads nodes, light pointers,
+ * bitmasks and the triangle records. */
 int rbh_build_light_tree(rbh_ctx ctx, const rbh_emissive_tri* tris, const float* flux, uint32_t n);
 enum {
     RBH_LIGHT_TREE_FP64 = 1,      /* FP64 power mode: flux sums, attribute merges and split costs in double */
@@ -228,10 +249,11 @@ int rbh_light_tree_build_host_ex(const rbh_emissive_tri* tris, const float* flux
 /* fillLightTreeDataBuffers (rendering_interface.cpp:1470-1510): uploads a
  * tree the caller built (nnodes packed nodes in pre-order: the left child of
  * an inner node follows it, its right child index is in coreData; leaves of
- * 1..15 lights), n light pointers (triangle lights), n bitmasks (the path of
- * each light from the root) and the n emissive-triangle records. The tree is
- * checked (every walk ends; every light's bitmask reaches the leaf holding
- * it) before the kernels use it. */
+ * 1..15 lights), n + m light pointers (type 0 for the n triangle lights, type
+ * 3 for the m shape lights uploaded with rbh_upload_shape_lights), n + m
+ * bitmasks (the path of each light from the root) and the n
+ * emissive-triangle records. The tree is checked (every walk ends; every
+ * light's bitmask reaches the leaf holding it) before the kernels use it. */
 int rbh_set_light_tree(rbh_ctx ctx, const rbh_packed_node* nodes, uint32_t nnodes, const uint32_t* lightPointers,
                        const uint64_t* bitmasks, const rbh_emissive_tri* tris, uint32_t n);
 /* Emitted flux of n emissive triangles of one material, as the glTF loader
@@ -250,9 +272,27 @@ int rbh_emissive_flux(const float* verts, const float* uvs, uint32_t n, const fl
  * (RBH_ERR_RANGE when it exceeds capacity). Host only. */
 int rbh_rasterise_uv_triangle(const float* a, const float* b, const float* c, int* xy, uint32_t capacity,
                               uint32_t* count);
-/* Read the built tree back: nodes (<= 2n), lightPointers (n), bitmasks (n). */
+/* Read the built tree back: nodes (<= 2(n+m)), lightPointers (n+m), bitmasks
+ * (n+m), for n triangle lights and m shape lights. */
 int rbh_get_light_tree(rbh_ctx ctx, rbh_packed_node* nodes, uint32_t* nnodes, uint32_t* lightPointers,
                        uint64_t* bitmasks);
+/* Disk and sphere lights (scene.h:319-492; the ADD_LIGHT / CHANGE_LIGHT_*
+ * commands of rendering_interface.cpp:1667-1740 re-upload them): replaces
+ * the context's m shape lights (m <= 4096; 0 removes them) and their flux
+ * (positive; the light tree's importance, e.g. pi x area x emitted radiance).
+ * Shape light i has light index n + i after the n triangle lights and light
+ * pointers of type 3 with that index. They join the light tree at the next
+ * rbh_build_light_tree / rbh_set_light_tree; a render refuses to run until
+ * then. Records are checked: finite position and frame, unit quaternion
+ * (|q| within 1e-3 of 1), radius > 0, type 0 or 1, materialIndex below the
+ * material count. */
+int rbh_upload_shape_lights(rbh_ctx ctx, const rbh_shape_light* lights, const float* flux, uint32_t m);
+/* rbh_light_tree_build_host_ex over n triangle lights and m shape lights:
+ * nodes (capacity 2(n+m)), lightPointers and bitmasks (n+m). Host only. */
+int rbh_light_tree_build_host_shapes(const rbh_emissive_tri* tris, const float* flux, uint32_t n,
+                                     const rbh_shape_light* shapes, const float* shapeFlux, uint32_t m,
+                                     rbh_packed_node* nodes, uint32_t* lightPointers, uint64_t* bitmasks,
+                                     uint32_t flags);
 
 /* ---------------------------------------------------------- environment map
  * loadEnvironmentMap + buildCDFLatLongRGBUpTo{4,8,16,32}k + invertCDFToIndices

@@ -9,6 +9,8 @@ int cdf_build_f64(const float*, int, int, int, double*, double*, double*);
 void cdf_invert(const float*, const float*, int, int, uint16_t*, uint16_t*);
 int cdf_block_size(int);
 int light_tree_build(const float*, const float*, uint32_t, std::vector<PackedNode>&, std::vector<uint32_t>&, std::vector<uint64_t>&);
+int light_tree_build(const float*, const float*, uint32_t, const orc_shape_light*, const float*, uint32_t,
+                     std::vector<PackedNode>&, std::vector<uint32_t>&, std::vector<uint64_t>&);
 int render(const orc_scene*, const orc_params*, double*, int, orc_render_stats*);
 }  // namespace orc
 
@@ -70,6 +72,19 @@ int orc_light_tree_build(const float* tv, const float* flux, uint32_t n, void* n
     std::vector<uint32_t> L;
     std::vector<uint64_t> B;
     int r = light_tree_build(tv, flux, n, N, L, B);
+    if (r < 0) return r;
+    std::memcpy(nodes, N.data(), N.size() * sizeof(PackedNode));
+    std::memcpy(lp, L.data(), L.size() * 4);
+    std::memcpy(bm, B.data(), B.size() * 8);
+    return r;
+}
+
+int orc_light_tree_build_shapes(const float* tv, const float* flux, uint32_t n, const orc_shape_light* shapes,
+                                const float* shape_flux, uint32_t m, void* nodes, uint32_t* lp, uint64_t* bm) {
+    std::vector<PackedNode> N;
+    std::vector<uint32_t> L;
+    std::vector<uint64_t> B;
+    int r = light_tree_build(tv, flux, n, shapes, shape_flux, m, N, L, B);
     if (r < 0) return r;
     std::memcpy(nodes, N.data(), N.size() * sizeof(PackedNode));
     std::memcpy(lp, L.data(), L.size() * 4);

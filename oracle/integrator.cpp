@@ -81,6 +81,103 @@ M3 quat_to_mat(const float q[4]) {
     return m;
 }
 
+// ---- shape lights: disks and spheres (DiskLightData / SphereLightData,
+// light_tree_types.h:24-39; the reference's programs for them are commented
+// out, pathtrace.cu:150-228, its samplers are sampling.h:32-110). A disk
+// emits along its frame's third column and occludes from both sides; a
+// sphere emits outwards; a path that hits either ends there.
+V3 shape_center(const orc_shape_light& s) { return {s.position[0], s.position[1], s.position[2]}; }
+
+// ray parameter of the hit in (0, tmax), else tmax (pathtrace.cu:150-190)
+float disk_t(const orc_shape_light& s, V3 o, V3 d, float tmax) {
+    V3 n = quat_to_mat(s.frame).c[2];
+    V3 oc = o - shape_center(s);
+    float t = -dot(n, oc) / dot(d, n);
+    V3 p = oc + d * t;
+    return (t > 0.0f && t < tmax && dot(p, p) < s.radius * s.radius) ? t : tmax;
+}
+// nearest root in (0, tmax) of |o + t d - c| = r, roots from the
+// closest-approach discriminant and the non-cancelling quadratic form
+float sphere_t(const orc_shape_light& s, V3 o, V3 d, float tmax) {
+    V3 oc = o - shape_center(s);
+    float b = dot(oc, d);
+    V3 qc = oc - d * b;
+    float r2 = s.radius * s.radius;
+    float disc = r2 - dot(qc, qc);
+    if (!(disc >= 0.0f)) return tmax;
+    float q = -b - std::copysign(std::sqrt(disc), b);
+    float c = dot(oc, oc) - r2;
+    float t0 = c / q, t1 = q;
+    if (t0 > t1) std::swap(t0, t1);
+    float t = t0 > 0.0f ? t0 : t1;
+    return (t > 0.0f && t < tmax) ? t : tmax;
+}
+float shape_t(const orc_shape_light& s, V3 o, V3 d, float tmax) {
+    return s.type == 0u ? disk_t(s, o, d, tmax) : sphere_t(s, o, d, tmax);
+}
+
+// 1 - cos(theta_max) of the cone a sphere subtends from distance dist, with
+// the small-angle form below sin^2 = 0.00068523 (sampling.h:79-92)
+float sphere_omc(float dist, float r, float& sinMax, float& sin2Max, float& cosMax) {
+    sinMax = r / dist;
+    sin2Max = sinMax * sinMax;
+    cosMax = std::max(0.0f, std::sqrt(std::max(0.0f, 1.0f - sin2Max)));
+    return sin2Max < 0.00068523f ? sin2Max / 2.0f : 1.0f - cosMax;
+}
+float sphere_cone_pdf(const orc_shape_light& s, V3 o) {
+    float a, b, c;
+    float omc = sphere_omc(length(shape_center(s) - o), s.radius, a, b, c);
+    return 1.0f / (2.0f * PI * omc);
+}
+
+// concentric square-to-disk map (sampling.h:34-51), centre to centre
+V2 concentric(V2 u) {
+    float a = u.x * 2.0f - 1.0f, b = u.y * 2.0f - 1.0f;
+    if (a == 0.0f && b == 0.0f) return {0.0f, 0.0f};
+    float r, th;
+    if (std::fabs(a) > std::fabs(b)) { r = a; th = (PI / 4.0f) * (b / a); }
+    else { r = b; th = (PI / 2.0f) - (PI / 4.0f) * (a / b); }
+    return {r * std::cos(th), r * std::sin(th)};
+}
+
+// cone sampling of a sphere seen from o (sampling.h:71-110)
+V3 sphere_sample(const orc_shape_light& s, V3 o, V2 u, float& pdf, bool& inside) {
+    V3 c = shape_center(s);
+    V3 w = o - c;
+    float dist = length(w);
+    inside = !(dist > s.radius);
+    if (inside) { pdf = 0.0f; return c; }
+    w = w / dist;
+    float sinMax, sin2Max, cosMax;
+    float omc = sphere_omc(dist, s.radius, sinMax, sin2Max, cosMax);
+    float cosT = (cosMax - 1.0f) * u.x + 1.0f;
+    float sin2T = 1.0f - cosT * cosT;
+    if (sin2Max < 0.00068523f) {
+        sin2T = sin2Max * u.x;
+        cosT = std::sqrt(1.0f - sin2T);
+    }
+    float cosA = sin2T / sinMax + cosT * std::sqrt(std::max(0.0f, 1.0f - sin2T / (sinMax * sinMax)));
+    float sinA = std::sqrt(std::max(0.0f, 1.0f - cosA * cosA));
+    float phi = u.y * 2.0f * PI;
+    // util.h:31-38 basis around w
+    float sign = std::copysign(1.0f, w.z);
+    float aa = -1.0f / (sign + w.z);
+    float bb = w.x * w.y * aa;
+    V3 bu{1.0f + sign * (w.x * w.x) * aa, sign * bb, -sign * w.x};
+    V3 bv{bb, sign + (w.y * w.y) * aa, -w.y};
+    V3 dir = w * cosA + bu * (sinA * std::cos(phi)) + bv * (sinA * std::sin(phi));
+    pdf = 1.0f / (2.0f * PI * omc);
+    return c + dir * s.radius;
+}
+
+// the material's emission (spectral, or RGB lifted; no texture) x power scale
+Spec shape_emission(const orc_scene* sc, const orc_shape_light& s, const Wavelengths& wl) {
+    const orc_material& m = sc->materials[s.materialIndex];
+    Spec e = (m.flags & MF_SPECTRAL_EMISSION) ? dense_sample(sc->spectra + (size_t)m.emissionSpectrum * NDENSE, wl) * m.emissive[0]
+                                              : rgb_spec(sc, {m.emissive[0], m.emissive[1], m.emissive[2]}, wl);
+    return e * s.powerScale;
+}
+
 struct Stats { uint64_t closest = 0, shadow = 0, vertices = 0; };
 
 void render_pixel(const Ctx& C, uint32_t px, uint32_t py, double res[4], Stats& st) {
@@ -130,7 +227,42 @@ void render_pixel(const Ctx& C, uint32_t px, uint32_t py, double res[4], Stats& 
             CpuBvh::Hit hit{};
             ++st.closest;
             ++st.vertices;
-            bool isHit = C.bvh.trace(ro, path.rD, FLT_MAX, false, hit);
+            // the nearest shape light bounds the triangle query
+            float shT = FLT_MAX;
+            uint32_t shIdx = 0xFFFFFFFFu;
+            for (uint32_t k = 0; k < sc->nshapes; ++k) {
+                float t = shape_t(sc->shapes[k], ro, path.rD, shT);
+                if (t < shT) { shT = t; shIdx = k; }
+            }
+            bool isHit = C.bvh.trace(ro, path.rD, shT, false, hit);
+            if (!isHit && shIdx != 0xFFFFFFFFu) {
+                // a shape light: its emission with the MIS weight of the previous
+                // BxDF sample, then the path ends
+                const orc_shape_light& sl = sc->shapes[shIdx];
+                const bool prevNEE = depth != 0 && !(path.flags & PF_PREVIOUS_HIT_SPECULAR);
+                float lightPdf = 0.0f;
+                bool noEmission;
+                if (sl.type == 0u) {
+                    V3 th = path.rD * shT;
+                    float d2 = th.x * th.x + th.y * th.y + th.z * th.z;
+                    float lcos = -dot(path.rD, quat_to_mat(sl.frame).c[2]);
+                    noEmission = lcos <= 0.0f;
+                    lightPdf = (1.0f / (PI * sl.radius * sl.radius)) * d2 / lcos;
+                } else {
+                    V3 oc = shape_center(sl) - ro;
+                    noEmission = !(dot(oc, oc) > sl.radius * sl.radius);
+                    if (!noEmission) lightPdf = sphere_cone_pdf(sl, ro);
+                }
+                if (!noEmission) {
+                    float treePdf = prevNEE ? light_tree_pdf(C.tree, LT_SHAPE, sc->nlights + shIdx, ro, lastN) : 1.0f;
+                    lightPdf *= treePdf;
+                    float w = prevNEE ? power_heuristic(path.bxdfPDF, lightPdf) : 1.0f;
+                    path.L += path.T * shape_emission(sc, sl, path.wl) * w;
+                }
+                path.flags |= PF_FINISHED;
+                finished = true;
+                break;
+            }
             uint32_t ltype = LT_NONE, lindex = 0;
             V3 gn{0, 0, 0}, hitPos{0, 0, 0}, hitInterp{0, 0, 0}, vtx[3];
             float b1 = 0, b2 = 0;
@@ -391,6 +523,35 @@ void render_pixel(const Ctx& C, uint32_t px, uint32_t py, double res[4], Stats& 
                         dl.occluded = lcos <= 0.0f;
                         lightPdf = (1.0f / area) * s2 / lcos;
                         dl.spectrum = triangle_emission(sc, tri, bw[0], bw[1], bw[2], path.wl);
+                    } else if (type == LT_SHAPE) {
+                        // a disk point (area measure) or a direction in the cone a
+                        // sphere subtends (solid-angle measure)
+                        const orc_shape_light& sl = sc->shapes[index - sc->nlights];
+                        V3 lp;
+                        bool inside = false;
+                        if (sl.type == 0u) {
+                            M3 f = quat_to_mat(sl.frame);
+                            V2 xy = concentric({rnd.x, rnd.y});
+                            lp = shape_center(sl) + (f.c[0] * xy.x + f.c[1] * xy.y) * sl.radius;
+                        } else {
+                            lp = sphere_sample(sl, hitPos, {rnd.x, rnd.y}, lightPdf, inside);
+                        }
+                        if (inside) {
+                            dl.occluded = true;
+                        } else {
+                            const bool inSample = dot(lastN, lp - hitPos) < 0.0f;
+                            lro = offset_point(inSample ? hitPos : hitInterp, inSample ? -gn : gn);
+                            V3 rl = lp - lro;
+                            float s2 = rl.x * rl.x + rl.y * rl.y + rl.z * rl.z;
+                            dToL = std::sqrt(s2);
+                            dl.dir = rl / dToL;
+                            if (sl.type == 0u) {
+                                float lcos = -dot(quat_to_mat(sl.frame).c[2], dl.dir);
+                                dl.occluded = lcos <= 0.0f;
+                                lightPdf = (1.0f / (PI * sl.radius * sl.radius)) * s2 / lcos;
+                            }
+                            dl.spectrum = shape_emission(sc, sl, path.wl);
+                        }
                     } else if (type == LT_SKY) {
                         float W = (float)sc->envWidth, H = (float)sc->envHeight;
                         float fcx = std::min(rnd.x * (W - 1.0f), W - 1.0001f);
@@ -418,7 +579,11 @@ void render_pixel(const Ctx& C, uint32_t px, uint32_t py, double res[4], Stats& 
                     if (!dl.occluded) {
                         ++st.shadow;
                         CpuBvh::Hit sh{};
-                        dl.occluded = C.bvh.trace(lro, dl.dir, dToL - 0.1f, true, sh);
+                        const float tmax = dToL - 0.1f;
+                        bool blocked = false;
+                        for (uint32_t k = 0; k < sc->nshapes && !blocked; ++k)
+                            blocked = shape_t(sc->shapes[k], lro, dl.dir, tmax) < tmax;
+                        dl.occluded = blocked || C.bvh.trace(lro, dl.dir, tmax, true, sh);
                     }
                 }
                 if (M->bxdf == 0) bxdf_conductor(sc, *M, dl, q, local, path);

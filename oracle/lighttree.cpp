@@ -10,6 +10,7 @@
 // (centroid[axis], light index), which is one valid nth_element outcome, so
 // topologies are comparable bit for bit.
 #include "rb_math.hpp"
+#include "rb_oracle.h"
 #include <vector>
 #include <limits>
 
@@ -71,13 +72,44 @@ static void pack_attr(PackedNode& p, const NodeAttr& a) {
     p.flux = a.flux;
 }
 
+// Unit normal of a disk light: the third column of its quaternion (x, y, z, w).
+static V3 shape_normal(const orc_shape_light& s) {
+    const float x = s.frame[0], y = s.frame[1], z = s.frame[2], w = s.frame[3];
+    return normalize(V3{2.0f * (x * z + w * y), 2.0f * (y * z - w * x), 1.0f - 2.0f * (x * x + y * y)});
+}
+
 struct Builder {
     const float* verts;  // per light: 9 floats (camera-relative world space)
+    const orc_shape_light* shapes = nullptr;   // light index ntri + i
+    uint32_t ntri = 0;
     uint32_t maxPerLeaf = 8, binCount = 32;
     TreeOut* out;
 
+    // Shape lights (no reference builder: scene.h:319-492 declares them):
+    // a disk emits like a triangle, a one-sided cosine lobe (sharpness of
+    // axis length 0.5) with the variance r^2/2 of a uniform disk; a sphere
+    // emits in every direction (sharpness 0) with variance r^2.
+    NodeAttr shape_attr(const SortData& sd) const {
+        const orc_shape_light& s = shapes[sd.light - ntri];
+        NodeAttr a{};
+        a.mean[0] = s.position[0]; a.mean[1] = s.position[1]; a.mean[2] = s.position[2];
+        if (s.type == 0u) {
+            V3 n = shape_normal(s);
+            a.dir[0] = n.x; a.dir[1] = n.y; a.dir[2] = n.z;
+            a.variance = s.radius * s.radius / 2.0f;
+            a.sharpness = vmf_axis_length_to_sharpness(0.5f);
+        } else {
+            a.dir[0] = 0.0f; a.dir[1] = 0.0f; a.dir[2] = 1.0f;
+            a.variance = s.radius * s.radius;
+            a.sharpness = 0.0f;
+        }
+        a.flux = sd.flux;
+        return a;
+    }
+
     // src/core/light_tree.cpp:14-43
     NodeAttr light_attr(const SortData& s) const {
+        if (s.light >= ntri) return shape_attr(s);
         const float* v = verts + (size_t)s.light * 9;
         V3 v0{v[0], v[1], v[2]}, v1{v[3], v[4], v[5]}, v2{v[6], v[7], v[8]};
         V3 e1 = v1 - v0, e2 = v2 - v0;
@@ -238,7 +270,7 @@ struct Builder {
         lf.coreData = (1u << 31) | ((cnt & ((1u << KLightCountBits) - 1u)) << KLightOffsetBits) | (begin & ((1u << KLightOffsetBits) - 1u));
         out->nodes[idx] = lf;
         for (uint32_t i = begin; i < end; ++i) {
-            out->lightPointers[i] = lptr_pack(LT_TRIANGLE, sd[i].light);
+            out->lightPointers[i] = lptr_pack(sd[i].light < ntri ? LT_TRIANGLE : LT_SHAPE, sd[i].light);
             out->bitmasks[sd[i].light] = mask;
         }
         return idx;
@@ -280,21 +312,53 @@ static SortData make_sort_data(const float* v, float flux, uint32_t light) {
     return s;
 }
 
-// Returns node count (>0) or a negative error.
-int light_tree_build(const float* tri_verts, const float* flux, uint32_t n, std::vector<PackedNode>& nodes,
+// Sort data of a shape light: a disk's exact box (half-extent
+// r sqrt(1 - n_k^2) on axis k, one ulp either side where that is 0, as
+// scene.h:355-369 bounds a disk) and its normal cone; a sphere's cube and the
+// whole sphere of directions.
+static SortData make_shape_sort_data(const orc_shape_light& s, float flux, uint32_t light) {
+    SortData d{};
+    V3 n = shape_normal(s);
+    const float nk[3] = {n.x, n.y, n.z};
+    for (int k = 0; k < 3; ++k) {
+        if (s.type == 0u) {
+            float span = std::max(0.0f, std::sqrt(1.0f - nk[k] * nk[k]));
+            d.bounds.mn[k] = span == 0.0f ? std::nextafter(s.position[k], -FLT_MAX) : s.position[k] - span * s.radius;
+            d.bounds.mx[k] = span == 0.0f ? std::nextafter(s.position[k], FLT_MAX) : s.position[k] + span * s.radius;
+            d.cone[k] = nk[k];
+        } else {
+            d.bounds.mn[k] = s.position[k] - s.radius;
+            d.bounds.mx[k] = s.position[k] + s.radius;
+            d.cone[k] = 0.0f;
+        }
+    }
+    d.cosCone = s.type == 0u ? 1.0f : -1.0f;
+    d.flux = flux;
+    d.light = light;
+    return d;
+}
+
+// Returns node count (>0) or a negative error. n triangle lights, m shape
+// lights (light indices n + i).
+int light_tree_build(const float* tri_verts, const float* flux, uint32_t n, const orc_shape_light* shapes,
+                     const float* shape_flux, uint32_t m, std::vector<PackedNode>& nodes,
                      std::vector<uint32_t>& lptrs, std::vector<uint64_t>& masks) {
-    if (n == 0) { nodes.clear(); lptrs.clear(); masks.clear(); return 0; }
-    if (n > KMaxLeafLightOffset + 8) return -1;
+    const uint32_t total = n + m;
+    if (total == 0) { nodes.clear(); lptrs.clear(); masks.clear(); return 0; }
+    if (total > KMaxLeafLightOffset + 8) return -1;
     TreeOut out;
-    out.lightPointers.assign(n, 0);
-    out.bitmasks.assign(n, ~0ull);
-    std::vector<SortData> sd(n);
+    out.lightPointers.assign(total, 0);
+    out.bitmasks.assign(total, ~0ull);
+    std::vector<SortData> sd(total);
     for (uint32_t i = 0; i < n; ++i) sd[i] = make_sort_data(tri_verts + (size_t)i * 9, flux[i], i);
+    for (uint32_t i = 0; i < m; ++i) sd[n + i] = make_shape_sort_data(shapes[i], shape_flux[i], n + i);
     Builder b;
     b.verts = tri_verts;
+    b.shapes = shapes;
+    b.ntri = n;
     b.out = &out;
-    b.build_node(sd, 0ull, 0u, 0u, n);
-    for (uint32_t i = 0; i < n; ++i)
+    b.build_node(sd, 0ull, 0u, 0u, total);
+    for (uint32_t i = 0; i < total; ++i)
         if (out.bitmasks[i] == ~0ull) return -2;
     b.fill_branch(0);
     nodes = std::move(out.nodes);
@@ -303,4 +367,11 @@ int light_tree_build(const float* tri_verts, const float* flux, uint32_t n, std:
     return (int)nodes.size();
 }
 
+}  // namespace orc
+
+namespace orc {
+int light_tree_build(const float* tri_verts, const float* flux, uint32_t n, std::vector<PackedNode>& nodes,
+                     std::vector<uint32_t>& lptrs, std::vector<uint64_t>& masks) {
+    return light_tree_build(tri_verts, flux, n, nullptr, nullptr, 0u, nodes, lptrs, masks);
+}
 }  // namespace orc

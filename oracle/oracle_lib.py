@@ -132,6 +132,7 @@ class OrcScene(ctypes.Structure):
         ("envChannels", ctypes.c_uint32),
         ("envTexels", ctypes.c_void_p), ("envCond", ctypes.c_void_p), ("envMarg", ctypes.c_void_p),
         ("envIntegral", ctypes.c_float),
+        ("shapes", ctypes.c_void_p), ("nshapes", ctypes.c_uint32),
     ]
 
 
@@ -154,17 +155,26 @@ class OrcStats(ctypes.Structure):
                 ("bvhSeconds", ctypes.c_double), ("renderSeconds", ctypes.c_double)]
 
 
-def light_tree_build(lights, flux):
+def light_tree_build(lights, flux, shapes=None, shape_flux=None):
+    """The oracle's tree over triangle lights and (optionally) shape lights."""
     n = len(lights)
     verts = np.ascontiguousarray(lights["vertices"], np.float32)
     flux = np.ascontiguousarray(flux, np.float32)
-    from ratbite_amd._lib import PACKED_NODE_DTYPE
-    nodes = np.zeros(max(1, 2 * n), PACKED_NODE_DTYPE)
-    lp = np.zeros(max(1, n), np.uint32)
-    bm = np.zeros(max(1, n), np.uint64)
-    rc = lib().orc_light_tree_build(p(verts), p(flux), n, p(nodes), p(lp), p(bm))
+    from ratbite_amd._lib import PACKED_NODE_DTYPE, SHAPE_LIGHT_DTYPE
+    m = 0 if shapes is None else len(shapes)
+    tot = n + m
+    nodes = np.zeros(max(1, 2 * tot), PACKED_NODE_DTYPE)
+    lp = np.zeros(max(1, tot), np.uint32)
+    bm = np.zeros(max(1, tot), np.uint64)
+    if m:
+        sh = np.ascontiguousarray(shapes, SHAPE_LIGHT_DTYPE)
+        sf = np.ascontiguousarray(shape_flux, np.float32)
+        rc = lib().orc_light_tree_build_shapes(p(verts) if n else None, p(flux) if n else None, n, p(sh), p(sf), m,
+                                               p(nodes), p(lp), p(bm))
+    else:
+        rc = lib().orc_light_tree_build(p(verts), p(flux), n, p(nodes), p(lp), p(bm))
     assert rc >= 0, rc
-    return nodes[:rc], lp[:n], bm[:n]
+    return nodes[:rc], lp[:tot], bm[:tot]
 
 
 class OracleScene:
@@ -198,8 +208,12 @@ class OracleScene:
         s.lutReflInner = keep(t["refl_inner"], np.float32)
         s.lutSheen = keep(t["sheen"], np.float32)
         s.lights, s.nlights = (keep(sc.lights), len(sc.lights)) if len(sc.lights) else (None, 0)
-        if tree is None and len(sc.lights):
-            tree = light_tree_build(sc.lights, sc.flux)
+        nshapes = getattr(sc, "nshapes", 0)
+        if nshapes:
+            s.shapes, s.nshapes = keep(sc.shapes), nshapes
+        if tree is None and len(sc.lights) + nshapes:
+            tree = light_tree_build(sc.lights, sc.flux, sc.shapes if nshapes else None,
+                                    sc.shape_flux if nshapes else None)
         self.tree = tree
         if tree is not None:
             s.nodes, s.nnodes = keep(tree[0]), len(tree[0])

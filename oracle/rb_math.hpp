@@ -366,7 +366,9 @@ inline float vmf_sharpness_to_axis_length(float s) {
 
 // ------------------------------------------------------------ light tree --
 // src/core/light_tree_types.h:10-18, 50-55, 191-207
-enum LightType : uint32_t { LT_TRIANGLE = 0, LT_SKY = 1, LT_NONE = 2 };
+// LT_SHAPE: a disk or sphere light (the free fourth value of the 2-bit type
+// field); its pointer index is its light index, after the triangle lights.
+enum LightType : uint32_t { LT_TRIANGLE = 0, LT_SKY = 1, LT_NONE = 2, LT_SHAPE = 3 };
 constexpr uint32_t KLightCountBits = 4, KLightOffsetBits = 27;
 constexpr uint32_t KMaxLeafLightCount = 15, KMaxLeafLightOffset = (1u << 27) - 1u;
 constexpr float KEnvImportance = 0.3f;

@@ -51,6 +51,16 @@ typedef struct orc_emissive_tri { /* EmissiveTriangleLightData, 68 bytes */
     uint32_t primitiveDataIndex;
 } orc_emissive_tri;
 
+typedef struct orc_shape_light { /* DiskLightData / SphereLightData, 48 bytes */
+    float position[3];
+    float powerScale;
+    float frame[4];           /* unit quaternion x, y, z, w; a disk's normal is its third column */
+    float radius;
+    uint32_t materialIndex;
+    uint32_t type;            /* 0 disk, 1 sphere */
+    uint32_t pad;
+} orc_shape_light;
+
 typedef struct orc_scene {
     const float* verts; uint32_t nverts;
     const uint32_t* indices; uint32_t ntris;
@@ -75,6 +85,7 @@ typedef struct orc_scene {
     const float* envTexels;
     const uint16_t* envCond; const uint16_t* envMarg;
     float envIntegral;
+    const orc_shape_light* shapes; uint32_t nshapes;   /* light indices nlights + i */
 } orc_scene;
 
 typedef struct orc_params {
@@ -112,6 +123,10 @@ int orc_cdf_block_size(int w);
 /* light tree */
 int orc_light_tree_build(const float* tri_verts, const float* flux, uint32_t n,
                          void* nodes /* >= 2n x 32 B */, uint32_t* light_ptrs, uint64_t* bitmasks);
+/* the same over n triangle lights and m shape lights (light indices n + i) */
+int orc_light_tree_build_shapes(const float* tri_verts, const float* flux, uint32_t n, const orc_shape_light* shapes,
+                                const float* shape_flux, uint32_t m, void* nodes /* >= 2(n+m) x 32 B */,
+                                uint32_t* light_ptrs, uint64_t* bitmasks);
 int orc_light_tree_pdf(const void* nodes, const uint32_t* lptrs, const uint64_t* masks, int env,
                        const uint32_t* lights, const float* pos_nrm /* n x 6 */, float* pdf, uint32_t n);
 int orc_light_tree_sample(const void* nodes, const uint32_t* lptrs, const uint64_t* masks, int env,

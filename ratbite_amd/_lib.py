@@ -57,6 +57,14 @@ EMISSIVE_TRI_DTYPE = np.dtype([
 ])
 assert EMISSIVE_TRI_DTYPE.itemsize == 68
 
+# rbh_shape_light: a disk (type 0) or sphere (type 1) light, 48 bytes
+SHAPE_LIGHT_DTYPE = np.dtype([
+    ("position", "<f4", (3,)), ("powerScale", "<f4"), ("frame", "<f4", (4,)), ("radius", "<f4"),
+    ("materialIndex", "<u4"), ("type", "<u4"), ("pad", "<u4"),
+])
+assert SHAPE_LIGHT_DTYPE.itemsize == 48
+SHAPE_DISK, SHAPE_SPHERE = 0, 1
+
 PACKED_NODE_DTYPE = np.dtype([
     ("spatialMean", "<f4", (3,)), ("spatialVariance", "<f4"), ("packedAverageDirection", "<u4"),
     ("sharpness", "<f4"), ("flux", "<f4"), ("coreData", "<u4"),
@@ -136,6 +144,9 @@ _SIGS = {
     "rbh_translate_scene": (c_int, [c_void_p, c_void_p]),
     "rbh_update_materials": (c_int, [c_void_p, c_uint32, c_uint32, c_void_p]),
     "rbh_get_light_tree": (c_int, [c_void_p, c_void_p, ctypes.POINTER(c_uint32), c_void_p, c_void_p]),
+    "rbh_upload_shape_lights": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32]),
+    "rbh_light_tree_build_host_shapes": (c_int, [c_void_p, c_void_p, c_uint32, c_void_p, c_void_p, c_uint32,
+                                                 c_void_p, c_void_p, c_void_p, c_uint32]),
     "rbh_envmap_build": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_uint32]),
     "rbh_envmap_rebuild": (c_int, [c_void_p, c_uint32, c_int]),
     "rbh_envmap_read": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_double)]),

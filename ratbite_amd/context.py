@@ -129,7 +129,18 @@ class Context:
         tris = np.ascontiguousarray(tris, L.EMISSIVE_TRI_DTYPE)
         n = len(tris)
         self.call("rbh_set_light_tree", L.ptr(nodes) if len(nodes) else None, len(nodes),
-                  L.ptr(pointers) if n else None, L.ptr(masks) if n else None, L.ptr(tris) if n else None, n)
+                  L.ptr(pointers) if len(pointers) else None, L.ptr(masks) if len(masks) else None,
+                  L.ptr(tris) if n else None, n)
+
+    def upload_shape_lights(self, shapes, flux):
+        """Disk and sphere lights (SHAPE_LIGHT_DTYPE) with their flux; they
+        join the light tree at its next build."""
+        shapes = np.ascontiguousarray(shapes, L.SHAPE_LIGHT_DTYPE)
+        flux = np.ascontiguousarray(flux, np.float32)
+        if len(flux) != len(shapes):
+            raise ValueError("one flux per shape light")
+        self.call("rbh_upload_shape_lights", L.ptr(shapes) if len(shapes) else None,
+                  L.ptr(flux) if len(flux) else None, len(shapes))
 
     def get_light_tree(self, nlights):
         nodes = np.zeros(max(1, 2 * nlights), L.PACKED_NODE_DTYPE)

@@ -50,12 +50,22 @@ struct EnvMap {
 };
 
 struct LightTreeDev {
-    uint32_t nnodes = 0, nlights = 0;
+    uint32_t nnodes = 0, nlights = 0;   // nlights: triangle lights (light indices [0, nlights))
+    uint32_t nshapes = 0;               // shape lights in the tree (light indices nlights + i)
+    bool stale = false;                 // the shape lights changed since the tree was built
     DevBuf nodes, lightPointers, bitmasks, tris;
     DevBuf nodesX;               // expanded nodes the kernels traverse (decoded direction, SG integral)
     std::vector<rbh_packed_node> hostNodes;
-    std::vector<uint32_t> hostPointers;
+    std::vector<uint32_t> hostPointers;   // nlights + nshapes entries
     std::vector<uint64_t> hostMasks;
+};
+
+// Disk and sphere lights (DiskLightData / SphereLightData) and their flux
+struct ShapeLights {
+    uint32_t n = 0;
+    DevBuf recs;
+    std::vector<rbh_shape_light> host;
+    std::vector<float> flux;
 };
 
 struct Geometry {
@@ -169,6 +179,7 @@ struct rbh_context {
     rbh::Geometry geom;
     rbh::Accel accel;
     rbh::LightTreeDev tree;
+    rbh::ShapeLights shapes;
     rbh::EnvMap env;
     rbh::RenderState rs;
     rbh_stats stats{};

@@ -10,7 +10,10 @@
 namespace rbh {
 namespace dev {
 
-constexpr uint32_t LT_TRIANGLE = 0, LT_SKY = 1, LT_NONE = 2;
+// Light-pointer types (light_tree_types.h:10-18, 191-207); LT_SHAPE, the free
+// fourth value of the 2-bit type field, marks a disk or sphere light whose
+// pointer index is its light index (after the triangle lights).
+constexpr uint32_t LT_TRIANGLE = 0, LT_SKY = 1, LT_NONE = 2, LT_SHAPE = 3;
 constexpr float kEnvImportance = 0.3f;
 
 __device__ __forceinline__ float sg_expm1(float x) {

@@ -97,6 +97,20 @@ struct EmissiveTri {
 };
 static_assert(sizeof(EmissiveTri) == 68, "EmissiveTri is 68 bytes");
 
+// DiskLightData / SphereLightData (src/core/light_tree_types.h:24-39) in one
+// record: type 0 disk, 1 sphere; frame is a unit quaternion (x, y, z, w)
+// whose third column is the disk's emitting normal.
+struct ShapeLight {
+    float pos[3];
+    float power;
+    float q[4];
+    float radius;
+    uint32_t material;
+    uint32_t type;
+    uint32_t pad;
+};
+static_assert(sizeof(ShapeLight) == 48, "ShapeLight is 48 bytes");
+
 struct Material {
     uint32_t bxdf, flags, attributes, attributeStride;
     uint32_t normalOffset, frameOffset, texCoord1Offset, texCoord2Offset;

@@ -13,6 +13,7 @@
 #include "guided.hpp"
 #include "lighttree.hpp"
 #include "qrng.hpp"
+#include "shapes.hpp"
 
 namespace rbh {
 namespace wf {
@@ -167,6 +168,11 @@ __device__ __forceinline__ void shade_miss(const Frame1& f, const SceneView& S, 
 // towards the viewer, outgoing direction, roughness factor, hero wavelength.
 __device__ __forceinline__ void guide_input(const Frame1& f, const SceneView& S, uint32_t pid, float* __restrict__ out) {
     const float4 hr = f.p.hit[pid], dr = f.p.dr[pid];
+    if (__float_as_uint(hr.w) & kShapeHit) {
+        // a shape-light hit ends the path: no vertex to guide
+        for (int k = 0; k < 16; ++k) out[k] = 0.0f;
+        return;
+    }
     const TriRecord& tr = S.tris[__float_as_uint(hr.w)];
     const V3 v0{tr.v0x, tr.v0y, tr.v0z}, v1{tr.v1x, tr.v1y, tr.v1z}, v2{tr.v2x, tr.v2y, tr.v2z};
     const float b1 = hr.y, b2 = hr.z, b0 = 1.0f - b1 - b2;
@@ -207,6 +213,46 @@ __device__ __forceinline__ S4 tri_emission(const SceneView& S, const EmissiveTri
         e = V3{e.x * t.x, e.y * t.y, e.z * t.z};
     }
     return ct.rgb(e, wl);
+}
+
+// Emission of a shape light: its material's emission (spectral, or RGB
+// lifted to a spectrum; no texture) times the light's power scale.
+__device__ __forceinline__ S4 shape_emission(const SceneView& S, const ShapeLight& sl, const Wl& wl,
+                                             const ColorTables& ct) {
+    const Material& m = S.materials[sl.material];
+    const S4 e = (m.flags & 4u) ? dense4(S.spectra + (size_t)m.emissionSpectrum * kDense, wl) * m.emissive[0]
+                                : ct.rgb(V3{m.emissive[0], m.emissive[1], m.emissive[2]}, wl);
+    return e * sl.power;
+}
+
+// A path whose ray hit a shape light: its emission with the MIS weight of
+// the previous vertex's BxDF sample (the light-tree and shape densities
+// below are those the NEE sampler uses), then the path ends.
+__device__ __forceinline__ void shade_shape_hit(const SceneView& S, const LightTree& tree, const ColorTables& ct,
+                                                uint32_t si, float t, V3 ro, V3 lastN, uint32_t depth, PathIO& P,
+                                                S4& L) {
+    const ShapeLight& sl = S.shapes[si];
+    const bool prevNEE = depth != 0 && !(P.flags & PF_PREV_SPECULAR);
+    float lightPdf = 0.0f;
+    bool noEm;
+    if (sl.type == SHAPE_DISK) {
+        const V3 th = P.rd * t;
+        const float d2 = th.x * th.x + th.y * th.y + th.z * th.z;
+        const float lcos = -dot(P.rd, shape_normal(sl));
+        noEm = lcos <= 0.0f;
+        lightPdf = (1.0f / (kShapePi * sl.radius * sl.radius)) * d2 / lcos;
+    } else {
+        const V3 oc = shape_center(sl) - ro;
+        noEm = !(dot(oc, oc) > sl.radius * sl.radius);   // from inside: the inner face does not emit
+        if (!noEm) lightPdf = sphere_cone_pdf(sl, ro);
+    }
+    if (!noEm) {
+        const float treePdf = prevNEE ? tree.pdf(LT_SHAPE, S.shapeBase + si, ro, lastN) : 1.0f;
+        lightPdf *= treePdf;
+        const float w = prevNEE ? power_heuristic(P.bxdfPdf, lightPdf) : 1.0f;
+        L += P.T * shape_emission(S, sl, P.wl, ct) * w;
+    }
+    P.flags |= PF_FINISHED;
 }
 
 __device__ __forceinline__ V2 attr2(const uint8_t* __restrict__ a, uint32_t stride, uint32_t off, uint32_t vi) {
@@ -253,6 +299,12 @@ __device__ __forceinline__ void shade_hit(const Frame1& f, const SceneView& S, u
     uint32_t px, py;
     slot_pixel(S, pix, px, py);
     P.wl = slot_wavelengths(S, sk, px, py);
+    if (__float_as_uint(hr.w) & kShapeHit) {
+        shade_shape_hit(S, tree, ct, __float_as_uint(hr.w) & ~kShapeHit, hr.x, ro, lastN, depth, P, L);
+        f.p.L[pid] = make_float4(L.v[0], L.v[1], L.v[2], L.v[3]);
+        f.p.nf[pid] = make_float4(lastN.x, lastN.y, lastN.z, __uint_as_float(P.flags));
+        return;
+    }
     const Qrng q = qrng_at(S.sample + sk, depth, pixel_hash(px, py));
 
     // ---- interaction (pathtrace.cu:86-149, 906-953): the hit's
@@ -504,6 +556,30 @@ __device__ __forceinline__ void shade_hit(const Frame1& f, const SceneView& S, u
                     dl.occluded = lcos <= 0.0f;
                     lightPdf = (1.0f / (nlen / 2.0f)) * s2 / lcos;
                     dl.spectrum = tri_emission(S, tri, ba, bb, bc, P.wl, ct);
+                } else if (type == LT_SHAPE) {
+                    // a disk point (area measure) or a direction in the cone a
+                    // sphere subtends (solid-angle measure)
+                    const ShapeLight& sl = S.shapes[index - S.shapeBase];
+                    V3 lp;
+                    bool inside = false;
+                    if (sl.type == SHAPE_DISK) lp = disk_sample(sl, V2{rnd.x, rnd.y});
+                    else lp = sphere_sample(sl, hitPos, V2{rnd.x, rnd.y}, lightPdf, inside);
+                    if (inside) {
+                        dl.occluded = true;
+                    } else {
+                        const bool inSample = dot(lastN, lp - hitPos) < 0.0f;
+                        lro = offset_point(inSample ? hitPos : hitInterp, inSample ? -gn : gn);
+                        const V3 rl = lp - lro;
+                        const float s2 = rl.x * rl.x + rl.y * rl.y + rl.z * rl.z;
+                        dToL = sqrtf(s2);
+                        dl.dir = rl / dToL;
+                        if (sl.type == SHAPE_DISK) {
+                            const float lcos = -dot(shape_normal(sl), dl.dir);
+                            dl.occluded = lcos <= 0.0f;
+                            lightPdf = (1.0f / (kShapePi * sl.radius * sl.radius)) * s2 / lcos;
+                        }
+                        dl.spectrum = shape_emission(S, sl, P.wl, ct);
+                    }
                 } else if (type == LT_SKY) {
                     const float W = (float)S.envW, H = (float)S.envH;
                     const float fcx = fminf(rnd.x * (W - 1.0f), W - 1.0001f);

@@ -163,17 +163,52 @@ T saoh(const Aabb& b, T flux, T cosT) {
     return flux * ac * oc;
 }
 
+// Third column of a shape light's unit quaternion (x, y, z, w): the disk's
+// emitting normal.
+template <typename T>
+void shape_normal(const ShapeLight& s, T* n) {
+    const T x = s.q[0], y = s.q[1], z = s.q[2], w = s.q[3];
+    n[0] = T(2) * (x * z + w * y);
+    n[1] = T(2) * (y * z - w * x);
+    n[2] = T(1) - T(2) * (x * x + y * y);
+    const T inv = T(1) / std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    for (int k = 0; k < 3; ++k) n[k] *= inv;
+}
+
 template <typename T>
 struct Builder {
     static constexpr bool kExact = sizeof(T) > sizeof(float);
     const EmissiveTri* tris;
+    const ShapeLight* shapes;   // light index ntri + i
+    uint32_t ntri;
     std::vector<Prim> prims;
     std::vector<PackedNode> nodes;
     std::vector<Attr<T>> leafAttrs;  // FP64 mode: unrounded leaf attributes by node index
     std::vector<uint32_t> pointers;
     std::vector<uint64_t> masks;
 
+    // Shape lights: a disk is a one-sided cosine emitter like a triangle
+    // (variance r^2/2 of a uniform disk); a sphere emits in every direction
+    // (sharpness 0; variance r^2 of a uniform sphere surface).
+    Attr<T> shape_attr(const ShapeLight& s) const {
+        Attr<T> a{};
+        for (int k = 0; k < 3; ++k) a.mean[k] = T(s.pos[k]);
+        const T r = T(s.radius);
+        if (s.type == 0u) {
+            shape_normal<T>(s, a.dir);
+            a.var = r * r / T(2);
+            a.sharp = axis_to_sharpness(T(0.5));
+        } else {
+            a.dir[0] = T(0); a.dir[1] = T(0); a.dir[2] = T(1);
+            a.var = r * r;
+            a.sharp = T(0);
+        }
+        a.flux = T(0);
+        return a;
+    }
+
     Attr<T> leaf_attr(uint32_t id) const {
+        if (id >= ntri) return shape_attr(shapes[id - ntri]);
         const float* v = tris[id].v;
         const T e1[3] = {T(v[3]) - T(v[0]), T(v[4]) - T(v[1]), T(v[5]) - T(v[2])};
         const T e2[3] = {T(v[6]) - T(v[0]), T(v[7]) - T(v[1]), T(v[8]) - T(v[2])};
@@ -303,7 +338,7 @@ struct Builder {
         const uint32_t cnt = end - begin;
         nodes[me].core = (1u << 31) | ((cnt & 15u) << 27) | (begin & ((1u << 27) - 1u));
         for (uint32_t i = begin; i < end; ++i) {
-            pointers[i] = (0u << 30) | (prims[i].id & ((1u << 30) - 1u));
+            pointers[i] = ((prims[i].id < ntri ? 0u : 3u) << 30) | (prims[i].id & ((1u << 30) - 1u));
             masks[prims[i].id] = mask;
         }
         return me;
@@ -333,12 +368,40 @@ struct Builder {
     }
 };
 
+// Sort data of a shape light: the disk's box as scene.h:355-369 gives it
+// (extent r sqrt(1 - n_k^2) along axis k, one ulp either side where that is
+// 0), the sphere's cube; the disk's normal cone, the whole sphere for a sphere.
+inline void shape_prim(const ShapeLight& s, float flux, uint32_t id, Prim& p) {
+    float n[3];
+    shape_normal<float>(s, n);
+    for (int k = 0; k < 3; ++k) {
+        if (s.type == 0u) {
+            const float span = std::max(0.0f, std::sqrt(1.0f - n[k] * n[k]));
+            p.box.lo[k] = span == 0.0f ? std::nextafter(s.pos[k], -std::numeric_limits<float>::max())
+                                       : s.pos[k] - span * s.radius;
+            p.box.hi[k] = span == 0.0f ? std::nextafter(s.pos[k], std::numeric_limits<float>::max())
+                                       : s.pos[k] + span * s.radius;
+            p.cone[k] = n[k];
+        } else {
+            p.box.lo[k] = s.pos[k] - s.radius;
+            p.box.hi[k] = s.pos[k] + s.radius;
+            p.cone[k] = 0.0f;
+        }
+    }
+    p.cosCone = s.type == 0u ? 1.0f : -1.0f;
+    p.flux = flux;
+    p.id = id;
+}
+
 template <typename T>
-int build(const EmissiveTri* tris, const float* flux, uint32_t n, std::vector<PackedNode>& nodes,
-          std::vector<uint32_t>& pointers, std::vector<uint64_t>& masks) {
+int build(const EmissiveTri* tris, const float* flux, uint32_t n, const ShapeLight* shapes, const float* shapeFlux,
+          uint32_t m, std::vector<PackedNode>& nodes, std::vector<uint32_t>& pointers, std::vector<uint64_t>& masks) {
     Builder<T> b;
     b.tris = tris;
-    b.prims.resize(n);
+    b.shapes = shapes;
+    b.ntri = n;
+    b.prims.resize(n + m);
+    for (uint32_t i = 0; i < m; ++i) shape_prim(shapes[i], shapeFlux[i], n + i, b.prims[n + i]);
     for (uint32_t i = 0; i < n; ++i) {
         const float* v = tris[i].v;
         Prim& p = b.prims[i];
@@ -359,11 +422,12 @@ int build(const EmissiveTri* tris, const float* flux, uint32_t n, std::vector<Pa
         p.flux = flux[i];
         p.id = i;
     }
-    b.pointers.assign(n, 0u);
-    b.masks.assign(n, ~0ull);
-    b.nodes.reserve(2 * (size_t)n);
-    b.node(0ull, 0u, 0u, n);
-    for (uint32_t i = 0; i < n; ++i)
+    const uint32_t total = n + m;
+    b.pointers.assign(total, 0u);
+    b.masks.assign(total, ~0ull);
+    b.nodes.reserve(2 * (size_t)total);
+    b.node(0ull, 0u, 0u, total);
+    for (uint32_t i = 0; i < total; ++i)
         if (b.masks[i] == ~0ull) return -2;
     b.fill(0);
     nodes = std::move(b.nodes);
@@ -374,17 +438,19 @@ int build(const EmissiveTri* tris, const float* flux, uint32_t n, std::vector<Pa
 
 }  // namespace
 
-// Returns the node count, 0 for no lights, or a negative error. fp64 selects
-// the FP64 power mode.
-int build_light_tree(const EmissiveTri* tris, const float* flux, uint32_t n, std::vector<PackedNode>& nodes,
+// Returns the node count, 0 for no lights, or a negative error. n triangle
+// lights (light indices [0, n)) and m shape lights (light indices n + i);
+// fp64 selects the FP64 power mode.
+int build_light_tree(const EmissiveTri* tris, const float* flux, uint32_t n, const ShapeLight* shapes,
+                     const float* shapeFlux, uint32_t m, std::vector<PackedNode>& nodes,
                      std::vector<uint32_t>& pointers, std::vector<uint64_t>& masks, bool fp64) {
     nodes.clear();
     pointers.clear();
     masks.clear();
-    if (n == 0) return 0;
-    if (n > (1u << 27) - 1u + kMaxPerLeaf) return -1;
-    return fp64 ? build<double>(tris, flux, n, nodes, pointers, masks)
-                : build<float>(tris, flux, n, nodes, pointers, masks);
+    if (n + (uint64_t)m == 0) return 0;
+    if (n + (uint64_t)m > (1u << 27) - 1u + kMaxPerLeaf) return -1;
+    return fp64 ? build<double>(tris, flux, n, shapes, shapeFlux, m, nodes, pointers, masks)
+                : build<float>(tris, flux, n, shapes, shapeFlux, m, nodes, pointers, masks);
 }
 
 }  // namespace host

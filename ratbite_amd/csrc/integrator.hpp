@@ -76,6 +76,9 @@ struct SceneView {
     const uint32_t* treePointers;
     const uint64_t* treeMasks;
     const EmissiveTri* lights;
+    const ShapeLight* shapes;          // disk and sphere lights
+    uint32_t nshapes;
+    uint32_t shapeBase;                // light index of shapes[0] (the triangle-light count)
     uint32_t envOn;
     uint32_t envW, envH, envC;
     const float* envTex;

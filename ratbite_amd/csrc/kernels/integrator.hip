@@ -207,6 +207,8 @@ __global__ __launch_bounds__(kTraceBlock, WAVES) void k_trace(Frame1 f, uint32_t
                 tris = S.tris;
                 const float4 od = f.p.od[pid], dr = f.p.dr[pid];
                 trav_init(ray, V3{od.x, od.y, od.z}, V3{dr.x, dr.y, dr.z}, 3.402823466e+38f, S.nnodes4);
+                // the nearest shape light bounds the walk; a nearer triangle replaces it
+                if (S.nshapes) shape_closest(S.shapes, S.nshapes, ray.o, ray.d, 3.402823466e+38f, ray.h.t, ray.h.leaf);
                 active = true;
                 ++nrays;
             }
@@ -231,13 +233,18 @@ __global__ __launch_bounds__(kTraceBlock, WAVES) void k_trace(Frame1 f, uint32_t
                 pid = identity ? i : queue[i];
                 const SceneView& S = scene_of<BATCH>(f, pid);
                 const float4 od = f.p.od[pid], dr = f.p.dr[pid];
-                const Hit h = traverse4<false>(scene_nodes<QN>(S), S.tris, S.nnodes4, V3{od.x, od.y, od.z},
-                                               V3{dr.x, dr.y, dr.z}, 3.402823466e+38f, st);
+                const V3 o{od.x, od.y, od.z}, d{dr.x, dr.y, dr.z};
+                // the nearest shape light bounds the walk; a nearer triangle replaces it
+                float ts = 3.402823466e+38f;
+                uint32_t shape = 0xFFFFFFFFu;
+                if (S.nshapes) shape_closest(S.shapes, S.nshapes, o, d, ts, ts, shape);
+                Hit h = traverse4<false>(scene_nodes<QN>(S), S.tris, S.nnodes4, o, d, ts, st);
+                if (h.leaf == 0xFFFFFFFFu && shape != 0xFFFFFFFFu) h = Hit{ts, 0.0f, 0.0f, shape};
                 hit = h.leaf != 0xFFFFFFFFu;
                 if (hit) {
                     f.p.hit[pid] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.leaf));
                     // material-sorted shading: conductor hits form their own run of the queue
-                    if (f.sortHits) back = S.materials[S.tris[h.leaf].material].bxdf == 0u;
+                    if (f.sortHits && !(h.leaf & kShapeHit)) back = S.materials[S.tris[h.leaf].material].bxdf == 0u;
                 }
                 ++nrays;
             }
@@ -302,8 +309,12 @@ __global__ __launch_bounds__(kTraceBlock, WAVES) void k_shadow(Frame1 f) {
                 nodes = S.nodes4;
                 tris = S.tris;
                 const float4 so = f.p.sro[pid], sd = f.p.srd[pid];
-                trav_init(ray, V3{so.x, so.y, so.z}, V3{sd.x, sd.y, sd.z}, so.w, S.nnodes4);
-                active = true;
+                const V3 o{so.x, so.y, so.z}, d{sd.x, sd.y, sd.z};
+                // a shape light in the way settles the ray without a walk
+                if (!(S.nshapes && shape_any(S.shapes, S.nshapes, o, d, so.w))) {
+                    trav_init(ray, o, d, so.w, S.nnodes4);
+                    active = true;
+                }
                 ++nrays;
             }
             if (chunk.done(__ballot(active) != 0ull)) break;
@@ -318,9 +329,11 @@ __global__ __launch_bounds__(kTraceBlock, WAVES) void k_shadow(Frame1 f) {
                 const uint32_t pid = f.p.sq[i];
                 const SceneView& S = scene_of<BATCH>(f, pid);
                 const float4 so = f.p.sro[pid], sd = f.p.srd[pid];
-                const Hit h = traverse4<true>(scene_nodes<QN>(S), S.tris, S.nnodes4, V3{so.x, so.y, so.z},
-                                              V3{sd.x, sd.y, sd.z}, so.w, st);
-                if (h.leaf == 0xFFFFFFFFu) add_nee(f, pid);
+                const V3 o{so.x, so.y, so.z}, d{sd.x, sd.y, sd.z};
+                if (!(S.nshapes && shape_any(S.shapes, S.nshapes, o, d, so.w))) {
+                    const Hit h = traverse4<true>(scene_nodes<QN>(S), S.tris, S.nnodes4, o, d, so.w, st);
+                    if (h.leaf == 0xFFFFFFFFu) add_nee(f, pid);
+                }
                 ++nrays;
             }
         }

@@ -18,7 +18,8 @@ hipError_t resolve_launch(rbh_ctx c, const float* colorspace, float expo, void* 
 size_t trace_spill_words(size_t lanes);
 hipError_t light_tree_expand(rbh_ctx c);
 namespace host {
-int build_light_tree(const EmissiveTri* tris, const float* flux, uint32_t n, std::vector<PackedNode>& nodes,
+int build_light_tree(const EmissiveTri* tris, const float* flux, uint32_t n, const ShapeLight* shapes,
+                     const float* shapeFlux, uint32_t m, std::vector<PackedNode>& nodes,
                      std::vector<uint32_t>& pointers, std::vector<uint64_t>& masks, bool fp64);
 }
 }  // namespace rbh
@@ -101,9 +102,18 @@ int rbh_translate_scene(rbh_ctx c, const float* delta) {
     Geometry& g = c->geom;
     RBH_HIP(c, translate_points(c, g.verts.as<float>(), g.nverts, 3, delta));
     LightTreeDev& t = c->tree;
+    ShapeLights& sh = c->shapes;
+    if (sh.n) {
+        for (auto& r : sh.host)
+            for (int a = 0; a < 3; ++a) r.position[a] -= delta[a];
+        RBH_HIP(c, hipMemcpyAsync(sh.recs.p, sh.host.data(), (size_t)sh.n * sizeof(rbh_shape_light),
+                                  hipMemcpyHostToDevice, c->stream));
+    }
     if (t.nlights) {
         // emitter corners (9 floats at the start of each 68-byte record)
         RBH_HIP(c, translate_points(c, t.tris.as<float>(), (size_t)t.nlights * 3, 0, delta));
+    }
+    if (t.nnodes) {
         for (auto& nd : t.hostNodes)
             for (int a = 0; a < 3; ++a) nd.spatialMean[a] -= delta[a];
         RBH_HIP(c, hipMemcpyAsync(t.nodes.p, t.hostNodes.data(), t.hostNodes.size() * sizeof(rbh_packed_node),
@@ -162,7 +172,8 @@ namespace {
 // pointer array, every light's bitmask path from the root ending in the leaf
 // that holds it (light_tree_types.h:119-206, light_tree.cpp:71-73).
 int check_tree(rbh_ctx c, const PackedNode* nodes, uint32_t nnodes, const uint32_t* ptrs, const uint64_t* masks,
-               uint32_t n) {
+               uint32_t ntri, uint32_t nshape) {
+    const uint32_t n = ntri + nshape;
     if (nnodes == 0 || nnodes > 2 * (uint64_t)n) return fail(c, RBH_ERR_INVALID, "%u nodes for %u lights", nnodes, n);
     for (uint32_t i = 0; i < nnodes; ++i) {
         const uint32_t core = nodes[i].core;
@@ -177,9 +188,13 @@ int check_tree(rbh_ctx c, const PackedNode* nodes, uint32_t nnodes, const uint32
     }
     std::vector<uint8_t> seen(n, 0);
     for (uint32_t k = 0; k < n; ++k) {
-        if ((ptrs[k] >> 30) != 0u || (ptrs[k] & ((1u << 30) - 1u)) >= n)
-            return fail(c, RBH_ERR_INVALID, "light pointer %u (0x%08x) is not a triangle light below %u", k, ptrs[k], n);
-        if (seen[ptrs[k]]++) return fail(c, RBH_ERR_INVALID, "light %u appears twice in the pointer array", ptrs[k]);
+        const uint32_t type = ptrs[k] >> 30, idx = ptrs[k] & ((1u << 30) - 1u);
+        const bool okTri = type == 0u && idx < ntri, okShape = type == 3u && idx >= ntri && idx < n;
+        if (!okTri && !okShape)
+            return fail(c, RBH_ERR_INVALID,
+                        "light pointer %u (0x%08x) is neither a triangle light below %u nor a shape light in [%u, %u)",
+                        k, ptrs[k], ntri, ntri, n);
+        if (seen[idx]++) return fail(c, RBH_ERR_INVALID, "light %u appears twice in the pointer array", idx);
     }
     for (uint32_t l = 0; l < n; ++l) {
         uint32_t cur = 0;
@@ -188,34 +203,68 @@ int check_tree(rbh_ctx c, const PackedNode* nodes, uint32_t nnodes, const uint32
         if (!(core >> 31)) return fail(c, RBH_ERR_INVALID, "light %u: its bitmask does not reach a leaf", l);
         const uint32_t cnt = (core >> 27) & 15u, off = core & ((1u << 27) - 1u);
         bool found = false;
-        for (uint32_t k = off; k < off + cnt; ++k) found = found || ptrs[k] == l;
+        for (uint32_t k = off; k < off + cnt; ++k) found = found || (ptrs[k] & ((1u << 30) - 1u)) == l;
         if (!found) return fail(c, RBH_ERR_INVALID, "light %u: its bitmask ends in leaf %u, which does not hold it", l, cur);
     }
     return RBH_OK;
 }
 
 // Uploads a checked tree with its triangle records and expands the nodes.
+// n triangle records; pointers and masks of the n triangle and the context's
+// shape lights.
 int upload_tree(rbh_ctx c, const PackedNode* nodes, uint32_t nnodes, const uint32_t* ptrs, const uint64_t* masks,
                 const rbh_emissive_tri* tris, uint32_t n) {
     LightTreeDev& t = c->tree;
+    const uint32_t m = nnodes ? c->shapes.n : 0u, total = n + m;
     t.hostNodes.resize(nnodes);
     if (nnodes) std::memcpy(t.hostNodes.data(), nodes, (size_t)nnodes * sizeof(PackedNode));
-    t.hostPointers.assign(ptrs, ptrs + n);
-    t.hostMasks.assign(masks, masks + n);
+    t.hostPointers.assign(ptrs, ptrs + total);
+    t.hostMasks.assign(masks, masks + total);
     t.nnodes = nnodes;
     t.nlights = n;
-    if (n == 0) return RBH_OK;
+    t.nshapes = m;
+    t.stale = false;
+    if (total == 0) return RBH_OK;
     RBH_ALLOC(c, t.nodes, (size_t)nnodes * sizeof(PackedNode));
-    RBH_ALLOC(c, t.lightPointers, (size_t)n * 4);
-    RBH_ALLOC(c, t.bitmasks, (size_t)n * 8);
-    RBH_ALLOC(c, t.tris, (size_t)n * sizeof(EmissiveTri));
+    RBH_ALLOC(c, t.lightPointers, (size_t)total * 4);
+    RBH_ALLOC(c, t.bitmasks, (size_t)total * 8);
     RBH_HIP(c, hipMemcpyAsync(t.nodes.p, nodes, (size_t)nnodes * sizeof(PackedNode), hipMemcpyHostToDevice, c->stream));
-    RBH_HIP(c, hipMemcpyAsync(t.lightPointers.p, ptrs, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
-    RBH_HIP(c, hipMemcpyAsync(t.bitmasks.p, masks, (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
-    RBH_HIP(c, hipMemcpyAsync(t.tris.p, tris, (size_t)n * sizeof(EmissiveTri), hipMemcpyHostToDevice, c->stream));
+    RBH_HIP(c, hipMemcpyAsync(t.lightPointers.p, ptrs, (size_t)total * 4, hipMemcpyHostToDevice, c->stream));
+    RBH_HIP(c, hipMemcpyAsync(t.bitmasks.p, masks, (size_t)total * 8, hipMemcpyHostToDevice, c->stream));
+    if (n) {
+        RBH_ALLOC(c, t.tris, (size_t)n * sizeof(EmissiveTri));
+        RBH_HIP(c, hipMemcpyAsync(t.tris.p, tris, (size_t)n * sizeof(EmissiveTri), hipMemcpyHostToDevice, c->stream));
+    }
     RBH_ALLOC(c, t.nodesX, (size_t)nnodes * 48);
     RBH_HIP(c, light_tree_expand(c));
     RBH_HIP(c, hipStreamSynchronize(c->stream));
+    return RBH_OK;
+}
+
+constexpr uint32_t kMaxShapeLights = 4096;
+
+// Shape-light records and flux; c may be null (host-only builds: no message,
+// no material check when nmaterials is ~0).
+int check_shapes(rbh_ctx c, const rbh_shape_light* s, const float* flux, uint32_t m, uint32_t nmaterials) {
+    auto bad = [&](const char* what, uint32_t i) {
+        return c ? fail(c, RBH_ERR_INVALID, "shape light %u: %s", i, what) : RBH_ERR_INVALID;
+    };
+    for (uint32_t i = 0; i < m; ++i) {
+        const rbh_shape_light& x = s[i];
+        if (x.type > 1u) return bad("type must be 0 (disk) or 1 (sphere)", i);
+        if (!(x.radius > 0.0f) || !std::isfinite(x.radius)) return bad("radius must be positive and finite", i);
+        if (!(flux[i] > 0.0f) || !std::isfinite(flux[i])) return bad("flux must be positive and finite", i);
+        if (!std::isfinite(x.powerScale)) return bad("power scale must be finite", i);
+        double q2 = 0.0;
+        for (int k = 0; k < 4; ++k) {
+            if (!std::isfinite(x.frame[k])) return bad("frame must be finite", i);
+            q2 += (double)x.frame[k] * x.frame[k];
+        }
+        if (std::fabs(std::sqrt(q2) - 1.0) > 1e-3) return bad("frame must be a unit quaternion", i);
+        for (int k = 0; k < 3; ++k)
+            if (!std::isfinite(x.position[k])) return bad("position must be finite", i);
+        if (nmaterials != ~0u && x.materialIndex >= nmaterials) return bad("material index out of range", i);
+    }
     return RBH_OK;
 }
 
@@ -242,8 +291,10 @@ int rbh_build_light_tree_ex(rbh_ctx c, const rbh_emissive_tri* tris, const float
     std::vector<PackedNode> nodes;
     std::vector<uint32_t> ptrs;
     std::vector<uint64_t> masks;
-    r = host::build_light_tree(et, flux, n, nodes, ptrs, masks, (flags & RBH_LIGHT_TREE_FP64) != 0);
-    if (r == -1) return fail(c, RBH_ERR_RANGE, "light count %u exceeds the tree's 27-bit leaf offsets", n);
+    const ShapeLights& sh = c->shapes;
+    r = host::build_light_tree(et, flux, n, reinterpret_cast<const ShapeLight*>(sh.host.data()), sh.flux.data(), sh.n,
+                               nodes, ptrs, masks, (flags & RBH_LIGHT_TREE_FP64) != 0);
+    if (r == -1) return fail(c, RBH_ERR_RANGE, "light count %u exceeds the tree's 27-bit leaf offsets", n + sh.n);
     if (r < 0) return fail(c, RBH_ERR_INVALID, "light tree produced an invalid bitmask");
     return upload_tree(c, nodes.data(), (uint32_t)nodes.size(), ptrs.data(), masks.data(), tris, n);
 }
@@ -251,13 +302,15 @@ int rbh_build_light_tree_ex(rbh_ctx c, const rbh_emissive_tri* tris, const float
 int rbh_set_light_tree(rbh_ctx c, const rbh_packed_node* nodes, uint32_t nnodes, const uint32_t* lightPointers,
                        const uint64_t* bitmasks, const rbh_emissive_tri* tris, uint32_t n) {
     if (!c) return RBH_ERR_INVALID;
-    if (n == 0) return upload_tree(c, nullptr, 0, nullptr, nullptr, nullptr, 0);
-    if (!nodes || !lightPointers || !bitmasks || !tris) return fail(c, RBH_ERR_INVALID, "light-tree arrays are null");
-    if (n > (1u << 27)) return fail(c, RBH_ERR_RANGE, "light count %u exceeds the tree's 27-bit leaf offsets", n);
+    const uint32_t m = c->shapes.n;
+    if (n == 0 && m == 0) return upload_tree(c, nullptr, 0, nullptr, nullptr, nullptr, 0);
+    if (!nodes || !lightPointers || !bitmasks || (n && !tris))
+        return fail(c, RBH_ERR_INVALID, "light-tree arrays are null");
+    if (n + (uint64_t)m > (1u << 27)) return fail(c, RBH_ERR_RANGE, "light count %u exceeds the tree's 27-bit leaf offsets", n + m);
     const PackedNode* pn = reinterpret_cast<const PackedNode*>(nodes);
     int r = check_light_records(c, tris, n);
     if (r) return r;
-    r = check_tree(c, pn, nnodes, lightPointers, bitmasks, n);
+    r = check_tree(c, pn, nnodes, lightPointers, bitmasks, n, m);
     if (r) return r;
     return upload_tree(c, pn, nnodes, lightPointers, bitmasks, tris, n);
 }
@@ -274,8 +327,8 @@ int rbh_light_tree_build_host_ex(const rbh_emissive_tri* tris, const float* flux
     std::vector<PackedNode> nv;
     std::vector<uint32_t> lp;
     std::vector<uint64_t> bm;
-    const int r = host::build_light_tree(reinterpret_cast<const EmissiveTri*>(tris), flux, n, nv, lp, bm,
-                                         (flags & RBH_LIGHT_TREE_FP64) != 0);
+    const int r = host::build_light_tree(reinterpret_cast<const EmissiveTri*>(tris), flux, n, nullptr, nullptr, 0u, nv,
+                                         lp, bm, (flags & RBH_LIGHT_TREE_FP64) != 0);
     if (r == -1) return RBH_ERR_RANGE;
     if (r < 0) return RBH_ERR_INVALID;
     if (r) {
@@ -289,10 +342,57 @@ int rbh_light_tree_build_host_ex(const rbh_emissive_tri* tris, const float* flux
 int rbh_get_light_tree(rbh_ctx c, rbh_packed_node* nodes, uint32_t* nnodes, uint32_t* lightPointers, uint64_t* bitmasks) {
     if (!c) return RBH_ERR_INVALID;
     const LightTreeDev& t = c->tree;
+    const size_t total = t.hostPointers.size();
     if (nnodes) *nnodes = t.nnodes;
     if (nodes && t.nnodes) std::memcpy(nodes, t.hostNodes.data(), (size_t)t.nnodes * sizeof(PackedNode));
-    if (lightPointers && t.nlights) std::memcpy(lightPointers, t.hostPointers.data(), (size_t)t.nlights * 4);
-    if (bitmasks && t.nlights) std::memcpy(bitmasks, t.hostMasks.data(), (size_t)t.nlights * 8);
+    if (lightPointers && total) std::memcpy(lightPointers, t.hostPointers.data(), total * 4);
+    if (bitmasks && total) std::memcpy(bitmasks, t.hostMasks.data(), total * 8);
+    return RBH_OK;
+}
+
+int rbh_light_tree_build_host_shapes(const rbh_emissive_tri* tris, const float* flux, uint32_t n,
+                                     const rbh_shape_light* shapes, const float* shapeFlux, uint32_t m,
+                                     rbh_packed_node* nodes, uint32_t* lightPointers, uint64_t* bitmasks,
+                                     uint32_t flags) {
+    if ((n && (!tris || !flux)) || (m && (!shapes || !shapeFlux)) || ((n || m) && (!nodes || !lightPointers || !bitmasks)))
+        return RBH_ERR_INVALID;
+    if (flags & ~(uint32_t)RBH_LIGHT_TREE_FP64) return RBH_ERR_INVALID;
+    if (m > kMaxShapeLights) return RBH_ERR_RANGE;
+    if (check_shapes(nullptr, shapes, shapeFlux, m, ~0u) != RBH_OK) return RBH_ERR_INVALID;
+    std::vector<PackedNode> nv;
+    std::vector<uint32_t> lp;
+    std::vector<uint64_t> bm;
+    const int r = host::build_light_tree(reinterpret_cast<const EmissiveTri*>(tris), flux, n,
+                                         reinterpret_cast<const ShapeLight*>(shapes), shapeFlux, m, nv, lp, bm,
+                                         (flags & RBH_LIGHT_TREE_FP64) != 0);
+    if (r == -1) return RBH_ERR_RANGE;
+    if (r < 0) return RBH_ERR_INVALID;
+    if (r) {
+        std::memcpy(nodes, nv.data(), nv.size() * sizeof(PackedNode));
+        std::memcpy(lightPointers, lp.data(), lp.size() * 4);
+        std::memcpy(bitmasks, bm.data(), bm.size() * 8);
+    }
+    return r;
+}
+
+int rbh_upload_shape_lights(rbh_ctx c, const rbh_shape_light* lights, const float* flux, uint32_t m) {
+    if (!c) return RBH_ERR_INVALID;
+    if (m && (!lights || !flux)) return fail(c, RBH_ERR_INVALID, "shape-light arrays are null");
+    if (m > kMaxShapeLights) return fail(c, RBH_ERR_RANGE, "%u shape lights (at most %u)", m, kMaxShapeLights);
+    const int r = check_shapes(c, lights, flux, m, c->nmaterials);
+    if (r) return r;
+    ShapeLights& sh = c->shapes;
+    sh.host.assign(lights, lights + m);
+    sh.flux.assign(flux, flux + m);
+    sh.n = m;
+    if (m) {
+        RBH_ALLOC(c, sh.recs, (size_t)m * sizeof(rbh_shape_light));
+        RBH_HIP(c, hipMemcpyAsync(sh.recs.p, lights, (size_t)m * sizeof(rbh_shape_light), hipMemcpyHostToDevice,
+                                  c->stream));
+        RBH_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    // the tree holds the previous set until it is rebuilt
+    c->tree.stale = c->tree.nshapes != m || m != 0;
     return RBH_OK;
 }
 
@@ -309,6 +409,8 @@ int check_scene(rbh_ctx c, const rbh_launch_params* lp) {
     if (lp->width == 0 || lp->height == 0) return fail(c, RBH_ERR_INVALID, "empty film");
     if ((uint64_t)lp->width * lp->height > (1ull << 28)) return fail(c, RBH_ERR_RANGE, "film above 2^28 pixels");
     if (lp->maxPathDepth == 0 || lp->maxPathDepth > 1024) return fail(c, RBH_ERR_INVALID, "maxPathDepth must be in [1, 1024]");
+    if (c->tree.stale || c->tree.nshapes != c->shapes.n)
+        return fail(c, RBH_ERR_STATE, "shape lights changed since the light tree was built: rebuild the light tree");
     if (c->geom.maxLight >= (int64_t)c->tree.nlights)
         return fail(c, RBH_ERR_STATE, "triangle light index %lld has no light record (%u lights built)",
                     (long long)c->geom.maxLight, c->tree.nlights);
@@ -332,7 +434,7 @@ int prepare_accum(rbh_ctx c, const rbh_launch_params* lp) {
 // Scene-level part of a pass view (the pass fields sample, S, slot0, first
 // are set per pass).
 void fill_view(rbh_ctx c, const rbh_launch_params* lp, SceneView& v) {
-    const bool treeOn = c->tree.nlights > 0;
+    const bool treeOn = c->tree.nnodes > 0;
     v = SceneView{};
     v.P = lp->width * lp->height;
     v.W = lp->width;
@@ -371,7 +473,10 @@ void fill_view(rbh_ctx c, const rbh_launch_params* lp, SceneView& v) {
     v.treeNodes = treeOn ? c->tree.nodesX.as<dev::TreeNodeX>() : nullptr;
     v.treePointers = treeOn ? c->tree.lightPointers.as<uint32_t>() : nullptr;
     v.treeMasks = treeOn ? c->tree.bitmasks.as<uint64_t>() : nullptr;
-    v.lights = treeOn ? c->tree.tris.as<EmissiveTri>() : nullptr;
+    v.lights = treeOn && c->tree.nlights ? c->tree.tris.as<EmissiveTri>() : nullptr;
+    v.shapes = c->shapes.n ? c->shapes.recs.as<ShapeLight>() : nullptr;
+    v.nshapes = c->shapes.n;
+    v.shapeBase = c->tree.nlights;
     v.envOn = c->env.enabled ? 1u : 0u;
     if (c->env.enabled) {
         v.envW = (uint32_t)c->env.width;

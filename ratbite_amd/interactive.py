@@ -3,7 +3,11 @@ RenderingInterface runs it (src/core/rendering_interface.cpp:1534-1879,
 src/core/command.h):
 
   * a command buffer (a stack: the last command pushed is processed first,
-    as CommandBuffer::pullCommand pops) of the reference's command types;
+    as CommandBuffer::pullCommand pops) of the reference's command types,
+    with the reference's enum values (src/core/command.h:9-43); the light
+    commands act on the disk and sphere lights (whose handling the reference
+    has commented out, rendering_interface.cpp:1667-1740): each re-uploads
+    the shape lights and rebuilds the light tree;
     processing sets restart / re-resolve / rebuild flags, then applies the
     rebuilds once: the light tree (full rebuild), the light records and
     light-tree spatial data after a camera move, the acceleration structure
@@ -41,7 +45,13 @@ class Cmd(enum.IntEnum):
     CHANGE_CAMERA_FIELD_OF_VIEW = 9
     ADD_ENVIRONMENT_MAP = 10
     REMOVE_ENVIRONMENT_MAP = 11
-    CHANGE_LIGHT_POWER = 15
+    ADD_LIGHT = 12
+    CHANGE_LIGHT_POSITION = 13
+    CHANGE_LIGHT_SIZE = 14
+    CHANGE_LIGHT_ORIENTATION = 15
+    CHANGE_LIGHT_POWER = 16
+    CHANGE_LIGHT_EMISSION_SPECTRUM = 17
+    REMOVE_LIGHT = 18
     CHANGE_MODEL_MATERIAL = 20
     CHANGE_MODEL_TRANSFORM = 21
 
@@ -101,7 +111,7 @@ class RenderSession:
     def process_commands(self):
         """Pops every command (last in, first out), then applies the
         rebuilds the commands asked for once, in the reference's order."""
-        restart = reresolve = rebuild_tree = refit = translate = False
+        restart = reresolve = rebuild_tree = refit = translate = reshape = False
         delta = np.zeros(3)
         while self.commands:
             cmd, payload = self.commands.pop()
@@ -148,9 +158,51 @@ class RenderSession:
                 self.r.ctx.envmap_disable()
                 restart = True
             elif cmd == Cmd.CHANGE_LIGHT_POWER:
-                # payload: new flux per emissive triangle
-                self.scene.flux = np.ascontiguousarray(payload, np.float32)
+                if isinstance(payload, tuple):
+                    # (shape light, new power scale)
+                    i, power = payload
+                    self._shapes()[int(i)]["powerScale"] = float(power)
+                    reshape = True
+                else:
+                    # new flux per emissive triangle
+                    self.scene.flux = np.ascontiguousarray(payload, np.float32)
                 rebuild_tree = restart = True
+            elif cmd == Cmd.ADD_LIGHT:
+                # payload: a shape-light record (camera-relative)
+                rec = np.asarray(payload, L.SHAPE_LIGHT_DTYPE).reshape(1)
+                sh = self._shapes()
+                self.scene.shapes = np.concatenate([sh, rec]) if sh is not None else rec.copy()
+                reshape = rebuild_tree = restart = True
+            elif cmd == Cmd.REMOVE_LIGHT:
+                self.scene.shapes = np.delete(self._shapes(), int(payload))
+                reshape = rebuild_tree = restart = True
+            elif cmd == Cmd.CHANGE_LIGHT_POSITION:
+                # (shape light, new camera-relative centre)
+                i, pos = payload
+                self._shapes()[int(i)]["position"] = np.asarray(pos, np.float32)
+                reshape = rebuild_tree = restart = True
+            elif cmd == Cmd.CHANGE_LIGHT_SIZE:
+                i, radius = payload
+                self._shapes()[int(i)]["radius"] = float(radius)
+                reshape = rebuild_tree = restart = True
+            elif cmd == Cmd.CHANGE_LIGHT_ORIENTATION:
+                # (disk light, new normal): the disk frame of scene.h:386-392
+                from .scenes import disk_light
+                i, normal = payload
+                r = self._shapes()[int(i)]
+                r["frame"] = disk_light(r["position"], normal, r["radius"], r["powerScale"],
+                                        r["materialIndex"])["frame"]
+                reshape = rebuild_tree = restart = True
+            elif cmd == Cmd.CHANGE_LIGHT_EMISSION_SPECTRUM:
+                # (shape light, spectrum index): its material emits that spectrum
+                i, spec = payload
+                mi = int(self._shapes()[int(i)]["materialIndex"])
+                m = self.scene.materials[mi:mi + 1].copy()
+                m["flags"] |= 4
+                m["emissionSpectrum"] = int(spec)
+                self.scene.materials[mi:mi + 1] = m
+                self.r.ctx.update_materials(mi, m)
+                reshape = rebuild_tree = restart = True
             elif cmd == Cmd.CHANGE_MODEL_MATERIAL:
                 first, mats = payload
                 self.scene.materials[first:first + len(mats)] = mats
@@ -173,6 +225,15 @@ class RenderSession:
                 refit = restart = True
             else:
                 raise ValueError("unknown command %r" % (cmd,))
+        if reshape:
+            from .scenes import shape_flux
+            sh = self._shapes()
+            if sh is None or len(sh) == 0:
+                self.scene.shapes = self.scene.shape_flux = None
+                self.r.ctx.upload_shape_lights(np.zeros(0, L.SHAPE_LIGHT_DTYPE), np.zeros(0, np.float32))
+            else:
+                self.scene.shape_flux = shape_flux(sh, self.scene.materials)
+                self.r.ctx.upload_shape_lights(sh, self.scene.shape_flux)
         if rebuild_tree:
             self.r.ctx.build_light_tree(self.scene.lights, self.scene.flux)
         if translate:
@@ -180,6 +241,8 @@ class RenderSession:
             self.r.ctx.translate_scene(d)                     # light records, tree means, vertices + refit
             self.scene.verts = (self.scene.verts - d).astype(np.float32)
             self.scene.lights["vertices"] = (self.scene.lights["vertices"].reshape(-1, 3, 3) - d).reshape(-1, 9)
+            if self.scene.nshapes:
+                self.scene.shapes["position"] -= d
         elif refit:
             self.r.ctx.refit_accel()
         if restart:
@@ -187,6 +250,9 @@ class RenderSession:
         elif reresolve and self.processed:
             self.image = self.resolve()
         return restart
+
+    def _shapes(self):
+        return self.scene.shapes
 
     # ------------------------------------------------------------ rendering
     def launch_params(self):

@@ -28,6 +28,11 @@ class Renderer:
         c.upload_materials(scene.materials)
         c.upload_geometry(scene.verts, scene.indices, scene.attribs, scene.tri_material, scene.tri_light)
         c.build_accel()
+        # (an empty set too: a reused context drops the previous scene's shape lights)
+        if scene.nshapes:
+            c.upload_shape_lights(scene.shapes, scene.shape_flux)
+        else:
+            c.upload_shape_lights(np.zeros(0, L.SHAPE_LIGHT_DTYPE), np.zeros(0, np.float32))
         c.build_light_tree(scene.lights, scene.flux, fp64=fp64_light_tree)
         if scene.env is not None:
             c.envmap_build(scene.env, fp64=fp64_cdf)
@@ -36,7 +41,7 @@ class Renderer:
         self.samples = 0
 
     def light_tree(self):
-        return self.ctx.get_light_tree(len(self.scene.lights))
+        return self.ctx.get_light_tree(len(self.scene.lights) + self.scene.nshapes)
 
     def render(self, spp=None, offset=None, guided=False, collect=False, no_drop=False, flags=0, fresh=False, **kw):
         """Renders spp samples starting at Sobol index `offset`. A launch with

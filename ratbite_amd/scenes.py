@@ -20,7 +20,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._lib import EMISSIVE_TRI_DTYPE, MATERIAL_DTYPE, TEXTURE_DTYPE, LaunchParams
+from ._lib import EMISSIVE_TRI_DTYPE, MATERIAL_DTYPE, SHAPE_LIGHT_DTYPE, TEXTURE_DTYPE, LaunchParams
 from .envmaps import lognormal_envmap
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -63,6 +63,12 @@ class Scene:
     spp: int = 16
     tables: dict = field(default_factory=dict)
     name: str = ""
+    shapes: np.ndarray = None       # disk and sphere lights (SHAPE_LIGHT_DTYPE), camera-relative
+    shape_flux: np.ndarray = None
+
+    @property
+    def nshapes(self):
+        return 0 if self.shapes is None else len(self.shapes)
 
     def launch_params(self, offset=0, count=None, width=None, height=None, depth=None, flags=0):
         p = LaunchParams()
@@ -187,6 +193,64 @@ def triangle_flux(rec, materials, tex_avg):
     return (f * tscale * area * np.pi).astype(np.float32)
 
 
+def disk_light(pos, normal, radius, power, material):
+    """A disk light record (world space) with the frame scene.h:386-392 builds
+    for a disk: tangent = normalize(cross(z, n)) (x instead of z when n is
+    within 0.9999 of +-z), bitangent = n x tangent."""
+    n = np.asarray(normal, np.float64)
+    n = n / np.linalg.norm(n)
+    a = np.array([0.0, 0.0, 1.0]) if abs(n[2]) < 0.9999 else np.array([1.0, 0.0, 0.0])
+    t = np.cross(a, n)
+    t /= np.linalg.norm(t)
+    b = np.cross(n, t)
+    r = np.zeros((), SHAPE_LIGHT_DTYPE)
+    r["position"], r["radius"], r["powerScale"] = pos, radius, power
+    r["frame"] = _quat_exact(t, b, n)
+    r["materialIndex"], r["type"] = material, 0
+    return r
+
+
+def sphere_light(pos, radius, power, material):
+    r = np.zeros((), SHAPE_LIGHT_DTYPE)
+    r["position"], r["radius"], r["powerScale"] = pos, radius, power
+    r["frame"] = (0.0, 0.0, 0.0, 1.0)
+    r["materialIndex"], r["type"] = material, 1
+    return r
+
+
+def shape_flux(shapes, materials):
+    """Flux of each shape light, as triangle_flux weighs an emitter: mean
+    emissive factor (spectral emitters: the spectrum's scale) x area x pi x
+    the light's power scale; a sphere's area is its whole surface."""
+    m = materials[shapes["materialIndex"]]
+    f = m["emissive"].mean(axis=1).astype(np.float64)
+    f = np.where((m["flags"] & MF_SPECTRAL_EMISSION) != 0, m["emissive"][:, 0], f)
+    r = shapes["radius"].astype(np.float64)
+    area = np.where(shapes["type"] == 0, np.pi * r * r, 4.0 * np.pi * r * r)
+    return (f * area * np.pi * shapes["powerScale"]).astype(np.float32)
+
+
+def _quat_exact(t, b, n):
+    """quat_from_frame of one frame without the w >= 1e-6 floor that vertex
+    frames need."""
+    m = np.stack([t, b, n], 1)
+    tr = m[0, 0] + m[1, 1] + m[2, 2]
+    if tr > 0:
+        r = math.sqrt(1.0 + tr) * 2.0
+        q = [(m[2, 1] - m[1, 2]) / r, (m[0, 2] - m[2, 0]) / r, (m[1, 0] - m[0, 1]) / r, 0.25 * r]
+    elif m[0, 0] > m[1, 1] and m[0, 0] > m[2, 2]:
+        r = math.sqrt(1.0 + m[0, 0] - m[1, 1] - m[2, 2]) * 2.0
+        q = [0.25 * r, (m[0, 1] + m[1, 0]) / r, (m[0, 2] + m[2, 0]) / r, (m[2, 1] - m[1, 2]) / r]
+    elif m[1, 1] > m[2, 2]:
+        r = math.sqrt(1.0 + m[1, 1] - m[0, 0] - m[2, 2]) * 2.0
+        q = [(m[0, 1] + m[1, 0]) / r, 0.25 * r, (m[1, 2] + m[2, 1]) / r, (m[0, 2] - m[2, 0]) / r]
+    else:
+        r = math.sqrt(1.0 + m[2, 2] - m[0, 0] - m[1, 1]) * 2.0
+        q = [(m[0, 2] + m[2, 0]) / r, (m[1, 2] + m[2, 1]) / r, 0.25 * r, (m[1, 0] - m[0, 1]) / r]
+    q = np.array(q)
+    return q / np.linalg.norm(q)
+
+
 def value_noise_texture(rng, size=256, cells=8):
     g = rng.random((cells + 1, cells + 1, 3))
     g[cells] = g[0]
@@ -298,7 +362,8 @@ def random_rotation(rng):
                      [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
 
 
-def _finish(parts, mats, spectra, tex_list, cam, env, width, height, depth, spp, name, tables, light_uv=None):
+def _finish(parts, mats, spectra, tex_list, cam, env, width, height, depth, spp, name, tables, light_uv=None,
+            shapes=None):
     verts = np.concatenate([p[0] for p in parts])
     norms = np.concatenate([p[2] for p in parts])
     uvs = np.concatenate([p[3] for p in parts])
@@ -332,15 +397,21 @@ def _finish(parts, mats, spectra, tex_list, cam, env, width, height, depth, spp,
         avgs.append(uv_footprint_average(t))
     texels = np.concatenate(blob) if blob else np.zeros(0, np.uint8)
     flux = triangle_flux(lights, materials, np.array(avgs))
+    sflux = None
+    if shapes is not None:
+        shapes = np.array(shapes, SHAPE_LIGHT_DTYPE).reshape(-1)
+        shapes["position"] = _transform(shapes["position"], cam["pos"])
+        sflux = shape_flux(shapes, materials)
     return Scene(verts=verts_c, indices=idx, attribs=pack_attribs(norms, uvs), tri_material=tri_material,
                  tri_light=tri_light, materials=materials,
                  spectra=np.ascontiguousarray(np.array(spectra, np.float32).reshape(-1, 471)),
                  textures=descs, texels=texels, lights=lights, flux=flux, env=env, camera=cam, width=width,
-                 height=height, depth=depth, spp=spp, tables=tables, name=name)
+                 height=height, depth=depth, spp=spp, tables=tables, name=name, shapes=shapes, shape_flux=sflux)
 
 
 # ----------------------------------------------------------------- configs
-def cornell(seed=0x5EED0001, width=256, height=256, spp=16, depth=8, light_up=False, occluder=False, bounce=False):
+def cornell(seed=0x5EED0001, width=256, height=256, spp=16, depth=8, light_up=False, occluder=False, bounce=False,
+            shapes=None):
     """Seeded Cornell box. With light_up the emitter faces the ceiling from
     just below it, so every other surface is lit only indirectly. With
     occluder a plate under the ceiling light hides it from the room, which is
@@ -352,7 +423,11 @@ def cornell(seed=0x5EED0001, width=256, height=256, spp=16, depth=8, light_up=Fa
     is well conditioned (no emitter within 0.4 of a lit surface); the spheres
     stay in the lower half of the box, out of the emitter's view, and every
     surface is at least 0.4 rough, so that no caustic path adds its own
-    variance."""
+    variance.
+
+    shapes="mixed" adds a disk light under the ceiling (facing down,
+    blackbody) and a sphere light (RGB emission) to the ceiling quad;
+    shapes="only" replaces the quad with them."""
     rng = np.random.default_rng(seed)
     tables = load_tables()
     mats, parts = [], []
@@ -388,8 +463,15 @@ def cornell(seed=0x5EED0001, width=256, height=256, spp=16, depth=8, light_up=Fa
         v, i, n, uv = quad((-0.3, 1.98, -0.3), (0.3, 1.98, -0.3), (0.3, 1.98, 0.3), (-0.3, 1.98, 0.3))
         power = 12.0
     mats.append(material(base=(0.8, 0.8, 0.8), metal=0.0, rough=1.0, spectral_emission=(0, power)))
-    mats[-1]["factors"] |= FC_EMISSION
-    parts.append((v, i, n, uv, len(mats) - 1))
+    if shapes != "only":
+        mats[-1]["factors"] |= FC_EMISSION
+        parts.append((v, i, n, uv, len(mats) - 1))
+    shape_recs = None
+    if shapes is not None:
+        mats.append(material(base=(0.0, 0.0, 0.0), metal=0.0, rough=1.0, spectral_emission=(0, 10.0)))
+        mats.append(material(base=(0.0, 0.0, 0.0), metal=0.0, rough=1.0, emissive=(3.0, 2.4, 1.6)))
+        shape_recs = [disk_light((0.55, 1.97, -0.45), (0.0, -1.0, 0.0), 0.22, 1.5, len(mats) - 2),
+                      sphere_light((-0.55, 1.55, 0.25), 0.1, 2.0, len(mats) - 1)]
     if occluder:
         # grey plate 0.12 below the light (intersection is two-sided)
         mats.append(material(base=(0.6, 0.6, 0.6), metal=0.0, rough=0.8, ior=1.5))
@@ -407,7 +489,60 @@ def cornell(seed=0x5EED0001, width=256, height=256, spp=16, depth=8, light_up=Fa
         parts.append((v, i, n, uv, len(mats) - 1))
     cam = camera_basis((0.0, 1.0, 3.4), (0.0, 1.0, 0.0), (0.0, 1.0, 0.0))
     cam["fov"] = math.radians(45.0)
-    return _finish(parts, mats, spectra, [], cam, None, width, height, depth, spp, "cornell", tables)
+    return _finish(parts, mats, spectra, [], cam, None, width, height, depth, spp, "cornell", tables,
+                   shapes=shape_recs)
+
+
+def shape_lit(kind="disk", triangles=False, width=48, height=48, spp=256, depth=4):
+    """A floor and a back wall lit by one shape light: a tilted disk (kind
+    "disk", blackbody emission) or a sphere (kind "sphere", RGB emission), or
+    with triangles=True the same emitter as emissive triangles: a 256-triangle
+    fan of the disk (area ratio 0.9999) or a level-4 icosphere of the sphere
+    (outward faces). Both versions have the same expectation; the camera sees
+    the light too."""
+    tables = load_tables()
+    spectra = [tables["blackbody4000"]]
+    floor = quad((-2, 0, -2), (-2, 0, 2), (2, 0, 2), (2, 0, -2))
+    wall = quad((-2, 0, -2), (2, 0, -2), (2, 3, -2), (-2, 3, -2))
+    mats = [material(base=(0.7, 0.7, 0.7), metal=0.0, rough=0.8, ior=1.5),
+            material(base=(0.5, 0.6, 0.7), metal=0.0, rough=0.6, ior=1.5)]
+    parts = [floor + (0,), wall + (1,)]
+    shapes = None
+    if kind == "disk":
+        c, n, r, power = np.array([0.2, 1.2, -0.3]), np.array([0.3, -1.0, 0.25]), 0.45, 1.5
+        rec = disk_light(c, n, r, power, 2)
+        if triangles:
+            mats.append(material(base=(0, 0, 0), metal=0.0, rough=1.0, spectral_emission=(0, 4.0 * power)))
+            mats[-1]["factors"] |= FC_EMISSION
+            q = rec["frame"].astype(np.float64)
+            x, y, z, w = q
+            t = np.array([1 - 2 * (y * y + z * z), 2 * (x * y + w * z), 2 * (x * z - w * y)])
+            b = np.array([2 * (x * y - w * z), 1 - 2 * (x * x + z * z), 2 * (y * z + w * x)])
+            k = 256
+            ang = 2 * np.pi * np.arange(k) / k
+            rim = c + r * (np.cos(ang)[:, None] * t + np.sin(ang)[:, None] * b)
+            v = np.concatenate([c[None], rim])
+            idx = np.stack([np.zeros(k), 1 + np.arange(k), 1 + (np.arange(k) + 1) % k], 1).astype(np.uint32)
+            nn = np.repeat((n / np.linalg.norm(n))[None], len(v), 0)
+            parts.append((v, idx, nn, np.zeros((len(v), 2), np.float32), 2))
+        else:
+            mats.append(material(base=(0, 0, 0), metal=0.0, rough=1.0, spectral_emission=(0, 4.0)))
+            shapes = [rec]
+    else:
+        c, r, power = np.array([0.0, 0.9, -0.4]), 0.35, 2.0
+        if triangles:
+            mats.append(material(base=(0, 0, 0), metal=0.0, rough=1.0,
+                                 emissive=(2.0 * power, 1.6 * power, 1.2 * power)))
+            mats[-1]["factors"] |= FC_EMISSION
+            iv, ii = icosphere(4)
+            parts.append((c + iv * r, ii.astype(np.uint32), iv, np.zeros((len(iv), 2), np.float32), 2))
+        else:
+            mats.append(material(base=(0, 0, 0), metal=0.0, rough=1.0, emissive=(2.0, 1.6, 1.2)))
+            shapes = [sphere_light(c, r, power, 2)]
+    cam = camera_basis((0.0, 1.6, 3.2), (0.0, 0.5, -0.5), (0.0, 1.0, 0.0))
+    cam["fov"] = math.radians(50.0)
+    return _finish(parts, mats, spectra, [], cam, None, width, height, depth, spp, "shape-" + kind, tables,
+                   light_uv=np.zeros((3, 2), np.float32), shapes=shapes)
 
 
 def furnace(inner="metal", rough=0.3, width=64, height=64, spp=64, depth=8, level=3, radius=1.5):

@@ -64,9 +64,28 @@ static void light_records(rbh_emissive_tri* t) {
     }
 }
 
+enum { NS = 2 };   /* shape lights */
+
+static void shape_records(rbh_shape_light* s) {
+    memset(s, 0, NS * sizeof(rbh_shape_light));
+    /* a disk facing -z (quaternion: 180 degrees about x) and a sphere */
+    s[0].position[0] = 0.6f; s[0].position[1] = 1.8f; s[0].position[2] = -3.0f;
+    s[0].powerScale = 2.0f;
+    s[0].frame[0] = 1.0f;
+    s[0].radius = 0.2f;
+    s[0].materialIndex = 1;
+    s[0].type = RBH_SHAPE_DISK;
+    s[1].position[0] = -0.6f; s[1].position[1] = 1.0f; s[1].position[2] = -3.0f;
+    s[1].powerScale = 1.0f;
+    s[1].frame[3] = 1.0f;
+    s[1].radius = 0.15f;
+    s[1].materialIndex = 1;
+    s[1].type = RBH_SHAPE_SPHERE;
+}
+
 static void host_part(void) {
     EXPECT(sizeof(rbh_material) == 152 && sizeof(rbh_texture_desc) == 48 && sizeof(rbh_emissive_tri) == 68 &&
-               sizeof(rbh_packed_node) == 32,
+               sizeof(rbh_packed_node) == 32 && sizeof(rbh_shape_light) == 48,
            "record sizes");
     EXPECT(RBH_GUIDE_RECORD == 16 && RBH_GUIDE_OUTPUTS == 41 && RBH_GUIDE_PARAMS == 20329, "guide sizes");
     rbh_emissive_tri t[NL];
@@ -82,6 +101,23 @@ static void host_part(void) {
         check_guard(nodes, 2 * NL * sizeof(rbh_packed_node), "rbh_light_tree_build_host nodes");
         check_guard(ptrs, NL * 4, "rbh_light_tree_build_host lightPointers");
         check_guard(masks, NL * 8, "rbh_light_tree_build_host bitmasks");
+        free(nodes); free(ptrs); free(masks);
+    }
+    {
+        rbh_shape_light s[NS];
+        shape_records(s);
+        const float sflux[NS] = {2.0f, 3.0f};
+        rbh_packed_node* nodes = (rbh_packed_node*)guarded(2 * (NL + NS) * sizeof(rbh_packed_node));
+        uint32_t* ptrs = (uint32_t*)guarded((NL + NS) * 4);
+        uint64_t* masks = (uint64_t*)guarded((NL + NS) * 8);
+        const int nn = rbh_light_tree_build_host_shapes(t, flux, NL, s, sflux, NS, nodes, ptrs, masks, 0);
+        EXPECT(nn >= 1 && nn <= 2 * (NL + NS), "light_tree_build_host_shapes returned %d", nn);
+        check_guard(nodes, 2 * (NL + NS) * sizeof(rbh_packed_node), "rbh_light_tree_build_host_shapes nodes");
+        check_guard(ptrs, (NL + NS) * 4, "rbh_light_tree_build_host_shapes lightPointers");
+        check_guard(masks, (NL + NS) * 8, "rbh_light_tree_build_host_shapes bitmasks");
+        int shapes = 0;
+        for (int i = 0; i < NL + NS; ++i) shapes += (ptrs[i] >> 30) == 3u;
+        EXPECT(shapes == NS, "%d shape-light pointers", shapes);
         free(nodes); free(ptrs); free(masks);
     }
 }
@@ -196,6 +232,30 @@ static void gpu_part(void) {
     const float cs[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     OK(c, rbh_resolve(c, cs, 1.0f, rgba));
     check_guard(rgba, (size_t)W * H * 4, "rbh_resolve");
+    /* shape lights: a render is refused until the light tree is rebuilt */
+    {
+        rbh_shape_light s[NS];
+        shape_records(s);
+        const float sflux[NS] = {2.0f, 3.0f};
+        OK(c, rbh_upload_shape_lights(c, s, sflux, NS));
+        EXPECT(rbh_render(c, &lp) == RBH_ERR_STATE, "render with a stale light tree accepted");
+        OK(c, rbh_build_light_tree(c, lt, flux, NL));
+        rbh_packed_node* nodes = (rbh_packed_node*)guarded(2 * (NL + NS) * sizeof(rbh_packed_node));
+        uint32_t* ptrs = (uint32_t*)guarded((NL + NS) * 4);
+        uint64_t* masks = (uint64_t*)guarded((NL + NS) * 8);
+        uint32_t nn = 0;
+        OK(c, rbh_get_light_tree(c, nodes, &nn, ptrs, masks));
+        check_guard(nodes, 2 * (NL + NS) * sizeof(rbh_packed_node), "rbh_get_light_tree nodes (shapes)");
+        check_guard(ptrs, (NL + NS) * 4, "rbh_get_light_tree lightPointers (shapes)");
+        check_guard(masks, (NL + NS) * 8, "rbh_get_light_tree bitmasks (shapes)");
+        OK(c, rbh_set_light_tree(c, nodes, nn, ptrs, masks, lt, NL));
+        OK(c, rbh_render(c, &lp));
+        s[1].radius = -1.0f;
+        EXPECT(rbh_upload_shape_lights(c, s, sflux, NS) == RBH_ERR_INVALID, "negative radius accepted");
+        OK(c, rbh_upload_shape_lights(c, NULL, NULL, 0));
+        OK(c, rbh_build_light_tree(c, lt, flux, NL));
+        free(nodes); free(ptrs); free(masks);
+    }
     rbh_tuning tu;
     OK(c, rbh_get_tuning(c, &tu));
     OK(c, rbh_set_tuning(c, &tu));
