@@ -255,46 +255,61 @@ __device__ __forceinline__ bool trav_step(const NODE* __restrict__ nodes, const 
             in[k] = cha[k] != kBvh4Empty && tnear <= tfar * 1.00000024f;
         }
     }
-    // triangle children first: a hit shortens the ray for the boxes
+    // Triangle children first (a hit shortens the ray for the boxes), in
+    // slot order. One loop over each lane's own triangle slots: a wave runs
+    // it as often as its lane with the most triangle children needs, instead
+    // of once for every slot in which some lane has one.
+    uint32_t triMask = 0;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        if (in[k] && cha[k] < 0) {
-            const TriRecord& tr = tris[~cha[k]];
-            float t, b1, b2;
-            if (tri_hit(r.sh, r.o, tr, r.h.t, t, b1, b2)) {
-                r.h.t = t;
-                r.h.b1 = b1;
-                r.h.b2 = b2;
-                r.h.leaf = (uint32_t)~cha[k];
-                if (ANY) return true;
-            }
+    for (int k = 0; k < 4; ++k) triMask |= (in[k] && cha[k] < 0) ? (1u << k) : 0u;
+    while (triMask) {
+        const uint32_t k = (uint32_t)__builtin_ctz(triMask);
+        triMask &= triMask - 1u;
+        const int c = k == 0u ? cha[0] : k == 1u ? cha[1] : k == 2u ? cha[2] : cha[3];
+        const TriRecord& tr = tris[~c];
+        float t, b1, b2;
+        if (tri_hit(r.sh, r.o, tr, r.h.t, t, b1, b2)) {
+            r.h.t = t;
+            r.h.b1 = b1;
+            r.h.b2 = b2;
+            r.h.leaf = (uint32_t)~c;
+            if (ANY) return true;
         }
     }
-    int ids[4];
-    float ds[4];
-    int cnt = 0;
+    // Surviving inner children nearest first. A child's key is the bit
+    // pattern of its entry distance (>= 0, so ordered as an unsigned
+    // integer; the sign bit cleared for -0) with its slot in the two low bits, so a five-comparator
+    // network on unsigned min/max sorts them without branches or indexed
+    // registers; children within 4 ulp of each other go in slot order.
+    // Culled and empty slots sort last as ~0.
     const float limit = r.h.t * 1.00000024f * 1.00000024f;
+    uint32_t key[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        if (in[k] && cha[k] >= 0 && tn[k] <= limit) {
-            // insertion by distance
-            int pos = cnt;
-            while (pos > 0 && ds[pos - 1] > tn[k]) {
-                ds[pos] = ds[pos - 1];
-                ids[pos] = ids[pos - 1];
-                --pos;
-            }
-            ds[pos] = tn[k];
-            ids[pos] = cha[k];
-            ++cnt;
-        }
+    for (int k = 0; k < 4; ++k)
+        key[k] = (in[k] && cha[k] >= 0 && tn[k] <= limit) ? ((__float_as_uint(tn[k]) & 0x7FFFFFFCu) | (uint32_t)k)
+                                                          : ~0u;
+    {
+        auto cx = [](uint32_t& a, uint32_t& b) {
+            const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+            a = lo;
+            b = hi;
+        };
+        cx(key[0], key[1]);
+        cx(key[2], key[3]);
+        cx(key[0], key[2]);
+        cx(key[1], key[3]);
+        cx(key[1], key[2]);
     }
-    if (cnt > 0) {
-        for (int k = cnt - 1; k >= 1; --k) {
-            stack.put(r.sp, ids[k]);
-            ++r.sp;
-        }
-        r.node = ids[0];
+    auto child = [&](uint32_t kk) {
+        const uint32_t s = kk & 3u;
+        return s == 0u ? cha[0] : s == 1u ? cha[1] : s == 2u ? cha[2] : cha[3];
+    };
+    if (key[0] != ~0u) {
+        // farthest pushed first, so the nearer ones are popped first
+        if (key[3] != ~0u) stack.put(r.sp++, child(key[3]));
+        if (key[2] != ~0u) stack.put(r.sp++, child(key[2]));
+        if (key[1] != ~0u) stack.put(r.sp++, child(key[1]));
+        r.node = child(key[0]);
         return false;
     }
     if (r.sp == 0) return true;
