@@ -352,7 +352,8 @@ typedef struct rbh_tuning {
     int32_t traceFetch;           /* 1: a trace/shadow lane takes a new ray as soon as its query completes; 0: waves take 64 rays at a time */
     int32_t traceWaves;           /* trace/shadow register budget (wave-synchronous kernels): 1 (unconstrained), 7 or 8 waves per SIMD */
     int32_t quantNodes;           /* 1: the wave-synchronous trace/shadow kernels read the four-wide nodes quantised to 64 bytes */
-    int32_t sortHits;             /* 1: material-sorted shading (conductor hits queued apart from the rest; wave-synchronous trace only) */
+    int32_t sortHits;             /* 1: material-sorted shading (hits queued by shading class: plain, transmissive and sheen
+                                     ComplexSurface, PureConductor, PureDielectric, shape light) */
     double samplesPerSecond;      /* measured by rbh_autotune */
 } rbh_tuning;
 /* Times trialSpp-sample renders of lp over a grid of slot budgets and grid

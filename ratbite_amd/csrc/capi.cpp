@@ -12,6 +12,7 @@ hipError_t envmap_cdf_launch(rbh_ctx c, bool fp64);
 hipError_t srgb_lut_fill(rbh_ctx c);
 size_t envmap_scratch_bytes(int h);
 hipError_t envmap_noalloc_init(int device);
+hipError_t refresh_tri_classes(rbh_ctx c);
 hipError_t envmap_cdf_noalloc(int device, hipStream_t s, bool fp64, const float* texels, int w, int h, int ch,
                               uint16_t* cond, uint16_t* marg, float* cdfY, double* normIntegral);
 }  // namespace rbh
@@ -159,8 +160,9 @@ int rbh_upload_materials(rbh_ctx c, const rbh_material* m, uint32_t n) {
     }
     RBH_ALLOC(c, c->materials, (size_t)n * sizeof(Material));
     RBH_HIP(c, hipMemcpyAsync(c->materials.p, m, (size_t)n * sizeof(Material), hipMemcpyHostToDevice, c->stream));
-    RBH_HIP(c, hipStreamSynchronize(c->stream));
     c->nmaterials = n;
+    RBH_HIP(c, refresh_tri_classes(c));
+    RBH_HIP(c, hipStreamSynchronize(c->stream));
     return RBH_OK;
 }
 

@@ -73,7 +73,8 @@ struct alignas(16) TriRecord {
     float v0x, v0y, v0z; uint32_t prim;
     float v1x, v1y, v1z; uint32_t material;
     float v2x, v2y, v2z; uint32_t light;
-    uint32_t i0, i1, i2, pad;
+    uint32_t i0, i1, i2;
+    uint32_t shadeClass;   // the material's shading class (SC_*), for the material-sorted hit queue
 };
 static_assert(sizeof(TriRecord) == 64, "TriRecord is 64 bytes");
 
@@ -126,6 +127,26 @@ struct Material {
     float emissive[3];
 };
 static_assert(sizeof(Material) == 152, "Material is 152 bytes");
+
+// Shading classes: hits are queued by the BxDF code path their material
+// takes in the shade kernel, so that a wave of the shade kernel mostly runs
+// one of them.
+enum : uint32_t {
+    SC_SURFACE = 0,        // ComplexSurface without transmission or sheen (diffuse / glossy / metallic)
+    SC_TRANSMISSIVE = 1,   // ComplexSurface with a transmission factor or texture
+    SC_SHEEN = 2,          // ComplexSurface with a sheen layer
+    SC_CONDUCTOR = 3,      // PureConductor
+    SC_DIELECTRIC = 4,     // PureDielectric
+    SC_SHAPE = 5,          // a disk or sphere light (the path ends there)
+    kShadeClasses = 6
+};
+__host__ __device__ inline uint32_t shade_class(const Material& m) {
+    if (m.bxdf == 0u) return SC_CONDUCTOR;
+    if (m.bxdf == 1u) return SC_DIELECTRIC;
+    if (m.flags & 2u) return SC_SHEEN;
+    if ((m.textures & 8u) || ((m.factors & 8u) && m.transmission != 0.0f)) return SC_TRANSMISSIVE;
+    return SC_SURFACE;
+}
 
 struct TextureDesc {
     uint32_t width, height, depth, format, srgb;

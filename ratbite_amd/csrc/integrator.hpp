@@ -13,11 +13,15 @@ constexpr int kTraceBlock = 128;   // lanes per block of the trace and shadow ke
 constexpr int kShadeBlock = 128;
 
 enum : uint32_t {
-    CNT_QUEUE0 = 0, CNT_QUEUE1 = 1, CNT_SHADOW = 2, CNT_HIT = 3, CNT_MISS = 4,
+    CNT_QUEUE0 = 0, CNT_QUEUE1 = 1, CNT_SHADOW = 2,
+    CNT_HIT = 3,                               // hits of shading class 0 (all hits when unsorted)
+    CNT_MISS = 4,
     CNT_TRACE_HEAD = 5, CNT_SHADOW_HEAD = 6,   // work-fetch positions of the trace and shadow queues
-    CNT_HIT_BACK = 7,                          // hits queued from the end of the hit queue (conductors)
-    CNT_COUNT = 8
+    CNT_HIT_CLASS1 = 7,                        // hits of shading classes 1 .. kShadeClasses-1
+    CNT_COUNT = CNT_HIT_CLASS1 + kShadeClasses - 1
 };
+// counter of the hits of shading class c
+__host__ __device__ inline uint32_t hit_counter(uint32_t c) { return c == 0u ? (uint32_t)CNT_HIT : CNT_HIT_CLASS1 + c - 1u; }
 enum : uint32_t { STAT_CLOSEST = 0, STAT_SHADOW = 1, STAT_VERTICES = 2, STAT_NONFINITE = 3, STAT_COUNT = 4 };
 
 // Structure-of-arrays path state, one entry per path slot of the pass.
@@ -34,8 +38,9 @@ struct PathBuffers {
     float4* srd;    // shadow ray direction
     uint32_t* q0;   // paths to trace (even / odd bounces)
     uint32_t* q1;
-    uint32_t* hq;   // paths whose closest hit found a triangle (to shade): CNT_HIT entries from the
-                    // front, CNT_HIT_BACK (conductor hits when sorting by material) from the end
+    uint32_t* hq;   // paths whose closest hit found a surface or shape light (to shade): kShadeClasses
+                    // segments of NP entries, segment c holding the hits of shading class c
+                    // (class 0 only when the hits are not sorted by material)
     uint32_t* mq;   // paths whose ray left the scene
     uint32_t* sq;   // paths with a deferred NEE shadow ray
     uint32_t* prec; // guided renders that collect training data: last record of the path
@@ -95,7 +100,7 @@ struct LaunchShape {
     int traceFetch;          // trace/shadow lanes fetch a new ray when theirs completes (1) or wave by wave (0)
     int traceWaves;          // register budget of the wave-synchronous trace/shadow kernels: 1 (free), 7 or 8
     int quantNodes;          // wave-synchronous trace/shadow kernels read the 64-byte quantised nodes
-    int sortHits;            // queue conductor hits apart from the rest (material-sorted shading)
+    int sortHits;            // queue the hits by shading class (material-sorted shading)
 };
 
 struct Frame1 {

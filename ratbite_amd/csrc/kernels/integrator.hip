@@ -71,7 +71,7 @@ __global__ __launch_bounds__(256) void k_camera(Frame1 f) {
         f.counters[CNT_MISS] = 0u;
         f.counters[CNT_TRACE_HEAD] = 0u;
         f.counters[CNT_SHADOW_HEAD] = 0u;
-        f.counters[CNT_HIT_BACK] = 0u;
+        for (uint32_t c = 1; c < kShadeClasses; ++c) f.counters[hit_counter(c)] = 0u;
     }
 }
 
@@ -82,21 +82,28 @@ __global__ void k_reset(uint32_t* __restrict__ counters, uint32_t which) {
 }
 
 __global__ void k_reset_hits(uint32_t* __restrict__ counters) {
-    if (threadIdx.x == 0) counters[CNT_HIT] = 0u;
-    if (threadIdx.x == 1) counters[CNT_MISS] = 0u;
-    if (threadIdx.x == 2) counters[CNT_TRACE_HEAD] = 0u;
-    if (threadIdx.x == 3) counters[CNT_HIT_BACK] = 0u;
+    if (threadIdx.x < kShadeClasses) counters[hit_counter(threadIdx.x)] = 0u;
+    if (threadIdx.x == kShadeClasses) counters[CNT_MISS] = 0u;
+    if (threadIdx.x == kShadeClasses + 1) counters[CNT_TRACE_HEAD] = 0u;
 }
 
-// Hit-queue entries in shading order: the CNT_HIT entries pushed from the
-// front, then the CNT_HIT_BACK entries pushed from the end of the NP-entry
-// queue (their total never exceeds the NP rays traced).
-__device__ __forceinline__ uint32_t hit_count(const Frame1& f) {
-    return f.counters[CNT_HIT] + f.counters[CNT_HIT_BACK];
-}
-__device__ __forceinline__ uint32_t hit_entry(const Frame1& f, uint32_t i, uint32_t nFront) {
-    return i < nFront ? f.p.hq[i] : f.p.hq[f.NP - 1u - (i - nFront)];
-}
+// Hit-queue entries in shading order: the segments of the shading classes
+// one after the other (class c's entries at hq[c * NP ...]).
+struct HitQueue {
+    uint32_t start[kShadeClasses + 1];   // first shading-order index of each class, then the total
+    __device__ __forceinline__ explicit HitQueue(const Frame1& f) {
+        start[0] = 0u;
+#pragma unroll
+        for (uint32_t c = 0; c < kShadeClasses; ++c) start[c + 1] = start[c] + f.counters[hit_counter(c)];
+    }
+    __device__ __forceinline__ uint32_t count() const { return start[kShadeClasses]; }
+    __device__ __forceinline__ uint32_t entry(const Frame1& f, uint32_t i) const {
+        uint32_t c = 0;
+#pragma unroll
+        for (uint32_t k = 1; k < kShadeClasses; ++k) c += i >= start[k] ? 1u : 0u;
+        return f.p.hq[(size_t)c * f.NP + (i - start[c])];
+    }
+};
 
 // Work fetch of the persistent trace and shadow kernels. A wave claims
 // chunks of kChunk consecutive queue entries with one atomic on *head and
@@ -168,6 +175,47 @@ struct WaveList {
     }
 };
 
+// The same staging for the hit queue, with each entry's shading class: a
+// flush writes every class present in the list as one run of that class's
+// queue segment, with one atomic on the class's counter.
+struct ClassList {
+    uint32_t* buf;   // 128 path slots of this wave
+    uint8_t* cls;    // their shading classes
+    uint32_t n;      // entries held (wave-uniform)
+    __device__ __forceinline__ void add(bool pred, uint32_t v, uint32_t c) {
+        const unsigned long long mask = __ballot(pred);
+        const uint32_t rank = (uint32_t)__popcll(mask & ((1ull << __lane_id()) - 1ull));
+        if (pred) {
+            buf[n + rank] = v;
+            cls[n + rank] = (uint8_t)c;
+        }
+        n += (uint32_t)__popcll(mask);
+    }
+    // at most 127 entries are held (64 added at a time, flushed from 64 on)
+    __device__ __forceinline__ void flush(const Frame1& f, uint32_t min) {
+        if (n == 0 || n < min) return;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const uint32_t lane = (uint32_t)__lane_id();
+        const bool v0 = lane < n, v1 = lane + 64u < n;
+        const uint32_t p0 = v0 ? buf[lane] : 0u, p1 = v1 ? buf[lane + 64u] : 0u;
+        const uint32_t c0 = v0 ? cls[lane] : (uint32_t)kShadeClasses, c1 = v1 ? cls[lane + 64u] : (uint32_t)kShadeClasses;
+        const unsigned long long lt = (1ull << lane) - 1ull;
+        for (uint32_t c = 0; c < kShadeClasses; ++c) {
+            const unsigned long long m0 = __ballot(c0 == c), m1 = __ballot(c1 == c);
+            const uint32_t k0 = (uint32_t)__popcll(m0), k = k0 + (uint32_t)__popcll(m1);
+            if (k == 0u) continue;
+            uint32_t base = 0;
+            if (lane == 0u) base = atomicAdd(f.counters + hit_counter(c), k);
+            base = __shfl(base, 0);
+            uint32_t* __restrict__ q = f.p.hq + (size_t)c * f.NP + base;
+            if (c0 == c) q[__popcll(m0 & lt)] = p0;
+            if (c1 == c) q[k0 + (uint32_t)__popcll(m1 & lt)] = p1;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        n = 0;
+    }
+};
+
 // The four-wide nodes a kernel traverses: 128-byte or quantised 64-byte
 template <bool QN>
 __device__ __forceinline__ auto scene_nodes(const SceneView& S) {
@@ -181,12 +229,18 @@ __device__ __forceinline__ auto scene_nodes(const SceneView& S) {
 template <bool BATCH, bool FETCH, int WAVES, bool QN>
 __global__ __launch_bounds__(kTraceBlock, WAVES) void k_trace(Frame1 f, uint32_t qsel) {
     __shared__ int stack[kStackLds * kTraceBlock];
-    __shared__ uint32_t staged[FETCH ? 2 * kTraceBlock * 2 : 1];   // per wave: 128 hits, 128 misses
+    __shared__ uint32_t staged[2 * kTraceBlock * 2];   // per wave: 128 hits, 128 misses
+    __shared__ uint8_t stagedCls[2 * kTraceBlock];     // per wave: the hits' shading classes
     const TravStack<kStackLds> st = lane_stack(f, stack);
     const uint32_t count = f.counters[qsel ? CNT_QUEUE1 : CNT_QUEUE0];
     const uint32_t* __restrict__ queue = qsel ? f.p.q1 : f.p.q0;
     const bool identity = f.bounce == 0;
     uint32_t nrays = 0;
+    // finished rays are staged per wave and queued 64 or more at a time;
+    // hits in their shading class's segment when sorting by material
+    const uint32_t wave = threadIdx.x >> 6;
+    ClassList hits{staged + wave * 256u, stagedCls + wave * 128u, 0u};
+    WaveList misses{staged + wave * 256u + 128u, 0u};
     if (FETCH) {
         // lanes take a new ray as soon as theirs completes
         TravRay ray;
@@ -195,8 +249,6 @@ __global__ __launch_bounds__(kTraceBlock, WAVES) void k_trace(Frame1 f, uint32_t
         uint32_t pid = 0;
         bool active = false;
         WaveChunk chunk;
-        const uint32_t wave = threadIdx.x >> 6;
-        WaveList hits{staged + wave * 256u, 0u}, misses{staged + wave * 256u + 128u, 0u};
         while (true) {
             const bool anyBusy = __ballot(active) != 0ull;
             const uint32_t idx = chunk.fetch(f.counters + CNT_TRACE_HEAD, count, !active, anyBusy);
@@ -216,18 +268,21 @@ __global__ __launch_bounds__(kTraceBlock, WAVES) void k_trace(Frame1 f, uint32_t
             bool done = false;
             if (active) done = trav_step<false>(nodes, tris, ray, st);
             const bool hit = done && ray.h.leaf != 0xFFFFFFFFu;
-            if (hit) f.p.hit[pid] = make_float4(ray.h.t, ray.h.b1, ray.h.b2, __uint_as_float(ray.h.leaf));
-            hits.add(hit, pid);
+            uint32_t cls = 0;
+            if (hit) {
+                f.p.hit[pid] = make_float4(ray.h.t, ray.h.b1, ray.h.b2, __uint_as_float(ray.h.leaf));
+                if (f.sortHits) cls = (ray.h.leaf & kShapeHit) ? (uint32_t)SC_SHAPE : tris[ray.h.leaf].shadeClass;
+            }
+            hits.add(hit, pid, cls);
             misses.add(done && !hit, pid);
-            hits.flush(f.counters + CNT_HIT, f.p.hq, 64u);
+            hits.flush(f, 64u);
             misses.flush(f.counters + CNT_MISS, f.p.mq, 64u);
             if (done) active = false;
         }
-        hits.flush(f.counters + CNT_HIT, f.p.hq, 1u);
-        misses.flush(f.counters + CNT_MISS, f.p.mq, 1u);
     } else {
         for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i - threadIdx.x < count; i += gridDim.x * blockDim.x) {
-            bool hit = false, back = false;
+            bool hit = false;
+            uint32_t cls = 0;
             uint32_t pid = 0;
             if (i < count) {
                 pid = identity ? i : queue[i];
@@ -243,31 +298,30 @@ __global__ __launch_bounds__(kTraceBlock, WAVES) void k_trace(Frame1 f, uint32_t
                 hit = h.leaf != 0xFFFFFFFFu;
                 if (hit) {
                     f.p.hit[pid] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.leaf));
-                    // material-sorted shading: conductor hits form their own run of the queue
-                    if (f.sortHits && !(h.leaf & kShapeHit)) back = S.materials[S.tris[h.leaf].material].bxdf == 0u;
+                    // material-sorted shading: the hit joins its shading class's segment
+                    if (f.sortHits) cls = (h.leaf & kShapeHit) ? (uint32_t)SC_SHAPE : S.tris[h.leaf].shadeClass;
                 }
                 ++nrays;
             }
             const bool live = i < count;
-            const uint32_t slot = wave_push(f.counters + CNT_HIT, live && hit && !back);
-            if (live && hit && !back) f.p.hq[slot] = pid;
-            if (f.sortHits) {
-                const uint32_t bslot = wave_push(f.counters + CNT_HIT_BACK, back);
-                if (back) f.p.hq[f.NP - 1u - bslot] = pid;
-            }
-            const uint32_t mslot = wave_push(f.counters + CNT_MISS, live && !hit);
-            if (live && !hit) f.p.mq[mslot] = pid;
+            hits.add(live && hit, pid, cls);
+            misses.add(live && !hit, pid);
+            hits.flush(f, 64u);
+            misses.flush(f.counters + CNT_MISS, f.p.mq, 64u);
         }
     }
+    hits.flush(f, 1u);
+    misses.flush(f.counters + CNT_MISS, f.p.mq, 1u);
     wave_count(f.stats + STAT_CLOSEST, nrays);
 }
 
 // Network input of every queued hit, one thread per hit (the gathers run
 // fully parallel), kept for the training records of the vertex.
 __global__ __launch_bounds__(256) void k_guide_features(Frame1 f) {
-    const uint32_t count = hit_count(f), nFront = f.counters[CNT_HIT];
+    const HitQueue hq(f);
+    const uint32_t count = hq.count();
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
-        guide_input(f, f.sc, hit_entry(f, i, nFront), f.gfeat + (size_t)i * guide::kRec);
+        guide_input(f, f.sc, hq.entry(f, i), f.gfeat + (size_t)i * guide::kRec);
 }
 
 // Mixture of every queued hit: wave-resident MLP inference over the
@@ -275,7 +329,7 @@ __global__ __launch_bounds__(256) void k_guide_features(Frame1 f) {
 template <bool FP8>
 __global__ __launch_bounds__(256, 2) void k_guide_mlp(Frame1 f) {
     __shared__ mlp::WaveLds L[4];
-    mlp::wave_infer<FP8, true>(L[threadIdx.x >> 6], f.gfeat, hit_count(f), f.gwf, f.gwq, f.gws, f.gbias, f.gmix);
+    mlp::wave_infer<FP8, true>(L[threadIdx.x >> 6], f.gfeat, HitQueue(f).count(), f.gwf, f.gwq, f.gws, f.gbias, f.gmix);
 }
 
 template <bool BATCH>
@@ -346,11 +400,16 @@ __global__ __launch_bounds__(kTraceBlock, WAVES) void k_shadow(Frame1 f) {
 // latency hiding.
 template <bool GUIDED, bool BATCH, int WAVES>
 __global__ __launch_bounds__(kShadeBlock, WAVES) void k_shade(Frame1 f, uint32_t qsel) {
-    // shades the paths whose ray hit a triangle (misses were finished by k_trace)
-    const uint32_t count = hit_count(f), nFront = f.counters[CNT_HIT];
+    // shades the paths whose ray hit a surface or a shape light (misses were finished by k_trace)
+    const HitQueue hq(f);
+    const uint32_t count = hq.count();
     uint32_t* __restrict__ qout = qsel ? f.p.q0 : f.p.q1;
     uint32_t* __restrict__ cout = f.counters + (qsel ? CNT_QUEUE0 : CNT_QUEUE1);
     uint32_t nvert = 0;
+    // continuing paths and shadow rays are staged per wave and queued 64 or more at a time
+    __shared__ uint32_t staged[2 * kShadeBlock * 2];
+    const uint32_t wave = threadIdx.x >> 6;
+    WaveList next{staged + wave * 256u, 0u}, shadows{staged + wave * 256u + 128u, 0u};
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i - threadIdx.x < count; i += gridDim.x * blockDim.x) {
         bool alive = false, shadow = false;
         bool recordSlot = false;      // guided + collecting: record this vertex's direction
@@ -359,15 +418,15 @@ __global__ __launch_bounds__(kShadeBlock, WAVES) void k_shade(Frame1 f, uint32_t
         S4 recT = s4(0.0f);           // throughput before the recorded vertex
         uint32_t pid = 0;
         if (i < count) {
-            pid = hit_entry(f, i, nFront);
+            pid = hq.entry(f, i);
             ++nvert;
             const SceneView& S = scene_of<BATCH>(f, pid);
             shade_hit<GUIDED>(f, S, pid, i, alive, shadow, recordSlot, recDir, recPb, recT);
         }
-        const uint32_t slot = wave_push(cout, alive);
-        if (alive) qout[slot] = pid;
-        const uint32_t sslot = wave_push(f.counters + CNT_SHADOW, shadow);
-        if (shadow) f.p.sq[sslot] = pid;
+        next.add(alive, pid);
+        shadows.add(shadow, pid);
+        next.flush(cout, qout, 64u);
+        shadows.flush(f.counters + CNT_SHADOW, f.p.sq, 64u);
         if (GUIDED && f.collect) {
             // a recorded direction needs the path to go on (its radiance is measured later)
             const bool rec = recordSlot && alive;
@@ -386,6 +445,8 @@ __global__ __launch_bounds__(kShadeBlock, WAVES) void k_shade(Frame1 f, uint32_t
             }
         }
     }
+    next.flush(cout, qout, 1u);
+    shadows.flush(f.counters + CNT_SHADOW, f.p.sq, 1u);
     wave_count(f.stats + STAT_VERTICES, nvert);
 }
 
@@ -485,7 +546,7 @@ static hipError_t launch_pass_t(rbh_ctx c, Frame1 f, uint32_t depthMax, const La
         if (kt && (e = kt->end(c->stream)) != hipSuccess) return e;  \
     } while (0)
     f.bounce = 0;
-    f.sortHits = ls.sortHits && !ls.traceFetch ? 1u : 0u;   // the work-fetch trace kernels push to the front only
+    f.sortHits = ls.sortHits ? 1u : 0u;
     hipLaunchKernelGGL(k_camera<BATCH>, dim3(nb), dim3(256), 0, c->stream, f);
     for (uint32_t b = 0; b < depthMax; ++b) {
         const uint32_t qsel = b & 1u;

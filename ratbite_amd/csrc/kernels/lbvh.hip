@@ -153,6 +153,7 @@ struct Box { float lo[3], hi[3]; };
 __global__ __launch_bounds__(kBuildBlock) void k_leaves(const float* __restrict__ v, const uint32_t* __restrict__ idx,
                                                         const uint32_t* __restrict__ triMaterial,
                                                         const uint32_t* __restrict__ triLight,
+                                                        const Material* __restrict__ mats, uint32_t nmats,
                                                         const unsigned long long* __restrict__ keys, int n, int idxBits,
                                                         float* __restrict__ lbox, TriRecord* __restrict__ tris) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -163,7 +164,8 @@ __global__ __launch_bounds__(kBuildBlock) void k_leaves(const float* __restrict_
     r.v0x = v[3 * a]; r.v0y = v[3 * a + 1]; r.v0z = v[3 * a + 2]; r.prim = tri;
     r.v1x = v[3 * b]; r.v1y = v[3 * b + 1]; r.v1z = v[3 * b + 2]; r.material = triMaterial[tri];
     r.v2x = v[3 * c]; r.v2y = v[3 * c + 1]; r.v2z = v[3 * c + 2]; r.light = triLight[tri];
-    r.i0 = a; r.i1 = b; r.i2 = c; r.pad = 0u;
+    r.i0 = a; r.i1 = b; r.i2 = c;
+    r.shadeClass = r.material < nmats ? shade_class(mats[r.material]) : (uint32_t)SC_SURFACE;
     tris[j] = r;
     float* lb = lbox + 6 * (size_t)j;
     lb[0] = fminf(fminf(r.v0x, r.v1x), r.v2x); lb[3] = fmaxf(fmaxf(r.v0x, r.v1x), r.v2x);
@@ -461,8 +463,8 @@ int build_boxes_and_nodes(rbh_ctx c, const BuildLayout& L) {
     const int nbi = (int)((nInt + kBuildBlock - 1) / kBuildBlock);
     const int cus = cu_count_of(c->device);
     hipLaunchKernelGGL(k_leaves, dim3(nb), dim3(kBuildBlock), 0, c->stream, g.verts.as<float>(), g.indices.as<uint32_t>(),
-                       g.triMaterial.as<uint32_t>(), g.triLight.as<uint32_t>(), keys2, n, acc.keyIndexBits, lbox,
-                       acc.tris.as<TriRecord>());
+                       g.triMaterial.as<uint32_t>(), g.triLight.as<uint32_t>(), c->materials.as<Material>(),
+                       c->nmaterials, keys2, n, acc.keyIndexBits, lbox, acc.tris.as<TriRecord>());
     if (n > 1) {
         const int lg = std::min(nbi, cus * 8);
         for (int d = kLevels - 1; d >= 0; --d)
@@ -581,6 +583,25 @@ int lbvh_build(rbh_ctx c) {
     acc.built = true;
     acc.stale = false;
     return RBH_OK;
+}
+
+__global__ __launch_bounds__(kBuildBlock) void k_tri_classes(TriRecord* __restrict__ tris, int n,
+                                                             const Material* __restrict__ mats, uint32_t nmats) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t m = tris[j].material;
+    tris[j].shadeClass = m < nmats ? shade_class(mats[m]) : (uint32_t)SC_SURFACE;
+}
+
+// Re-derives the triangle records' shading classes after the materials
+// changed (the classes only order the shade kernel's queue).
+hipError_t refresh_tri_classes(rbh_ctx c) {
+    const Accel& acc = c->accel;
+    if (!acc.built || acc.ntris == 0) return hipSuccess;
+    const int n = (int)acc.ntris;
+    hipLaunchKernelGGL(k_tri_classes, dim3((n + kBuildBlock - 1) / kBuildBlock), dim3(kBuildBlock), 0, c->stream,
+                       acc.tris.as<TriRecord>(), n, c->materials.as<Material>(), c->nmaterials);
+    return hipGetLastError();
 }
 
 hipError_t translate_points(rbh_ctx c, float* p, size_t npoints, uint32_t strideFloats, const float* delta) {

@@ -17,6 +17,7 @@ hipError_t launch_pass(rbh_ctx c, const Frame1& f, uint32_t depthMax, const Laun
 hipError_t resolve_launch(rbh_ctx c, const float* colorspace, float expo, void* out);
 size_t trace_spill_words(size_t lanes);
 hipError_t light_tree_expand(rbh_ctx c);
+hipError_t refresh_tri_classes(rbh_ctx c);
 namespace host {
 int build_light_tree(const EmissiveTri* tris, const float* flux, uint32_t n, const ShapeLight* shapes,
                      const float* shapeFlux, uint32_t m, std::vector<PackedNode>& nodes,
@@ -44,7 +45,7 @@ size_t layout_paths(char* base, size_t P, PathBuffers* pb, uint32_t** counters, 
     pb->depth = reinterpret_cast<uint32_t*>(take(P * 4));
     pb->q0 = reinterpret_cast<uint32_t*>(take(P * 4));
     pb->q1 = reinterpret_cast<uint32_t*>(take(P * 4));
-    pb->hq = reinterpret_cast<uint32_t*>(take(P * 4));
+    pb->hq = reinterpret_cast<uint32_t*>(take(P * 4 * kShadeClasses));   // one segment per shading class
     pb->mq = reinterpret_cast<uint32_t*>(take(P * 4));
     pb->prec = reinterpret_cast<uint32_t*>(take(P * 4));
     pb->sq = reinterpret_cast<uint32_t*>(take(P * 4));
@@ -158,6 +159,7 @@ int rbh_update_materials(rbh_ctx c, uint32_t first, uint32_t count, const rbh_ma
     }
     RBH_HIP(c, hipMemcpyAsync(c->materials.as<char>() + (size_t)first * sizeof(Material), m,
                               (size_t)count * sizeof(Material), hipMemcpyHostToDevice, c->stream));
+    RBH_HIP(c, refresh_tri_classes(c));
     RBH_HIP(c, hipStreamSynchronize(c->stream));
     return RBH_OK;
 }
@@ -909,17 +911,18 @@ int rbh_autotune(rbh_ctx c, const rbh_launch_params* lp, uint32_t trialSpp, rbh_
         rs.shadeBlocksPerCU = b.shadeBlocksPerCU;
         rs.shadeWaves = b.shadeWaves ? b.shadeWaves : 1;
         rs.traceWaves = b.traceWaves;
+        rs.sortHits = b.sortHits;
         rs.traceFetch = 1;
         const int rc = rbh_render(c, &t);
         rs.traceFetch = 0;
+        rs.sortHits = 0;
         if (rc) return rc;
         const double rate = (double)P * trialSpp / (c->stats.renderMs * 1e-3);
         if (rate > bestRate) {
             bestRate = rate;
             b.samplesPerSecond = rate;
             b.traceFetch = 1;
-            b.quantNodes = 0;    // the work-fetch kernels read the 128-byte nodes and push hits unsorted
-            b.sortHits = 0;
+            b.quantNodes = 0;    // the work-fetch kernels read the 128-byte nodes
         }
     }
     if (bestRate >= 0.0) {

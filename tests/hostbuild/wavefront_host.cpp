@@ -111,7 +111,7 @@ int build_bvh(const float* v, const uint32_t* idx, uint32_t n, const uint32_t* t
         r.v0x = v[3 * a]; r.v0y = v[3 * a + 1]; r.v0z = v[3 * a + 2]; r.prim = t;
         r.v1x = v[3 * b]; r.v1y = v[3 * b + 1]; r.v1z = v[3 * b + 2]; r.material = triMat[t];
         r.v2x = v[3 * c]; r.v2y = v[3 * c + 1]; r.v2z = v[3 * c + 2]; r.light = triLight[t];
-        r.i0 = a; r.i1 = b; r.i2 = c; r.pad = 0u;
+        r.i0 = a; r.i1 = b; r.i2 = c; r.shadeClass = 0u;
         out.tris[j] = r;
         float* lb = &lbox[6 * (size_t)j];
         lb[0] = std::fmin(std::fmin(r.v0x, r.v1x), r.v2x); lb[3] = std::fmax(std::fmax(r.v0x, r.v1x), r.v2x);
@@ -383,7 +383,7 @@ int host_render(const HostScene* hs, const rbh_launch_params* lp, uint32_t passS
     pb.nf = new float4[NPmax]; pb.hit = new float4[NPmax]; pb.Lnee = new float4[NPmax];
     pb.sro = new float4[NPmax]; pb.srd = new float4[NPmax];
     pb.depth = new uint32_t[NPmax];
-    pb.q0 = new uint32_t[NPmax]; pb.q1 = new uint32_t[NPmax]; pb.hq = new uint32_t[NPmax];
+    pb.q0 = new uint32_t[NPmax]; pb.q1 = new uint32_t[NPmax]; pb.hq = new uint32_t[NPmax * kShadeClasses];
     pb.mq = new uint32_t[NPmax]; pb.sq = new uint32_t[NPmax]; pb.prec = new uint32_t[NPmax];
     uint32_t* counters = new uint32_t[CNT_COUNT]();
     unsigned long long* st = new unsigned long long[STAT_COUNT]();
@@ -412,7 +412,10 @@ int host_render(const HostScene* hs, const rbh_launch_params* lp, uint32_t passS
         for (uint32_t b = 0; b < sv.maxDepth; ++b) {
             const uint32_t qsel = b & 1u;
             f.bounce = b;
-            if (b > 0) counters[CNT_HIT] = counters[CNT_MISS] = counters[CNT_TRACE_HEAD] = counters[CNT_HIT_BACK] = 0u;
+            if (b > 0) {
+                counters[CNT_MISS] = counters[CNT_TRACE_HEAD] = 0u;
+                for (uint32_t c = 0; c < kShadeClasses; ++c) counters[hit_counter(c)] = 0u;
+            }
             // closest hit (k_trace, wave-synchronous form)
             {
                 const uint32_t count = counters[qsel ? CNT_QUEUE1 : CNT_QUEUE0];
@@ -440,9 +443,9 @@ int host_render(const HostScene* hs, const rbh_launch_params* lp, uint32_t passS
                                 ++an.hitNonFinite;
                                 an.note("non-finite hit", sv, pid, b, pb.hit[pid], dr);
                             }
-                            const bool back = f.sortHits && materials[tris[h.leaf].material].bxdf == 0u;
-                            if (back) pb.hq[NP - 1u - push(counters + CNT_HIT_BACK)] = pid;
-                            else pb.hq[push(counters + CNT_HIT)] = pid;
+                            // material-sorted shading: one queue segment per shading class
+                            const uint32_t cls = f.sortHits ? shade_class(materials[tris[h.leaf].material]) : 0u;
+                            pb.hq[(size_t)cls * NP + push(counters + hit_counter(cls))] = pid;
                         } else {
                             pb.mq[push(counters + CNT_MISS)] = pid;
                         }
@@ -450,7 +453,9 @@ int host_render(const HostScene* hs, const rbh_launch_params* lp, uint32_t passS
                     delete[] spill;
                 }
                 st[STAT_CLOSEST] += nrays;
-                if ((uint64_t)counters[CNT_HIT] + counters[CNT_HIT_BACK] > NP) { ++an.queueOverflow; break; }
+                uint64_t hits = 0;
+                for (uint32_t c = 0; c < kShadeClasses; ++c) hits += counters[hit_counter(c)];
+                if (hits > NP) { ++an.queueOverflow; break; }
             }
             // sky hits (k_miss)
             {
@@ -463,13 +468,17 @@ int host_render(const HostScene* hs, const rbh_launch_params* lp, uint32_t passS
             counters[CNT_SHADOW_HEAD] = 0u;
             // surface shading (k_shade)
             {
-                const uint32_t nFront = counters[CNT_HIT], count = nFront + counters[CNT_HIT_BACK];
+                uint32_t start[kShadeClasses + 1] = {0u};
+                for (uint32_t c = 0; c < kShadeClasses; ++c) start[c + 1] = start[c] + counters[hit_counter(c)];
+                const uint32_t count = start[kShadeClasses];
                 uint32_t* qout = qsel ? pb.q0 : pb.q1;
                 uint32_t* cout = counters + (qsel ? CNT_QUEUE0 : CNT_QUEUE1);
 #pragma omp parallel for schedule(dynamic, 4096)
                 for (int64_t ii = 0; ii < (int64_t)count; ++ii) {
                     const uint32_t i = (uint32_t)ii;
-                    const uint32_t pid = i < nFront ? pb.hq[i] : pb.hq[NP - 1u - (i - nFront)];
+                    uint32_t cls = 0;
+                    while (cls + 1 < kShadeClasses && i >= start[cls + 1]) ++cls;
+                    const uint32_t pid = pb.hq[(size_t)cls * NP + (i - start[cls])];
                     bool alive = false, shadow = false, rec = false;
                     V3 recDir{0.0f, 0.0f, 0.0f};
                     float recPb = 0.0f;
