@@ -699,52 +699,89 @@ int rbh_render_batch(rbh_ctx owner, const rbh_ctx* scenes, const rbh_launch_para
         depth = std::max(depth, std::max(1u, lps[s].maxPathDepth));
     }
     RenderState& rs = owner->rs;
+    // Pass schedule: each pass packs the scenes' remaining samples shortest
+    // remaining work first (pixels x samples left), whole samples per scene,
+    // up to the pass's slot count: small scenes finish in the first passes
+    // and large ones stream through the rest (shortest-job-first completion
+    // times at full passes). A pass holds at least the largest film and at
+    // most the slot budget; without an explicit budget it is sized for about
+    // kBatchPasses passes.
+    constexpr uint64_t kBatchPasses = 8;
+    uint64_t maxFilm = 0;
+    for (uint32_t s = 0; s < n; ++s) maxFilm = std::max<uint64_t>(maxFilm, (uint64_t)lps[s].width * lps[s].height);
     const size_t budget = slot_budget(owner, (size_t)totalSamples, 200);
-    const uint32_t S = (uint32_t)std::max<uint64_t>(1, budget / std::max<uint64_t>(1, totalP));
+    uint64_t passSlots = rs.slotBudget ? (uint64_t)budget
+                                       : std::min<uint64_t>(budget, (totalSamples + kBatchPasses - 1) / kBatchPasses);
+    passSlots = std::max<uint64_t>(passSlots, maxFilm);
+    if (passSlots > (1ull << 30)) return fail(owner, RBH_ERR_RANGE, "a film of %llu pixels exceeds a pass",
+                                              (unsigned long long)maxFilm);
     std::vector<SceneView> base(n);
     for (uint32_t s = 0; s < n; ++s) {
         fill_view(scenes[s], &lps[s], base[s]);
         // a scene may need fewer bounces than the pass runs; its paths end at its own caps
     }
+    // the schedule, planned on the host: views per pass (sample, S, slot0, first)
+    std::vector<std::vector<SceneView>> tables;
+    std::vector<int> finishedAt(n, -1);
+    {
+        std::vector<uint32_t> done(n, 0u);
+        uint32_t left = n;
+        for (uint32_t s = 0; s < n; ++s)
+            if (lps[s].sampleCount == 0) { --left; finishedAt[s] = -1; }
+        std::vector<uint32_t> order(n);
+        while (left > 0) {
+            for (uint32_t s = 0; s < n; ++s) order[s] = s;
+            std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+                const uint64_t wa = (uint64_t)base[a].P * (lps[a].sampleCount - done[a]);
+                const uint64_t wb = (uint64_t)base[b].P * (lps[b].sampleCount - done[b]);
+                return wa < wb;
+            });
+            std::vector<SceneView> t;
+            uint64_t room = passSlots;
+            uint32_t slot = 0;
+            for (uint32_t s : order) {
+                if (done[s] >= lps[s].sampleCount) continue;
+                const uint64_t fit = room / base[s].P;
+                if (fit == 0) continue;
+                SceneView v = base[s];
+                v.sample = lps[s].sampleOffset + done[s];
+                v.S = (uint32_t)std::min<uint64_t>(fit, lps[s].sampleCount - done[s]);
+                v.slot0 = slot;
+                v.first = (lps[s].sampleOffset == 0 && done[s] == 0) ? 1u : 0u;
+                slot += v.P * v.S;
+                room -= (uint64_t)v.P * v.S;
+                done[s] += v.S;
+                if (done[s] >= lps[s].sampleCount) {
+                    finishedAt[s] = (int)tables.size();
+                    --left;
+                }
+                t.push_back(v);
+            }
+            // slots in ascending order, as the kernels' scene lookup needs
+            tables.push_back(std::move(t));
+        }
+    }
     Frame1 f{};
     LaunchShape ls;
-    int rc = prepare_pass_state(owner, (size_t)totalP * S, f, ls);
+    int rc = prepare_pass_state(owner, (size_t)passSlots, f, ls);
     if (rc) return rc;
     f.noDrop = 0u;
     for (uint32_t s = 0; s < n; ++s)
         if (lps[s].flags & RBH_RENDER_NO_DROP) f.noDrop = 1u;
-    uint32_t maxPasses = 0;
-    for (uint32_t s = 0; s < n; ++s) maxPasses = std::max(maxPasses, (lps[s].sampleCount + S - 1) / S);
-    RBH_ALLOC(owner, rs.sceneTable, (size_t)std::max(1u, maxPasses) * n * sizeof(SceneView));
+    RBH_ALLOC(owner, rs.sceneTable, std::max<size_t>(1, tables.size()) * n * sizeof(SceneView));
     KernelTimer* kt = (lps[0].flags & RBH_RENDER_TIMING) ? &rs.timer : nullptr;
     if (kt) kt->reset();
-    std::vector<uint32_t> done(n, 0u);
-    std::vector<int> finishedAt(n, -1);
-    std::vector<std::vector<SceneView>> tables;   // kept alive until the stream has consumed them
     std::vector<hipEvent_t> passEnd;
     RBH_HIP(owner, hipEventRecord(owner->ev0, owner->stream));
     hipLaunchKernelGGL(k_zero_stats, dim3(1), dim3(64), 0, owner->stream, f.stats);
     uint32_t passes = 0;
     int err = RBH_OK;
-    while (true) {
-        std::vector<SceneView> t;
+    for (const std::vector<SceneView>& tb : tables) {
         uint32_t slot = 0, maxP = 0;
-        for (uint32_t s = 0; s < n; ++s) {
-            if (done[s] >= lps[s].sampleCount) continue;
-            SceneView v = base[s];
-            v.sample = lps[s].sampleOffset + done[s];
-            v.S = std::min(S, lps[s].sampleCount - done[s]);
-            v.slot0 = slot;
-            v.first = (lps[s].sampleOffset == 0 && done[s] == 0) ? 1u : 0u;
-            slot += v.P * v.S;
+        for (const SceneView& v : tb) {
+            slot = v.slot0 + v.P * v.S;
             maxP = std::max(maxP, v.P);
-            done[s] += v.S;
-            if (done[s] >= lps[s].sampleCount) finishedAt[s] = (int)passes;
-            t.push_back(v);
         }
-        if (t.empty()) break;
-        tables.push_back(std::move(t));
-        const std::vector<SceneView>& tb = tables.back();
         // each pass reads its own table slice, so uploads never overwrite a table in use
         SceneView* dtab = rs.sceneTable.as<SceneView>() + (size_t)passes * n;
         if (hipMemcpyAsync(dtab, tb.data(), tb.size() * sizeof(SceneView), hipMemcpyHostToDevice, owner->stream) !=
