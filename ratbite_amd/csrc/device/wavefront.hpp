@@ -74,8 +74,8 @@ __device__ __forceinline__ Wl path_wavelengths(const SceneView& S, uint32_t pid)
     return slot_wavelengths(S, sk, x, y);
 }
 
-// Camera ray of path slot i and a fresh path state (pathtrace.cu:1114-1160).
-__device__ __forceinline__ void camera_slot(const Frame1& f, const SceneView& S, uint32_t i) {
+// Camera ray of path slot i (pathtrace.cu:1114-1160).
+__device__ __forceinline__ void camera_ray(const SceneView& S, uint32_t i, V3& o, V3& d) {
     const uint32_t local = i - S.slot0;
     const uint32_t k = local / S.P, pix = local - k * S.P;
     uint32_t x, y;
@@ -83,7 +83,7 @@ __device__ __forceinline__ void camera_slot(const Frame1& f, const SceneView& S,
     const Qrng q = qrng_at(S.sample + k, 0, pixel_hash(x, y));
     const V2 fo = gaussian_offset(q.s2(QD_FILTER));
     const float cx = (float)x + fo.x, cy = (float)y + fo.y;
-    V3 o{0.0f, 0.0f, 0.0f}, d;
+    o = V3{0.0f, 0.0f, 0.0f};
     if (S.dof) {
         // src/device/dir_gen.h:358-374
         const V2 ls = q.s2(QD_LENS);
@@ -101,12 +101,28 @@ __device__ __forceinline__ void camera_slot(const Frame1& f, const SceneView& S,
         const float zs = 2.0f * cy * S.invH - 1.0f;
         d = nrm(S.camV + S.camU * xs * S.psW + S.camW * zs * S.psH);
     }
+}
+
+// The state a path starts with. The first bounce's kernels use it instead of
+// reading it: only the camera ray's origin and direction are stored for
+// every slot (by the first closest-hit pass), the rest is first written by
+// the kernel that shades the slot's first hit or miss.
+__device__ __forceinline__ float4 init_L() { return make_float4(0.0f, 0.0f, 0.0f, 0.0f); }
+__device__ __forceinline__ float4 init_T() { return make_float4(1.0f, 1.0f, 1.0f, 1.0f); }
+// last shading normal 0, no state flags
+__device__ __forceinline__ float4 init_nf() { return make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u)); }
+
+// Camera ray and the whole fresh path state of slot i in memory (the host
+// build of these bodies starts its passes with it).
+__device__ __forceinline__ void camera_slot(const Frame1& f, const SceneView& S, uint32_t i) {
+    V3 o, d;
+    camera_ray(S, i, o, d);
     const PathBuffers& p = f.p;
     p.od[i] = make_float4(o.x, o.y, o.z, 1.0f);           // bxdfPdf = 1
     p.dr[i] = make_float4(d.x, d.y, d.z, 1.0f);           // refractionScale = 1
-    p.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    p.T[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-    p.nf[i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
+    p.L[i] = init_L();
+    p.T[i] = init_T();
+    p.nf[i] = init_nf();
     p.depth[i] = 0u;
     if (f.collect) p.prec[i] = kNoRec;
 }
@@ -130,12 +146,14 @@ __device__ __forceinline__ void snapshot_radiance(const Frame1& f, uint32_t pid,
 // previous vertex's BxDF sample, then the path ends (pathtrace.cu:1185-1274,
 // the sky branch; 1240-1252 for the lookup and its pdf).
 __device__ __forceinline__ void shade_miss(const Frame1& f, const SceneView& S, uint32_t pid) {
-    float4 nf = f.p.nf[pid];
+    const bool first = f.bounce == 0u;
+    float4 nf = first ? init_nf() : f.p.nf[pid];
     uint32_t flags = __float_as_uint(nf.w);
     flags &= ~PF_TRIANGULAR;
-    if (f.collect) snapshot_radiance(f, pid, f.p.L[pid]);
+    if (f.collect) snapshot_radiance(f, pid, first ? init_L() : f.p.L[pid]);
+    if (first && !S.envOn) f.p.L[pid] = init_L();
     if (S.envOn) {
-        const uint32_t depth = f.p.depth[pid];
+        const uint32_t depth = first ? 0u : f.p.depth[pid];
         const float4 od = f.p.od[pid], dr = f.p.dr[pid];
         const V3 rd{dr.x, dr.y, dr.z};
         const LightTree tree{S.treeNodes, S.treePointers, S.treeMasks, true};
@@ -151,8 +169,8 @@ __device__ __forceinline__ void shade_miss(const Frame1& f, const SceneView& S, 
         const S4 Le = ColorTables{S.colors}.rgb(sky, wl);
         lightPdf *= treePdf;
         const float w = prevNEE ? power_heuristic(od.w, lightPdf) : 1.0f;
-        const float4 Tv = f.p.T[pid];
-        float4 L = f.p.L[pid];
+        const float4 Tv = first ? init_T() : f.p.T[pid];
+        float4 L = first ? init_L() : f.p.L[pid];
         L.x += Tv.x * Le.v[0] * w;
         L.y += Tv.y * Le.v[1] * w;
         L.z += Tv.z * Le.v[2] * w;
@@ -281,19 +299,20 @@ __device__ __forceinline__ void shade_hit(const Frame1& f, const SceneView& S, u
     const Luts lut{S.luts};
     const LightTree tree{S.treeNodes, S.treePointers, S.treeMasks, S.envOn != 0};
     const bool treeOn = S.treeNodes != nullptr;
-    const float4 od = f.p.od[pid], dr = f.p.dr[pid], nf = f.p.nf[pid], hr = f.p.hit[pid];
+    const bool first = f.bounce == 0u;
+    const float4 od = f.p.od[pid], dr = f.p.dr[pid], nf = first ? init_nf() : f.p.nf[pid], hr = f.p.hit[pid];
     PathIO P;
     P.rd = V3{dr.x, dr.y, dr.z};
     P.refr = dr.w;
     P.bxdfPdf = od.w;
     P.flags = __float_as_uint(nf.w);
-    float4 Lv = f.p.L[pid], Tv = f.p.T[pid];
+    float4 Lv = first ? init_L() : f.p.L[pid], Tv = first ? init_T() : f.p.T[pid];
     if (GUIDED && f.collect) snapshot_radiance(f, pid, Lv);
     S4 L{{Lv.x, Lv.y, Lv.z, Lv.w}};
     P.T = S4{{Tv.x, Tv.y, Tv.z, Tv.w}};
     V3 ro{od.x, od.y, od.z};
     V3 lastN{nf.x, nf.y, nf.z};
-    uint32_t depth = f.p.depth[pid];
+    uint32_t depth = first ? 0u : f.p.depth[pid];
     const uint32_t local = pid - S.slot0;
     const uint32_t sk = local / S.P, pix = local - sk * S.P;
     uint32_t px, py;

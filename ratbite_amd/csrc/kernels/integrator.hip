@@ -8,7 +8,9 @@
 // paths is paid once per pass rather than once per sample; path state lives in
 // structure-of-arrays buffers in HBM. Per bounce:
 //   k_trace    closest hit for the queued paths           (pathtrace.cu:1162-1177),
-//              sorting them into a hit queue and a miss queue
+//              sorting them into a hit queue (by shading class) and a miss
+//              queue; at the first bounce it generates the camera rays
+//              (pathtrace.cu:1114-1160)
 //   k_miss     sky emission + MIS for the misses; the path ends
 //              (pathtrace.cu:1185-1274, sky branch)
 //   k_shade    emission + MIS, surface attributes, light-tree NEE, BxDF,
@@ -17,7 +19,8 @@
 //              the bounce when the light is visible        (pathtrace.cu:1704-1720)
 // and after the last bounce k_accumulate adds each sample's XYZ to the FP64
 // accumulation buffer (pathtrace.cu:1789-1812). Queues are compacted with
-// wave ballots (one atomic per wave). Queue lengths stay on the device; every
+// wave ballots; the trace and shade kernels stage finished paths in per-wave
+// LDS lists and write them 64 or more at a time (one atomic per run). Queue lengths stay on the device; every
 // kernel reads its count and exits early, so the host issues a fixed launch
 // sequence per pass.
 #include "../context.hpp"
@@ -57,13 +60,10 @@ __device__ __forceinline__ void wave_count(unsigned long long* __restrict__ coun
     if (__lane_id() == 0 && s) atomicAdd(counter, (unsigned long long)s);
 }
 
-template <bool BATCH>
-__global__ __launch_bounds__(256) void k_camera(Frame1 f) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= f.NP) return;
-    const SceneView& S = scene_of<BATCH>(f, i);
-    camera_slot(f, S, i);
-    if (i == 0) {
+// The queue counters at the start of a pass: every slot traces its camera
+// ray (the first closest-hit pass generates it).
+__global__ void k_pass_start(Frame1 f) {
+    if (threadIdx.x == 0) {
         f.counters[CNT_QUEUE0] = f.NP;
         f.counters[CNT_QUEUE1] = 0u;
         f.counters[CNT_SHADOW] = 0u;
@@ -72,6 +72,23 @@ __global__ __launch_bounds__(256) void k_camera(Frame1 f) {
         f.counters[CNT_TRACE_HEAD] = 0u;
         f.counters[CNT_SHADOW_HEAD] = 0u;
         for (uint32_t c = 1; c < kShadeClasses; ++c) f.counters[hit_counter(c)] = 0u;
+    }
+}
+
+// The ray a closest-hit pass traces for path slot pid: at the first bounce
+// the camera ray, generated here and stored for the shading kernels (with a
+// fresh training-record chain when collecting), afterwards the stored ray.
+template <bool BATCH>
+__device__ __forceinline__ void pass_ray(const Frame1& f, const SceneView& S, uint32_t pid, bool first, V3& o, V3& d) {
+    if (first) {
+        camera_ray(S, pid, o, d);
+        f.p.od[pid] = make_float4(o.x, o.y, o.z, 1.0f);   // bxdfPdf = 1
+        f.p.dr[pid] = make_float4(d.x, d.y, d.z, 1.0f);   // refractionScale = 1
+        if (f.collect) f.p.prec[pid] = kNoRec;
+    } else {
+        const float4 od = f.p.od[pid], dr = f.p.dr[pid];
+        o = V3{od.x, od.y, od.z};
+        d = V3{dr.x, dr.y, dr.z};
     }
 }
 
@@ -257,8 +274,9 @@ __global__ __launch_bounds__(kTraceBlock, WAVES) void k_trace(Frame1 f, uint32_t
                 const SceneView& S = scene_of<BATCH>(f, pid);
                 nodes = S.nodes4;
                 tris = S.tris;
-                const float4 od = f.p.od[pid], dr = f.p.dr[pid];
-                trav_init(ray, V3{od.x, od.y, od.z}, V3{dr.x, dr.y, dr.z}, 3.402823466e+38f, S.nnodes4);
+                V3 o, d;
+                pass_ray<BATCH>(f, S, pid, identity, o, d);
+                trav_init(ray, o, d, 3.402823466e+38f, S.nnodes4);
                 // the nearest shape light bounds the walk; a nearer triangle replaces it
                 if (S.nshapes) shape_closest(S.shapes, S.nshapes, ray.o, ray.d, 3.402823466e+38f, ray.h.t, ray.h.leaf);
                 active = true;
@@ -287,8 +305,8 @@ __global__ __launch_bounds__(kTraceBlock, WAVES) void k_trace(Frame1 f, uint32_t
             if (i < count) {
                 pid = identity ? i : queue[i];
                 const SceneView& S = scene_of<BATCH>(f, pid);
-                const float4 od = f.p.od[pid], dr = f.p.dr[pid];
-                const V3 o{od.x, od.y, od.z}, d{dr.x, dr.y, dr.z};
+                V3 o, d;
+                pass_ray<BATCH>(f, S, pid, identity, o, d);
                 // the nearest shape light bounds the walk; a nearer triangle replaces it
                 float ts = 3.402823466e+38f;
                 uint32_t shape = 0xFFFFFFFFu;
@@ -536,7 +554,6 @@ static hipError_t launch_pass_t(rbh_ctx c, Frame1 f, uint32_t depthMax, const La
     int cus = 256;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0) cus = 256;
     const int mlpGrid = 2 * cus;
-    const int nb = (int)((f.NP + 255) / 256);
     const dim3 accGrid((f.maxP + 255) / 256, BATCH ? f.nscenes : 1u);
     hipError_t e = hipSuccess;
 #define RBH_TIMED(cls, launch)                                       \
@@ -547,7 +564,7 @@ static hipError_t launch_pass_t(rbh_ctx c, Frame1 f, uint32_t depthMax, const La
     } while (0)
     f.bounce = 0;
     f.sortHits = ls.sortHits ? 1u : 0u;
-    hipLaunchKernelGGL(k_camera<BATCH>, dim3(nb), dim3(256), 0, c->stream, f);
+    hipLaunchKernelGGL(k_pass_start, dim3(1), dim3(64), 0, c->stream, f);
     for (uint32_t b = 0; b < depthMax; ++b) {
         const uint32_t qsel = b & 1u;
         f.bounce = b;
