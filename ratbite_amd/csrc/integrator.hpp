@@ -110,6 +110,7 @@ struct Frame1 {
     uint32_t NP;                  // path slots of the pass (all scenes)
     uint32_t maxP;                // largest pixel count of a scene in the pass
     uint32_t bounce;
+    uint32_t camOrigin0;          // no scene of the pass uses depth of field: camera rays start at 0
     uint32_t sortHits;            // LaunchShape::sortHits
     // state
     PathBuffers p;

@@ -243,7 +243,12 @@ __device__ __forceinline__ auto scene_nodes(const SceneView& S) {
 // WAVES: minimum resident waves per SIMD the register allocation must allow
 // (1 = unconstrained), traded by the launch shape against spills; QN: the
 // quantised 64-byte nodes
-template <bool BATCH, bool FETCH, int WAVES, bool QN>
+// CAM0: the first bounce of a pass without depth of field, whose camera
+// rays all start at the camera position, the origin of the camera-relative
+// scene: the origin is the constant 0 there, which removes the origin
+// subtractions of every box and triangle test (x - 0 is x exactly, so the
+// walk is the same).
+template <bool BATCH, bool FETCH, int WAVES, bool QN, bool CAM0>
 __global__ __launch_bounds__(kTraceBlock, WAVES) void k_trace(Frame1 f, uint32_t qsel) {
     __shared__ int stack[kStackLds * kTraceBlock];
     __shared__ uint32_t staged[2 * kTraceBlock * 2];   // per wave: 128 hits, 128 misses
@@ -276,6 +281,7 @@ __global__ __launch_bounds__(kTraceBlock, WAVES) void k_trace(Frame1 f, uint32_t
                 tris = S.tris;
                 V3 o, d;
                 pass_ray<BATCH>(f, S, pid, identity, o, d);
+                if constexpr (CAM0) o = V3{0.0f, 0.0f, 0.0f};
                 trav_init(ray, o, d, 3.402823466e+38f, S.nnodes4);
                 // the nearest shape light bounds the walk; a nearer triangle replaces it
                 if (S.nshapes) shape_closest(S.shapes, S.nshapes, ray.o, ray.d, 3.402823466e+38f, ray.h.t, ray.h.leaf);
@@ -307,6 +313,7 @@ __global__ __launch_bounds__(kTraceBlock, WAVES) void k_trace(Frame1 f, uint32_t
                 const SceneView& S = scene_of<BATCH>(f, pid);
                 V3 o, d;
                 pass_ray<BATCH>(f, S, pid, identity, o, d);
+                if constexpr (CAM0) o = V3{0.0f, 0.0f, 0.0f};
                 // the nearest shape light bounds the walk; a nearer triangle replaces it
                 float ts = 3.402823466e+38f;
                 uint32_t shape = 0xFFFFFFFFu;
@@ -514,9 +521,15 @@ size_t trace_spill_words(size_t lanes) { return lanes * (size_t)(kMaxStack4 - kS
 template <bool BATCH, bool FETCH, bool QN>
 static void launch_trace_q(rbh_ctx c, const Frame1& f, uint32_t qsel, const LaunchShape& ls) {
     const dim3 g(ls.traceGrid), b(kTraceBlock);
-    if (ls.traceWaves >= 8) hipLaunchKernelGGL((k_trace<BATCH, FETCH, 8, QN>), g, b, 0, c->stream, f, qsel);
-    else if (ls.traceWaves == 7) hipLaunchKernelGGL((k_trace<BATCH, FETCH, 7, QN>), g, b, 0, c->stream, f, qsel);
-    else hipLaunchKernelGGL((k_trace<BATCH, FETCH, 1, QN>), g, b, 0, c->stream, f, qsel);
+    if (f.bounce == 0u && f.camOrigin0) {
+        if (ls.traceWaves >= 8) hipLaunchKernelGGL((k_trace<BATCH, FETCH, 8, QN, true>), g, b, 0, c->stream, f, qsel);
+        else if (ls.traceWaves == 7) hipLaunchKernelGGL((k_trace<BATCH, FETCH, 7, QN, true>), g, b, 0, c->stream, f, qsel);
+        else hipLaunchKernelGGL((k_trace<BATCH, FETCH, 1, QN, true>), g, b, 0, c->stream, f, qsel);
+        return;
+    }
+    if (ls.traceWaves >= 8) hipLaunchKernelGGL((k_trace<BATCH, FETCH, 8, QN, false>), g, b, 0, c->stream, f, qsel);
+    else if (ls.traceWaves == 7) hipLaunchKernelGGL((k_trace<BATCH, FETCH, 7, QN, false>), g, b, 0, c->stream, f, qsel);
+    else hipLaunchKernelGGL((k_trace<BATCH, FETCH, 1, QN, false>), g, b, 0, c->stream, f, qsel);
 }
 
 // the work-fetch variants read the 128-byte nodes

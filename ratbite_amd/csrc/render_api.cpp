@@ -628,6 +628,7 @@ int rbh_render(rbh_ctx c, const rbh_launch_params* lp) {
     if (rc) return rc;
     f.maxP = (uint32_t)P;
     f.noDrop = (guided || (lp->flags & RBH_RENDER_NO_DROP)) ? 1u : 0u;
+    f.camOrigin0 = lp->depthOfField ? 0u : 1u;
     if (guided) {
         RBH_ALLOC(c, rs.gfeat, NP * 16 * sizeof(float));
         RBH_ALLOC(c, rs.gmix, NP * (guide::kOut) * sizeof(float));   // lobes + selection per hit
@@ -766,8 +767,11 @@ int rbh_render_batch(rbh_ctx owner, const rbh_ctx* scenes, const rbh_launch_para
     int rc = prepare_pass_state(owner, (size_t)passSlots, f, ls);
     if (rc) return rc;
     f.noDrop = 0u;
-    for (uint32_t s = 0; s < n; ++s)
+    f.camOrigin0 = 1u;
+    for (uint32_t s = 0; s < n; ++s) {
         if (lps[s].flags & RBH_RENDER_NO_DROP) f.noDrop = 1u;
+        if (lps[s].depthOfField) f.camOrigin0 = 0u;
+    }
     RBH_ALLOC(owner, rs.sceneTable, std::max<size_t>(1, tables.size()) * n * sizeof(SceneView));
     KernelTimer* kt = (lps[0].flags & RBH_RENDER_TIMING) ? &rs.timer : nullptr;
     if (kt) kt->reset();
