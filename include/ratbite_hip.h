@@ -308,8 +308,8 @@ int rbh_envmap_read(rbh_ctx ctx, uint16_t* condIdx, uint16_t* margIdx, float* cd
  * hipMalloc). The build and inversion of rbh_envmap_build, from and into
  * caller-owned DEVICE buffers: texels (w*h*channels floats), condIdx (w*h),
  * margIdx (h), cdfY (h) and integralOut (one double: the integral divided by
- * w*h, as rbh_envmap_read returns it). The build's scratch (h row terms,
- * a counter) is a fixed array of the library on each device, so
+ * w*h, as rbh_envmap_read returns it). The build's scratch (h row terms
+ * and the raw integral) is a fixed array of the library on each device, so
  * h <= RBH_ENVMAP_NOALLOC_MAX_ROWS; the call allocates nothing and only
  * launches kernels on the context's stream (no host synchronisation: the
  * outputs are ready when the stream reaches them). Calls that overlap on one

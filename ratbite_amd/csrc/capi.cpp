@@ -279,9 +279,6 @@ int rbh_envmap_build(rbh_ctx c, const float* texels, int w, int h, int ch, uint3
     RBH_ALLOC(c, e.marg, (size_t)h * sizeof(uint16_t));
     RBH_ALLOC(c, e.cdfY, (size_t)h * sizeof(float));
     RBH_ALLOC(c, e.integralDev, envmap_scratch_bytes(h));
-    bool fresh = e.counter.p == nullptr;
-    RBH_ALLOC(c, e.counter, 256);
-    if (fresh) RBH_HIP(c, hipMemset(e.counter.p, 0, 256));
     e.width = w; e.height = h; e.channels = ch;
     RBH_HIP(c, hipMemcpyAsync(e.texels.p, texels, (size_t)w * h * ch * sizeof(float), hipMemcpyHostToDevice, c->stream));
     int r = envmap_run(c, flags);

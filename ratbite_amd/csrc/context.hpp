@@ -46,7 +46,6 @@ struct EnvMap {
     DevBuf cond, marg;           // uint16 inverted-CDF indices
     DevBuf cdfY;                 // h floats (normalised marginal CDF)
     DevBuf integralDev;          // raw total (float) + double slot
-    DevBuf counter;              // last-block counter
 };
 
 struct LightTreeDev {

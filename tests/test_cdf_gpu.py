@@ -13,8 +13,9 @@ from ratbite_amd.envmaps import lognormal_envmap
 
 pytestmark = pytest.mark.gpu
 
-# one to four rows per workgroup (LDS rows of max(w, h) floats), one to eight
-# chunks of the reference's block per row, non-power-of-two sizes
+# one to eight chunks of the reference's block per row (the marginal up to
+# four), rows with and without the linear inversion's marks (up to 16384 and
+# above), non-power-of-two sizes
 SIZES = [(64, 32, 3), (256, 128, 4), (1024, 512, 3), (2048, 1024, 3), (4096, 2048, 3), (1000, 500, 3),
          (8192, 16, 4), (4, 4, 3), (16384, 8, 3), (32768, 4, 3), (12, 4096, 3)]
 
