@@ -45,6 +45,16 @@ int rbh_test_lbvh_read(rbh_ctx ctx, uint64_t* keys, void* nodes, void* tris, uin
  * counterpart): calibrates HBM byte counters and measures copy bandwidth;
  * *ms = average time per copy. */
 int rbh_test_copy(rbh_ctx ctx, uint64_t bytes, int repeats, float* ms);
+/* The LBVH build's stable radix sort (replaces the sort inside
+ * optixAccelBuild; no reference source): n 64-bit keys (and, when vals is
+ * non-null, n 32-bit values) sorted by bits [beginBit, endBit) into
+ * outKeys / outVals. Host buffers. */
+int rbh_test_radix_sort(rbh_ctx ctx, const uint64_t* keys, const uint32_t* vals, uint32_t n, int beginBit, int endBit,
+                        uint64_t* outKeys, uint32_t* outVals);
+/* The guide's training-set compaction (no reference counterpart): the
+ * indices i < n with weights[i] > 0, in increasing order, into out; their
+ * number into *count. Host buffers. */
+int rbh_test_select_positive(rbh_ctx ctx, const float* weights, uint32_t n, uint32_t* out, uint32_t* count);
 /* traverseLightTree (src/kernels/pathtrace.cu:1002-1049) on the built tree.
  * lights: n light indices; posNrm: n x 6; out: n pdfs */
 int rbh_test_light_pdf(rbh_ctx ctx, const uint32_t* lights, const float* posNrm, float* pdf, uint32_t n);

@@ -242,6 +242,8 @@ register({
     "rbh_test_oct": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint32]),
     "rbh_test_copy": (c_int, [c_void_p, c_uint64, c_int, ctypes.POINTER(ctypes.c_float)]),
     "rbh_test_lbvh_read": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_uint32)]),
+    "rbh_test_radix_sort": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_int, c_int, c_void_p, c_void_p]),
+    "rbh_test_select_positive": (c_int, [c_void_p, c_void_p, c_uint32, c_void_p, ctypes.POINTER(c_uint32)]),
     "rbh_test_trace": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_int]),
     "rbh_test_light_pdf": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint32]),
     "rbh_test_light_sample": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint32]),

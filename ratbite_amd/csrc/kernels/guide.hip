@@ -16,7 +16,7 @@
 #include <cstring>
 #include <vector>
 
-#include <hipcub/hipcub.hpp>
+#include "radix.hpp"
 
 #include "../context.hpp"
 #include "../guide.hpp"
@@ -616,11 +616,8 @@ static int prepare_collected(rbh_ctx c, uint32_t n, uint32_t* m) {
     DevBuf idx, tmp;
     RBH_ALLOC(c, idx, (size_t)n * 4 + 16);
     uint32_t* cnt = idx.as<uint32_t>() + n + (4 - n % 4) % 4;
-    size_t tb = 0;
-    const hipcub::CountingInputIterator<uint32_t> it(0u);
-    RBH_HIP(c, hipcub::DeviceSelect::If(nullptr, tb, it, idx.as<uint32_t>(), cnt, (int)n, PositiveWeight{recs}, c->stream));
-    RBH_ALLOC(c, tmp, std::max<size_t>(tb, 16));
-    RBH_HIP(c, hipcub::DeviceSelect::If(tmp.p, tb, it, idx.as<uint32_t>(), cnt, (int)n, PositiveWeight{recs}, c->stream));
+    RBH_ALLOC(c, tmp, radix::select_scratch_bytes(n));
+    RBH_HIP(c, radix::select_indices(tmp.p, n, PositiveWeight{recs}, idx.as<uint32_t>(), cnt, c->stream));
     uint32_t k = 0;
     RBH_HIP(c, hipMemcpyAsync(&k, cnt, 4, hipMemcpyDeviceToHost, c->stream));
     RBH_HIP(c, hipStreamSynchronize(c->stream));

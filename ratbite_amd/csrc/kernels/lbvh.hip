@@ -6,7 +6,7 @@
 //      needs, the Morton code as many of the rest as fit three axes (at most
 //      21 per axis; 14 for 4.1M triangles, where 10 bits per axis would put
 //      84% of the triangles in cells shared with others)
-//   3. LSD radix sort of the keys (rocPRIM via hipCUB)
+//   3. stable LSD radix sort of the keys (radix.hpp: ballot-ranked 8-bit digits)
 //   4. Karras (HPG 2012) hierarchy: internal node i covers a key range and
 //      splits at the highest differing bit
 //   5. refit level by level from the deepest internal nodes up (one launch
@@ -21,7 +21,7 @@
 // Every root-to-leaf path strictly lengthens the common key prefix, so the
 // depth is at most 64 and a 64-entry traversal stack never overflows.
 #include "../context.hpp"
-#include <hipcub/hipcub.hpp>
+#include "radix.hpp"
 
 namespace rbh {
 
@@ -424,14 +424,7 @@ hipError_t build_layout(rbh_ctx c, int n, BuildLayout& L) {
     L.oListA = off; off = align_up(off + nInt * 8);
     L.oListB = off; off = align_up(off + nInt * 8);
     L.oCounts = off; off = align_up(off + 16);
-    size_t sortBytes = 0, levelSortBytes = 0;
-    hipError_t e = hipcub::DeviceRadixSort::SortKeys(nullptr, sortBytes, (unsigned long long*)nullptr,
-                                                     (unsigned long long*)nullptr, n, 0, 64, c->stream);
-    if (e != hipSuccess) return e;
-    e = hipcub::DeviceRadixSort::SortPairs(nullptr, levelSortBytes, (unsigned int*)nullptr, (unsigned int*)nullptr,
-                                           (int*)nullptr, (int*)nullptr, (int)nInt, 0, 7, c->stream);
-    if (e != hipSuccess) return e;
-    L.sortBytes = std::max(sortBytes, levelSortBytes);
+    L.sortBytes = std::max(radix::sort_scratch_bytes(n, 8, 0), radix::sort_scratch_bytes(nInt, 4, 4));
     L.oSort = off; off = align_up(off + L.sortBytes);
     L.total = off;
     return hipSuccess;
@@ -546,7 +539,8 @@ int lbvh_build(rbh_ctx c) {
     acc.keyIndexBits = idxBits;
     hipLaunchKernelGGL(k_morton, dim3(nb), dim3(kBuildBlock), 0, c->stream, v, idx, (uint32_t)n, bounds, axisBits,
                        idxBits, keys);
-    RBH_HIP(c, hipcub::DeviceRadixSort::SortKeys(s + L.oSort, L.sortBytes, keys, keys2, n, 0, 64, c->stream));
+    RBH_HIP(c, (radix::sort<unsigned long long, uint32_t>(s + L.oSort, keys, keys2, nullptr, nullptr, (uint32_t)n, 0, 64,
+                                                          c->stream)));
     const int nbi = (int)((nInt + kBuildBlock - 1) / kBuildBlock);
     if (n > 1) {
         hipLaunchKernelGGL(k_karras, dim3(nbi), dim3(kBuildBlock), 0, c->stream, keys2, n, child, parent);
@@ -562,8 +556,7 @@ int lbvh_build(rbh_ctx c) {
         int* ids = reinterpret_cast<int*>(jb);                       // free after the last round
         unsigned int* dsorted = reinterpret_cast<unsigned int*>(ids + nInt);
         hipLaunchKernelGGL(k_depth_keys, dim3(nbi), dim3(kBuildBlock), 0, c->stream, (int)nInt, ja, dkey, ids);
-        RBH_HIP(c, hipcub::DeviceRadixSort::SortPairs(s + L.oSort, L.sortBytes, dkey, dsorted, ids, order, (int)nInt, 0,
-                                                      7, c->stream));
+        RBH_HIP(c, (radix::sort<unsigned int, int>(s + L.oSort, dkey, dsorted, ids, order, (uint32_t)nInt, 0, 7, c->stream)));
         hipLaunchKernelGGL(k_level_bounds, dim3(1), dim3(128), 0, c->stream, dsorted, (int)nInt, lstart);
     }
     hipLaunchKernelGGL(k_depth, dim3(nb), dim3(kBuildBlock), 0, c->stream, n, parent, bounds + 6);

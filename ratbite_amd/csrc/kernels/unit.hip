@@ -19,6 +19,7 @@
 #include "../device/lighttree.hpp"
 #include "../device/qrng.hpp"
 #include "../device/traverse.hpp"
+#include "radix.hpp"
 
 namespace rbh {
 using namespace dev;
@@ -179,6 +180,11 @@ struct Staging {
 
 inline dim3 grid(uint32_t n, int b) { return dim3(n ? (n + b - 1) / b : 1u); }
 
+struct PositiveAt {
+    const float* w;
+    __device__ __forceinline__ bool operator()(uint32_t i) const { return w[i] > 0.0f; }
+};
+
 // plain streaming copy, one 16-byte element per lane and no loop (a grid of
 // n / 256 blocks): the reference point for the HBM byte counters and the
 // measured bandwidth ceiling. On MI355X this form moves 6.3 TB/s read+write at
@@ -323,6 +329,37 @@ int rbh_test_copy(rbh_ctx c, uint64_t bytes, int repeats, float* ms) {
     float t = 0.0f;
     RBH_HIP(c, hipEventElapsedTime(&t, c->ev0, c->ev1));
     if (ms) *ms = t / repeats;
+    return RBH_OK;
+}
+
+int rbh_test_radix_sort(rbh_ctx c, const uint64_t* keys, const uint32_t* vals, uint32_t n, int beginBit, int endBit,
+                        uint64_t* outKeys, uint32_t* outVals) {
+    if (!c || !keys || !outKeys || (vals && !outVals)) return RBH_ERR_INVALID;
+    if (beginBit < 0 || endBit > 64 || beginBit > endBit) return fail(c, RBH_ERR_INVALID, "bad bit range");
+    Staging s{c};
+    const uint64_t* dk = static_cast<const uint64_t*>(s.in(keys, (size_t)n * 8));
+    const uint32_t* dv = vals ? static_cast<const uint32_t*>(s.in(vals, (size_t)n * 4)) : nullptr;
+    uint64_t* dko = static_cast<uint64_t*>(s.out((size_t)n * 8));
+    uint32_t* dvo = vals ? static_cast<uint32_t*>(s.out((size_t)n * 4)) : nullptr;
+    void* scratch = s.out(radix::sort_scratch_bytes(n, 8, vals ? 4 : 0));
+    STAGE_CHECK(dk && dko && scratch && (!vals || (dv && dvo)));
+    RBH_HIP(c, (radix::sort<uint64_t, uint32_t>(scratch, dk, dko, dv, dvo, n, beginBit, endBit, 0)));
+    FINISH(outKeys, dko, (size_t)n * 8);
+    if (vals) RBH_HIP(c, hipMemcpy(outVals, dvo, (size_t)n * 4, hipMemcpyDeviceToHost));
+    return RBH_OK;
+}
+
+int rbh_test_select_positive(rbh_ctx c, const float* weights, uint32_t n, uint32_t* out, uint32_t* count) {
+    if (!c || !weights || !out || !count) return RBH_ERR_INVALID;
+    Staging s{c};
+    const float* dw = static_cast<const float*>(s.in(weights, (size_t)n * 4));
+    uint32_t* dout = static_cast<uint32_t*>(s.out((size_t)n * 4 + 4));
+    void* scratch = s.out(radix::select_scratch_bytes(n));
+    STAGE_CHECK(dw && dout && scratch);
+    RBH_HIP(c, radix::select_indices(scratch, n, PositiveAt{dw}, dout, dout + n, 0));
+    FINISH(count, dout + n, 4);
+    if (*count > n) return fail(c, RBH_ERR_INVALID, "selection count %u above %u", *count, n);
+    RBH_HIP(c, hipMemcpy(out, dout, (size_t)*count * 4, hipMemcpyDeviceToHost));
     return RBH_OK;
 }
 

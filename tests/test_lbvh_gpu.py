@@ -183,11 +183,12 @@ def _read_lbvh(ctx, n):
     return keys, nodes[:nn.value], tris
 
 
-@pytest.mark.parametrize("which", ["soup", "spheres"])
+@pytest.mark.parametrize("which", ["soup", "spheres", "soup_large"])
 def test_morton_keys_bit_exact_and_tree_invariants(ctx, which):
-    if which == "soup":
+    # soup_large: 300k triangles, 74 tiles of the radix sort's 4096 keys
+    if which.startswith("soup"):
         rng = np.random.default_rng(21)
-        ntri = 3000
+        ntri = 3000 if which == "soup" else 300000
         c = rng.uniform(-5, 5, (ntri, 3))
         v = (c[:, None, :] + rng.normal(scale=0.5, size=(ntri, 3, 3))).reshape(-1, 3).astype(np.float32)
         idx = np.arange(3 * ntri, dtype=np.uint32).reshape(-1, 3)
