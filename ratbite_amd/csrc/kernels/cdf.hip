@@ -338,16 +338,32 @@ __device__ __forceinline__ void invert_group(const float* __restrict__ cdf, uint
         invert_search(cdf, n, out, pb, pe);
         return;
     }
-    // one pass over the probed entries: the order check and the marks
+    // one pass over the probed entries: the order check and the marks. Each
+    // lane takes G consecutive entries; the first query above the entry after
+    // its last one comes from the next lane (lane 63 computes it), so
+    // first_above runs G + 1 times per G entries instead of 2 G.
+    constexpr int G = 4;
     bool bad = false;
-    for (int m = 1 + (int)threadIdx.x; m <= n - 2; m += 64 * WPR) {
-        const float c = cdf[m];
-        const bool last = m + 1 > n - 2;
-        const float cn = last ? 0.0f : cdf[m + 1];
-        bad = bad || !(c >= 0.0f) || (!last && !(cn >= c));
-        const int jm = first_above(c, n);
-        const int jn = last ? n : first_above(cn, n);
-        if (jm < jn && jm < n) marks[jm] = (uint16_t)(m + 1);
+    for (int base = 1 + G * (int)threadIdx.x; base - G * lane <= n - 2; base += G * 64 * WPR) {
+        float c[G + 1];
+        int j[G + 1];
+#pragma unroll
+        for (int g = 0; g <= G; ++g) c[g] = base + g <= n - 1 ? cdf[base + g] : 0.0f;
+#pragma unroll
+        for (int g = 0; g < G; ++g) j[g] = base + g <= n - 2 ? first_above(c[g], n) : n;
+        j[G] = __shfl_down(j[0], 1);
+        if (lane == 63) j[G] = base + G <= n - 2 ? first_above(c[G], n) : n;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int m = base + g;
+            if (m <= n - 2) {
+                const bool last = m + 1 > n - 2;
+                bad = bad || !(c[g] >= 0.0f) || (!last && !(c[g + 1] >= c[g]));
+                const int jm = j[g];
+                const int jn = last ? n : j[g + 1];
+                if (jm < jn && jm < n) marks[jm] = (uint16_t)(m + 1);
+            }
+        }
     }
     if (__ballot(bad) != 0ull && lane == 0) *badFlag = 1;
     __syncthreads();
