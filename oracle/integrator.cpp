@@ -560,7 +560,10 @@ void render_pixel(const Ctx& C, uint32_t px, uint32_t py, double res[4], Stats& 
                         float frx = fcx - flx, fry = fcy - fly;
                         uint32_t ix = (uint32_t)flx, iy = (uint32_t)fly;
                         float m0 = sc->envMarg[iy], m1 = sc->envMarg[iy + 1];
-                        size_t row = (size_t)(uint32_t)(m0 * W);
+                        // pathtrace.cu:1670 forms the row offset as a float product; where
+                        // it rounds (W with a large odd factor, m0 * W above 2^24) the read
+                        // could pass the table's end, so the offset is capped at the last row
+                        size_t row = std::min((size_t)(uint32_t)(m0 * W), (size_t)(sc->envHeight - 1) * sc->envWidth);
                         float c0 = sc->envCond[row + ix], c1 = sc->envCond[row + ix + 1];
                         float ux = mixf(c0, c1, frx) * (1.0f / W);
                         float uy = mixf(m0, m1, fry) * (1.0f / H);

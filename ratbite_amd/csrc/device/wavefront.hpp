@@ -606,7 +606,10 @@ __device__ __forceinline__ void shade_hit(const Frame1& f, const SceneView& S, u
                     const float flx = floorf(fcx), fly = floorf(fcy);
                     const uint32_t ix = (uint32_t)flx, iy = (uint32_t)fly;
                     const float m0 = (float)S.envMarg[iy], m1 = (float)S.envMarg[iy + 1];
-                    const size_t row = (size_t)(uint32_t)(m0 * W);
+                    // the reference's float row offset (pathtrace.cu:1670), capped at
+                    // the last row: where m0 * W rounds (W with a large odd factor,
+                    // products above 2^24) it could otherwise read past the table
+                    const size_t row = min((size_t)(uint32_t)(m0 * W), (size_t)(S.envH - 1) * (size_t)S.envW);
                     const float c0 = (float)S.envCond[row + ix], c1 = (float)S.envCond[row + ix + 1];
                     const float ux = lerpf(c0, c1, fcx - flx) * (1.0f / W);
                     const float uy = lerpf(m0, m1, fcy - fly) * (1.0f / H);
