@@ -388,13 +388,12 @@ def guided_workload(args):
     r.close()
 
 
-def cdf_workload(args):
-    """The env-map CDF build (k_envmap_rows: row CDFs and their inversion,
-    then k_envmap_marginal) on seeded log-normal maps with a sun at the two
-    sizes the configurations use, 2048x1024 and 8192x4096, FP32 and FP64
-    normalisation; each build repeated --steps times after a warm-up, timed
-    with HIP events on the build stream around both launches. Algorithmic
-    bytes: 12 B of RGB texel read + 2 B of index written per texel."""
+def cdf_timings(steps):
+    """Env-map CDF builds (k_envmap_rows + k_envmap_marginal) of seeded
+    log-normal maps with a sun at 2048x1024 and 8192x4096, FP32 and FP64
+    normalisation, each rebuilt `steps` times on a context of its own and
+    timed with HIP events on its stream around both launches; GB/s from the
+    algorithmic 14 B per RGB texel (12 B read, 2 B index written)."""
     from ratbite_amd import Context
     from ratbite_amd.envmaps import lognormal_envmap
     res = {}
@@ -403,13 +402,24 @@ def cdf_workload(args):
             tex = lognormal_envmap(w, h, sigma=2.0, seed=3, sun=True)
             for fp64 in (False, True):
                 ctx.envmap_build(tex, fp64=fp64)
-                ctx.envmap_rebuild(fp64=fp64, iterations=max(1, args.steps))
+                ctx.envmap_rebuild(fp64=fp64, iterations=steps)
                 ms = ctx.stats()["cdfMs"]
                 gbs = 14.0 * w * h / (ms * 1e-3) / 1e9
                 key = "%dx%d_%s" % (w, h, "fp64" if fp64 else "fp32")
                 res[key] = {"ms": ms, "GB_s": gbs, "frac": gbs / HBM_PEAK_GBS}
                 print("cdf %s: %.4f ms, %.0f GB/s, %.1f%% of peak" % (key, ms, gbs, 100 * gbs / HBM_PEAK_GBS),
                       file=sys.stderr, flush=True)
+    return res
+
+
+def cdf_workload(args):
+    """The env-map CDF build (k_envmap_rows: row CDFs and their inversion,
+    then k_envmap_marginal) on seeded log-normal maps with a sun at the two
+    sizes the configurations use, 2048x1024 and 8192x4096, FP32 and FP64
+    normalisation; each build repeated --steps times after a warm-up, timed
+    with HIP events on the build stream around both launches. Algorithmic
+    bytes: 12 B of RGB texel read + 2 B of index written per texel."""
+    res = cdf_timings(max(1, args.steps))
     top = res["2048x1024_fp32"]
     out = {"metric": "env-map CDF build GB/s", "value": top["GB_s"], "unit": "GB/s", "n_gpus": 1,
            "steps": args.steps, "warmup": 1, "ms_per_step": top["ms"], "higher_is_better": True, "scaling": "weak",
@@ -674,6 +684,10 @@ def main():
                                      args.train_iters, args.train_steps, args.train_batch, args.train_lr,
                                      bool(args.fixed_selection))
         out["guided"]["workload"] = "the same scene and frame size, plain vs guided, no-drop sample semantics"
+    # the env-map CDF build at the configurations' map sizes (its own
+    # context; the frame above is unaffected)
+    out["envmap_cdf"] = {"kernels": "k_envmap_rows + k_envmap_marginal", "bytes_per_texel": 14,
+                         "sizes": cdf_timings(20)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, args, r.light_tree())
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
